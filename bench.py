@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Throughput of ExaMiniMD's time step on one MI355X.
+
+Workload (default): LJ melt, 4 000 000 atoms (FCC 100^3 cells, density
+0.8442, cutoff 2.5, skin 0.3, rebuild every 20 steps, dt 0.005, T* 1.44,
+full CSR neighbour list, serial periodic images), run in FP64 and in mixed
+precision (FP32 pair arithmetic and accumulation, FP64 positions/velocities).
+A step is one velocity-Verlet step of the whole system; the timed region is
+K steps with the state resident in HBM.  `value` is the mixed-precision
+atom-steps/s; the FP64 figure and the force error of the mixed path stand
+beside it.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload lj4m|lj32k|snap2k]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6     # MI355X vector FP64
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="lj4m", choices=["lj4m", "lj32k"])
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def lj_run(emd, decks, tmpdir, n, precision, warmup, steps):
+    deck = decks.write(os.path.join(tmpdir, f"in.lj.{precision}"),
+                       decks.lj_deck(n=n, steps=steps))
+    app = emd.App(["-il", deck, "--neigh-type", "CSR", "--comm-type", "SERIAL",
+                   "--precision", precision]).init()
+    app.set_phase_sync(False)
+    L = emd.hip_lib()
+    ctx = app.ctx
+    if warmup:
+        app.run(warmup)
+    L.emd_synchronize(ctx)
+    L.emd_profile_enable(ctx, 1)
+    L.emd_profile_reset(ctx)
+    t0 = time.perf_counter()
+    app.run(steps)
+    L.emd_synchronize(ctx)
+    wall = time.perf_counter() - t0
+    prof = {}
+    buf = C.create_string_buffer(1 << 16)
+    L.emd_profile_report(ctx, buf, len(buf))
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.split()
+        prof[name] = (int(cnt), float(ms))
+    L.emd_profile_enable(ctx, 0)
+    row, num, ent = app.neigh()
+    natoms = app.natoms
+    nlist = float(len(ent)) / natoms
+    return app, wall, prof, nlist
+
+
+def force_error(emd, decks, tmpdir, n):
+    """force RMS error of the mixed path against FP64 on identical positions"""
+    import numpy as np
+    res = {}
+    for prec in ("fp64", "mixed"):
+        deck = decks.write(os.path.join(tmpdir, f"in.lj.err.{prec}"), decks.lj_deck(n=n, steps=0))
+        app = emd.App(["-il", deck, "--neigh-type", "CSR", "--precision", prec]).init()
+        res[prec] = app.arrays()
+        app.close()
+    df = res["mixed"]["f"] - res["fp64"]["f"]
+    return float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+
+
+def cpu_baseline(n, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if threads <= 0:
+        threads = min(os.cpu_count() or 1, 16)
+    cfg = pyoracle.make_config(n=(n, n, n), gaussian=True, nthreads=threads)
+    m = pyoracle.MD(cfg)
+    m.init()
+    t0 = time.perf_counter()
+    m.run(steps)
+    dt = time.perf_counter() - t0
+    natoms = m.system.N
+    m.close()
+    return {"value": natoms * steps / dt, "unit": "atom-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"CPU oracle (C/OpenMP restatement of the reference), same LJ "
+                      f"{natoms}-atom workload, {steps} steps incl. 1 rebuild, {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is design-only)"}))
+        sys.exit(2)
+    import examinimd_amd as emd
+    from examinimd_amd import decks
+
+    n = 100 if a.workload == "lj4m" else 20
+    tmpdir = tempfile.mkdtemp(prefix="emd_bench_")
+    out = {}
+    for prec in ("fp64", "mixed"):
+        app, wall, prof, nlist = lj_run(emd, decks, tmpdir, n, prec, a.warmup, a.steps)
+        natoms = app.natoms
+        kname = "lj_force_mixed" if prec == "mixed" else "lj_force_fp64"
+        cnt, ms = prof.get(kname, (0, 0.0))
+        out[prec] = {"wall": wall, "natoms": natoms, "prof": prof, "nlist": nlist,
+                     "kernel": kname, "kernel_launches": cnt, "kernel_ms": ms}
+        app.close()
+
+    mixed, fp64 = out["mixed"], out["fp64"]
+    natoms = mixed["natoms"]
+    value = natoms * a.steps / mixed["wall"]
+    value_fp64 = natoms * a.steps / fp64["wall"]
+    # roofline of the dominant kernel (LJ force, mixed): algorithmic bytes per
+    # atom = 4*nlist (CSR entries) + 8 (row offset, length) + 24 (x_i)
+    # + 24 (each x_j fetched once) + 24 (f32 read-modify-write)
+    bytes_per_atom = 4.0 * mixed["nlist"] + 8 + 24 + 24 + 24
+    avg_ms = mixed["kernel_ms"] / max(mixed["kernel_launches"], 1)
+    achieved = bytes_per_atom * natoms / (avg_ms * 1e-3) / 1e9
+    ferr = force_error(emd, decks, tmpdir, 20)
+
+    line = {
+        "metric": "atom-steps/s on 1 MI355X, LJ 4M & SNAP 2k atoms: mixed vs FP64 at fixed error",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * mixed["wall"] / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair math / f64 state",
+        "data": "synthetic FCC lattice, seeded Gaussian velocities",
+        "config": {"workload": f"LJ melt {natoms} atoms (FCC {n}^3, rho 0.8442, rc 2.5, "
+                               f"skin 0.3, rebuild every 20), mixed precision",
+                   "atoms": natoms, "neighbors_per_atom": round(mixed["nlist"], 3)},
+        "fp64": {"value": value_fp64, "ms_per_step": 1e3 * fp64["wall"] / a.steps,
+                 "dtype": "f64"},
+        "mixed_over_fp64": value / value_fp64,
+        "mixed_force_rms_error_vs_fp64": ferr,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": mixed["kernel"], "avg_ms": avg_ms,
+                     "algorithmic_bytes_per_atom": bytes_per_atom},
+        "kernels_ms_per_step": {k: v[1] / a.steps for k, v in mixed["prof"].items()},
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n, a.cpu_steps, a.cpu_threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

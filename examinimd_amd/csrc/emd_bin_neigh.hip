@@ -1,0 +1,406 @@
+// emd_bin_neigh.hip -- cell binning (counting sort) and neighbour lists.
+//
+// Binning: bin id per atom (Kokkos BinOp3D arithmetic: int(mul*(x-min)),
+// mul = nbin/(max-min), x-major flattening), atomic histogram, exclusive
+// scan -> offsets, atomic scatter, then a per-bin insertion sort so that the
+// permutation lists each bin's atoms in ascending index (the order the
+// reference produces on its serial backend; deterministic here).
+//
+// Neighbour build: one 256-thread block per interior bin.  The block stages
+// the 27 stencil bins' candidates (index + FP64 position) in LDS in stencil
+// order; each wavefront takes atoms of the centre bin and tests 64
+// candidates per step, compacting hits with ballot + popcount so every row
+// comes out in stencil-then-permutation order.  Two passes (count, fill)
+// with an exclusive scan between them, as NeighborCSR does.
+#include "emd_internal.hpp"
+
+// ------------------------------------------------------------------ binning
+__global__ void k_bin_assign(const double *__restrict__ x, int r0, int n, double minx,
+                             double miny, double minz, double mulx, double muly, double mulz,
+                             int nbx, int nby, int nbz, int *__restrict__ binid,
+                             int *__restrict__ count, int *__restrict__ err) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *xi = x + 3 * (size_t)(r0 + i);
+  int bx = (int)(mulx * (xi[0] - minx));
+  int by = (int)(muly * (xi[1] - miny));
+  int bz = (int)(mulz * (xi[2] - minz));
+  if (bx < 0 || bx >= nbx || by < 0 || by >= nby || bz < 0 || bz >= nbz) {
+    atomicOr(err, 1);
+    bx = bx < 0 ? 0 : (bx >= nbx ? nbx - 1 : bx);
+    by = by < 0 ? 0 : (by >= nby ? nby - 1 : by);
+    bz = bz < 0 ? 0 : (bz >= nbz ? nbz - 1 : bz);
+  }
+  const int b = (bx * nby + by) * nbz + bz;
+  binid[i] = b;
+  atomicAdd(&count[b], 1);
+}
+
+struct bin_offsets_op {
+  const int *count;
+  int *offset;
+  int *cursor;
+  int *maxc;
+  __device__ int value(int b) const { return count[b]; }
+  __device__ void emit(int b, int excl, int v) const {
+    offset[b] = excl;
+    cursor[b] = excl;
+    if (v) atomicMax(maxc, v);
+  }
+};
+
+__global__ void k_bin_fill(const int *__restrict__ binid, int n, int *__restrict__ cursor,
+                           int *__restrict__ permute) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int slot = atomicAdd(&cursor[binid[i]], 1);
+  permute[slot] = i;
+}
+
+__global__ void k_bin_sort(const int *__restrict__ count, const int *__restrict__ offset,
+                           int *__restrict__ permute, int nbins) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int *p = permute + offset[b];
+  const int c = count[b];
+  for (int k = 1; k < c; k++) {
+    const int v = p[k];
+    int m = k - 1;
+    while (m >= 0 && p[m] > v) {
+      p[m + 1] = p[m];
+      m--;
+    }
+    p[m + 1] = v;
+  }
+}
+
+// BinSort::sort over x, v, f, type, id, q: new[i] = old[permute[i]]
+__global__ void k_bin_gather(const int *__restrict__ perm, int r0, int n,
+                             const double *__restrict__ x, const double *__restrict__ v,
+                             const double *__restrict__ f, const double *__restrict__ q,
+                             const int *__restrict__ type, const int *__restrict__ id,
+                             double *__restrict__ tx, double *__restrict__ tv,
+                             double *__restrict__ tf, double *__restrict__ tq,
+                             int *__restrict__ ttype, int *__restrict__ tid) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t s = (size_t)r0 + perm[i];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    tx[3 * (size_t)i + d] = x[3 * s + d];
+    tv[3 * (size_t)i + d] = v[3 * s + d];
+    tf[3 * (size_t)i + d] = f[3 * s + d];
+  }
+  tq[i] = q[s];
+  ttype[i] = type[s];
+  tid[i] = id[s];
+}
+
+__global__ void k_fill_int(int *p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+extern "C" emd_status emd_binning(emd_ctx *ctx, emd_system *s, double dx_in, double dy_in,
+                                  double dz_in, int halo_depth, int do_local, int do_ghost,
+                                  int sort, emd_bins *b) {
+  if (!ctx || !s || !b) return EMD_ERR_ARG;
+  if (!(do_local || do_ghost)) return EMD_OK;
+  EMD_ARG(ctx, dx_in > 0 && dy_in > 0 && dz_in > 0, "bin sizes must be positive");
+  b->nhalo = halo_depth;
+  const int r0 = do_local ? 0 : s->n_local;
+  const int r1 = do_ghost ? s->n_local + s->n_ghost : s->n_local;
+  const int n = r1 - r0;
+  const double sub[3] = {s->hi[0] - s->lo[0], s->hi[1] - s->lo[1], s->hi[2] - s->lo[2]};
+  const double din[3] = {dx_in, dy_in, dz_in};
+  int nb[3];
+  double dd[3];
+  for (int d = 0; d < 3; d++) {
+    nb[d] = (int)(sub[d] / din[d]);
+    if (nb[d] == 0) nb[d] = 1;
+    dd[d] = sub[d] / nb[d];
+  }
+  for (int d = 0; d < 3; d++) nb[d] += 2 * halo_depth;
+  const double eps = dd[0] / 1000;  // binning_kksort.cpp:92 uses dx for every dimension
+  double mn[3], mx[3], mul[3];
+  for (int d = 0; d < 3; d++) {
+    mn[d] = -dd[d] * halo_depth - eps + s->lo[d];
+    mx[d] = dd[d] * halo_depth + eps + s->hi[d];
+    mul[d] = 1.0 * nb[d] / (mx[d] - mn[d]);
+  }
+  b->nbinx = nb[0]; b->nbiny = nb[1]; b->nbinz = nb[2];
+  b->minx = mn[0]; b->maxx = mx[0]; b->miny = mn[1]; b->maxy = mx[1];
+  b->minz = mn[2]; b->maxz = mx[2];
+  const long nbt = (long)nb[0] * nb[1] * nb[2];
+  EMD_ARG(ctx, nbt < (1l << 30), "bin grid too large");
+  b->nbins = (int)nbt;
+  b->n_range = n;
+  int cap_b2 = b->cap_bins;
+  EMD_TRY(emd_ensure_int(ctx, &b->bincount, &b->cap_bins, nbt));
+  EMD_TRY(emd_ensure_int(ctx, &b->binoffsets, &cap_b2, nbt));
+  b->cap_bins = b->cap_bins < cap_b2 ? b->cap_bins : cap_b2;
+  int cap_r2 = b->cap_range;
+  EMD_TRY(emd_ensure_int(ctx, &b->permute, &b->cap_range, n));
+  EMD_TRY(emd_ensure_int(ctx, &b->binid, &cap_r2, n));
+  b->cap_range = b->cap_range < cap_r2 ? b->cap_range : cap_r2;
+  // cursor scratch
+  EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)nbt * sizeof(int)));
+  int *cursor = (int *)ctx->d_tmp2;
+
+  EMD_LAUNCH(ctx, "binning", k_fill_int, emd_blocks(nbt, 256), 256, 0, b->bincount, (int)nbt, 0);
+  EMD_LAUNCH(ctx, "binning", k_fill_int, 1, 64, 0, ctx->d_small + 1, 2, 0);
+  if (n > 0)
+    EMD_LAUNCH(ctx, "binning", k_bin_assign, emd_blocks(n, 256), 256, 0, s->x, r0, n, mn[0], mn[1],
+               mn[2], mul[0], mul[1], mul[2], nb[0], nb[1], nb[2], b->binid, b->bincount,
+               ctx->d_small + 2);
+  bin_offsets_op op{b->bincount, b->binoffsets, cursor, ctx->d_small + 1};
+  EMD_TRY(emd_scan(ctx, op, (int)nbt, "binning", nullptr));
+  if (n > 0) {
+    EMD_LAUNCH(ctx, "binning", k_bin_fill, emd_blocks(n, 256), 256, 0, b->binid, n, cursor,
+               b->permute);
+    EMD_LAUNCH(ctx, "binning", k_bin_sort, emd_blocks(nbt, 128), 128, 0, b->bincount,
+               b->binoffsets, b->permute, (int)nbt);
+  }
+  EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, 2 * sizeof(int),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  b->max_count = ctx->h_small[1];
+  if (ctx->h_small[2])
+    return emd_set_error(ctx, EMD_ERR_OVERFLOW,
+                         "emd_binning: an atom lies outside the bin grid (lost atom?)");
+
+  if (sort && n > 0) {
+    size_t bytes = (size_t)n * (72 + 8 + 8);
+    EMD_TRY(emd_ensure(ctx, &ctx->d_tmp, &ctx->tmp_cap, bytes));
+    char *t = (char *)ctx->d_tmp;
+    double *tx = (double *)t, *tv = tx + 3 * (size_t)n, *tf = tv + 3 * (size_t)n,
+           *tq = tf + 3 * (size_t)n;
+    int *tt = (int *)(tq + n), *ti = tt + n;
+    EMD_LAUNCH(ctx, "binning_sort", k_bin_gather, emd_blocks(n, 256), 256, 0, b->permute, r0, n,
+               s->x, s->v, s->f, s->q, s->type, s->id, tx, tv, tf, tq, tt, ti);
+    hipStream_t st = ctx->stream;
+    const size_t o3 = 3 * (size_t)r0, n3 = 3 * (size_t)n * sizeof(double);
+    EMD_HIP(ctx, hipMemcpyAsync(s->x + o3, tx, n3, hipMemcpyDeviceToDevice, st));
+    EMD_HIP(ctx, hipMemcpyAsync(s->v + o3, tv, n3, hipMemcpyDeviceToDevice, st));
+    EMD_HIP(ctx, hipMemcpyAsync(s->f + o3, tf, n3, hipMemcpyDeviceToDevice, st));
+    EMD_HIP(ctx, hipMemcpyAsync(s->q + r0, tq, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+    EMD_HIP(ctx, hipMemcpyAsync(s->type + r0, tt, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    EMD_HIP(ctx, hipMemcpyAsync(s->id + r0, ti, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  }
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_bins_free(emd_ctx *ctx, emd_bins *b) {
+  if (!ctx || !b) return EMD_ERR_ARG;
+  hipStreamSynchronize(ctx->stream);
+  int *ptrs[] = {b->bincount, b->binoffsets, b->permute, b->binid};
+  for (int *p : ptrs)
+    if (p) hipFree(p);
+  b->bincount = b->binoffsets = b->permute = b->binid = nullptr;
+  b->cap_bins = b->cap_range = 0;
+  return EMD_OK;
+}
+
+// ---------------------------------------------------------------- neighbours
+constexpr int NB_THREADS = 256;
+constexpr int NB_WAVES = NB_THREADS / EMD_WAVE;
+constexpr int NB_MAXC = 1024;  // candidates staged per LDS chunk
+
+struct neigh_geom {
+  int nbx, nby, nbz, nhalo;  // full grid incl. halo
+  int nix, niy, niz;         // interior bins
+};
+
+// NeighborCSR Tag{Count,Fill}Neigh{Full,Half}, src/neighbor_types/neighbor_csr.h:176-358
+template <bool HALF, bool FILL>
+__global__ void __launch_bounds__(NB_THREADS)
+k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
+        const int *__restrict__ binoffsets, const int *__restrict__ permute, neigh_geom g,
+        int n_local, double cutsq, int newton, int *__restrict__ num, const int *__restrict__ row,
+        int pitch, int *__restrict__ entries, int *__restrict__ cursor, int *__restrict__ maxrow) {
+#pragma clang fp contract(off)
+  __shared__ double s_x[NB_MAXC], s_y[NB_MAXC], s_z[NB_MAXC];
+  __shared__ int s_j[NB_MAXC];
+  __shared__ int s_off[27], s_pre[28];
+
+  const int lr = emd_xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = lr / (g.niy * g.niz) + g.nhalo;
+  const int by = (lr / g.niz) % g.niy + g.nhalo;
+  const int bz = lr % g.niz + g.nhalo;
+  const int tid = threadIdx.x;
+  const int wave = tid / EMD_WAVE, lane = tid % EMD_WAVE;
+
+  if (tid < 27) {
+    const int ddx = tid / 9 - 1, ddy = (tid / 3) % 3 - 1, ddz = tid % 3 - 1;
+    const int b = ((bx + ddx) * g.nby + (by + ddy)) * g.nbz + (bz + ddz);
+    s_off[tid] = binoffsets[b];
+    s_pre[tid + 1] = bincount[b];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    s_pre[0] = 0;
+    for (int k = 1; k <= 27; k++) s_pre[k] += s_pre[k - 1];
+  }
+  __syncthreads();
+  const int T = s_pre[27];
+  const int cb = (bx * g.nby + by) * g.nbz + bz;
+  const int c_off = binoffsets[cb], c_cnt = bincount[cb];
+  const unsigned long long lt = emd_lanemask_lt();
+  int wave_max = 0;  // longest row seen by this wavefront (count pass)
+
+  for (int cs = 0; cs < T || (cs == 0 && T == 0); cs += NB_MAXC) {
+    const int cn = (T - cs) < NB_MAXC ? (T - cs) : NB_MAXC;
+    for (int c = tid; c < cn; c += NB_THREADS) {
+      const int cc = cs + c;
+      int k = 0;
+      while (s_pre[k + 1] <= cc) k++;
+      const int j = permute[s_off[k] + (cc - s_pre[k])];
+      s_j[c] = j;
+      s_x[c] = x[3 * (size_t)j];
+      s_y[c] = x[3 * (size_t)j + 1];
+      s_z[c] = x[3 * (size_t)j + 2];
+    }
+    __syncthreads();
+    const bool last = (cs + NB_MAXC >= T);
+    for (int bi = wave; bi < c_cnt; bi += NB_WAVES) {
+      const int i = permute[c_off + bi];
+      if (i >= n_local) continue;
+      const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
+      int run = 0;
+      if (cs > 0) run = FILL ? cursor[i] : num[i];
+      size_t base = 0;
+      if (FILL) base = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+      for (int c0 = 0; c0 < cn; c0 += EMD_WAVE) {
+        const int c = c0 + lane;
+        bool keep = false;
+        int j = -1;
+        if (c < cn) {
+          j = s_j[c];
+          const double x_j = s_x[c], y_j = s_y[c], z_j = s_z[c];
+          const double dx = x_i - x_j, dy = y_i - y_j, dz = z_i - z_j;
+          const double rsq = dx * dx + dy * dy + dz * dz;
+          if (HALF) {
+            const bool drop =
+                ((j == i) || (j < n_local || newton)) &&
+                !((x_j > x_i) || ((x_j == x_i) && ((y_j > y_i) || ((y_j == y_i) && (z_j > z_i)))));
+            keep = !drop && (rsq <= cutsq);
+          } else {
+            keep = (rsq <= cutsq) && (i != j);
+          }
+        }
+        const unsigned long long m = __ballot(keep);
+        if (FILL && keep) entries[base + run + __popcll(m & lt)] = j;
+        run += __popcll(m);
+      }
+      if (lane == 0) {
+        if (FILL) {
+          if (!last) cursor[i] = run;
+        } else {
+          num[i] = run;
+        }
+      }
+      if (!FILL && last && run > wave_max) wave_max = run;
+    }
+    __syncthreads();
+    if (T == 0) break;
+  }
+  // one atomic per wavefront for the longest row (a per-atom atomic on a
+  // single address serialises millions of updates)
+  if (!FILL && lane == 0 && wave_max > 0) atomicMax(maxrow, wave_max);
+}
+
+struct row_offsets_op {
+  const int *num;
+  int *row;
+  int n;
+  __device__ int value(int i) const { return num[i]; }
+  __device__ void emit(int i, int excl, int v) const {
+    row[i] = excl;
+    if (i == n - 1) row[n] = excl + v;
+  }
+};
+
+extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const emd_bins *b,
+                                      double neigh_cut, int half, int newton, int layout,
+                                      emd_neigh *nl) {
+  if (!ctx || !s || !b || !nl) return EMD_ERR_ARG;
+  EMD_ARG(ctx, layout == EMD_NEIGH_CSR || layout == EMD_NEIGH_2D, "bad layout");
+  const int n = s->n_local;
+  nl->layout = layout;
+  nl->n = n;
+  neigh_geom g;
+  g.nbx = b->nbinx; g.nby = b->nbiny; g.nbz = b->nbinz; g.nhalo = b->nhalo;
+  g.nix = b->nbinx - 2 * b->nhalo; g.niy = b->nbiny - 2 * b->nhalo; g.niz = b->nbinz - 2 * b->nhalo;
+  const long nib = (long)g.nix * g.niy * g.niz;
+  EMD_ARG(ctx, b->nhalo >= 1, "neighbour build needs a halo of at least one bin");
+  const double cutsq = neigh_cut * neigh_cut;
+
+  int rcap = nl->cap_rows;
+  EMD_TRY(emd_ensure_int(ctx, &nl->num, &nl->cap_rows, n + 1));
+  int rcap2 = rcap;
+  EMD_TRY(emd_ensure_int(ctx, &nl->row, &rcap2, n + 1));
+  nl->cap_rows = nl->cap_rows < rcap2 ? nl->cap_rows : rcap2;
+  EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)(n + 1) * sizeof(int)));
+  int *cursor = (int *)ctx->d_tmp2;
+  if (n > 0) EMD_LAUNCH(ctx, "neigh_count", k_fill_int, emd_blocks(n, 256), 256, 0, nl->num, n, 0);
+  EMD_LAUNCH(ctx, "neigh_count", k_fill_int, 1, 64, 0, ctx->d_small + 4, 1, 0);
+
+#define NB_ARGS(F)                                                                          \
+  s->x, b->bincount, b->binoffsets, b->permute, g, n, cutsq, newton, nl->num,              \
+      (F) ? (layout == EMD_NEIGH_CSR ? nl->row : nullptr) : nullptr, nl->maxneighs,        \
+      nl->entries, cursor, ctx->d_small + 4
+  if (nib > 0) {
+    if (half) EMD_LAUNCH(ctx, "neigh_count", (k_neigh<true, false>), nib, NB_THREADS, 0, NB_ARGS(0));
+    else EMD_LAUNCH(ctx, "neigh_count", (k_neigh<false, false>), nib, NB_THREADS, 0, NB_ARGS(0));
+  }
+  // row offsets (CSR) and the longest row
+  int total = 0;
+  row_offsets_op op{nl->num, nl->row, n};
+  if (n > 0) EMD_TRY(emd_scan(ctx, op, n, "neigh_offsets", &total));
+  else total = 0;
+  EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 4, ctx->d_small + 4, sizeof(int),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int maxrow = ctx->h_small[4];
+  long need;
+  if (layout == EMD_NEIGH_CSR) {
+    nl->maxneighs = maxrow;
+    need = total;
+  } else {
+    // Neighbor2D grows maxneighs x1.2 until every row fits (neighbor_2d.h:304-330)
+    if (nl->maxneighs <= 0) nl->maxneighs = 16;
+    while (maxrow > nl->maxneighs) nl->maxneighs = (int)(maxrow * 1.2);
+    need = (long)n * nl->maxneighs;
+  }
+  nl->total = total;
+  if (need > nl->cap_entries || !nl->entries) {
+    if (nl->entries) {
+      EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      EMD_HIP(ctx, hipFree(nl->entries));
+    }
+    nl->entries = nullptr;
+    long want = need + need / 8 + 64;
+    if (hipMalloc(&nl->entries, (size_t)want * sizeof(int)) != hipSuccess)
+      return emd_set_error(ctx, EMD_ERR_ALLOC, "neighbour list: hipMalloc of %ld ints failed",
+                           want);
+    nl->cap_entries = want;
+  }
+  if (nib > 0) {
+    if (half) EMD_LAUNCH(ctx, "neigh_fill", (k_neigh<true, true>), nib, NB_THREADS, 0, NB_ARGS(1));
+    else EMD_LAUNCH(ctx, "neigh_fill", (k_neigh<false, true>), nib, NB_THREADS, 0, NB_ARGS(1));
+  }
+#undef NB_ARGS
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_neigh_free(emd_ctx *ctx, emd_neigh *nl) {
+  if (!ctx || !nl) return EMD_ERR_ARG;
+  hipStreamSynchronize(ctx->stream);
+  if (nl->row) hipFree(nl->row);
+  if (nl->num) hipFree(nl->num);
+  if (nl->entries) hipFree(nl->entries);
+  nl->row = nl->num = nl->entries = nullptr;
+  nl->cap_entries = 0;
+  nl->cap_rows = 0;
+  return EMD_OK;
+}

@@ -1,0 +1,383 @@
+// emd_core.hip -- context, device memory, kernel timing and the System image.
+#include "emd_internal.hpp"
+
+// -------------------------------------------------------------- errors
+emd_status emd_set_error(emd_ctx *ctx, emd_status code, const char *fmt, ...) {
+  if (ctx) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+extern "C" const char *emd_last_error(const emd_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+extern "C" const char *emd_version(void) { return "examinimd-amd 0.1 (gfx950)"; }
+
+// ------------------------------------------------------------ scan helper
+__global__ void __launch_bounds__(1024) k_scan_blocks(int *bs, int nb, int *total) {
+  __shared__ int s_w[16];
+  __shared__ int s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  const int wid = threadIdx.x / EMD_WAVE;
+  for (int base = 0; base < nb; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = (i < nb) ? bs[i] : 0;
+    int inc = emd_wave_incl_scan(v);
+    if (__lane_id() == EMD_WAVE - 1) s_w[wid] = inc;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wid; w++) woff += s_w[w];
+    int carry = s_carry;
+    if (i < nb) bs[i] = carry + woff + inc - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) s_carry = carry + woff + inc;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) total[0] = s_carry;
+}
+
+// ------------------------------------------------------------- context
+extern "C" emd_status emd_create(int device, emd_ctx **out) {
+  if (!out) return EMD_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return EMD_ERR_HIP;
+  if (device < 0 || device >= ndev) return EMD_ERR_ARG;
+  emd_ctx *ctx = new emd_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete ctx; return EMD_ERR_HIP; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return EMD_ERR_HIP;
+  }
+  bool ok = hipMalloc(&ctx->d_small, 64 * sizeof(int)) == hipSuccess &&
+            hipMalloc(&ctx->d_dsmall, 64 * sizeof(double)) == hipSuccess &&
+            hipHostMalloc(&ctx->h_small, 64 * sizeof(int), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&ctx->h_dsmall, 64 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+  if (!ok) {
+    emd_destroy(ctx);
+    return EMD_ERR_ALLOC;
+  }
+  hipMemset(ctx->d_small, 0, 64 * sizeof(int));
+  // lanes per atom of the LJ kernel: EMD_LJ_LANES (both precisions) or
+  // EMD_LJ_LANES_FP64 / EMD_LJ_LANES_MIXED; 0 or unset = built-in default
+  if (const char *e = getenv("EMD_LJ_LANES")) ctx->lj_lanes[0] = ctx->lj_lanes[1] = atoi(e);
+  if (const char *e = getenv("EMD_LJ_LANES_FP64")) ctx->lj_lanes[0] = atoi(e);
+  if (const char *e = getenv("EMD_LJ_LANES_MIXED")) ctx->lj_lanes[1] = atoi(e);
+  for (int p = 0; p < 2; p++) {
+    const int k = ctx->lj_lanes[p];
+    if (!(k == 0 || k == 1 || k == 2 || k == 4 || k == 8 || k == 16)) ctx->lj_lanes[p] = 0;
+  }
+  *out = ctx;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_destroy(emd_ctx *ctx) {
+  if (!ctx) return EMD_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  for (auto &r : ctx->prof_pending) { hipEventDestroy(r.start); hipEventDestroy(r.stop); }
+  for (auto e : ctx->prof_pool) hipEventDestroy(e);
+  if (ctx->d_small) hipFree(ctx->d_small);
+  if (ctx->d_dsmall) hipFree(ctx->d_dsmall);
+  if (ctx->h_small) hipHostFree(ctx->h_small);
+  if (ctx->h_dsmall) hipHostFree(ctx->h_dsmall);
+  if (ctx->d_scan) hipFree(ctx->d_scan);
+  if (ctx->d_tmp) hipFree(ctx->d_tmp);
+  if (ctx->d_tmp2) hipFree(ctx->d_tmp2);
+  if (ctx->d_red) hipFree(ctx->d_red);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_synchronize(emd_ctx *ctx) {
+  if (!ctx) return EMD_ERR_ARG;
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return EMD_OK;
+}
+
+extern "C" void *emd_get_stream(emd_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+// --------------------------------------------------------------- memory
+emd_status emd_ensure(emd_ctx *ctx, void **p, size_t *cap, size_t bytes) {
+  if (bytes <= *cap && *p) return EMD_OK;
+  if (*p) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_HIP(ctx, hipFree(*p));
+  }
+  *p = nullptr;
+  *cap = 0;
+  size_t want = bytes + bytes / 8 + 256;
+  if (hipMalloc(p, want) != hipSuccess)
+    return emd_set_error(ctx, EMD_ERR_ALLOC, "hipMalloc of %zu bytes failed", want);
+  *cap = want;
+  return EMD_OK;
+}
+
+emd_status emd_ensure_int(emd_ctx *ctx, int **p, int *cap, long count) {
+  if (count <= *cap && *p) return EMD_OK;
+  size_t c = 0;
+  void *q = *p;
+  if (q) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_HIP(ctx, hipFree(q));
+  }
+  *p = nullptr;
+  *cap = 0;
+  long want = count + count / 8 + 64;
+  if (hipMalloc(&q, (size_t)want * sizeof(int)) != hipSuccess)
+    return emd_set_error(ctx, EMD_ERR_ALLOC, "hipMalloc of %ld ints failed", want);
+  (void)c;
+  *p = (int *)q;
+  *cap = (int)want;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_malloc(emd_ctx *ctx, void **ptr, size_t bytes) {
+  if (!ctx || !ptr) return EMD_ERR_ARG;
+  if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess)
+    return emd_set_error(ctx, EMD_ERR_ALLOC, "hipMalloc of %zu bytes failed", bytes);
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_free(emd_ctx *ctx, void *ptr) {
+  if (!ctx) return EMD_ERR_ARG;
+  if (ptr) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_HIP(ctx, hipFree(ptr));
+  }
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_memcpy_h2d(emd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return EMD_ERR_ARG;
+  if (!bytes) return EMD_OK;
+  EMD_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_memcpy_d2h(emd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return EMD_ERR_ARG;
+  if (!bytes) return EMD_OK;
+  EMD_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_memcpy_d2d(emd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return EMD_ERR_ARG;
+  if (!bytes) return EMD_OK;
+  EMD_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return EMD_OK;
+}
+
+// -------------------------------------------------------------- timing
+static int prof_id(emd_ctx *ctx, const char *name) {
+  auto it = ctx->prof_ids.find(name);
+  if (it != ctx->prof_ids.end()) return it->second;
+  int id = (int)ctx->prof_names.size();
+  ctx->prof_names.push_back(name);
+  ctx->prof_ids[name] = id;
+  ctx->prof_ms.push_back(0.0);
+  ctx->prof_count.push_back(0);
+  return id;
+}
+
+static hipEvent_t prof_event(emd_ctx *ctx) {
+  if (!ctx->prof_pool.empty()) {
+    hipEvent_t e = ctx->prof_pool.back();
+    ctx->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void emd_prof_begin(emd_ctx *ctx, const char *name, emd_prof_rec *rec) {
+  rec->name = -1;
+  if (!ctx->profile) return;
+  rec->name = prof_id(ctx, name);
+  rec->start = prof_event(ctx);
+  rec->stop = prof_event(ctx);
+  if (rec->start) hipEventRecord(rec->start, ctx->stream);
+}
+
+void emd_prof_end(emd_ctx *ctx, emd_prof_rec *rec) {
+  if (rec->name < 0 || !rec->start || !rec->stop) return;
+  hipEventRecord(rec->stop, ctx->stream);
+  ctx->prof_pending.push_back(*rec);
+}
+
+static void prof_collect(emd_ctx *ctx) {
+  if (ctx->prof_pending.empty()) return;
+  hipStreamSynchronize(ctx->stream);
+  for (auto &r : ctx->prof_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess) {
+      ctx->prof_ms[r.name] += ms;
+      ctx->prof_count[r.name] += 1;
+    }
+    ctx->prof_pool.push_back(r.start);
+    ctx->prof_pool.push_back(r.stop);
+  }
+  ctx->prof_pending.clear();
+}
+
+extern "C" emd_status emd_profile_enable(emd_ctx *ctx, int on) {
+  if (!ctx) return EMD_ERR_ARG;
+  prof_collect(ctx);
+  ctx->profile = on != 0;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_profile_reset(emd_ctx *ctx) {
+  if (!ctx) return EMD_ERR_ARG;
+  prof_collect(ctx);
+  for (auto &m : ctx->prof_ms) m = 0.0;
+  for (auto &c : ctx->prof_count) c = 0;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_profile_get(emd_ctx *ctx, const char *name, double *ms, long *n) {
+  if (!ctx || !name) return EMD_ERR_ARG;
+  prof_collect(ctx);
+  auto it = ctx->prof_ids.find(name);
+  if (ms) *ms = it == ctx->prof_ids.end() ? 0.0 : ctx->prof_ms[it->second];
+  if (n) *n = it == ctx->prof_ids.end() ? 0 : ctx->prof_count[it->second];
+  return EMD_OK;
+}
+
+extern "C" size_t emd_profile_report(emd_ctx *ctx, char *buf, size_t len) {
+  if (!ctx) return 0;
+  prof_collect(ctx);
+  std::string s;
+  char line[256];
+  for (size_t i = 0; i < ctx->prof_names.size(); i++) {
+    snprintf(line, sizeof(line), "%s %ld %.6f\n", ctx->prof_names[i].c_str(), ctx->prof_count[i],
+             ctx->prof_ms[i]);
+    s += line;
+  }
+  if (buf && len) {
+    size_t n = s.size() < len - 1 ? s.size() : len - 1;
+    memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  return s.size() + 1;
+}
+
+// --------------------------------------------------------------- system
+template <class T>
+static emd_status grow_one(emd_ctx *ctx, T **p, int old_n, int new_n, int width) {
+  T *q = nullptr;
+  size_t nb = (size_t)new_n * width * sizeof(T);
+  if (hipMalloc(&q, nb ? nb : 1) != hipSuccess)
+    return emd_set_error(ctx, EMD_ERR_ALLOC, "system grow: hipMalloc of %zu bytes failed", nb);
+  size_t ob = (size_t)old_n * width * sizeof(T);
+  if (*p && ob) EMD_HIP(ctx, hipMemcpyAsync(q, *p, ob, hipMemcpyDeviceToDevice, ctx->stream));
+  if (nb > ob) EMD_HIP(ctx, hipMemsetAsync((char *)q + ob, 0, nb - ob, ctx->stream));
+  if (*p) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_HIP(ctx, hipFree(*p));
+  }
+  *p = q;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_system_grow(emd_ctx *ctx, emd_system *s, int n) {
+  if (!ctx || !s || n < 0) return EMD_ERR_ARG;
+  if (n <= s->n_max && s->x) return EMD_OK;
+  int old = s->x ? s->n_max : 0;
+  EMD_TRY(grow_one(ctx, &s->x, old, n, 3));
+  EMD_TRY(grow_one(ctx, &s->v, old, n, 3));
+  EMD_TRY(grow_one(ctx, &s->f, old, n, 3));
+  EMD_TRY(grow_one(ctx, &s->f32, old, n, 3));
+  EMD_TRY(grow_one(ctx, &s->type, old, n, 1));
+  EMD_TRY(grow_one(ctx, &s->id, old, n, 1));
+  EMD_TRY(grow_one(ctx, &s->q, old, n, 1));
+  if (!s->mass) {
+    EMD_HIP(ctx, hipMalloc(&s->mass, 16 * sizeof(double)));
+    EMD_HIP(ctx, hipMemsetAsync(s->mass, 0, 16 * sizeof(double), ctx->stream));
+  }
+  s->n_max = n;
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_system_free(emd_ctx *ctx, emd_system *s) {
+  if (!ctx || !s) return EMD_ERR_ARG;
+  hipStreamSynchronize(ctx->stream);
+  void *ptrs[] = {s->x, s->v, s->f, s->f32, s->type, s->id, s->q, s->mass};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  s->x = s->v = s->f = s->q = s->mass = nullptr;
+  s->f32 = nullptr;
+  s->type = s->id = nullptr;
+  s->n_max = 0;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_system_upload(emd_ctx *ctx, emd_system *s, int n, const double *x,
+                                        const double *v, const int *type, const int *id,
+                                        const double *q) {
+  if (!ctx || !s || n < 0) return EMD_ERR_ARG;
+  EMD_TRY(emd_system_grow(ctx, s, n));
+  hipStream_t st = ctx->stream;
+  if (x) EMD_HIP(ctx, hipMemcpyAsync(s->x, x, (size_t)n * 24, hipMemcpyHostToDevice, st));
+  if (v) EMD_HIP(ctx, hipMemcpyAsync(s->v, v, (size_t)n * 24, hipMemcpyHostToDevice, st));
+  if (type) EMD_HIP(ctx, hipMemcpyAsync(s->type, type, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  if (id) EMD_HIP(ctx, hipMemcpyAsync(s->id, id, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  if (q) EMD_HIP(ctx, hipMemcpyAsync(s->q, q, (size_t)n * 8, hipMemcpyHostToDevice, st));
+  EMD_HIP(ctx, hipStreamSynchronize(st));
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_system_download(emd_ctx *ctx, const emd_system *s, int n, double *x,
+                                          double *v, double *f, int *type, int *id, double *q) {
+  if (!ctx || !s || n < 0 || n > s->n_max) return EMD_ERR_ARG;
+  hipStream_t st = ctx->stream;
+  if (x) EMD_HIP(ctx, hipMemcpyAsync(x, s->x, (size_t)n * 24, hipMemcpyDeviceToHost, st));
+  if (v) EMD_HIP(ctx, hipMemcpyAsync(v, s->v, (size_t)n * 24, hipMemcpyDeviceToHost, st));
+  if (f) EMD_HIP(ctx, hipMemcpyAsync(f, s->f, (size_t)n * 24, hipMemcpyDeviceToHost, st));
+  if (type) EMD_HIP(ctx, hipMemcpyAsync(type, s->type, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  if (id) EMD_HIP(ctx, hipMemcpyAsync(id, s->id, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  if (q) EMD_HIP(ctx, hipMemcpyAsync(q, s->q, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+  EMD_HIP(ctx, hipStreamSynchronize(st));
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_system_set_mass(emd_ctx *ctx, emd_system *s, int ntypes,
+                                          const double *mass) {
+  if (!ctx || !s || ntypes < 1 || ntypes > 16 || !mass) return EMD_ERR_ARG;
+  if (!s->mass) EMD_HIP(ctx, hipMalloc(&s->mass, 16 * sizeof(double)));
+  EMD_HIP(ctx, hipMemcpyAsync(s->mass, mass, ntypes * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  s->ntypes = ntypes;
+  return EMD_OK;
+}
+
+emd_status emd_grow_keep_int(emd_ctx *ctx, int **p, int *cap, long count, long keep) {
+  if (count <= *cap && *p) return EMD_OK;
+  long want = count + count / 4 + 64;
+  int *q = nullptr;
+  if (hipMalloc(&q, (size_t)want * sizeof(int)) != hipSuccess)
+    return emd_set_error(ctx, EMD_ERR_ALLOC, "hipMalloc of %ld ints failed", want);
+  if (*p && keep > 0)
+    EMD_HIP(ctx, hipMemcpyAsync(q, *p, (size_t)keep * sizeof(int), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+  if (*p) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_HIP(ctx, hipFree(*p));
+  }
+  *p = q;
+  *cap = (int)want;
+  return EMD_OK;
+}

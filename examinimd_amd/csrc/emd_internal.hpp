@@ -1,0 +1,267 @@
+// emd_internal.hpp -- shared internals of libemd_hip.so (gfx950 / CDNA4).
+//
+// Context object, error plumbing, HIP-event kernel timing, wavefront
+// primitives and the device-wide scan used by binning, halo compaction and
+// neighbour-row offsets.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "emd_hip.h"
+
+// ----------------------------------------------------------------- context
+struct emd_prof_rec {
+  int name;
+  hipEvent_t start, stop;
+};
+
+struct emd_ctx {
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  char err[1024] = {0};
+
+  // small device + pinned host scratch for scalar results
+  int *d_small = nullptr;      // 64 ints
+  double *d_dsmall = nullptr;  // 64 doubles
+  int *h_small = nullptr;      // pinned, 64 ints
+  double *h_dsmall = nullptr;  // pinned, 64 doubles
+
+  // scan block sums
+  int *d_scan = nullptr;
+  size_t scan_cap = 0;
+  // generic device scratch (grown on demand)
+  void *d_tmp = nullptr;
+  size_t tmp_cap = 0;
+  void *d_tmp2 = nullptr;
+  size_t tmp2_cap = 0;
+  double *d_red = nullptr;     // reduction partials
+  size_t red_cap = 0;
+
+  // kernel timing
+  bool profile = false;
+  std::vector<std::string> prof_names;
+  std::map<std::string, int> prof_ids;
+  std::vector<emd_prof_rec> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  std::vector<double> prof_ms;
+  std::vector<long> prof_count;
+
+  // tuning
+  int lj_lanes[2] = {0, 0};
+};
+
+emd_status emd_set_error(emd_ctx *ctx, emd_status code, const char *fmt, ...);
+
+#define EMD_HIP(ctx, call)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return emd_set_error((ctx), EMD_ERR_HIP, "%s failed: %s (%s:%d)", #call,    \
+                           hipGetErrorString(e_), __FILE__, __LINE__);            \
+  } while (0)
+
+#define EMD_TRY(expr)                       \
+  do {                                      \
+    emd_status s_ = (expr);                 \
+    if (s_ != EMD_OK) return s_;            \
+  } while (0)
+
+#define EMD_ARG(ctx, cond, msg)                                                  \
+  do {                                                                           \
+    if (!(cond)) return emd_set_error((ctx), EMD_ERR_ARG, "%s: %s", __func__, msg); \
+  } while (0)
+
+// timing brackets (no-ops unless profiling is enabled)
+void emd_prof_begin(emd_ctx *ctx, const char *name, emd_prof_rec *rec);
+void emd_prof_end(emd_ctx *ctx, emd_prof_rec *rec);
+
+// launch with an optional timing bracket and a launch-error check
+#define EMD_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                    \
+  do {                                                                            \
+    emd_prof_rec rec_;                                                            \
+    emd_prof_begin((ctx), (name), &rec_);                                         \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), (shmem), (ctx)->stream,   \
+                       __VA_ARGS__);                                              \
+    emd_prof_end((ctx), &rec_);                                                   \
+    EMD_HIP((ctx), hipGetLastError());                                            \
+  } while (0)
+
+// grow a device buffer (contents not preserved)
+emd_status emd_ensure(emd_ctx *ctx, void **p, size_t *cap, size_t bytes);
+template <class T>
+inline emd_status emd_ensure_t(emd_ctx *ctx, T **p, size_t *cap, size_t count) {
+  return emd_ensure(ctx, reinterpret_cast<void **>(p), cap, count * sizeof(T));
+}
+emd_status emd_ensure_int(emd_ctx *ctx, int **p, int *cap, long count);
+// grow an int buffer preserving its first `keep` entries
+emd_status emd_grow_keep_int(emd_ctx *ctx, int **p, int *cap, long count, long keep);
+
+// -------------------------------------------------------- wave primitives
+#define EMD_WAVE 64
+
+__device__ __forceinline__ int emd_lane() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long emd_lanemask_lt() {
+  const int l = __lane_id();
+  return (l == 0) ? 0ull : (~0ull >> (64 - l));
+}
+
+// inclusive prefix sum across the 64 lanes of a wavefront
+__device__ __forceinline__ int emd_wave_incl_scan(int v) {
+  const int l = __lane_id();
+#pragma unroll
+  for (int o = 1; o < EMD_WAVE; o <<= 1) {
+    int t = __shfl_up(v, o, EMD_WAVE);
+    if (l >= o) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int emd_wave_sum(int v) {
+#pragma unroll
+  for (int o = EMD_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, EMD_WAVE);
+  return v;
+}
+
+// map a physical block onto a logical tile so that each of the 8 XCDs (which
+// receive blocks round-robin) works on one contiguous range of tiles and
+// shares its L2 with spatially neighbouring atoms
+__device__ __forceinline__ int emd_xcd_tile(int b, int nb) {
+  const int nx = 8;
+  const int q = nb / nx, r = nb % nx;
+  const int x = b % nx, k = b / nx;
+  // XCD x owns q (+1 for the first r XCDs) consecutive tiles
+  const int base = x * q + (x < r ? x : r);
+  const int len = q + (x < r ? 1 : 0);
+  return (k < len) ? base + k : b;  // k < len always holds for b < nb
+}
+
+// --------------------------------------------------------- device-wide scan
+// Exclusive scan of v(i), i in [0,n), in three launches.  The functor type
+// provides   int value(int i) const   and   void emit(int i, int excl, int v) const.
+// The total is written to *total_dev.
+constexpr int EMD_SCAN_THREADS = 256;
+constexpr int EMD_SCAN_ITEMS = 8;
+constexpr int EMD_SCAN_TILE = EMD_SCAN_THREADS * EMD_SCAN_ITEMS;
+
+template <class F>
+__global__ void __launch_bounds__(EMD_SCAN_THREADS)
+k_scan_reduce(F f, int n, int *block_sums) {
+  __shared__ int s_w[EMD_SCAN_THREADS / EMD_WAVE];
+  const int base = blockIdx.x * EMD_SCAN_TILE;
+  int acc = 0;
+#pragma unroll
+  for (int k = 0; k < EMD_SCAN_ITEMS; k++) {
+    int i = base + k * EMD_SCAN_THREADS + threadIdx.x;
+    if (i < n) acc += f.value(i);
+  }
+  acc = emd_wave_sum(acc);
+  if (__lane_id() == 0) s_w[threadIdx.x / EMD_WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < EMD_SCAN_THREADS / EMD_WAVE; w++) t += s_w[w];
+    block_sums[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_scan_blocks(int *block_sums, int nblocks, int *total);
+
+template <class F>
+__global__ void __launch_bounds__(EMD_SCAN_THREADS)
+k_scan_apply(F f, int n, const int *block_offsets) {
+  __shared__ int s_w[EMD_SCAN_THREADS / EMD_WAVE];
+  __shared__ int s_carry;
+  const int base = blockIdx.x * EMD_SCAN_TILE;
+  if (threadIdx.x == 0) s_carry = block_offsets[blockIdx.x];
+  __syncthreads();
+  const int wid = threadIdx.x / EMD_WAVE;
+  for (int k = 0; k < EMD_SCAN_ITEMS; k++) {
+    int i = base + k * EMD_SCAN_THREADS + threadIdx.x;
+    int v = (i < n) ? f.value(i) : 0;
+    int inc = emd_wave_incl_scan(v);
+    if (__lane_id() == EMD_WAVE - 1) s_w[wid] = inc;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wid; w++) woff += s_w[w];
+    int carry = s_carry;
+    if (i < n) f.emit(i, carry + woff + inc - v, v);
+    __syncthreads();
+    if (threadIdx.x == EMD_SCAN_THREADS - 1) s_carry = carry + woff + inc;
+    __syncthreads();
+  }
+}
+
+// host driver, first half: per-block sums and their exclusive scan; the
+// total lands in ctx->d_small[0] (and in *total_host after a sync if asked)
+template <class F>
+emd_status emd_scan_count(emd_ctx *ctx, const F &f, int n, const char *name, int *total_host) {
+  int nb = (n + EMD_SCAN_TILE - 1) / EMD_SCAN_TILE;
+  if (nb < 1) nb = 1;
+  if ((size_t)nb > ctx->scan_cap) {
+    if (ctx->d_scan) {
+      EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      EMD_HIP(ctx, hipFree(ctx->d_scan));
+    }
+    ctx->d_scan = nullptr;
+    size_t cap = (size_t)nb * 2;
+    EMD_HIP(ctx, hipMalloc(&ctx->d_scan, cap * sizeof(int)));
+    ctx->scan_cap = cap;
+  }
+  if (n > 0) EMD_LAUNCH(ctx, name, (k_scan_reduce<F>), nb, EMD_SCAN_THREADS, 0, f, n, ctx->d_scan);
+  else EMD_HIP(ctx, hipMemsetAsync(ctx->d_scan, 0, sizeof(int), ctx->stream));
+  EMD_LAUNCH(ctx, "scan_blocks", k_scan_blocks, 1, 1024, 0, ctx->d_scan, nb, ctx->d_small);
+  if (total_host) {
+    EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *total_host = ctx->h_small[0];
+  }
+  return EMD_OK;
+}
+
+// second half: rescans each block and calls f.emit (same n as the count)
+template <class F>
+emd_status emd_scan_apply(emd_ctx *ctx, const F &f, int n, const char *name) {
+  int nb = (n + EMD_SCAN_TILE - 1) / EMD_SCAN_TILE;
+  if (n > 0) EMD_LAUNCH(ctx, name, (k_scan_apply<F>), nb, EMD_SCAN_THREADS, 0, f, n, ctx->d_scan);
+  return EMD_OK;
+}
+
+template <class F>
+emd_status emd_scan(emd_ctx *ctx, const F &f, int n, const char *name, int *total_host) {
+  EMD_TRY(emd_scan_count(ctx, f, n, name, nullptr));
+  EMD_TRY(emd_scan_apply(ctx, f, n, name));
+  if (total_host) {
+    EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *total_host = ctx->h_small[0];
+  }
+  return EMD_OK;
+}
+
+// plain exclusive scan of an int array into out (out[n] = total if room_total)
+struct emd_scan_array {
+  const int *in;
+  int *out;
+  int n;
+  int write_total;
+  __device__ int value(int i) const { return in[i]; }
+  __device__ void emit(int i, int excl, int v) const {
+    out[i] = excl;
+    if (write_total && i == n - 1) out[n] = excl + v;
+  }
+};
+
+// ------------------------------------------------------------ misc helpers
+inline int emd_blocks(long n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -1,0 +1,464 @@
+// emd_lj.hip -- LJ pair force/energy, force zeroing, NVE and thermo sums.
+//
+// LJ force: K lanes per atom (K in {1,2,4,8,16}); the lanes of one atom read
+// consecutive CSR entries (coalesced) and gather x_j, then reduce their
+// partial forces with cross-lane shuffles.  FP64 path: separate multiply and
+// add roundings (no contraction) so that K == 1 reproduces the reference's
+// serial arithmetic exactly.  Mixed path: the distance and the cutoff test
+// stay FP64 (identical pair set), the pair arithmetic and the per-atom
+// accumulation are FP32, the result goes to the FP32 force array f32.
+#include "emd_internal.hpp"
+
+constexpr int LJ_THREADS = 256;
+constexpr int LJ_MAXT = 16;
+
+struct lj_table {
+  double lj1[LJ_MAXT * LJ_MAXT], lj2[LJ_MAXT * LJ_MAXT], cutsq[LJ_MAXT * LJ_MAXT];
+};
+
+template <class T>
+__global__ void k_zero3(T *__restrict__ f, long n3) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) f[i] = T(0);
+}
+
+extern "C" emd_status emd_force_zero(emd_ctx *ctx, emd_system *s, int precision) {
+  if (!ctx || !s) return EMD_ERR_ARG;
+  const long n3 = 3l * (s->n_local + s->n_ghost);
+  if (n3 <= 0) return EMD_OK;
+  if (precision == EMD_MIXED)
+    EMD_LAUNCH(ctx, "force_zero", k_zero3<float>, emd_blocks(n3, 256), 256, 0, s->f32, n3);
+  else
+    EMD_LAUNCH(ctx, "force_zero", k_zero3<double>, emd_blocks(n3, 256), 256, 0, s->f, n3);
+  return EMD_OK;
+}
+
+template <int K, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = K / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, K);
+  return v;
+}
+
+// ForceLJNeigh TagFullNeigh, src/force_types/force_lj_neigh_impl.h:158-206
+template <int K, bool ONE_TYPE>
+__global__ void __launch_bounds__(LJ_THREADS)
+k_lj_full_fp64(const double *__restrict__ x, const int *__restrict__ type,
+               const int *__restrict__ row, const int *__restrict__ num,
+               const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
+               double cutsq_0, const lj_table *__restrict__ tab, int ntypes,
+               double *__restrict__ f) {
+#pragma clang fp contract(off)
+  const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
+  const int gid = tile * LJ_THREADS + threadIdx.x;
+  const int i = gid / K, sub = gid % K;
+  if (i >= n) return;
+  const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
+  const int type_i = ONE_TYPE ? 0 : type[i];
+  const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+  const int cnt = num[i];
+  double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+  for (int jj = sub; jj < cnt; jj += K) {
+    const int j = entries[start + jj];
+    const double dx = x_i - x[3 * (size_t)j];
+    const double dy = y_i - x[3 * (size_t)j + 1];
+    const double dz = z_i - x[3 * (size_t)j + 2];
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    double cutsq = cutsq_0, lj1 = lj1_0, lj2 = lj2_0;
+    if (!ONE_TYPE) {
+      const int t = type_i * LJ_MAXT + type[j];
+      cutsq = tab->cutsq[t]; lj1 = tab->lj1[t]; lj2 = tab->lj2[t];
+    }
+    if (rsq < cutsq) {
+      const double r2inv = 1.0 / rsq;
+      const double r6inv = r2inv * r2inv * r2inv;
+      const double fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
+      fxi += dx * fpair;
+      fyi += dy * fpair;
+      fzi += dz * fpair;
+    }
+  }
+  if (K > 1) {
+    fxi = group_sum<K>(fxi);
+    fyi = group_sum<K>(fyi);
+    fzi = group_sum<K>(fzi);
+  }
+  if (sub == 0) {
+    f[3 * (size_t)i] += fxi;
+    f[3 * (size_t)i + 1] += fyi;
+    f[3 * (size_t)i + 2] += fzi;
+  }
+}
+
+template <int K, bool ONE_TYPE>
+__global__ void __launch_bounds__(LJ_THREADS)
+k_lj_full_mixed(const double *__restrict__ x, const int *__restrict__ type,
+                const int *__restrict__ row, const int *__restrict__ num,
+                const int *__restrict__ entries, int pitch, int n, float lj1_0, float lj2_0,
+                double cutsq_0, const lj_table *__restrict__ tab, int ntypes,
+                float *__restrict__ f32) {
+  const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
+  const int gid = tile * LJ_THREADS + threadIdx.x;
+  const int i = gid / K, sub = gid % K;
+  if (i >= n) return;
+  const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
+  const int type_i = ONE_TYPE ? 0 : type[i];
+  const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+  const int cnt = num[i];
+  float fxi = 0.f, fyi = 0.f, fzi = 0.f;
+  for (int jj = sub; jj < cnt; jj += K) {
+    const int j = entries[start + jj];
+    const double dx = x_i - x[3 * (size_t)j];
+    const double dy = y_i - x[3 * (size_t)j + 1];
+    const double dz = z_i - x[3 * (size_t)j + 2];
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    double cutsq = cutsq_0;
+    float lj1 = lj1_0, lj2 = lj2_0;
+    if (!ONE_TYPE) {
+      const int t = type_i * LJ_MAXT + type[j];
+      cutsq = tab->cutsq[t]; lj1 = (float)tab->lj1[t]; lj2 = (float)tab->lj2[t];
+    }
+    if (rsq < cutsq) {
+      const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
+      const float r6inv = r2inv * r2inv * r2inv;
+      const float fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
+      fxi += (float)dx * fpair;
+      fyi += (float)dy * fpair;
+      fzi += (float)dz * fpair;
+    }
+  }
+  if (K > 1) {
+    fxi = group_sum<K>(fxi);
+    fyi = group_sum<K>(fyi);
+    fzi = group_sum<K>(fzi);
+  }
+  if (sub == 0) {
+    f32[3 * (size_t)i] += fxi;
+    f32[3 * (size_t)i + 1] += fyi;
+    f32[3 * (size_t)i + 2] += fzi;
+  }
+}
+
+// ForceLJNeigh TagHalfNeigh, force_lj_neigh_impl.h:208-254 (atomic j updates)
+template <class T>
+__global__ void __launch_bounds__(LJ_THREADS)
+k_lj_half(const double *__restrict__ x, const int *__restrict__ type, const int *__restrict__ row,
+          const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
+          const lj_table *__restrict__ tab, T *__restrict__ f) {
+  const int i = blockIdx.x * LJ_THREADS + threadIdx.x;
+  if (i >= n) return;
+  const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
+  const int type_i = type[i];
+  const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+  const int cnt = num[i];
+  T fxi = 0, fyi = 0, fzi = 0;
+  for (int jj = 0; jj < cnt; jj++) {
+    const int j = entries[start + jj];
+    const double dx = x_i - x[3 * (size_t)j];
+    const double dy = y_i - x[3 * (size_t)j + 1];
+    const double dz = z_i - x[3 * (size_t)j + 2];
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    const int t = type_i * LJ_MAXT + type[j];
+    if (rsq < tab->cutsq[t]) {
+      const T r2inv = T(1) / T(rsq);
+      const T r6inv = r2inv * r2inv * r2inv;
+      const T fpair = (r6inv * (T(tab->lj1[t]) * r6inv - T(tab->lj2[t]))) * r2inv;
+      fxi += T(dx) * fpair;
+      fyi += T(dy) * fpair;
+      fzi += T(dz) * fpair;
+      atomicAdd(&f[3 * (size_t)j], -T(dx) * fpair);
+      atomicAdd(&f[3 * (size_t)j + 1], -T(dy) * fpair);
+      atomicAdd(&f[3 * (size_t)j + 2], -T(dz) * fpair);
+    }
+  }
+  atomicAdd(&f[3 * (size_t)i], fxi);
+  atomicAdd(&f[3 * (size_t)i + 1], fyi);
+  atomicAdd(&f[3 * (size_t)i + 2], fzi);
+}
+
+static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p, lj_table **dtab) {
+  static_assert(sizeof(lj_table) <= 8 * 1024, "table size");
+  lj_table h;
+  memset(&h, 0, sizeof(h));
+  for (int a = 0; a < p->ntypes; a++)
+    for (int b = 0; b < p->ntypes; b++) {
+      h.lj1[a * LJ_MAXT + b] = p->lj1[a][b];
+      h.lj2[a * LJ_MAXT + b] = p->lj2[a][b];
+      h.cutsq[a * LJ_MAXT + b] = p->cutsq[a][b];
+    }
+  // the table lives in the small double scratch area's neighbour: d_red head
+  EMD_TRY(emd_ensure(ctx, (void **)&ctx->d_red, &ctx->red_cap, sizeof(lj_table) + (1 << 20)));
+  EMD_HIP(ctx, hipMemcpyAsync(ctx->d_red, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *dtab = (lj_table *)ctx->d_red;
+  return EMD_OK;
+}
+
+static int default_lanes(int precision) { return precision == EMD_MIXED ? 4 : 4; }
+
+extern "C" emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes) {
+  if (!ctx || (precision != EMD_FP64 && precision != EMD_MIXED)) return EMD_ERR_ARG;
+  if (!(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16))
+    return emd_set_error(ctx, EMD_ERR_ARG, "lanes must be 0,1,2,4,8 or 16");
+  ctx->lj_lanes[precision] = lanes;
+  return EMD_OK;
+}
+
+template <int K>
+static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                              const emd_lj_params *p, int precision, const lj_table *tab) {
+  const int n = s->n_local;
+  const int nb = emd_blocks((long)n * K, LJ_THREADS);
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  const bool one = p->ntypes == 1;
+  if (precision == EMD_MIXED) {
+    const float a = (float)p->lj1[0][0], b = (float)p->lj2[0][0];
+    if (one)
+      EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full_mixed<K, true>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, n, a, b, p->cutsq[0][0], tab,
+                 p->ntypes, s->f32);
+    else
+      EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full_mixed<K, false>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, n, a, b, p->cutsq[0][0], tab,
+                 p->ntypes, s->f32);
+  } else {
+    if (one)
+      EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_full_fp64<K, true>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0],
+                 p->lj2[0][0], p->cutsq[0][0], tab, p->ntypes, s->f);
+    else
+      EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_full_fp64<K, false>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0],
+                 p->lj2[0][0], p->cutsq[0][0], tab, p->ntypes, s->f);
+  }
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                   const emd_lj_params *p, int precision, int half) {
+  if (!ctx || !s || !nl || !p) return EMD_ERR_ARG;
+  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
+  EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
+  if (s->n_local <= 0) return EMD_OK;
+  lj_table *tab = nullptr;
+  if (p->ntypes > 1 || half) EMD_TRY(upload_table(ctx, p, &tab));
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  if (half) {
+    const int nb = emd_blocks(s->n_local, LJ_THREADS);
+    if (precision == EMD_MIXED)
+      EMD_LAUNCH(ctx, "lj_force_half", k_lj_half<float>, nb, LJ_THREADS, 0, s->x, s->type, row,
+                 nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f32);
+    else
+      EMD_LAUNCH(ctx, "lj_force_half", k_lj_half<double>, nb, LJ_THREADS, 0, s->x, s->type, row,
+                 nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f);
+    return EMD_OK;
+  }
+  int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
+  if (K == 0) K = default_lanes(precision);
+  switch (K) {
+    case 1: return launch_full<1>(ctx, s, nl, p, precision, tab);
+    case 2: return launch_full<2>(ctx, s, nl, p, precision, tab);
+    case 4: return launch_full<4>(ctx, s, nl, p, precision, tab);
+    case 8: return launch_full<8>(ctx, s, nl, p, precision, tab);
+    default: return launch_full<16>(ctx, s, nl, p, precision, tab);
+  }
+}
+
+// ------------------------------------------------------ compensated sums
+// error-free transformation a + b = s + e (Knuth TwoSum)
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// block reduction of (hi, lo) pairs; results land in out[2*blockIdx.x + {0,1}]
+__device__ void block_dd_sum(double hi, double lo, double *out) {
+  __shared__ double s_hi[16], s_lo[16];
+#pragma unroll
+  for (int o = EMD_WAVE / 2; o > 0; o >>= 1) {
+    const double oh = __shfl_xor(hi, o, EMD_WAVE), ol = __shfl_xor(lo, o, EMD_WAVE);
+    double s, e;
+    two_sum(hi, oh, s, e);
+    hi = s;
+    lo = lo + ol + e;
+  }
+  const int w = threadIdx.x / EMD_WAVE;
+  if (__lane_id() == 0) { s_hi[w] = hi; s_lo[w] = lo; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double H = 0.0, Lo = 0.0;
+    for (int k = 0; k < (int)(blockDim.x / EMD_WAVE); k++) {
+      double s, e;
+      two_sum(H, s_hi[k], s, e);
+      H = s;
+      Lo += s_lo[k] + e;
+    }
+    out[2 * blockIdx.x] = H;
+    out[2 * blockIdx.x + 1] = Lo;
+  }
+}
+
+// ForceLJNeigh Tag{Full,Half}NeighPE, force_lj_neigh_impl.h:256-343
+__global__ void __launch_bounds__(256)
+k_lj_energy(const double *__restrict__ x, const int *__restrict__ type,
+            const int *__restrict__ row, const int *__restrict__ num,
+            const int *__restrict__ entries, int pitch, int n, int half,
+            const lj_table *__restrict__ tab, double *__restrict__ part) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double pe = 0.0;
+  if (i < n) {
+    const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
+    const int type_i = type[i];
+    const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+    const int cnt = num[i];
+    for (int jj = 0; jj < cnt; jj++) {
+      const int j = entries[start + jj];
+      const double dx = x_i - x[3 * (size_t)j];
+      const double dy = y_i - x[3 * (size_t)j + 1];
+      const double dz = z_i - x[3 * (size_t)j + 2];
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      const int t = type_i * LJ_MAXT + type[j];
+      const double cutsq = tab->cutsq[t], lj1 = tab->lj1[t], lj2 = tab->lj2[t];
+      if (rsq < cutsq) {
+        const double r2inv = 1.0 / rsq;
+        const double r6inv = r2inv * r2inv * r2inv;
+        const double r2invc = 1.0 / cutsq;
+        const double r6invc = r2invc * r2invc * r2invc;
+        const double fac = half ? ((j < n) ? 1.0 : 0.5) : 0.5;
+        pe += fac * r6inv * (0.5 * lj1 * r6inv - lj2) / 6.0;
+        pe -= fac * r6invc * (0.5 * lj1 * r6invc - lj2) / 6.0;
+      }
+    }
+  }
+  block_dd_sum(pe, 0.0, part);
+}
+
+static emd_status finish_dd(emd_ctx *ctx, double *part, int nb, double *out) {
+  std::vector<double> h(2 * (size_t)nb);
+  EMD_HIP(ctx, hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // Neumaier summation of the block (hi, lo) pairs
+  double sum = 0.0, c = 0.0;
+  for (size_t k = 0; k < h.size(); k++) {
+    const double v = h[k];
+    const double t = sum + v;
+    if (fabs(sum) >= fabs(v)) c += (sum - t) + v;
+    else c += (v - t) + sum;
+    sum = t;
+  }
+  *out = sum + c;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                    const emd_lj_params *p, int half, double *pe) {
+  if (!ctx || !s || !nl || !p || !pe) return EMD_ERR_ARG;
+  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
+  const int n = s->n_local;
+  if (n <= 0) { *pe = 0.0; return EMD_OK; }
+  lj_table *tab;
+  EMD_TRY(upload_table(ctx, p, &tab));
+  const int nb = emd_blocks(n, 256);
+  EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)nb * 2 * sizeof(double)));
+  double *part = (double *)ctx->d_tmp2;
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  EMD_LAUNCH(ctx, "lj_energy", k_lj_energy, nb, 256, 0, s->x, s->type, row, nl->num, nl->entries,
+             nl->maxneighs, n, half, tab, part);
+  return finish_dd(ctx, part, nb, pe);
+}
+
+// ---------------------------------------------------------------- NVE
+// IntegratorNVE InitialIntegrateFunctor, src/integrator_nve.cpp:65-74
+template <bool MIXED>
+__global__ void k_nve_initial(double *__restrict__ x, double *__restrict__ v,
+                              const double *__restrict__ f, const float *__restrict__ f32,
+                              const int *__restrict__ type, const double *__restrict__ mass,
+                              int n, double dtf, double dtv) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dtfm = dtf / mass[type[i]];
+  const size_t o = 3 * (size_t)i;
+  const double f0 = MIXED ? (double)f32[o] : f[o];
+  const double f1 = MIXED ? (double)f32[o + 1] : f[o + 1];
+  const double f2 = MIXED ? (double)f32[o + 2] : f[o + 2];
+  const double v0 = v[o] + dtfm * f0, v1 = v[o + 1] + dtfm * f1, v2 = v[o + 2] + dtfm * f2;
+  v[o] = v0; v[o + 1] = v1; v[o + 2] = v2;
+  x[o] += dtv * v0;
+  x[o + 1] += dtv * v1;
+  x[o + 2] += dtv * v2;
+}
+
+// IntegratorNVE FinalIntegrateFunctor, src/integrator_nve.cpp:104-112
+template <bool MIXED>
+__global__ void k_nve_final(double *__restrict__ v, const double *__restrict__ f,
+                            const float *__restrict__ f32, const int *__restrict__ type,
+                            const double *__restrict__ mass, int n, double dtf) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dtfm = dtf / mass[type[i]];
+  const size_t o = 3 * (size_t)i;
+  const double f0 = MIXED ? (double)f32[o] : f[o];
+  const double f1 = MIXED ? (double)f32[o + 1] : f[o + 1];
+  const double f2 = MIXED ? (double)f32[o + 2] : f[o + 2];
+  v[o] += dtfm * f0;
+  v[o + 1] += dtfm * f1;
+  v[o + 2] += dtfm * f2;
+}
+
+extern "C" emd_status emd_nve_initial(emd_ctx *ctx, emd_system *s, double dtf, double dtv,
+                                      int precision) {
+  if (!ctx || !s) return EMD_ERR_ARG;
+  const int n = s->n_local;
+  if (n <= 0) return EMD_OK;
+  if (precision == EMD_MIXED)
+    EMD_LAUNCH(ctx, "nve_initial", k_nve_initial<true>, emd_blocks(n, 256), 256, 0, s->x, s->v,
+               s->f, s->f32, s->type, s->mass, n, dtf, dtv);
+  else
+    EMD_LAUNCH(ctx, "nve_initial", k_nve_initial<false>, emd_blocks(n, 256), 256, 0, s->x, s->v,
+               s->f, s->f32, s->type, s->mass, n, dtf, dtv);
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_nve_final(emd_ctx *ctx, emd_system *s, double dtf, int precision) {
+  if (!ctx || !s) return EMD_ERR_ARG;
+  const int n = s->n_local;
+  if (n <= 0) return EMD_OK;
+  if (precision == EMD_MIXED)
+    EMD_LAUNCH(ctx, "nve_final", k_nve_final<true>, emd_blocks(n, 256), 256, 0, s->v, s->f,
+               s->f32, s->type, s->mass, n, dtf);
+  else
+    EMD_LAUNCH(ctx, "nve_final", k_nve_final<false>, emd_blocks(n, 256), 256, 0, s->v, s->f,
+               s->f32, s->type, s->mass, n, dtf);
+  return EMD_OK;
+}
+
+// Temperature / KinE summand, src/property_temperature.h:54-56
+__global__ void __launch_bounds__(256)
+k_mv2(const double *__restrict__ v, const int *__restrict__ type,
+      const double *__restrict__ mass, int n, double *__restrict__ part) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double t = 0.0;
+  if (i < n) {
+    const size_t o = 3 * (size_t)i;
+    t = (v[o] * v[o] + v[o + 1] * v[o + 1] + v[o + 2] * v[o + 2]) * mass[type[i]];
+  }
+  block_dd_sum(t, 0.0, part);
+}
+
+extern "C" emd_status emd_reduce_mv2(emd_ctx *ctx, const emd_system *s, double *mv2) {
+  if (!ctx || !s || !mv2) return EMD_ERR_ARG;
+  const int n = s->n_local;
+  if (n <= 0) { *mv2 = 0.0; return EMD_OK; }
+  const int nb = emd_blocks(n, 256);
+  EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)nb * 2 * sizeof(double)));
+  double *part = (double *)ctx->d_tmp2;
+  EMD_LAUNCH(ctx, "reduce_mv2", k_mv2, nb, 256, 0, s->v, s->type, s->mass, n, part);
+  return finish_dd(ctx, part, nb, mv2);
+}

@@ -1,0 +1,56 @@
+// force.cpp -- Force base (src/force.cpp) and the LJ module.
+#include "force.h"
+
+#include <cmath>
+
+Force::Force(char**, System*, bool half_neigh_) : half_neigh(half_neigh_), comm_newton(false) {}
+void Force::init_coeff(int, char**) {}
+void Force::compute(System*, Binning*, Neighbor*) {}
+const char* Force::name() { return "ForceNone"; }
+
+// ForceLJNeigh ctor, src/force_types/force_lj_neigh_impl.h:43-58
+ForceLJNeighHIP::ForceLJNeighHIP(char** args, System* system, bool half_neigh_)
+    : Force(args, system, half_neigh_) {
+  ntypes = system->ntypes;
+  use_stackparams = (ntypes <= MAX_TYPES_STACKPARAMS);
+  memset(&params, 0, sizeof(params));
+  params.ntypes = ntypes;
+  if (ntypes > 16) { fprintf(stderr, "ForceLJNeighHIP supports at most 16 types\n"); exit(1); }
+}
+
+// ForceLJNeigh::init_coeff, force_lj_neigh_impl.h:60-98.  With at most
+// MAX_TYPES_STACKPARAMS types the reference writes every (i,j) entry from each
+// pair_coeff line, so the last line sets all pairs; kept as is.
+void ForceLJNeighHIP::init_coeff(int nargs, char** args) {
+  if (nargs < 6) { fprintf(stderr, "pair_coeff lj/cut needs: i j eps sigma cut\n"); exit(1); }
+  int t1 = atoi(args[1]) - 1, t2 = atoi(args[2]) - 1;
+  double eps = atof(args[3]), sigma = atof(args[4]), cut = atof(args[5]);
+  double lj1 = 48.0 * eps * pow(sigma, 12.0);
+  double lj2 = 24.0 * eps * pow(sigma, 6.0);
+  if (use_stackparams) {
+    for (int i = 0; i < ntypes; i++)
+      for (int j = 0; j < ntypes; j++) {
+        params.lj1[i][j] = lj1;
+        params.lj2[i][j] = lj2;
+        params.cutsq[i][j] = cut * cut;
+      }
+  } else {
+    params.lj1[t1][t2] = params.lj1[t2][t1] = lj1;
+    params.lj2[t1][t2] = params.lj2[t2][t1] = lj2;
+    params.cutsq[t1][t2] = params.cutsq[t2][t1] = cut * cut;
+  }
+}
+
+// ForceLJNeigh::compute, force_lj_neigh_impl.h:100-126
+void ForceLJNeighHIP::compute(System* system, Binning*, Neighbor* neighbor) {
+  EMD_CALL(system->ctx, emd_lj_force(system->ctx, system->view(), neighbor->get_neigh_list(),
+                                     &params, system->precision, half_neigh ? 1 : 0));
+}
+
+// ForceLJNeigh::compute_energy, force_lj_neigh_impl.h:128-156
+T_F_FLOAT ForceLJNeighHIP::compute_energy(System* system, Binning*, Neighbor* neighbor) {
+  double pe = 0.0;
+  EMD_CALL(system->ctx, emd_lj_energy(system->ctx, system->view(), neighbor->get_neigh_list(),
+                                      &params, half_neigh ? 1 : 0, &pe));
+  return pe;
+}

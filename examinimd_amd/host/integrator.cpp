@@ -1,0 +1,21 @@
+// integrator.cpp -- Integrator base (src/integrator.cpp) and IntegratorNVEHIP
+// (src/integrator_nve.cpp:41-121).
+#include "integrator.h"
+
+Integrator::Integrator(System* s) : system(s), timestep_size(0) {}
+Integrator::~Integrator() {}
+void Integrator::initial_integrate() {}
+void Integrator::final_integrate() {}
+
+IntegratorNVEHIP::IntegratorNVEHIP(System* s) : Integrator(s) {
+  dtv = system->dt;
+  dtf = 0.5 * system->dt / system->mvv2e;
+}
+
+void IntegratorNVEHIP::initial_integrate() {
+  EMD_CALL(system->ctx, emd_nve_initial(system->ctx, system->view(), dtf, dtv, system->precision));
+}
+
+void IntegratorNVEHIP::final_integrate() {
+  EMD_CALL(system->ctx, emd_nve_final(system->ctx, system->view(), dtf, system->precision));
+}

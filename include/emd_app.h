@@ -1,0 +1,52 @@
+/*
+ * emd_app.h -- C ABI of the whole ExaMiniMD driver (init + time-step loop)
+ * running on the GPU modules.  It wraps the C++ class ExaMiniMD, whose init()
+ * and run() follow src/examinimd.cpp:60-294 of the reference; the arguments
+ * are the reference's command line (src/input.cpp:151-224), e.g.
+ *   {"ExaMiniMD", "-il", "in.lj", "--neigh-type", "CSR", "--precision", "mixed"}
+ * Every call returns 0 on success.  The driver prints nothing when quiet != 0.
+ */
+#ifndef EMD_APP_H
+#define EMD_APP_H
+
+#include "emd_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct emd_app emd_app;
+
+int      emd_app_create(int argc, const char **argv, int quiet, emd_app **out);
+/* ExaMiniMD::init: build modules, lattice, first neighbour list and forces */
+int      emd_app_init(emd_app *app);
+/* ExaMiniMD::run(nsteps) */
+int      emd_app_run(emd_app *app, int nsteps);
+/* synchronise the device between phases (per-phase timers) or not */
+int      emd_app_set_phase_sync(emd_app *app, int on);
+int      emd_app_destroy(emd_app *app);
+
+int      emd_app_nsteps(const emd_app *app);       /* the deck's `run` count */
+int      emd_app_natoms(const emd_app *app);       /* N (global) */
+int      emd_app_nlocal(const emd_app *app);
+int      emd_app_nghost(const emd_app *app);
+/* T, PE/N, (PE+KE)/N */
+int      emd_app_thermo(emd_app *app, double out[3]);
+/* seconds of the last run(): 0 total, 1 force, 2 neigh, 3 comm, 4 other */
+double   emd_app_time(const emd_app *app, int which);
+/* host copies of the local atoms (any pointer may be NULL); f is FP64 */
+int      emd_app_download(emd_app *app, double *x, double *v, double *f, int *type, int *id,
+                          double *q);
+/* reference-format binary dump of the local atoms to `path` */
+int      emd_app_dump(emd_app *app, const char *path);
+/* device objects for callers that go below the driver */
+emd_ctx          *emd_app_ctx(emd_app *app);
+emd_system       *emd_app_system(emd_app *app);
+const emd_neigh  *emd_app_neigh(emd_app *app);
+const emd_halo   *emd_app_halo(emd_app *app);
+int               emd_app_precision(const emd_app *app);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,218 @@
+/*
+ * emd_hip.h -- C ABI of the MI355X (gfx950) implementation of ExaMiniMD's
+ * per-step hot path: serial ghost exchange, binning, neighbour lists, LJ and
+ * SNAP forces, the NVE integrator and the thermo reductions.
+ *
+ * The reference exposes this path only as C++ virtual methods of its module
+ * base classes; each entry point below names the method it replaces
+ * (paths relative to the ExaMiniMD source tree).  A host-side module class
+ * (Binning, Neighbor, Force, Integrator, Comm subclass) calls exactly one of
+ * these per virtual call.
+ *
+ * Conventions
+ *  - Every function returns an emd_status (0 = EMD_OK).  Nothing calls exit();
+ *    the text of the last error is available from emd_last_error().
+ *  - All array pointers inside the structs are DEVICE pointers owned by the
+ *    library (allocated through the *_grow / *_free calls) unless stated.
+ *  - Work is ordered on the context's single HIP stream.  Only calls that
+ *    must return a count or a scalar to the host synchronise (the same points
+ *    at which the reference synchronises).  A context is not thread-safe.
+ *  - Layouts in HBM: x, v, f are AoS double[n_max][3] (the reference's t_x
+ *    layout, src/types.h:88); f32 is AoS float[n_max][3] used by the
+ *    mixed-precision path; type, id are int32[n_max]; q double[n_max];
+ *    mass double[ntypes].
+ */
+#ifndef EMD_HIP_H
+#define EMD_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int emd_status;
+enum {
+  EMD_OK = 0,
+  EMD_ERR_ARG = 1,       /* invalid argument / inconsistent sizes */
+  EMD_ERR_HIP = 2,       /* HIP runtime error */
+  EMD_ERR_ALLOC = 3,     /* device allocation failed */
+  EMD_ERR_OVERFLOW = 4,  /* an atom left the bin grid / a capacity was exceeded */
+  EMD_ERR_STATE = 5      /* call made in the wrong state */
+};
+
+enum { EMD_FP64 = 0, EMD_MIXED = 1 };       /* force precision */
+enum { EMD_NEIGH_CSR = 0, EMD_NEIGH_2D = 1 }; /* neighbour list layout */
+
+typedef struct emd_ctx emd_ctx;
+
+/* ------------------------------------------------------------ context */
+/* replaces Kokkos::initialize / finalize (src/main.cpp:54,70) */
+emd_status  emd_create(int device, emd_ctx **out);
+emd_status  emd_destroy(emd_ctx *ctx);
+const char *emd_last_error(const emd_ctx *ctx);
+emd_status  emd_synchronize(emd_ctx *ctx);
+void       *emd_get_stream(emd_ctx *ctx);   /* the hipStream_t all work runs on */
+const char *emd_version(void);
+
+/* device memory for callers (fixtures, tests, host modules) */
+emd_status emd_malloc(emd_ctx *ctx, void **ptr, size_t bytes);
+emd_status emd_free(emd_ctx *ctx, void *ptr);
+emd_status emd_memcpy_h2d(emd_ctx *ctx, void *dst, const void *src, size_t bytes);
+emd_status emd_memcpy_d2h(emd_ctx *ctx, void *dst, const void *src, size_t bytes);
+emd_status emd_memcpy_d2d(emd_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* per-kernel timing with HIP events recorded on the context stream */
+emd_status emd_profile_enable(emd_ctx *ctx, int on);
+emd_status emd_profile_reset(emd_ctx *ctx);
+/* total milliseconds and launch count of kernel `name` since the last reset */
+emd_status emd_profile_get(emd_ctx *ctx, const char *name, double *ms, long *launches);
+/* newline-separated "name launches total_ms" table; returns bytes needed */
+size_t     emd_profile_report(emd_ctx *ctx, char *buf, size_t len);
+
+/* ------------------------------------------------------------- system */
+/* Device image of System (src/system.h:57-143). */
+typedef struct {
+  double *x, *v, *f;
+  float  *f32;
+  int    *type, *id;
+  double *q;
+  double *mass;
+  int n_local, n_ghost, n_max, ntypes, n_global;
+  double domain[3], lo[3], hi[3];
+  double boltz, mvv2e, dt;
+} emd_system;
+
+/* System::init / System::grow (src/system.cpp:71-109): allocate or grow all
+ * per-atom arrays to n (contents preserved, new tail zeroed) */
+emd_status emd_system_grow(emd_ctx *ctx, emd_system *s, int n);
+emd_status emd_system_free(emd_ctx *ctx, emd_system *s);
+/* host <-> device copies of the first n atoms (AoS double[n][3]) */
+emd_status emd_system_upload(emd_ctx *ctx, emd_system *s, int n, const double *x,
+                             const double *v, const int *type, const int *id,
+                             const double *q);
+emd_status emd_system_download(emd_ctx *ctx, const emd_system *s, int n, double *x,
+                               double *v, double *f, int *type, int *id, double *q);
+emd_status emd_system_set_mass(emd_ctx *ctx, emd_system *s, int ntypes, const double *mass);
+
+/* ----------------------------------------------------- comm (serial) */
+typedef struct {
+  int num_ghost[6];
+  int *pack[6];     /* device: source index of each ghost, per phase */
+  int cap[6];
+  int *root;        /* device [n_ghost]: owning local atom of each ghost */
+  int *image;       /* device [n_ghost]: packed periodic shift (3 x 2 bits) */
+  int root_cap;
+  int n_ghost;
+} emd_halo;
+
+/* CommSerial::exchange, src/comm_types/comm_serial.cpp:47-53 */
+emd_status emd_comm_exchange(emd_ctx *ctx, emd_system *s);
+/* CommSerial::exchange_halo, comm_serial.cpp:55-97 (grows s as needed;
+ * ghost order within a phase is ascending source index) */
+emd_status emd_comm_exchange_halo(emd_ctx *ctx, emd_system *s, double comm_depth,
+                                  emd_halo *h);
+/* CommSerial::update_halo, comm_serial.cpp:99-110 (positions only) */
+emd_status emd_comm_update_halo(emd_ctx *ctx, emd_system *s, const emd_halo *h);
+/* CommSerial::update_force, comm_serial.cpp:112-127 (folds f, or f32 when
+ * precision == EMD_MIXED) */
+emd_status emd_comm_update_force(emd_ctx *ctx, emd_system *s, const emd_halo *h,
+                                 int precision);
+emd_status emd_halo_free(emd_ctx *ctx, emd_halo *h);
+
+/* ------------------------------------------------------------ binning */
+typedef struct {
+  int nbinx, nbiny, nbinz, nhalo;
+  double minx, maxx, miny, maxy, minz, maxz;
+  int nbins, n_range;
+  int *bincount;    /* device [nbins], x-major (ix*nby + iy)*nbz + iz */
+  int *binoffsets;  /* device [nbins] */
+  int *permute;     /* device [n_range] */
+  int *binid;       /* device [n_range] scratch */
+  int cap_bins, cap_range;
+  int max_count;    /* largest bin population (host copy) */
+} emd_bins;
+
+/* BinningKKSort::create_binning, src/binning_types/binning_kksort.cpp:71-140 */
+emd_status emd_binning(emd_ctx *ctx, emd_system *s, double dx, double dy, double dz,
+                       int halo_depth, int do_local, int do_ghost, int sort, emd_bins *b);
+emd_status emd_bins_free(emd_ctx *ctx, emd_bins *b);
+
+/* --------------------------------------------------------- neighbours */
+typedef struct {
+  int layout;       /* EMD_NEIGH_CSR or EMD_NEIGH_2D */
+  int n;            /* rows = n_local */
+  long total;       /* number of entries */
+  int maxneighs;    /* 2D row pitch; largest row for CSR */
+  int *row;         /* CSR: device [n+1] offsets */
+  int *num;         /* device [n] row lengths */
+  int *entries;     /* CSR: [total]; 2D: [n][maxneighs] row-major */
+  long cap_entries;
+  int cap_rows;
+} emd_neigh;
+
+/* NeighborCSR::create_neigh_list (src/neighbor_types/neighbor_csr.h:370-431)
+ * and Neighbor2D::create_neigh_list (neighbor_2d.h:280-331).  Row order is
+ * the 27-bin stencil order then permutation order (deterministic). */
+emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const emd_bins *b,
+                           double neigh_cut, int half, int newton, int layout,
+                           emd_neigh *nl);
+emd_status emd_neigh_free(emd_ctx *ctx, emd_neigh *nl);
+
+/* ----------------------------------------------------------------- LJ */
+typedef struct {
+  int ntypes;
+  double lj1[16][16], lj2[16][16], cutsq[16][16];
+} emd_lj_params;
+
+/* deep_copy(system->f, 0.0), src/examinimd.cpp:232 (f, or f32 for mixed) */
+emd_status emd_force_zero(emd_ctx *ctx, emd_system *s, int precision);
+/* ForceLJNeigh::compute, src/force_types/force_lj_neigh_impl.h:100-126 */
+emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                        const emd_lj_params *p, int precision, int half);
+/* ForceLJNeigh::compute_energy, force_lj_neigh_impl.h:128-156 (compensated) */
+emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                         const emd_lj_params *p, int half, double *pe);
+/* lanes per atom of the LJ force kernel (0 = autotuned default) */
+emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes);
+
+/* ---------------------------------------------------------------- NVE */
+/* IntegratorNVE::initial_integrate / final_integrate,
+ * src/integrator_nve.cpp:76-121 (reads f32 when precision == EMD_MIXED) */
+emd_status emd_nve_initial(emd_ctx *ctx, emd_system *s, double dtf, double dtv,
+                           int precision);
+emd_status emd_nve_final(emd_ctx *ctx, emd_system *s, double dtf, int precision);
+
+/* Temperature / KinE reductions, src/property_temperature.cpp:43-62,
+ * src/property_kine.cpp:43-61: returns sum_i m_i v_i^2 (compensated) */
+emd_status emd_reduce_mv2(emd_ctx *ctx, const emd_system *s, double *mv2);
+
+/* --------------------------------------------------------------- SNAP */
+typedef struct {
+  int twojmax;
+  double rfac0, rmin0, rcutfac;
+  int switchflag, bzeroflag;
+  double radelem, wjelem;
+  int ncoeff;            /* bispectrum components (55 for twojmax 8) */
+  const double *coeff;   /* HOST pointer: beta_0 .. beta_ncoeff */
+} emd_snap_params;
+
+typedef struct emd_snap emd_snap;
+/* ForceSNAP::init_coeff + SNA::init (src/force_types/force_snap_neigh_impl.h:
+ * 258-338, sna_impl.hpp:139-143) */
+emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, emd_snap **out);
+emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn);
+emd_status emd_snap_set_coeff(emd_ctx *ctx, emd_snap *sn, const double *coeff);
+/* ForceSNAP::compute, force_snap_neigh_impl.h:159-208 (+ operator() 589-725);
+ * forces land on locals and ghosts, fold with emd_comm_update_force */
+emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, emd_snap *sn,
+                          int precision);
+/* SNAP energy (no reference counterpart: the reference's ForceSNAP has no
+ * compute_energy, src/force.h:54); eatom may be NULL or a device [n_local] */
+emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                           emd_snap *sn, double *energy, double *eatom);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,195 @@
+/*
+ * oracle.h -- CPU restatement of ExaMiniMD's per-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it, and there only to compute the expected answer or to time the CPU
+ * baseline.  Nothing in examinimd_amd/ links, loads or calls it.
+ *
+ * Every routine restates the reference algorithm named in its comment
+ * (paths relative to the ExaMiniMD source tree).  Arithmetic is FP64 and is
+ * written in the same operation order as the reference so that, compiled with
+ * -ffp-contract=off, results are the Kokkos-Serial results of the reference.
+ * Parallel loops (OpenMP) only split work whose result does not depend on the
+ * split; every floating-point sum keeps the serial order.
+ *
+ * Third-party arithmetic restated here: Kokkos BinOp3D/BinSort (not vendored
+ * by the reference, version unpinned, >= 3.0; CMakeLists.txt:7): bin index
+ * int(mul*(x-min)) with mul = nbin/(max-min), x-major flattening, counts ->
+ * exclusive scan -> permutation in ascending atom order within a bin (the
+ * Serial execution order).
+ */
+#ifndef EMD_ORACLE_H
+#define EMD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- system */
+typedef struct {
+  int N;          /* global atom count */
+  int N_local, N_ghost, N_max;
+  int ntypes;
+  double *x, *v, *f;  /* AoS [N_max][3] */
+  int *type, *id;
+  double *q;
+  double mass[16];
+  double domain[3], lo[3], hi[3];
+  double boltz, mvv2e, dt;
+} or_system;
+
+/* ------------------------------------------------------------------ halo */
+typedef struct {
+  int num_ghost[6];
+  int *pack[6];       /* source index of every ghost, per phase */
+  int cap[6];
+} or_halo;
+
+/* --------------------------------------------------------------- binning */
+typedef struct {
+  int nbin[3], nhalo;
+  double min[3], max[3];
+  int nbins_total, n_range;
+  int *count;         /* [nbins_total], x-major */
+  int *offset;        /* [nbins_total] */
+  int *permute;       /* [n_range] */
+} or_bins;
+
+/* ------------------------------------------------------------- neighbour */
+typedef struct {
+  int n;              /* rows (N_local) */
+  int total;
+  int *row;           /* [n+1] CSR offsets */
+  int *entries;       /* [total] */
+} or_csr;
+
+typedef struct {
+  int n, maxneighs;
+  int *num;           /* [n] */
+  int *neighs;        /* [n][maxneighs] row-major */
+} or_list2d;
+
+/* ------------------------------------------------------------------ SNAP */
+typedef struct {
+  int twojmax;
+  double rfac0, rmin0, rcutfac;
+  int switchflag, bzeroflag;
+  double radelem, wjelem;
+  int ncoeff;             /* number of bispectrum components (55 for 2J=8) */
+  double coeff[1024];     /* beta_0 .. beta_ncoeff */
+} or_snap_params;
+
+/* ---------------------------------------------------------------- config */
+enum { OR_UNITS_REAL = 0, OR_UNITS_LJ = 1, OR_UNITS_METAL = 2 };
+enum { OR_LATTICE_SC = 0, OR_LATTICE_FCC = 1, OR_LATTICE_BCC = 2 };
+enum { OR_FORCE_LJ = 0, OR_FORCE_SNAP = 1 };
+enum { OR_NEIGH_CSR = 0, OR_NEIGH_2D = 1 };
+
+typedef struct {
+  int units;
+  int lattice_style;
+  double lattice_arg;      /* sc/bcc: constant; fcc: reduced density */
+  double lattice_offset[3];
+  int nx, ny, nz;
+  int ntypes;
+  double mass[16];
+  double temperature;
+  int seed;
+  int vel_gaussian;        /* 0: reference uniform geom; 1: gaussian (extension) */
+  int force_type;
+  double force_cutoff;
+  double lj_eps, lj_sigma, lj_cut;
+  double skin;
+  int exchange_rate;
+  int neigh_type;
+  int half_neigh;
+  int newton;
+  double dt;               /* <= 0: units default */
+  int thermo_rate;
+  int nthreads;            /* OpenMP threads for the parallel loops (0: default) */
+} or_config;
+
+/* ---------------------------------------------------------- RNG/lattice */
+double or_rng_uniform(int *seed);
+double or_rng_gaussian(int *seed);
+int    or_rng_reset(int ibase, const double coord[3]);
+unsigned or_hash_bytes(const signed char *s, int n, unsigned hash);
+void   or_units(or_system *s, int units);
+int    or_create_lattice(or_system *s, int style, double arg, const double off[3],
+                         int nx, int ny, int nz, double T, int seed, int gaussian);
+double or_temperature(const or_system *s);
+double or_kinetic_energy(const or_system *s);
+
+/* -------------------------------------------------------------- system */
+void or_system_init(or_system *s);
+void or_system_free(or_system *s);
+void or_system_grow(or_system *s, int n);
+
+/* ---------------------------------------------------------------- comm */
+void or_comm_exchange(or_system *s);
+void or_comm_exchange_halo(or_system *s, or_halo *h, double comm_depth);
+void or_comm_update_halo(or_system *s, const or_halo *h);
+void or_comm_update_force(or_system *s, const or_halo *h);
+void or_halo_free(or_halo *h);
+
+/* ------------------------------------------------------------- binning */
+void or_binning(or_system *s, or_bins *b, double dx, double dy, double dz,
+                int halo, int do_local, int do_ghost, int sort);
+void or_bins_free(or_bins *b);
+
+/* ----------------------------------------------------------- neighbour */
+void or_neigh_csr(const or_system *s, const or_bins *b, double cut, int half,
+                  int newton, or_csr *out);
+void or_neigh_2d(const or_system *s, const or_bins *b, double cut, int half,
+                 int newton, or_list2d *out);
+void or_csr_free(or_csr *c);
+void or_list2d_free(or_list2d *l);
+void or_brute_neigh_count(const or_system *s, double cut, int *counts);
+
+/* ------------------------------------------------------------------ LJ */
+void   or_lj_force(or_system *s, const or_csr *c, double eps, double sigma,
+                   double cut, int half);
+double or_lj_energy(const or_system *s, const or_csr *c, double eps,
+                    double sigma, double cut, int half);
+
+/* -------------------------------------------------------------- NVE */
+void or_nve_initial(or_system *s);
+void or_nve_final(or_system *s);
+
+/* ------------------------------------------------------------------ SNAP */
+int    or_snap_ncoeff(int twojmax);
+int    or_snap_idxj(int twojmax, int *j1, int *j2, int *j);  /* returns count */
+double or_snap_cg(int twojmax, int j1, int j2, int j, int m1, int m2);
+void   or_snap_force(or_system *s, const or_csr *c, const or_snap_params *p);
+double or_snap_energy(const or_system *s, const or_csr *c, const or_snap_params *p,
+                      double *eatom);
+void   or_snap_bispectrum(const or_snap_params *p, int nnei, const double *rij,
+                          double *B);
+void   or_snap_ui(const or_snap_params *p, int nnei, const double *rij,
+                  double *ur, double *ui);
+void   or_snap_dbidrj(const or_snap_params *p, int nnei, const double *rij,
+                      int jj, double *dB);
+
+/* ------------------------------------------------------------ dumps */
+int or_dump_write(const or_system *s, const char *path);
+int or_dump_compare(const or_system *s, const char *path, double out[6]);
+
+/* -------------------------------------------------------- MD driver */
+typedef struct or_md or_md;
+or_md     *or_md_create(const or_config *cfg, const or_snap_params *snap);
+void       or_md_destroy(or_md *m);
+int        or_md_init(or_md *m);
+int        or_md_run(or_md *m, int nsteps);
+or_system *or_md_system(or_md *m);
+const or_csr *or_md_csr(or_md *m);
+const or_halo *or_md_halo(or_md *m);
+void       or_md_thermo(or_md *m, double out[3]);  /* T, PE/N, (PE+KE)/N */
+double     or_md_potential(or_md *m);
+double     or_md_time(or_md *m, int which);        /* 0 total,1 force,2 neigh,3 comm,4 other */
+int        or_md_dump(or_md *m, const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+"""GPU parity of the LJ path (wrap, halo, binning, neighbour list, LJ force,
+NVE) against the CPU oracle, end to end through the C ABI driver.
+
+The oracle runs the reference algorithm in its serial order; the GPU path is
+deterministic and, with one lane per atom in the LJ kernel, performs the same
+FP64 operations in the same order, so positions, velocities and forces are
+expected to be bit-identical.  The default multi-lane kernel changes only the
+summation order of each atom's force: bound 1e-12 relative over 100 steps.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import examinimd_amd as emd
+from examinimd_amd import decks
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+N_CELLS = 20      # 32 000 atoms: the CPU-runnable LJ configuration
+STEPS = 100
+
+
+def oracle_run(oracle, steps, n=N_CELLS, half=False, newton=False, gaussian=True):
+    cfg = oracle.make_config(n=(n, n, n), gaussian=gaussian, half=half, newton=newton)
+    m = oracle.MD(cfg)
+    m.init()
+    if steps:
+        m.run(steps)
+    return m
+
+
+def gpu_app(tmp_path, steps, n=N_CELLS, neigh="CSR", precision="fp64", half=False,
+            newton=False, gaussian=True, lanes=None):
+    deck = decks.write(str(tmp_path / "in.lj"),
+                       decks.lj_deck(n=n, steps=steps, newton=newton, gaussian=gaussian))
+    if lanes is not None:
+        os.environ["EMD_LJ_LANES"] = str(lanes)
+    try:
+        args = ["-il", deck, "--neigh-type", neigh, "--comm-type", "SERIAL",
+                "--force-iteration", "NEIGH_HALF" if half else "NEIGH_FULL",
+                "--precision", precision]
+        app = emd.App(args).init()
+    finally:
+        os.environ.pop("EMD_LJ_LANES", None)
+    return app
+
+
+def compare(app, m, tol, exact=False):
+    g = app.arrays()
+    o = m.arrays()
+    n = app.nlocal
+    assert n == m.system.N_local
+    assert np.array_equal(g["id"], o["id"][:n]), "atom order differs"
+    for k in ("x", "v", "f"):
+        if exact:
+            assert np.array_equal(g[k], o[k][:n]), f"{k} not bit-identical: max diff " \
+                f"{np.max(np.abs(g[k] - o[k][:n]))}"
+        else:
+            e = rel_err(g[k], o[k][:n])
+            assert e <= tol, f"{k}: relative error {e} > {tol}"
+
+
+def test_lj32k_init_bitexact(tmp_path, oracle):
+    print("case: LJ 32k init, CSR, 1 lane", flush=True)
+    app = gpu_app(tmp_path, 0, lanes=1)
+    m = oracle_run(oracle, 0)
+    compare(app, m, 0.0, exact=True)
+    # neighbour list: identical rows in identical order
+    row, num, ent = app.neigh()
+    orow, oent = m.csr()
+    assert np.array_equal(row, orow)
+    assert np.array_equal(ent, oent)
+    assert app.nghost == m.system.N_ghost
+
+
+def test_lj32k_100_steps_bitexact(tmp_path, oracle):
+    print("case: LJ 32k, 100 steps, CSR, 1 lane", flush=True)
+    app = gpu_app(tmp_path, STEPS, lanes=1)
+    app.run(STEPS)
+    m = oracle_run(oracle, STEPS)
+    compare(app, m, 0.0, exact=True)
+    th_g, th_o = app.thermo(), m.thermo()
+    assert abs(th_g[0] - th_o[0]) <= 1e-12 * abs(th_o[0])
+    assert abs(th_g[1] - th_o[1]) <= 1e-12 * abs(th_o[1])
+    assert abs(th_g[2] - th_o[2]) <= 1e-12 * abs(th_o[2])
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16])
+def test_lj32k_100_steps_lanes(tmp_path, oracle, lanes):
+    print(f"case: LJ 32k, 100 steps, CSR, {lanes} lanes", flush=True)
+    app = gpu_app(tmp_path, STEPS, lanes=lanes)
+    app.run(STEPS)
+    m = oracle_run(oracle, STEPS)
+    compare(app, m, 1e-12)
+
+
+def test_lj32k_2d_list(tmp_path, oracle):
+    print("case: LJ 32k, 100 steps, 2D list", flush=True)
+    app = gpu_app(tmp_path, STEPS, neigh="2D", lanes=1)
+    app.run(STEPS)
+    m = oracle_run(oracle, STEPS)
+    compare(app, m, 0.0, exact=True)
+
+
+@pytest.mark.parametrize("newton", [False, True])
+def test_lj_half_list(tmp_path, oracle, newton):
+    print(f"case: LJ 4000 atoms half list newton={newton}", flush=True)
+    app = gpu_app(tmp_path, 40, n=10, half=True, newton=newton)
+    m = oracle_run(oracle, 0, n=10, half=True, newton=newton)
+    # the half-list rows hold the same pairs as the oracle's
+    row, num, ent = app.neigh()
+    orow, oent = m.csr()
+    assert np.array_equal(row, orow)
+    assert np.array_equal(ent, oent)
+    assert np.array_equal(app.arrays()["x"], m.arrays()["x"][:app.nlocal])
+    # on the perfect lattice every force is rounding noise (~1e-14) whose
+    # value depends on the order of the atomic updates: absolute bound
+    assert np.abs(app.arrays()["f"] - m.arrays()["f"][:app.nlocal]).max() < 1e-12
+    app.run(40)
+    m.run(40)
+    compare(app, m, 1e-11)
+
+
+def test_lj_uniform_velocity_reference_deck(tmp_path, oracle):
+    """the reference's own generator (uniform 'loop geom' velocities)"""
+    print("case: LJ 4000 atoms uniform velocities", flush=True)
+    app = gpu_app(tmp_path, 30, n=10, gaussian=False, lanes=1)
+    m = oracle_run(oracle, 0, n=10, gaussian=False)
+    compare(app, m, 0.0, exact=True)
+    app.run(30)
+    m.run(30)
+    compare(app, m, 0.0, exact=True)
+
+
+def test_lj_mixed_force_error(tmp_path, oracle):
+    """FP32 pair arithmetic on the FP64 positions: force error at step 0"""
+    print("case: LJ 32k mixed precision force error", flush=True)
+    app = gpu_app(tmp_path, 0, precision="mixed")
+    m = oracle_run(oracle, 0)
+    g = app.arrays()
+    o = m.arrays()
+    n = app.nlocal
+    assert np.array_equal(g["x"], o["x"][:n])
+    df = g["f"] - o["f"][:n]
+    rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+    print("mixed force RMS error", rms)
+    assert rms <= 1e-5
+    # identical pair set: the neighbour list is the FP64 one
+    row, num, ent = app.neigh()
+    orow, oent = m.csr()
+    assert np.array_equal(ent, oent)

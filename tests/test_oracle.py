@@ -1,0 +1,439 @@
+"""Pins the CPU oracle against known answers, closed forms and invariants.
+
+The reference ships no tests or golden vectors, so each oracle function is
+checked against something independent of its own formula: a published check
+value, a library routine (sympy CG), a closed form on a perfect lattice, a
+brute-force count, or an identity (F = -grad E by central differences,
+momentum conservation, rotation invariance, unitarity).
+"""
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, REF_INPUT, read_snapcoeff
+
+
+# ------------------------------------------------------------------- RNG
+def test_park_miller_check_value(oracle):
+    """minimal standard generator: seed 1 -> 1043618065 after 10000 draws
+    (Park & Miller, CACM 31 (1988) 1192)"""
+    L = oracle.lib()
+    s = C.c_int(1)
+    for _ in range(10000):
+        L.or_rng_uniform(C.byref(s))
+    assert s.value == 1043618065
+
+
+def _oat_final(h):
+    h = (h + (h << 3)) & 0xFFFFFFFF
+    h ^= h >> 11
+    h = (h + (h << 15)) & 0xFFFFFFFF
+    return h
+
+
+@pytest.mark.parametrize("text,expect", [(b"a", 0xca2e9442),
+                                         (b"The quick brown fox jumps over the lazy dog",
+                                          0x519e91f5)])
+def test_jenkins_one_at_a_time_vectors(oracle, text, expect):
+    """published one-at-a-time hash vectors (ASCII input)"""
+    h = oracle.lib().or_hash_bytes(text, len(text), 0)
+    assert _oat_final(h) == expect
+
+
+def test_gaussian_moments(oracle):
+    L = oracle.lib()
+    s = C.c_int(12345)
+    g = np.array([L.or_rng_gaussian(C.byref(s)) for _ in range(200000)])
+    assert abs(g.mean()) < 0.01 and abs(g.std() - 1.0) < 0.01
+
+
+# --------------------------------------------------------------- lattice
+@pytest.mark.parametrize("style,arg,n,per_cell", [("sc", 1.3, (3, 4, 5), 1),
+                                                  ("fcc", 0.8442, (4, 4, 4), 4),
+                                                  ("bcc", 3.1803, (5, 3, 4), 2)])
+def test_lattice_counts_and_box(oracle, style, arg, n, per_cell):
+    L = oracle.lib()
+    s = oracle.System()
+    L.or_system_init(C.byref(s))
+    s.mass[0] = 1.0
+    L.or_units(C.byref(s), 1)
+    off = (C.c_double * 3)(0, 0, 0)
+    L.or_create_lattice(C.byref(s), oracle.LATTICE[style], arg, off, n[0], n[1], n[2], 1.0, 42, 0)
+    N = per_cell * n[0] * n[1] * n[2]
+    assert s.N_local == N
+    a = (4.0 / arg) ** (1 / 3) if style == "fcc" else arg
+    x = oracle.arr(s.x, 3 * N).reshape(N, 3)
+    ids = oracle.arr(s.id, N, np.int32)
+    assert np.array_equal(ids, np.arange(1, N + 1))
+    for d in range(3):
+        assert abs(s.domain[d] - a * n[d]) < 1e-12
+        assert x[:, d].min() >= 0 and x[:, d].max() < s.domain[d]
+    # all sites distinct, nearest-neighbour distance of the lattice
+    d2 = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1) + np.eye(N) * 1e9
+    nn = {"sc": a, "fcc": a / math.sqrt(2), "bcc": a * math.sqrt(3) / 2}[style]
+    assert abs(math.sqrt(d2.min()) - nn) < 1e-12
+    L.or_system_free(C.byref(s))
+
+
+@pytest.mark.parametrize("gaussian", [0, 1])
+def test_velocity_zero_momentum_and_temperature(oracle, gaussian):
+    """sum m v = 0 and T equals the target (closed form after rescaling)"""
+    L = oracle.lib()
+    s = oracle.System()
+    L.or_system_init(C.byref(s))
+    s.mass[0] = 2.0
+    L.or_units(C.byref(s), 1)
+    off = (C.c_double * 3)(0, 0, 0)
+    L.or_create_lattice(C.byref(s), 1, 0.8442, off, 6, 6, 6, 1.44, 87287, gaussian)
+    N = s.N_local
+    v = oracle.arr(s.v, 3 * N).reshape(N, 3)
+    assert np.abs((2.0 * v).sum(0)).max() < 1e-10
+    T = (2.0 * v * v).sum() / (3 * N - 3)
+    assert abs(T - 1.44) < 1e-12
+    assert abs(L.or_temperature(C.byref(s)) - 1.44) < 1e-12
+    if not gaussian:
+        # uniform draws in [-0.5,0.5)/sqrt(m), shifted by v_cm, then scaled
+        assert np.abs(v).max() < 1.0 * math.sqrt(1.44 / (T / 1.44)) * 10
+    L.or_system_free(C.byref(s))
+
+
+def _sc_system(oracle, n, a, jitter=0.0, seed=0, mass=1.0):
+    L = oracle.lib()
+    s = oracle.System()
+    L.or_system_init(C.byref(s))
+    s.mass[0] = mass
+    L.or_units(C.byref(s), 1)
+    off = (C.c_double * 3)(0, 0, 0)
+    L.or_create_lattice(C.byref(s), 0, a, off, n, n, n, 1.0, 1, 0)
+    if jitter:
+        N = s.N_local
+        x = np.ctypeslib.as_array(s.x, shape=(3 * N,))
+        x += np.random.default_rng(seed).normal(0, jitter, 3 * N)
+        L.or_comm_exchange(C.byref(s))
+    return s
+
+
+# ------------------------------------------------------------------ halo
+@pytest.mark.parametrize("n,a,depth", [(4, 1.0, 1.5), (5, 1.0, 2.2), (4, 3.1803, 5.73442)])
+def test_halo_closed_form(oracle, n, a, depth):
+    """ghost count of a perfect SC lattice: (n + k_hi + k_lo)^3 - n^3 with
+    k_hi planes at x >= L - depth and k_lo planes at x <= depth"""
+    L = oracle.lib()
+    s = _sc_system(oracle, n, a)
+    h = oracle.Halo()
+    L.or_comm_exchange_halo(C.byref(s), C.byref(h), depth)
+    Lx = n * a
+    planes = [k * a for k in range(n)]
+    k_hi = sum(1 for p in planes if p >= Lx - depth)
+    k_lo = sum(1 for p in planes if p <= depth)
+    assert s.N_ghost == (n + k_hi + k_lo) ** 3 - n ** 3
+    # every ghost is its owner shifted by a lattice vector of the box
+    N, G = s.N_local, s.N_ghost
+    x = oracle.arr(s.x, 3 * (N + G)).reshape(-1, 3)
+    ids = oracle.arr(s.id, N + G, np.int32)
+    owner = {int(i): k for k, i in enumerate(ids[:N])}
+    sh = x[N:] - x[[owner[int(i)] for i in ids[N:]]]
+    assert np.allclose(np.round(sh / Lx) * Lx, sh, atol=1e-12)
+    # update_force conserves the total force
+    f = np.ctypeslib.as_array(s.f, shape=(3 * s.N_max,))
+    f[:3 * (N + G)] = np.random.default_rng(1).normal(size=3 * (N + G))
+    tot = f[:3 * (N + G)].reshape(-1, 3).sum(0)
+    L.or_comm_update_force(C.byref(s), C.byref(h))
+    assert np.allclose(f[:3 * N].reshape(-1, 3).sum(0), tot, atol=1e-10)
+    L.or_halo_free(C.byref(h))
+    L.or_system_free(C.byref(s))
+
+
+# ------------------------------------------------------- binning/neighbour
+def _build(oracle, s, cut, half=0, newton=0):
+    L = oracle.lib()
+    h, b, c = oracle.Halo(), oracle.Bins(), oracle.CSR()
+    L.or_binning(C.byref(s), C.byref(b), cut, cut, cut, 1, 1, 0, 1)
+    L.or_comm_exchange_halo(C.byref(s), C.byref(h), cut)
+    L.or_binning(C.byref(s), C.byref(b), cut, cut, cut, 1, 1, 1, 0)
+    L.or_neigh_csr(C.byref(s), C.byref(b), cut, half, newton, C.byref(c))
+    return h, b, c
+
+
+def test_binning_invariants(oracle):
+    L = oracle.lib()
+    s = _sc_system(oracle, 6, 1.1, jitter=0.2)
+    h, b, c = _build(oracle, s, 1.9)
+    nb = list(b.nbin)
+    cnt = oracle.arr(b.count, b.nbins_total, np.int32)
+    off = oracle.arr(b.offset, b.nbins_total, np.int32)
+    perm = oracle.arr(b.permute, b.n_range, np.int32)
+    assert cnt.sum() == s.N_local + s.N_ghost
+    assert np.array_equal(off[1:], np.cumsum(cnt)[:-1]) and off[0] == 0
+    assert np.array_equal(np.sort(perm), np.arange(b.n_range))
+    x = oracle.arr(s.x, 3 * b.n_range).reshape(-1, 3)
+    lo = np.array(list(b.min))
+    w = (np.array(list(b.max)) - lo) / np.array(nb)
+    for k in range(b.nbins_total):
+        idx = perm[off[k]:off[k] + cnt[k]]
+        assert np.all(np.diff(idx) > 0)      # ascending within a bin
+        bx, by, bz = k // (nb[1] * nb[2]), (k // nb[2]) % nb[1], k % nb[2]
+        cell = np.array([bx, by, bz])
+        assert np.all(x[idx] >= lo + cell * w - 1e-9) and np.all(x[idx] < lo + (cell + 1) * w + 1e-9)
+
+
+@pytest.mark.parametrize("cut", [1.5, 2.3])
+def test_neighbour_counts_brute_force(oracle, cut):
+    L = oracle.lib()
+    s = _sc_system(oracle, 6, 1.0, jitter=0.15, seed=3)
+    h, b, c = _build(oracle, s, cut)
+    row, ent = oracle.csr_arrays(c)
+    brute = np.zeros(s.N_local, np.int32)
+    L.or_brute_neigh_count(C.byref(s), cut, brute.ctypes.data_as(C.POINTER(C.c_int)))
+    assert np.array_equal(np.diff(row), brute)
+    # half list (newton on): every unordered local/ghost pair exactly once
+    s2 = _sc_system(oracle, 6, 1.0, jitter=0.15, seed=3)
+    h2, b2, c2 = _build(oracle, s2, cut, half=1, newton=1)
+    row2, ent2 = oracle.csr_arrays(c2)
+    assert row2[-1] * 2 == row[-1]
+
+
+# --------------------------------------------------------------------- LJ
+def _lj_pair(r, eps=1.0, sig=1.0, rc=2.5):
+    f = 24 * eps * (2 * sig ** 12 / r ** 13 - sig ** 6 / r ** 7)
+    e = 4 * eps * ((sig / r) ** 12 - (sig / r) ** 6) - 4 * eps * ((sig / rc) ** 12 - (sig / rc) ** 6)
+    return f, e
+
+
+def test_lj_dimer_closed_form(oracle):
+    L = oracle.lib()
+    s = oracle.System()
+    L.or_system_init(C.byref(s))
+    L.or_system_grow(C.byref(s), 2)
+    s.N_local = s.N = 2
+    s.mass[0] = 1.0
+    for d in range(3):
+        s.domain[d] = s.hi[d] = 20.0
+    x = np.ctypeslib.as_array(s.x, shape=(6,))
+    r = 1.123
+    x[:] = [5.0, 5.0, 5.0, 5.0 + r, 5.0, 5.0]
+    h, b, c = _build(oracle, s, 2.8)
+    f = np.ctypeslib.as_array(s.f, shape=(3 * s.N_max,))
+    f[:] = 0
+    L.or_lj_force(C.byref(s), C.byref(c), 1.0, 1.0, 2.5, 0)
+    fr, er = _lj_pair(r)
+    assert abs(f[0] + fr) < 1e-12 and abs(f[3] - fr) < 1e-12
+    pe = L.or_lj_energy(C.byref(s), C.byref(c), 1.0, 1.0, 2.5, 0)
+    assert abs(pe - er) < 1e-12
+
+
+def test_lj_fcc_lattice_energy_and_zero_force(oracle):
+    """perfect FCC at rho 0.8442: shells 12@a/sqrt2, 6@a, 24@a*sqrt(3/2),
+    12@a*sqrt2 inside rc = 2.5; forces vanish by symmetry"""
+    m = oracle.MD(oracle.make_config(n=(6, 6, 6)))
+    m.init()
+    a = (4 / 0.8442) ** (1 / 3)
+    shells = [(12, a / math.sqrt(2)), (6, a), (24, a * math.sqrt(1.5)), (12, a * math.sqrt(2))]
+    e = 0.5 * sum(k * _lj_pair(r)[1] for k, r in shells)
+    assert abs(m.potential() / m.system.N - e) < 1e-12
+    assert np.abs(m.arrays()["f"][:m.system.N_local]).max() < 1e-10
+
+
+def _energy_force(oracle, xl, box, force, eps=1.0, sig=1.0, rc=2.5, snap=None, cut=2.8):
+    """oracle energy and forces of a configuration (for finite differences)"""
+    L = oracle.lib()
+    s = oracle.System()
+    L.or_system_init(C.byref(s))
+    n = len(xl)
+    L.or_system_grow(C.byref(s), n)
+    s.N_local = s.N = n
+    s.mass[0] = 1.0
+    for d in range(3):
+        s.domain[d] = s.hi[d] = box
+    np.ctypeslib.as_array(s.x, shape=(3 * n,))[:] = xl.ravel()
+    np.ctypeslib.as_array(s.id, shape=(n,))[:] = np.arange(1, n + 1)
+    L.or_comm_exchange(C.byref(s))
+    h, b, c = _build(oracle, s, cut, newton=1 if force == "snap" else 0)
+    np.ctypeslib.as_array(s.f, shape=(3 * s.N_max,))[:] = 0
+    if force == "lj":
+        L.or_lj_force(C.byref(s), C.byref(c), eps, sig, rc, 0)
+        E = L.or_lj_energy(C.byref(s), C.byref(c), eps, sig, rc, 0)
+    else:
+        L.or_snap_force(C.byref(s), C.byref(c), C.byref(snap))
+        L.or_comm_update_force(C.byref(s), C.byref(h))
+        E = L.or_snap_energy(C.byref(s), C.byref(c), C.byref(snap), None)
+    F = np.zeros((n, 3))
+    # binning sorted the atoms: return forces in input order (by id)
+    F[oracle.arr(s.id, n, np.int32) - 1] = oracle.arr(s.f, 3 * n).reshape(n, 3)
+    L.or_system_free(C.byref(s))
+    L.or_halo_free(C.byref(h))
+    L.or_bins_free(C.byref(b))
+    L.or_csr_free(C.byref(c))
+    return E, F
+
+
+def test_lj_force_is_minus_energy_gradient(oracle):
+    rng = np.random.default_rng(7)
+    n, a = 5, 1.5
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) * a
+    x = g + rng.normal(0, 0.08, g.shape)
+    E, F = _energy_force(oracle, x, n * a, "lj")
+    assert np.abs(F.sum(0)).max() < 1e-10           # momentum conservation
+    h = 1e-6
+    for (i, d) in [(0, 0), (17, 1), (60, 2), (99, 0)]:
+        xp, xm = x.copy(), x.copy()
+        xp[i, d] += h
+        xm[i, d] -= h
+        fd = -(_energy_force(oracle, xp, n * a, "lj")[0] - _energy_force(oracle, xm, n * a, "lj")[0]) / (2 * h)
+        assert abs(fd - F[i, d]) < 1e-6 * max(1.0, abs(F[i, d]))
+
+
+# ------------------------------------------------------------------- NVE
+def test_nve_reversibility_and_energy(oracle):
+    m = oracle.MD(oracle.make_config(n=(8, 8, 8), every=10))
+    m.init()
+    e0 = m.thermo()[2]
+    m.run(100)
+    e1 = m.thermo()[2]
+    assert abs(e1 - e0) < 2e-4                       # NVE drift over 100 steps
+
+
+def test_nve_free_flight(oracle):
+    """zero force: x(t) = x0 + t v exactly in steps of dt"""
+    L = oracle.lib()
+    s = _sc_system(oracle, 3, 1.0)
+    N = s.N_local
+    x0 = oracle.arr(s.x, 3 * N).copy()
+    v0 = oracle.arr(s.v, 3 * N).copy()
+    np.ctypeslib.as_array(s.f, shape=(3 * s.N_max,))[:] = 0
+    for _ in range(10):
+        L.or_nve_initial(C.byref(s))
+        L.or_nve_final(C.byref(s))
+    x = oracle.arr(s.x, 3 * N)
+    assert np.allclose(x, x0 + 10 * s.dt * v0, atol=1e-13)
+    assert np.array_equal(oracle.arr(s.v, 3 * N), v0)
+
+
+# ------------------------------------------------------------------ SNAP
+def test_cg_against_sympy(oracle):
+    g = json.load(open(os.path.join(GOLDEN, "cg_2j8.json")))
+    L = oracle.lib()
+    err = 0.0
+    for j1, j2, j, k1, k2, c in g["cg"][::7]:
+        err = max(err, abs(L.or_snap_cg(8, j1, j2, j, k1, k2) - c))
+    assert err < 1e-12
+
+
+def test_snap_ncoeff(oracle):
+    L = oracle.lib()
+    assert L.or_snap_ncoeff(8) == 55       # W.snapcoeff: 56 = 55 + beta_0
+    assert L.or_snap_ncoeff(6) == 30       # Ta06A.snapcoeff: 31
+
+
+def _shell(rng, n=14, rmin=2.3, rmax=4.5):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return d * rng.uniform(rmin, rmax, n)[:, None]
+
+
+def _sp(oracle, **kw):
+    beta = read_snapcoeff(os.path.join(REF_INPUT, "W.snapcoeff"))
+    return oracle.snap_params(beta, **kw)
+
+
+def test_snap_u_unitary(oracle):
+    """a single neighbour's Wigner U_j is unitary (switching off: U_tot - I)"""
+    L = oracle.lib()
+    sp = _sp(oracle, switchflag=0)
+    rij = np.array([[1.2, -0.7, 2.1]])
+    ur = np.zeros(9 ** 3)
+    ui = np.zeros(9 ** 3)
+    L.or_snap_ui(C.byref(sp), 1, oracle.dptr(rij), oracle.dptr(ur), oracle.dptr(ui))
+    U = (ur + 1j * ui).reshape(9, 9, 9)
+    for j in range(9):
+        Uj = U[j, :j + 1, :j + 1] - np.eye(j + 1)
+        assert np.allclose(Uj @ Uj.conj().T, np.eye(j + 1), atol=1e-12)
+
+
+def test_snap_bispectrum_rotation_invariant(oracle):
+    L = oracle.lib()
+    sp = _sp(oracle)
+    rng = np.random.default_rng(5)
+    rij = _shell(rng)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] *= -1
+    B1, B2 = np.zeros(55), np.zeros(55)
+    L.or_snap_bispectrum(C.byref(sp), len(rij), oracle.dptr(rij), oracle.dptr(B1))
+    r2 = np.ascontiguousarray(rij @ q.T)
+    L.or_snap_bispectrum(C.byref(sp), len(rij), oracle.dptr(r2), oracle.dptr(B2))
+    assert np.abs(B1 - B2).max() < 1e-10 * max(1.0, np.abs(B1).max())
+
+
+def test_snap_dbdr_matches_finite_difference(oracle):
+    L = oracle.lib()
+    sp = _sp(oracle)
+    rng = np.random.default_rng(9)
+    rij = _shell(rng, n=10)
+    dB = np.zeros(55 * 3)
+    jj = 3
+    L.or_snap_dbidrj(C.byref(sp), len(rij), oracle.dptr(rij), jj, oracle.dptr(dB))
+    dB = dB.reshape(55, 3)
+    h = 1e-6
+    for d in range(3):
+        rp, rm = rij.copy(), rij.copy()
+        rp[jj, d] += h
+        rm[jj, d] -= h
+        Bp, Bm = np.zeros(55), np.zeros(55)
+        L.or_snap_bispectrum(C.byref(sp), len(rij), oracle.dptr(rp), oracle.dptr(Bp))
+        L.or_snap_bispectrum(C.byref(sp), len(rij), oracle.dptr(rm), oracle.dptr(Bm))
+        fd = (Bp - Bm) / (2 * h)
+        assert np.abs(fd - dB[:, d]).max() < 1e-6 * max(1.0, np.abs(dB).max())
+
+
+def test_snap_force_is_minus_energy_gradient(oracle):
+    sp = _sp(oracle)
+    rng = np.random.default_rng(11)
+    n, a = 4, 3.1803
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) * a
+    g = np.concatenate([g, g + a / 2])                         # BCC
+    x = g + rng.normal(0, 0.08, g.shape)
+    kw = dict(snap=sp, cut=5.73442)
+    E, F = _energy_force(oracle, x, n * a, "snap", **kw)
+    assert np.abs(F.sum(0)).max() < 1e-9
+    h = 1e-5
+    for (i, d) in [(0, 0), (33, 1), (90, 2)]:
+        xp, xm = x.copy(), x.copy()
+        xp[i, d] += h
+        xm[i, d] -= h
+        fd = -(_energy_force(oracle, xp, n * a, "snap", **kw)[0] -
+               _energy_force(oracle, xm, n * a, "snap", **kw)[0]) / (2 * h)
+        assert abs(fd - F[i, d]) < 1e-6 * max(1.0, abs(F[i, d]))
+
+
+def test_snap_perfect_bcc_zero_force(oracle):
+    sp = _sp(oracle)
+    n, a = 4, 3.1803
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) * a
+    g = np.concatenate([g, g + a / 2]) + 0.01
+    E, F = _energy_force(oracle, g, n * a, "snap", snap=sp, cut=5.73442)
+    assert np.abs(F).max() < 1e-9
+
+
+# ------------------------------------------------------------------ dump
+def test_dump_layout_round_trip(oracle, tmp_path):
+    m = oracle.MD(oracle.make_config(n=(3, 3, 3)))
+    m.init()
+    p = str(tmp_path / "output.0000000000.000")
+    assert m.dump(p) == 0
+    raw = open(p, "rb").read()
+    n = np.frombuffer(raw[:4], np.int32)[0]
+    assert len(raw) == 4 + 88 * n
+    o = 4
+    ids = np.frombuffer(raw[o:o + 4 * n], np.int32); o += 4 * n
+    o += 4 * n + 8 * n
+    x = np.frombuffer(raw[o:o + 24 * n], np.float64).reshape(n, 3)
+    a = m.arrays()
+    assert np.array_equal(ids, a["id"][:n]) and np.array_equal(x, a["x"][:n])
+    out = (C.c_double * 6)()
+    assert oracle.lib().or_dump_compare(C.byref(m.system), p.encode(), out) == 0
+    assert max(out) == 0.0
