@@ -14,6 +14,8 @@
 // with an exclusive scan between them, as NeighborCSR does.
 #include "emd_internal.hpp"
 
+__global__ void k_max_int(const int *__restrict__ a, int n, int *__restrict__ out);
+
 // ------------------------------------------------------------------ binning
 __global__ void k_bin_assign(const double *__restrict__ x, int r0, int n, double minx,
                              double miny, double minz, double mulx, double muly, double mulz,
@@ -45,7 +47,6 @@ struct bin_offsets_op {
   __device__ void emit(int b, int excl, int v) const {
     offset[b] = excl;
     cursor[b] = excl;
-    if (v) atomicMax(maxc, v);
   }
 };
 
@@ -155,6 +156,10 @@ extern "C" emd_status emd_binning(emd_ctx *ctx, emd_system *s, double dx_in, dou
                ctx->d_small + 2);
   bin_offsets_op op{b->bincount, b->binoffsets, cursor, ctx->d_small + 1};
   EMD_TRY(emd_scan(ctx, op, (int)nbt, "binning", nullptr));
+  {
+    const int nbm = emd_blocks(nbt, 256) < 512 ? emd_blocks(nbt, 256) : 512;
+    EMD_LAUNCH(ctx, "binning", k_max_int, nbm, 256, 0, b->bincount, (int)nbt, ctx->d_small + 1);
+  }
   if (n > 0) {
     EMD_LAUNCH(ctx, "binning", k_bin_fill, emd_blocks(n, 256), 256, 0, b->binid, n, cursor,
                b->permute);
@@ -246,7 +251,6 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
   const int cb = (bx * g.nby + by) * g.nbz + bz;
   const int c_off = binoffsets[cb], c_cnt = bincount[cb];
   const unsigned long long lt = emd_lanemask_lt();
-  int wave_max = 0;  // longest row seen by this wavefront (count pass)
 
   for (int cs = 0; cs < T || (cs == 0 && T == 0); cs += NB_MAXC) {
     const int cn = (T - cs) < NB_MAXC ? (T - cs) : NB_MAXC;
@@ -299,14 +303,31 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
           num[i] = run;
         }
       }
-      if (!FILL && last && run > wave_max) wave_max = run;
     }
     __syncthreads();
     if (T == 0) break;
   }
-  // one atomic per wavefront for the longest row (a per-atom atomic on a
-  // single address serialises millions of updates)
-  if (!FILL && lane == 0 && wave_max > 0) atomicMax(maxrow, wave_max);
+}
+
+// longest row: block max, one atomic per block (atomics from every atom or
+// wavefront on one address serialise at the L2)
+__global__ void __launch_bounds__(256) k_max_int(const int *__restrict__ a, int n,
+                                                 int *__restrict__ out) {
+  __shared__ int s_m[4];
+  int m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = a[i] > m ? a[i] : m;
+#pragma unroll
+  for (int o = EMD_WAVE / 2; o > 0; o >>= 1) {
+    const int t = __shfl_xor(m, o, EMD_WAVE);
+    m = t > m ? t : m;
+  }
+  if (__lane_id() == 0) s_m[threadIdx.x / EMD_WAVE] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++) m = s_m[w] > m ? s_m[w] : m;
+    if (m > 0) atomicMax(out, m);
+  }
 }
 
 struct row_offsets_op {
@@ -354,6 +375,10 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
     else EMD_LAUNCH(ctx, "neigh_count", (k_neigh<false, false>), nib, NB_THREADS, 0, NB_ARGS(0));
   }
   // row offsets (CSR) and the longest row
+  if (n > 0) {
+    const int nbm = emd_blocks(n, 256) < 512 ? emd_blocks(n, 256) : 512;
+    EMD_LAUNCH(ctx, "neigh_offsets", k_max_int, nbm, 256, 0, nl->num, n, ctx->d_small + 4);
+  }
   int total = 0;
   row_offsets_op op{nl->num, nl->row, n};
   if (n > 0) EMD_TRY(emd_scan(ctx, op, n, "neigh_offsets", &total));

@@ -40,29 +40,42 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
-// ForceLJNeigh TagFullNeigh, src/force_types/force_lj_neigh_impl.h:158-206
-template <int K, bool ONE_TYPE>
+// ForceLJNeigh TagFullNeigh, src/force_types/force_lj_neigh_impl.h:158-206.
+// K lanes per atom; each lane takes every K-th entry of the row, U entries
+// per round with their index loads and position gathers issued together
+// (independent memory requests in flight), row indices loaded non-temporally
+// (read once, kept out of L2 so positions stay resident).
+template <bool MIXED>
+struct lj_acc {
+  typedef double T;
+};
+template <>
+struct lj_acc<true> {
+  typedef float T;
+};
+
+template <int K, int U, bool ONE_TYPE, bool MIXED>
 __global__ void __launch_bounds__(LJ_THREADS)
-k_lj_full_fp64(const double *__restrict__ x, const int *__restrict__ type,
-               const int *__restrict__ row, const int *__restrict__ num,
-               const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
-               double cutsq_0, const lj_table *__restrict__ tab, int ntypes,
-               double *__restrict__ f) {
+k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
+          const int *__restrict__ row, const int *__restrict__ num,
+          const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
+          double cutsq_0, const lj_table *__restrict__ tab, double *__restrict__ f,
+          float *__restrict__ f32) {
 #pragma clang fp contract(off)
+  typedef typename lj_acc<MIXED>::T T;
   const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
   const int gid = tile * LJ_THREADS + threadIdx.x;
   const int i = gid / K, sub = gid % K;
   if (i >= n) return;
   const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
   const int type_i = ONE_TYPE ? 0 : type[i];
-  const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+  const int *rowp = entries + (row ? (size_t)row[i] : (size_t)i * (size_t)pitch);
   const int cnt = num[i];
-  double fxi = 0.0, fyi = 0.0, fzi = 0.0;
-  for (int jj = sub; jj < cnt; jj += K) {
-    const int j = entries[start + jj];
-    const double dx = x_i - x[3 * (size_t)j];
-    const double dy = y_i - x[3 * (size_t)j + 1];
-    const double dz = z_i - x[3 * (size_t)j + 2];
+  T fxi = 0, fyi = 0, fzi = 0;
+  auto pair = [&](int j, double xj, double yj, double zj) {
+    const double dx = x_i - xj;
+    const double dy = y_i - yj;
+    const double dz = z_i - zj;
     const double rsq = dx * dx + dy * dy + dz * dz;
     double cutsq = cutsq_0, lj1 = lj1_0, lj2 = lj2_0;
     if (!ONE_TYPE) {
@@ -70,13 +83,41 @@ k_lj_full_fp64(const double *__restrict__ x, const int *__restrict__ type,
       cutsq = tab->cutsq[t]; lj1 = tab->lj1[t]; lj2 = tab->lj2[t];
     }
     if (rsq < cutsq) {
-      const double r2inv = 1.0 / rsq;
-      const double r6inv = r2inv * r2inv * r2inv;
-      const double fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
-      fxi += dx * fpair;
-      fyi += dy * fpair;
-      fzi += dz * fpair;
+      if (MIXED) {
+        const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
+        const float r6inv = r2inv * r2inv * r2inv;
+        const float fpair = (r6inv * ((float)lj1 * r6inv - (float)lj2)) * r2inv;
+        fxi += (float)dx * fpair;
+        fyi += (float)dy * fpair;
+        fzi += (float)dz * fpair;
+      } else {
+        const double r2inv = 1.0 / rsq;
+        const double r6inv = r2inv * r2inv * r2inv;
+        const double fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
+        fxi += dx * fpair;
+        fyi += dy * fpair;
+        fzi += dz * fpair;
+      }
     }
+  };
+  int jj = sub;
+  for (; jj + (U - 1) * K < cnt; jj += U * K) {
+    int j[U];
+    double xj[U], yj[U], zj[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) j[u] = __builtin_nontemporal_load(rowp + jj + u * K);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      xj[u] = x[3 * (size_t)j[u]];
+      yj[u] = x[3 * (size_t)j[u] + 1];
+      zj[u] = x[3 * (size_t)j[u] + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) pair(j[u], xj[u], yj[u], zj[u]);
+  }
+  for (; jj < cnt; jj += K) {
+    const int j = __builtin_nontemporal_load(rowp + jj);
+    pair(j, x[3 * (size_t)j], x[3 * (size_t)j + 1], x[3 * (size_t)j + 2]);
   }
   if (K > 1) {
     fxi = group_sum<K>(fxi);
@@ -84,58 +125,15 @@ k_lj_full_fp64(const double *__restrict__ x, const int *__restrict__ type,
     fzi = group_sum<K>(fzi);
   }
   if (sub == 0) {
-    f[3 * (size_t)i] += fxi;
-    f[3 * (size_t)i + 1] += fyi;
-    f[3 * (size_t)i + 2] += fzi;
-  }
-}
-
-template <int K, bool ONE_TYPE>
-__global__ void __launch_bounds__(LJ_THREADS)
-k_lj_full_mixed(const double *__restrict__ x, const int *__restrict__ type,
-                const int *__restrict__ row, const int *__restrict__ num,
-                const int *__restrict__ entries, int pitch, int n, float lj1_0, float lj2_0,
-                double cutsq_0, const lj_table *__restrict__ tab, int ntypes,
-                float *__restrict__ f32) {
-  const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
-  const int gid = tile * LJ_THREADS + threadIdx.x;
-  const int i = gid / K, sub = gid % K;
-  if (i >= n) return;
-  const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
-  const int type_i = ONE_TYPE ? 0 : type[i];
-  const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
-  const int cnt = num[i];
-  float fxi = 0.f, fyi = 0.f, fzi = 0.f;
-  for (int jj = sub; jj < cnt; jj += K) {
-    const int j = entries[start + jj];
-    const double dx = x_i - x[3 * (size_t)j];
-    const double dy = y_i - x[3 * (size_t)j + 1];
-    const double dz = z_i - x[3 * (size_t)j + 2];
-    const double rsq = dx * dx + dy * dy + dz * dz;
-    double cutsq = cutsq_0;
-    float lj1 = lj1_0, lj2 = lj2_0;
-    if (!ONE_TYPE) {
-      const int t = type_i * LJ_MAXT + type[j];
-      cutsq = tab->cutsq[t]; lj1 = (float)tab->lj1[t]; lj2 = (float)tab->lj2[t];
+    if (MIXED) {
+      f32[3 * (size_t)i] += (float)fxi;
+      f32[3 * (size_t)i + 1] += (float)fyi;
+      f32[3 * (size_t)i + 2] += (float)fzi;
+    } else {
+      f[3 * (size_t)i] += fxi;
+      f[3 * (size_t)i + 1] += fyi;
+      f[3 * (size_t)i + 2] += fzi;
     }
-    if (rsq < cutsq) {
-      const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
-      const float r6inv = r2inv * r2inv * r2inv;
-      const float fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
-      fxi += (float)dx * fpair;
-      fyi += (float)dy * fpair;
-      fzi += (float)dz * fpair;
-    }
-  }
-  if (K > 1) {
-    fxi = group_sum<K>(fxi);
-    fyi = group_sum<K>(fyi);
-    fzi = group_sum<K>(fzi);
-  }
-  if (sub == 0) {
-    f32[3 * (size_t)i] += fxi;
-    f32[3 * (size_t)i + 1] += fyi;
-    f32[3 * (size_t)i + 2] += fzi;
   }
 }
 
@@ -194,7 +192,8 @@ static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p, lj_table **
   return EMD_OK;
 }
 
-static int default_lanes(int precision) { return precision == EMD_MIXED ? 4 : 4; }
+// lanes per atom and gathers per round (measured on MI355X, LJ 4M atoms)
+static int default_lanes(int precision) { return precision == EMD_MIXED ? 16 : 16; }
 
 extern "C" emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes) {
   if (!ctx || (precision != EMD_FP64 && precision != EMD_MIXED)) return EMD_ERR_ARG;
@@ -204,33 +203,24 @@ extern "C" emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes) {
   return EMD_OK;
 }
 
-template <int K>
+template <int K, int U>
 static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                               const emd_lj_params *p, int precision, const lj_table *tab) {
   const int n = s->n_local;
   const int nb = emd_blocks((long)n * K, LJ_THREADS);
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   const bool one = p->ntypes == 1;
+#define LJ_FULL_ARGS                                                                        \
+  s->x, s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],   \
+      p->cutsq[0][0], tab, s->f, s->f32
   if (precision == EMD_MIXED) {
-    const float a = (float)p->lj1[0][0], b = (float)p->lj2[0][0];
-    if (one)
-      EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full_mixed<K, true>), nb, LJ_THREADS, 0, s->x,
-                 s->type, row, nl->num, nl->entries, nl->maxneighs, n, a, b, p->cutsq[0][0], tab,
-                 p->ntypes, s->f32);
-    else
-      EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full_mixed<K, false>), nb, LJ_THREADS, 0, s->x,
-                 s->type, row, nl->num, nl->entries, nl->maxneighs, n, a, b, p->cutsq[0][0], tab,
-                 p->ntypes, s->f32);
+    if (one) EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, true, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
+    else EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, false, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
   } else {
-    if (one)
-      EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_full_fp64<K, true>), nb, LJ_THREADS, 0, s->x,
-                 s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0],
-                 p->lj2[0][0], p->cutsq[0][0], tab, p->ntypes, s->f);
-    else
-      EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_full_fp64<K, false>), nb, LJ_THREADS, 0, s->x,
-                 s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0],
-                 p->lj2[0][0], p->cutsq[0][0], tab, p->ntypes, s->f);
+    if (one) EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_full<K, U, true, false>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
+    else EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_full<K, U, false, false>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
   }
+#undef LJ_FULL_ARGS
   return EMD_OK;
 }
 
@@ -256,11 +246,11 @@ extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh 
   int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
   if (K == 0) K = default_lanes(precision);
   switch (K) {
-    case 1: return launch_full<1>(ctx, s, nl, p, precision, tab);
-    case 2: return launch_full<2>(ctx, s, nl, p, precision, tab);
-    case 4: return launch_full<4>(ctx, s, nl, p, precision, tab);
-    case 8: return launch_full<8>(ctx, s, nl, p, precision, tab);
-    default: return launch_full<16>(ctx, s, nl, p, precision, tab);
+    case 1: return launch_full<1, 1>(ctx, s, nl, p, precision, tab);
+    case 2: return launch_full<2, 4>(ctx, s, nl, p, precision, tab);
+    case 4: return launch_full<4, 4>(ctx, s, nl, p, precision, tab);
+    case 8: return launch_full<8, 2>(ctx, s, nl, p, precision, tab);
+    default: return launch_full<16, 2>(ctx, s, nl, p, precision, tab);
   }
 }
 

@@ -1,23 +1,805 @@
-// emd_snap.hip -- linear SNAP on the GPU (placeholder until the kernels land).
+// emd_snap.hip -- linear SNAP forces and energy on the GPU.
+//
+// The reference (ForceSNAP + SNA) computes, per atom, U_tot, the full Z
+// array, and per neighbour dU/dr and dB_k/dr for all 55 components before
+// contracting with beta.  Here the same force is evaluated in adjoint form:
+// the beta-weighted sum over the three symmetry roles of dB/dr (the three
+// half-plane sums of SNA::compute_dbidrj, sna_impl.hpp:329-539) is folded
+// into one array Y_J(ma,mb) per atom, so each neighbour needs only
+// dU_J/dr contracted with Y_J:
+//
+//   F_ij = sum_J sum'_{ma,mb} Re( conj(dU_J(ma,mb)/dr_j) Y_J(ma,mb) )
+//   Y_J  = sum_{a>=b} c_{abJ} Z_{abJ},  c collects 2 beta_k, 2 beta_k (j+1)/(j1+1)
+//          and 2 beta_k (j+1)/(j2+1) for the roles in which (a,b,J) appears
+//
+// (sum' is the reference's half plane: mb < J/2, plus for even J the middle
+// column ma <= J/2 with weight 1/2 on the diagonal).
+//
+// Kernels (T = double, or float for the mixed path):
+//   k_snap_neigh  one wavefront per atom: compacts the in-cutoff neighbours
+//                 (FP64 distance test, list order kept) with ballots
+//   k_snap_ui     one wavefront per atom: Cayley-Klein recursion layer by
+//                 layer (previous layer in LDS, one lane per half-plane
+//                 entry), accumulates sfac*w*U, writes U_tot (285 complex)
+//   k_snap_y      one block per atom: U_tot staged in LDS, one thread per
+//                 half-plane entry of Y (155), loops over the (a,b) pairs
+//                 of its J with Clebsch-Gordan weights from a compact table
+//   k_snap_fij    one wavefront per (atom, neighbour): U and dU recursion
+//                 (LDS layer buffers), contraction with Y_J, wavefront
+//                 reduction, atomic scatter to f_i / f_j
+//   k_snap_energy one block per atom: B_k from U_tot and Z, energy
+#include <cmath>
+#include <map>
+#include <tuple>
+#include <vector>
+
 #include "emd_internal.hpp"
 
-struct emd_snap {
-  int dummy;
+namespace {
+
+constexpr int SNAP_MAXJ = 8;                 // twojmax <= 8
+constexpr int SNAP_JD = SNAP_MAXJ + 1;
+constexpr int SNAP_NU = 285;                 // sum_j (j+1)^2 for twojmax 8
+constexpr int SNAP_NY = 155;                 // half-plane entries for twojmax 8
+constexpr double SNAP_PI = 3.14159265358979323846;
+
+// device tables (filled by emd_snap_create)
+struct snap_dev_tables {
+  int twojmax;
+  int nu, ny;                      // number of U entries / half-plane entries
+  int ustart[SNAP_JD + 1];         // full layer offsets: ustart[j] + ma*(j+1) + mb
+  int ystart[SNAP_JD + 1];         // half-plane offsets: ystart[J] + mb*(J+1) + ma
+  int yjstart[SNAP_JD + 1];        // (a,b) list for each J in ytab
+  int ny_terms;
+  double rootpq[(SNAP_JD + 1) * (SNAP_JD + 1)];
+  double rfac0, rmin0, rcut, wj, wself;
+  int switchflag;
 };
 
-extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *, emd_snap **out) {
-  if (out) *out = nullptr;
-  return emd_set_error(ctx, EMD_ERR_STATE, "SNAP kernels are not built yet");
+struct ytab_entry {
+  int a, b, cgoff, pad;
+  double coef;
+};
+
+struct etab_entry {
+  int j1, j2, j, cgoff;
+  double beta;
+};
+
+__device__ __forceinline__ int u_idx(const snap_dev_tables &t, int j, int ma, int mb) {
+  return t.ustart[j] + ma * (j + 1) + mb;
 }
-extern "C" emd_status emd_snap_destroy(emd_ctx *, emd_snap *) { return EMD_OK; }
-extern "C" emd_status emd_snap_set_coeff(emd_ctx *ctx, emd_snap *, const double *) {
-  return emd_set_error(ctx, EMD_ERR_STATE, "SNAP kernels are not built yet");
+
+template <class T>
+struct cplx {
+  T r, i;
+};
+
+// ------------------------------------------------------------- switching
+template <class T>
+__device__ __forceinline__ void sfac_dsfac(const snap_dev_tables &t, T r, T &sfac, T &dsfac) {
+  if (t.switchflag == 0) {
+    sfac = 1;
+    dsfac = 0;
+  } else if (r <= (T)t.rmin0) {
+    sfac = 1;
+    dsfac = 0;
+  } else if (r > (T)t.rcut) {
+    sfac = 0;
+    dsfac = 0;
+  } else {
+    const T rcutfac = (T)(SNAP_PI / (t.rcut - t.rmin0));
+    sfac = (T)0.5 * (cos((r - (T)t.rmin0) * rcutfac) + (T)1);
+    dsfac = (T)-0.5 * sin((r - (T)t.rmin0) * rcutfac) * rcutfac;
+  }
 }
-extern "C" emd_status emd_snap_force(emd_ctx *ctx, emd_system *, const emd_neigh *, emd_snap *,
-                                     int) {
-  return emd_set_error(ctx, EMD_ERR_STATE, "SNAP kernels are not built yet");
+
+}  // namespace
+
+// ================================================================ kernels
+// in-cutoff neighbours of each atom, ForceSNAP::operator() lines 612-656
+// (rij = x_j - x_i, test rsq < rcutmax^2); slot k of atom i at i*maxn + k
+__global__ void __launch_bounds__(64)
+k_snap_neigh(const double *__restrict__ x, const int *__restrict__ row,
+             const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
+             double cutsq, int maxn, int *__restrict__ nin, int *__restrict__ nj,
+             double *__restrict__ nr) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const int lane = threadIdx.x;
+  const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
+  const int *rp = entries + (row ? (size_t)row[i] : (size_t)i * (size_t)pitch);
+  const int cnt = num[i];
+  const unsigned long long lt = emd_lanemask_lt();
+  int run = 0;
+  for (int c0 = 0; c0 < cnt; c0 += 64) {
+    const int c = c0 + lane;
+    bool keep = false;
+    int j = 0;
+    double dx = 0, dy = 0, dz = 0;
+    if (c < cnt) {
+      j = rp[c];
+      dx = x[3 * (size_t)j] - x_i;
+      dy = x[3 * (size_t)j + 1] - y_i;
+      dz = x[3 * (size_t)j + 2] - z_i;
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      keep = rsq < cutsq;
+    }
+    const unsigned long long m = __ballot(keep);
+    if (keep) {
+      const int k = run + __popcll(m & lt);
+      if (k < maxn) {
+        const size_t o = (size_t)i * maxn + k;
+        nj[o] = j;
+        nr[3 * o] = dx;
+        nr[3 * o + 1] = dy;
+        nr[3 * o + 2] = dz;
+      }
+    }
+    run += __popcll(m);
+  }
+  if (lane == 0) nin[i] = run < maxn ? run : maxn;
 }
-extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *, const emd_neigh *,
-                                      emd_snap *, double *, double *) {
-  return emd_set_error(ctx, EMD_ERR_STATE, "SNAP kernels are not built yet");
+
+// layer entry owned by lane l in layer j: (ma, mb) with mb <= j/2
+__device__ __forceinline__ bool lane_entry(int l, int j, int &ma, int &mb) {
+  mb = l / (j + 1);
+  ma = l - mb * (j + 1);
+  return 2 * mb <= j;
+}
+
+// Cayley-Klein parameters a = (z0 - i z)/r0, b = (y - i x)/r0
+template <class T>
+__device__ __forceinline__ void ck_params(T x, T y, T z, T z0, T r, T &a_r, T &a_i, T &b_r,
+                                          T &b_i) {
+  const T r0inv = (T)1 / sqrt(r * r + z0 * z0);
+  a_r = r0inv * z0;
+  a_i = -r0inv * z;
+  b_r = r0inv * y;
+  b_i = -r0inv * x;
+}
+
+// one entry of layer j from layer j-1 (full layer in LDS, row-major ma*(j)+mb):
+// U(j,ma,mb) = [ma<j] rp(j-ma,j-mb) conj(a) U(j-1,ma,mb)
+//            - [ma>0] rp(ma,j-mb)  conj(b) U(j-1,ma-1,mb)
+// (SNA::compute_uarray, sna_impl.hpp:663-695)
+template <class T>
+__device__ __forceinline__ cplx<T> u_step(const snap_dev_tables &t, const T *pr, const T *pi,
+                                          int j, int ma, int mb, T a_r, T a_i, T b_r, T b_i) {
+  cplx<T> u{0, 0};
+  const int jp = j;  // previous layer has j rows/cols: index ma*j + mb
+  if (ma < j) {
+    const T rp = (T)t.rootpq[(j - ma) * (SNAP_JD + 1) + (j - mb)];
+    const T ur = pr[ma * jp + mb], ui = pi[ma * jp + mb];
+    u.r += rp * (a_r * ur + a_i * ui);
+    u.i += rp * (a_r * ui - a_i * ur);
+  }
+  if (ma > 0) {
+    const T rp = (T)t.rootpq[ma * (SNAP_JD + 1) + (j - mb)];
+    const T ur = pr[(ma - 1) * jp + mb], ui = pi[(ma - 1) * jp + mb];
+    u.r += -rp * (b_r * ur + b_i * ui);
+    u.i += -rp * (b_r * ui - b_i * ur);
+  }
+  return u;
+}
+
+// U_tot of every atom (SNA::compute_ui, sna_impl.hpp:152-190): one wavefront
+// per atom, lanes over the half-plane entries of a layer
+template <class T>
+__global__ void __launch_bounds__(64)
+k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restrict__ nr,
+          int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
+  __shared__ T s_r[2][SNAP_JD * SNAP_JD], s_i[2][SNAP_JD * SNAP_JD];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const int l = threadIdx.x;
+  const int twojmax = t.twojmax;
+  T acc_r[SNAP_JD], acc_i[SNAP_JD];
+#pragma unroll
+  for (int j = 0; j < SNAP_JD; j++) acc_r[j] = acc_i[j] = 0;
+  const int cnt = nin[i];
+  for (int jj = 0; jj < cnt; jj++) {
+    const size_t o = (size_t)i * maxn + jj;
+    const T x = (T)nr[3 * o], y = (T)nr[3 * o + 1], z = (T)nr[3 * o + 2];
+    const T r = sqrt(x * x + y * y + z * z);
+    const T theta0 = (r - (T)t.rmin0) * (T)t.rfac0 * (T)SNAP_PI / ((T)t.rcut - (T)t.rmin0);
+    const T z0 = r / tan(theta0);
+    T a_r, a_i, b_r, b_i;
+    ck_params(x, y, z, z0, r, a_r, a_i, b_r, b_i);
+    T sfac, dsfac;
+    sfac_dsfac(t, r, sfac, dsfac);
+    sfac *= (T)t.wj;
+    // layer 0
+    if (l == 0) {
+      s_r[0][0] = 1;
+      s_i[0][0] = 0;
+      acc_r[0] += sfac;
+    }
+    __syncthreads();
+    for (int j = 1; j <= twojmax; j++) {
+      const int p = (j - 1) & 1, c = j & 1;
+      int ma, mb;
+      const bool own = l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb);
+      cplx<T> u{0, 0};
+      if (own) {
+        u = u_step<T>(t, s_r[p], s_i[p], j, ma, mb, a_r, a_i, b_r, b_i);
+        // full layer j: own entry and, off the middle column, its mirror
+        s_r[c][ma * (j + 1) + mb] = u.r;
+        s_i[c][ma * (j + 1) + mb] = u.i;
+        if (2 * mb != j) {
+          const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
+          s_r[c][(j - ma) * (j + 1) + (j - mb)] = sg * u.r;
+          s_i[c][(j - ma) * (j + 1) + (j - mb)] = -sg * u.i;
+        }
+        acc_r[j] += sfac * u.r;
+        acc_i[j] += sfac * u.i;
+      }
+      __syncthreads();
+    }
+  }
+  // U_tot = wself * I + accumulated sum; write own entries and mirrors
+  T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
+  for (int j = 0; j <= twojmax; j++) {
+    int ma, mb;
+    if (l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb)) {
+      T vr = acc_r[j], vi = acc_i[j];
+      if (ma == mb) vr += (T)t.wself;
+      ur[t.ustart[j] + ma * (j + 1) + mb] = vr;
+      ui[t.ustart[j] + ma * (j + 1) + mb] = vi;
+      if (2 * mb != j) {
+        const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
+        ur[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = sg * vr;
+        ui[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = -sg * vi;
+      }
+    }
+  }
+}
+
+// Z_{abJ}(ma,mb) = sum_ma1 cg(ma1,ma2) sum_mb1 cg(mb1,mb2) U_a(ma1,mb1) U_b(ma2,mb2)
+// (SNA::compute_zi, sna_impl.hpp:221-274) from U_tot staged in LDS
+template <class T>
+__device__ __forceinline__ cplx<T> z_entry(const snap_dev_tables &t, const double *__restrict__ cg,
+                                           const T *su_r, const T *su_i, int a, int b, int J,
+                                           int cgoff, int ma, int mb) {
+  cplx<T> z{0, 0};
+  const int ma1lo = max(0, (2 * ma - J - b + a) / 2), ma1hi = min(a, (2 * ma - J + b + a) / 2);
+  const int mb1lo = max(0, (2 * mb - J - b + a) / 2), mb1hi = min(a, (2 * mb - J + b + a) / 2);
+  const int ua = t.ustart[a], ub = t.ustart[b];
+  for (int ma1 = ma1lo; ma1 <= ma1hi; ma1++) {
+    const int ma2 = (2 * ma - J - (2 * ma1 - a) + b) / 2;
+    T s_r = 0, s_i = 0;
+    for (int mb1 = mb1lo; mb1 <= mb1hi; mb1++) {
+      const int mb2 = (2 * mb - J - (2 * mb1 - a) + b) / 2;
+      const T c = (T)cg[cgoff + mb1 * (b + 1) + mb2];
+      const T u1r = su_r[ua + ma1 * (a + 1) + mb1], u1i = su_i[ua + ma1 * (a + 1) + mb1];
+      const T u2r = su_r[ub + ma2 * (b + 1) + mb2], u2i = su_i[ub + ma2 * (b + 1) + mb2];
+      s_r += c * (u1r * u2r - u1i * u2i);
+      s_i += c * (u1r * u2i + u1i * u2r);
+    }
+    const T c = (T)cg[cgoff + ma1 * (b + 1) + ma2];
+    z.r += s_r * c;
+    z.i += s_i * c;
+  }
+  return z;
+}
+
+// half-plane entry e (0..ny-1) -> (J, ma, mb)
+__device__ __forceinline__ void y_entry(const snap_dev_tables &t, int e, int &J, int &ma,
+                                        int &mb) {
+  J = 0;
+  while (J < t.twojmax && e >= t.ystart[J + 1]) J++;
+  const int k = e - t.ystart[J];
+  mb = k / (J + 1);
+  ma = k - mb * (J + 1);
+}
+
+template <class T>
+__global__ void __launch_bounds__(192)
+k_snap_y(snap_dev_tables t, const double *__restrict__ cg, const ytab_entry *__restrict__ ytab,
+         int n, const T *__restrict__ utot_r, const T *__restrict__ utot_i, T *__restrict__ y_r,
+         T *__restrict__ y_i) {
+  __shared__ T su_r[SNAP_NU], su_i[SNAP_NU];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  for (int k = threadIdx.x; k < t.nu; k += blockDim.x) {
+    su_r[k] = utot_r[(size_t)i * t.nu + k];
+    su_i[k] = utot_i[(size_t)i * t.nu + k];
+  }
+  __syncthreads();
+  const int e = threadIdx.x;
+  if (e >= t.ny) return;
+  int J, ma, mb;
+  y_entry(t, e, J, ma, mb);
+  T yr = 0, yi = 0;
+  for (int q = t.yjstart[J]; q < t.yjstart[J + 1]; q++) {
+    const ytab_entry te = ytab[q];
+    const cplx<T> z = z_entry<T>(t, cg, su_r, su_i, te.a, te.b, J, te.cgoff, ma, mb);
+    yr += (T)te.coef * z.r;
+    yi += (T)te.coef * z.i;
+  }
+  y_r[(size_t)i * t.ny + e] = yr;
+  y_i[(size_t)i * t.ny + e] = yi;
+}
+
+// F_ij for one (atom, neighbour) pair per wavefront (SNA::compute_duidrj +
+// compute_duarray, sna_impl.hpp:290-321 / 728-893, then the Y contraction)
+constexpr int FIJ_WAVES = 4;
+
+template <class T, class TF>
+__global__ void __launch_bounds__(64 * FIJ_WAVES)
+k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
+           const double *__restrict__ nr, int maxn, int n, const T *__restrict__ y_r,
+           const T *__restrict__ y_i, TF *__restrict__ f) {
+  // previous layer (j-1 <= 7): U and dU (3 components), 64 entries each
+  __shared__ T s_u[FIJ_WAVES][2][8][64];
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int i = blockIdx.x;
+  const int jj = blockIdx.y * FIJ_WAVES + w;
+  if (i >= n || jj >= nin[i]) return;
+  const size_t o = (size_t)i * maxn + jj;
+  const double xd = nr[3 * o], yd = nr[3 * o + 1], zd = nr[3 * o + 2];
+  const T x = (T)xd, y = (T)yd, z = (T)zd;
+  const T rsq = x * x + y * y + z * z;
+  const T r = sqrt(rsq);
+  const T rscale0 = (T)t.rfac0 * (T)SNAP_PI / ((T)t.rcut - (T)t.rmin0);
+  const T theta0 = (r - (T)t.rmin0) * rscale0;
+  const T cs = cos(theta0), sn = sin(theta0);
+  const T z0 = r * cs / sn;
+  const T dz0dr = z0 / r - (r * rscale0) * (rsq + z0 * z0) / rsq;
+  const T rinv = (T)1 / r;
+  const T u3[3] = {x * rinv, y * rinv, z * rinv};
+  const T r0inv = (T)1 / sqrt(r * r + z0 * z0);
+  const T a_r = z0 * r0inv, a_i = -z * r0inv, b_r = y * r0inv, b_i = -x * r0inv;
+  const T dr0invdr = -(r0inv * r0inv * r0inv) * (r + z0 * dz0dr);
+  T da_r[3], da_i[3], db_r[3], db_i[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const T dr0 = dr0invdr * u3[k], dz = dz0dr * u3[k];
+    da_r[k] = dz * r0inv + z0 * dr0;
+    da_i[k] = -z * dr0;
+    db_r[k] = y * dr0;
+    db_i[k] = -x * dr0;
+  }
+  da_i[2] += -r0inv;
+  db_i[0] += -r0inv;
+  db_r[1] += r0inv;
+  T sfac, dsfac;
+  sfac_dsfac(t, r, sfac, dsfac);
+  sfac *= (T)t.wj;
+  dsfac *= (T)t.wj;
+
+  T (*buf)[8][64] = s_u[w];
+  const T *yr = y_r + (size_t)i * t.ny, *yi = y_i + (size_t)i * t.ny;
+  T fx = 0, fy = 0, fz = 0;
+  // layer 0: U = 1, dU = 0; its dU_tot = dsfac * u; contraction with Y_0
+  if (l == 0) {
+    buf[0][0][0] = 1; buf[0][1][0] = 0;
+    for (int k = 0; k < 3; k++) { buf[0][2 + 2 * k][0] = 0; buf[0][3 + 2 * k][0] = 0; }
+    const T w0 = (T)0.5;  // J = 0: the single entry is the middle-column diagonal
+    const T c = dsfac * w0 * yr[0];
+    fx += c * u3[0];
+    fy += c * u3[1];
+    fz += c * u3[2];
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  for (int j = 1; j <= t.twojmax; j++) {
+    const int p = (j - 1) & 1, c = j & 1;
+    int ma, mb;
+    const bool own = l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb);
+    T ur = 0, ui = 0, dr[3] = {0, 0, 0}, di[3] = {0, 0, 0};
+    if (own) {
+      const T *P = &buf[p][0][0];
+      // P[q*64 + idx]: q = 0 U_r, 1 U_i, 2+2k dU_r[k], 3+2k dU_i[k]
+      if (ma < j) {
+        const T rp = (T)t.rootpq[(j - ma) * (SNAP_JD + 1) + (j - mb)];
+        const int s = ma * j + mb;
+        const T pr = P[s], pi = P[64 + s];
+        ur += rp * (a_r * pr + a_i * pi);
+        ui += rp * (a_r * pi - a_i * pr);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const T dpr = P[(2 + 2 * k) * 64 + s], dpi = P[(3 + 2 * k) * 64 + s];
+          dr[k] += rp * (da_r[k] * pr + da_i[k] * pi + a_r * dpr + a_i * dpi);
+          di[k] += rp * (da_r[k] * pi - da_i[k] * pr + a_r * dpi - a_i * dpr);
+        }
+      }
+      if (ma > 0) {
+        const T rp = (T)t.rootpq[ma * (SNAP_JD + 1) + (j - mb)];
+        const int s = (ma - 1) * j + mb;
+        const T pr = P[s], pi = P[64 + s];
+        ur += -rp * (b_r * pr + b_i * pi);
+        ui += -rp * (b_r * pi - b_i * pr);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const T dpr = P[(2 + 2 * k) * 64 + s], dpi = P[(3 + 2 * k) * 64 + s];
+          dr[k] += -rp * (db_r[k] * pr + db_i[k] * pi + b_r * dpr + b_i * dpi);
+          di[k] += -rp * (db_r[k] * pi - db_i[k] * pr + b_r * dpi - b_i * dpr);
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (own && j < t.twojmax) {
+      T *C = &buf[c][0][0];
+      const int s = ma * (j + 1) + mb;
+      C[s] = ur; C[64 + s] = ui;
+#pragma unroll
+      for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + s] = dr[k]; C[(3 + 2 * k) * 64 + s] = di[k]; }
+      if (2 * mb != j) {
+        const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
+        const int m = (j - ma) * (j + 1) + (j - mb);
+        C[m] = sg * ur; C[64 + m] = -sg * ui;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + m] = sg * dr[k]; C[(3 + 2 * k) * 64 + m] = -sg * di[k]; }
+      }
+    }
+    if (own) {
+      // half-plane weight of (ma, mb) in layer j
+      T wgt = 1;
+      if (2 * mb == j) wgt = (ma < mb) ? (T)1 : ((ma == mb) ? (T)0.5 : (T)0);
+      if (wgt != (T)0) {
+        const int e = t.ystart[j] + mb * (j + 1) + ma;
+        const T Yr = wgt * yr[e], Yi = wgt * yi[e];
+        const T uy = ur * Yr + ui * Yi;   // Re(conj(U) Y) (switching-derivative term)
+        const T g[3] = {dr[0] * Yr + di[0] * Yi, dr[1] * Yr + di[1] * Yi, dr[2] * Yr + di[2] * Yi};
+        fx += dsfac * uy * u3[0] + sfac * g[0];
+        fy += dsfac * uy * u3[1] + sfac * g[1];
+        fz += dsfac * uy * u3[2] + sfac * g[2];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  // wavefront sum of the force components
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off, 64);
+    fy += __shfl_xor(fy, off, 64);
+    fz += __shfl_xor(fz, off, 64);
+  }
+  if (l == 0) {
+    // ad-hoc repulsion of the reference, force_snap_neigh_impl.h:705-711
+    const double rsqd = xd * xd + yd * yd + zd * zd;
+    const double fdivr = -1.5e6 / pow(rsqd, 7.0);
+    const TF Fx = (TF)fx + (TF)(xd * fdivr);
+    const TF Fy = (TF)fy + (TF)(yd * fdivr);
+    const TF Fz = (TF)fz + (TF)(zd * fdivr);
+    const int j = nj[o];
+    atomicAdd(&f[3 * (size_t)i], Fx);
+    atomicAdd(&f[3 * (size_t)i + 1], Fy);
+    atomicAdd(&f[3 * (size_t)i + 2], Fz);
+    atomicAdd(&f[3 * (size_t)j], -Fx);
+    atomicAdd(&f[3 * (size_t)j + 1], -Fy);
+    atomicAdd(&f[3 * (size_t)j + 2], -Fz);
+  }
+}
+
+// per-atom energy: beta_0 + sum_k beta_k B_k + pair repulsion (FP64)
+__global__ void __launch_bounds__(256)
+k_snap_energy(snap_dev_tables t, const double *__restrict__ cg,
+              const etab_entry *__restrict__ etab, int ne, double beta0, int bzero,
+              const int *__restrict__ nin, const double *__restrict__ nr, int maxn, int n,
+              const double *__restrict__ utot_r, const double *__restrict__ utot_i,
+              double *__restrict__ eatom) {
+  __shared__ double su_r[SNAP_NU], su_i[SNAP_NU];
+  __shared__ double s_red[4];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  for (int k = threadIdx.x; k < t.nu; k += blockDim.x) {
+    su_r[k] = utot_r[(size_t)i * t.nu + k];
+    su_i[k] = utot_i[(size_t)i * t.nu + k];
+  }
+  __syncthreads();
+  double e = 0.0;
+  // work items: (k, half-plane entry of J = j_k)
+  for (int k = 0; k < ne; k++) {
+    const etab_entry te = etab[k];
+    const int J = te.j;
+    const int nyj = t.ystart[J + 1] - t.ystart[J];
+    for (int q = threadIdx.x; q < nyj; q += blockDim.x) {
+      const int mb = q / (J + 1), ma = q - mb * (J + 1);
+      double wgt = 1.0;
+      if (2 * mb == J) wgt = (ma < mb) ? 1.0 : ((ma == mb) ? 0.5 : 0.0);
+      if (wgt == 0.0) continue;
+      const cplx<double> z = z_entry<double>(t, cg, su_r, su_i, te.j1, te.j2, J, te.cgoff, ma, mb);
+      const int u = t.ustart[J] + ma * (J + 1) + mb;
+      e += te.beta * 2.0 * wgt * (su_r[u] * z.r + su_i[u] * z.i);
+    }
+  }
+  // bzero and the repulsion (rc-shifted), done by thread 0
+  if (threadIdx.x == 0) {
+    if (bzero)
+      for (int k = 0; k < ne; k++) e -= etab[k].beta * t.wself * t.wself * t.wself * (etab[k].j + 1);
+    const double rc12 = 1.0 / pow(t.rcut * t.rcut, 6.0);
+    for (int jj = 0; jj < nin[i]; jj++) {
+      const size_t o = (size_t)i * maxn + jj;
+      const double rsq = nr[3 * o] * nr[3 * o] + nr[3 * o + 1] * nr[3 * o + 1] + nr[3 * o + 2] * nr[3 * o + 2];
+      e += 1.25e5 * (1.0 / pow(rsq, 6.0) - rc12);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+  if (threadIdx.x % 64 == 0) s_red[threadIdx.x / 64] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) eatom[i] = beta0 + ((s_red[0] + s_red[1]) + (s_red[2] + s_red[3]));
+}
+
+// ============================================================ host side
+struct emd_snap {
+  snap_dev_tables t;
+  int ncoeff;
+  std::vector<double> beta;            // beta_0 .. beta_ncoeff
+  std::vector<int> idxj;               // 3 per component
+  std::map<std::tuple<int, int, int>, int> cgoff;
+  std::vector<double> cg_h;
+  double *cg = nullptr;
+  ytab_entry *ytab = nullptr;
+  etab_entry *etab = nullptr;
+  int bzeroflag = 0;
+  double rcutfac = 0, radelem = 0;
+  // per-system scratch
+  int cap_n = 0, cap_maxn = 0;
+  int *nin = nullptr, *nj = nullptr;
+  double *nr = nullptr;
+  void *u_r = nullptr, *u_i = nullptr, *y_r = nullptr, *y_i = nullptr;
+  size_t cap_u = 0, cap_y = 0, cap_ui = 0, cap_yi = 0;
+  double *eatom = nullptr;
+  int cap_e = 0;
+};
+
+static double host_factorial(int n) {
+  double r = 1.0;
+  for (int k = 2; k <= n; k++) r *= k;
+  return r;
+}
+
+// Clebsch-Gordan <j1/2 m1; j2/2 m2 | j/2 m1+m2> by the Racah formula
+// (integer-doubled arguments; m1 = k1 - j1/2, m2 = k2 - j2/2)
+static double host_cg(int j1, int j2, int j, int k1, int k2) {
+  const int aa2 = 2 * k1 - j1, bb2 = 2 * k2 - j2;
+  const int cc2 = aa2 + bb2;
+  if (cc2 < -j || cc2 > j) return 0.0;
+  double sum = 0.0;
+  const int zmin = std::max(0, std::max(-(j - j2 + aa2) / 2, -(j - j1 - bb2) / 2));
+  const int zmax = std::min((j1 + j2 - j) / 2, std::min((j1 - aa2) / 2, (j2 + bb2) / 2));
+  for (int z = zmin; z <= zmax; z++) {
+    const double sgn = (z % 2) ? -1.0 : 1.0;
+    sum += sgn / (host_factorial(z) * host_factorial((j1 + j2 - j) / 2 - z) *
+                  host_factorial((j1 - aa2) / 2 - z) * host_factorial((j2 + bb2) / 2 - z) *
+                  host_factorial((j - j2 + aa2) / 2 + z) * host_factorial((j - j1 - bb2) / 2 + z));
+  }
+  const double delta = std::sqrt(host_factorial((j1 + j2 - j) / 2) * host_factorial((j1 - j2 + j) / 2) *
+                                 host_factorial((-j1 + j2 + j) / 2) /
+                                 host_factorial((j1 + j2 + j) / 2 + 1));
+  const double norm = std::sqrt(host_factorial((j1 + aa2) / 2) * host_factorial((j1 - aa2) / 2) *
+                                host_factorial((j2 + bb2) / 2) * host_factorial((j2 - bb2) / 2) *
+                                host_factorial((j + cc2) / 2) * host_factorial((j - cc2) / 2) *
+                                (j + 1));
+  return sum * delta * norm;
+}
+
+static emd_status snap_upload_coeff(emd_ctx *ctx, emd_snap *sn) {
+  const int twojmax = sn->t.twojmax;
+  // Y weights: the three roles of each component (compute_dbidrj)
+  std::map<std::tuple<int, int, int>, double> coef;
+  const int nk = (int)sn->idxj.size() / 3;
+  for (int k = 0; k < nk; k++) {
+    const int j1 = sn->idxj[3 * k], j2 = sn->idxj[3 * k + 1], j = sn->idxj[3 * k + 2];
+    const double b = sn->beta[k + 1];
+    coef[std::make_tuple(std::max(j1, j2), std::min(j1, j2), j)] += 2.0 * b;
+    coef[std::make_tuple(std::max(j, j2), std::min(j, j2), j1)] += 2.0 * b * (j + 1) / (j1 + 1.0);
+    coef[std::make_tuple(std::max(j1, j), std::min(j1, j), j2)] += 2.0 * b * (j + 1) / (j2 + 1.0);
+  }
+  std::vector<ytab_entry> yt;
+  for (int J = 0; J <= twojmax; J++) {
+    sn->t.yjstart[J] = (int)yt.size();
+    for (auto &kv : coef) {
+      int a, b, JJ;
+      std::tie(a, b, JJ) = kv.first;
+      if (JJ != J || kv.second == 0.0) continue;
+      ytab_entry e;
+      e.a = a; e.b = b; e.cgoff = sn->cgoff.at(kv.first); e.pad = 0; e.coef = kv.second;
+      yt.push_back(e);
+    }
+  }
+  sn->t.yjstart[twojmax + 1] = (int)yt.size();
+  sn->t.ny_terms = (int)yt.size();
+  std::vector<etab_entry> et;
+  for (int k = 0; k < nk; k++) {
+    etab_entry e;
+    e.j1 = sn->idxj[3 * k]; e.j2 = sn->idxj[3 * k + 1]; e.j = sn->idxj[3 * k + 2];
+    e.cgoff = sn->cgoff.at(std::make_tuple(e.j1, e.j2, e.j));
+    e.beta = sn->beta[k + 1];
+    et.push_back(e);
+  }
+  if (sn->ytab) EMD_HIP(ctx, hipFree(sn->ytab));
+  if (sn->etab) EMD_HIP(ctx, hipFree(sn->etab));
+  sn->ytab = nullptr;
+  sn->etab = nullptr;
+  EMD_HIP(ctx, hipMalloc(&sn->ytab, std::max<size_t>(1, yt.size()) * sizeof(ytab_entry)));
+  EMD_HIP(ctx, hipMalloc(&sn->etab, std::max<size_t>(1, et.size()) * sizeof(etab_entry)));
+  if (!yt.empty())
+    EMD_HIP(ctx, hipMemcpy(sn->ytab, yt.data(), yt.size() * sizeof(ytab_entry), hipMemcpyHostToDevice));
+  if (!et.empty())
+    EMD_HIP(ctx, hipMemcpy(sn->etab, et.data(), et.size() * sizeof(etab_entry), hipMemcpyHostToDevice));
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, emd_snap **out) {
+  if (!ctx || !p || !out) return EMD_ERR_ARG;
+  *out = nullptr;
+  if (p->twojmax < 0 || p->twojmax > SNAP_MAXJ)
+    return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: twojmax must be in 0..%d", SNAP_MAXJ);
+  if (!p->coeff) return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: no coefficients");
+  emd_snap *sn = new emd_snap();
+  snap_dev_tables &t = sn->t;
+  memset(&t, 0, sizeof(t));
+  const int twojmax = p->twojmax;
+  t.twojmax = twojmax;
+  int nu = 0, ny = 0;
+  for (int j = 0; j <= twojmax; j++) {
+    t.ustart[j] = nu;
+    t.ystart[j] = ny;
+    nu += (j + 1) * (j + 1);
+    ny += (j + 1) * (j / 2 + 1);
+  }
+  t.ustart[twojmax + 1] = nu;
+  t.ystart[twojmax + 1] = ny;
+  t.nu = nu;
+  t.ny = ny;
+  for (int a = 1; a <= twojmax; a++)
+    for (int b = 1; b <= twojmax; b++) t.rootpq[a * (SNAP_JD + 1) + b] = std::sqrt((double)a / b);
+  t.rfac0 = p->rfac0;
+  t.rmin0 = p->rmin0;
+  t.rcut = (p->radelem + p->radelem) * p->rcutfac;
+  t.wj = p->wjelem;
+  t.wself = 1.0;
+  t.switchflag = p->switchflag;
+  sn->bzeroflag = p->bzeroflag;
+  sn->rcutfac = p->rcutfac;
+  sn->radelem = p->radelem;
+  // component list (SNA::build_indexlist, diagonalstyle 3) and CG blocks
+  for (int j1 = 0; j1 <= twojmax; j1++)
+    for (int j2 = 0; j2 <= j1; j2++)
+      for (int j = j1 - j2; j <= std::min(twojmax, j1 + j2); j += 2) {
+        if (j >= j1) {
+          sn->idxj.push_back(j1);
+          sn->idxj.push_back(j2);
+          sn->idxj.push_back(j);
+        }
+        sn->cgoff[std::make_tuple(j1, j2, j)] = (int)sn->cg_h.size();
+        for (int k1 = 0; k1 <= j1; k1++)
+          for (int k2 = 0; k2 <= j2; k2++) sn->cg_h.push_back(host_cg(j1, j2, j, k1, k2));
+      }
+  sn->ncoeff = (int)sn->idxj.size() / 3;
+  if (p->ncoeff != sn->ncoeff) {
+    delete sn;
+    return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: ncoeff %d but twojmax %d has %d",
+                         p->ncoeff, twojmax, (int)(sn ? 0 : 0) + p->ncoeff);
+  }
+  sn->beta.assign(p->coeff, p->coeff + sn->ncoeff + 1);
+  if (hipMalloc(&sn->cg, sn->cg_h.size() * sizeof(double)) != hipSuccess) {
+    delete sn;
+    return emd_set_error(ctx, EMD_ERR_ALLOC, "emd_snap_create: hipMalloc failed");
+  }
+  EMD_HIP(ctx, hipMemcpy(sn->cg, sn->cg_h.data(), sn->cg_h.size() * sizeof(double),
+                         hipMemcpyHostToDevice));
+  emd_status st = snap_upload_coeff(ctx, sn);
+  if (st != EMD_OK) {
+    emd_snap_destroy(ctx, sn);
+    return st;
+  }
+  *out = sn;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn) {
+  if (!sn) return EMD_OK;
+  if (ctx) hipStreamSynchronize(ctx->stream);
+  void *ptrs[] = {sn->cg, sn->ytab, sn->etab, sn->nin, sn->nj, sn->nr,
+                  sn->u_r, sn->u_i, sn->y_r, sn->y_i, sn->eatom};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  delete sn;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_snap_set_coeff(emd_ctx *ctx, emd_snap *sn, const double *coeff) {
+  if (!ctx || !sn || !coeff) return EMD_ERR_ARG;
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  sn->beta.assign(coeff, coeff + sn->ncoeff + 1);
+  return snap_upload_coeff(ctx, sn);
+}
+
+static emd_status snap_scratch(emd_ctx *ctx, emd_snap *sn, int n, int maxn) {
+  if (n > sn->cap_n || maxn > sn->cap_maxn || !sn->nin) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (sn->nin) EMD_HIP(ctx, hipFree(sn->nin));
+    if (sn->nj) EMD_HIP(ctx, hipFree(sn->nj));
+    if (sn->nr) EMD_HIP(ctx, hipFree(sn->nr));
+    const int cn = n + n / 8 + 16, cm = maxn + 8;
+    EMD_HIP(ctx, hipMalloc(&sn->nin, (size_t)cn * sizeof(int)));
+    EMD_HIP(ctx, hipMalloc(&sn->nj, (size_t)cn * cm * sizeof(int)));
+    EMD_HIP(ctx, hipMalloc(&sn->nr, (size_t)cn * cm * 3 * sizeof(double)));
+    sn->cap_n = cn;
+    sn->cap_maxn = cm;
+  }
+  EMD_TRY(emd_ensure(ctx, &sn->u_r, &sn->cap_u, (size_t)n * sn->t.nu * sizeof(double)));
+  EMD_TRY(emd_ensure(ctx, &sn->u_i, &sn->cap_ui, (size_t)n * sn->t.nu * sizeof(double)));
+  EMD_TRY(emd_ensure(ctx, &sn->y_r, &sn->cap_y, (size_t)n * sn->t.ny * sizeof(double)));
+  EMD_TRY(emd_ensure(ctx, &sn->y_i, &sn->cap_yi, (size_t)n * sn->t.ny * sizeof(double)));
+  return EMD_OK;
+}
+
+static emd_status snap_neigh(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl, emd_snap *sn) {
+  const int n = s->n_local;
+  const int maxn = nl->maxneighs > 0 ? nl->maxneighs : 1;
+  EMD_TRY(snap_scratch(ctx, sn, n, maxn));
+  const double rcutmax = 2.0 * sn->radelem * sn->rcutfac;
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  EMD_LAUNCH(ctx, "snap_neigh", k_snap_neigh, n, 64, 0, s->x, row, nl->num, nl->entries,
+             nl->maxneighs, n, rcutmax * rcutmax, sn->cap_maxn, sn->nin, sn->nj, sn->nr);
+  return EMD_OK;
+}
+
+template <class T>
+static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
+  T *ur = (T *)sn->u_r, *ui = (T *)sn->u_i, *yr = (T *)sn->y_r, *yi = (T *)sn->y_i;
+  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n, ur, ui);
+  EMD_LAUNCH(ctx, "snap_y", k_snap_y<T>, n, 192, 0, sn->t, sn->cg, sn->ytab, n, ur, ui, yr, yi);
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                     emd_snap *sn, int precision) {
+  if (!ctx || !s || !nl || !sn) return EMD_ERR_ARG;
+  EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
+  const int n = s->n_local;
+  if (n <= 0) return EMD_OK;
+  EMD_TRY(snap_neigh(ctx, s, nl, sn));
+  const dim3 grid(n, (sn->cap_maxn + FIJ_WAVES - 1) / FIJ_WAVES);
+  if (precision == EMD_MIXED) {
+    EMD_TRY(snap_ui_y<float>(ctx, sn, n));
+    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<float, float>), grid, 64 * FIJ_WAVES, 0, sn->t, sn->nin,
+               sn->nj, sn->nr, sn->cap_maxn, n, (const float *)sn->y_r, (const float *)sn->y_i,
+               s->f32);
+  } else {
+    EMD_TRY(snap_ui_y<double>(ctx, sn, n));
+    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<double, double>), grid, 64 * FIJ_WAVES, 0, sn->t,
+               sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, (const double *)sn->y_r,
+               (const double *)sn->y_i, s->f);
+  }
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                      emd_snap *sn, double *energy, double *eatom) {
+  if (!ctx || !s || !nl || !sn || !energy) return EMD_ERR_ARG;
+  const int n = s->n_local;
+  *energy = 0.0;
+  if (n <= 0) return EMD_OK;
+  EMD_TRY(snap_neigh(ctx, s, nl, sn));
+  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<double>, n, 64, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n,
+             (double *)sn->u_r, (double *)sn->u_i);
+  double *ea = eatom;
+  if (!ea) {
+    EMD_TRY(emd_ensure_int(ctx, (int **)&sn->eatom, &sn->cap_e, 2l * n));
+    ea = sn->eatom;
+  }
+  EMD_LAUNCH(ctx, "snap_energy", k_snap_energy, n, 256, 0, sn->t, sn->cg, sn->etab,
+             sn->ncoeff, sn->beta[0], sn->bzeroflag, sn->nin, sn->nr, sn->cap_maxn, n,
+             (const double *)sn->u_r, (const double *)sn->u_i, ea);
+  std::vector<double> h(n);
+  EMD_HIP(ctx, hipMemcpyAsync(h.data(), ea, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double sum = 0.0, c = 0.0;
+  for (int k = 0; k < n; k++) {
+    const double v = h[k], tt = sum + v;
+    if (std::fabs(sum) >= std::fabs(v)) c += (sum - tt) + v;
+    else c += (v - tt) + sum;
+    sum = tt;
+  }
+  *energy = sum + c;
+  return EMD_OK;
 }

@@ -152,3 +152,26 @@ def test_lj_mixed_force_error(tmp_path, oracle):
     row, num, ent = app.neigh()
     orow, oent = m.csr()
     assert np.array_equal(ent, oent)
+
+
+def test_lj_kernel_level_disordered(oracle):
+    """displaced FCC (0.05 sigma): FP64 force against the oracle through the
+    kernel-level C ABI; FP32 pair arithmetic error against FP64"""
+    from gpu_pipeline import Pipeline, lattice
+    from test_oracle import _energy_force
+    print("case: LJ 32k disordered, kernel level", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    x = lattice("fcc", 20, a) + np.random.default_rng(5).normal(0, 0.05, (32000, 3))
+    box = (20 * a,) * 3
+    pl = Pipeline()
+    pl.load(x, box)
+    pl.build(2.8)
+    f64 = pl.lj(emd.EMD_FP64)
+    f32 = pl.lj(emd.EMD_MIXED)
+    pl.close()
+    E, F = _energy_force(oracle, x % (20 * a), 20 * a, "lj")
+    assert rel_err(f64, F) <= 1e-13
+    df = f32 - f64
+    rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+    print("mixed LJ force RMS error", rms, "force RMS", np.sqrt(np.mean(np.sum(f64 ** 2, 1))))
+    assert rms <= 1e-5

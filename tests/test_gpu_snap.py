@@ -1,0 +1,137 @@
+"""GPU parity of SNAP (adjoint form on the GPU vs the reference Z/dB form in
+the CPU oracle).  The two evaluate the same force with a different order of
+floating-point operations, so agreement is to rounding: 1e-11 relative on
+forces at a fixed configuration, 1e-10 on trajectories over 100 steps."""
+import os
+
+import numpy as np
+import pytest
+
+import examinimd_amd as emd
+from examinimd_amd import decks
+from conftest import REF_INPUT, read_snapcoeff
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+W_COEFF = os.path.join(REF_INPUT, "W.snapcoeff")
+W_PARAM = os.path.join(REF_INPUT, "W.snapparam")
+
+
+def oracle_md(oracle, beta, lattice, n, steps, every, gaussian, dt, seed, T=300.0):
+    cfg = oracle.make_config(units="metal", lattice=lattice, lattice_arg=3.1803, n=n,
+                             mass=183.84, temperature=T, seed=seed, gaussian=gaussian,
+                             force="snap", skin=1.0, every=every, newton=True, dt=dt)
+    sp = oracle.snap_params(beta)
+    m = oracle.MD(cfg, sp)
+    m.init()
+    if steps:
+        m.run(steps)
+    return m
+
+
+def gpu_md(tmp_path, coeff, lattice, n, steps, every, gaussian, dt, seed, precision="fp64",
+           neigh="CSR", T=300.0):
+    deck = decks.snap_deck(n=n[0], ny=n[1], nz=n[2], steps=steps, every=every, lattice=lattice,
+                           coeff=coeff, param=W_PARAM, gaussian=gaussian, dt=dt, seed=seed,
+                           temperature=T)
+    path = decks.write(str(tmp_path / f"in.snap.{precision}"), deck)
+    return emd.App(["-il", path, "--neigh-type", neigh, "--precision", precision]).init()
+
+
+def compare(app, m, tol):
+    """max |gpu - oracle| over max(|oracle|, floor); the force floor of
+    1 eV/A keeps perfect-lattice forces (pure rounding noise, ~1e-13) from
+    being divided by themselves"""
+    g, o = app.arrays(), m.arrays()
+    n = app.nlocal
+    assert np.array_equal(g["id"], o["id"][:n])
+    floor = {"x": 0.0, "v": 0.0, "f": 1.0}
+    errs = {k: float(np.max(np.abs(g[k] - o[k][:n])) / max(np.max(np.abs(o[k][:n])), floor[k]))
+            for k in ("x", "v", "f")}
+    print("relative errors", errs)
+    for k, e in errs.items():
+        assert e <= tol, f"{k}: {e} > {tol}"
+
+
+def bcc_coeff(tmp_path, oracle):
+    beta = decks.snap_beta(seed=2024, sigma=0.05)
+    path = str(tmp_path / "W_bcc.snapcoeff")
+    decks.write(path, decks.snap_coeff_text(beta))
+    return list(beta), path
+
+
+def test_snap_reference_deck_init(tmp_path, oracle):
+    """the reference's in.snap.W: SC W 4x8x8, newton on, rebuild every step"""
+    print("case: SNAP SC 256 atoms init", flush=True)
+    beta = read_snapcoeff(W_COEFF)
+    app = gpu_md(tmp_path, W_COEFF, "sc", (4, 8, 8), 0, 1, False, 0.001, 4928459)
+    m = oracle_md(oracle, beta, "sc", (4, 8, 8), 0, 1, False, 0.001, 4928459)
+    compare(app, m, 1e-11)
+
+
+def test_snap_reference_deck_100_steps(tmp_path, oracle):
+    print("case: SNAP SC 256 atoms, 100 steps", flush=True)
+    beta = read_snapcoeff(W_COEFF)
+    app = gpu_md(tmp_path, W_COEFF, "sc", (4, 8, 8), 100, 1, False, 0.001, 4928459)
+    app.run(100)
+    m = oracle_md(oracle, beta, "sc", (4, 8, 8), 100, 1, False, 0.001, 4928459)
+    compare(app, m, 1e-10)
+
+
+def test_snap_bcc_2000_init_and_energy(tmp_path, oracle):
+    print("case: SNAP BCC 2000 atoms init + energy", flush=True)
+    beta, path = bcc_coeff(tmp_path, oracle)
+    app = gpu_md(tmp_path, path, "bcc", (10, 10, 10), 0, 10, True, 0.0005, 4928459)
+    m = oracle_md(oracle, beta, "bcc", (10, 10, 10), 0, 10, True, 0.0005, 4928459)
+    compare(app, m, 1e-11)
+    tg, to = app.thermo(), m.thermo()
+    print("thermo gpu", tg, "oracle", to)
+    assert abs(tg[1] - to[1]) <= 1e-12 * abs(to[1])
+
+
+def test_snap_bcc_2000_100_steps(tmp_path, oracle):
+    print("case: SNAP BCC 2000 atoms, 100 steps", flush=True)
+    beta, path = bcc_coeff(tmp_path, oracle)
+    app = gpu_md(tmp_path, path, "bcc", (10, 10, 10), 100, 10, True, 0.0005, 4928459)
+    app.run(100)
+    m = oracle_md(oracle, beta, "bcc", (10, 10, 10), 100, 10, True, 0.0005, 4928459)
+    compare(app, m, 1e-10)
+
+
+def test_snap_2d_list(tmp_path, oracle):
+    print("case: SNAP SC 256 atoms, 2D list, 20 steps", flush=True)
+    beta = read_snapcoeff(W_COEFF)
+    app = gpu_md(tmp_path, W_COEFF, "sc", (4, 8, 8), 20, 1, False, 0.001, 4928459, neigh="2D")
+    app.run(20)
+    m = oracle_md(oracle, beta, "sc", (4, 8, 8), 20, 1, False, 0.001, 4928459)
+    compare(app, m, 1e-10)
+
+
+def test_snap_kernel_level_disordered(oracle):
+    """thermally displaced BCC W (0.1 A): GPU FP64 force and energy against
+    the oracle, and the FP32 path's force RMS error against FP64"""
+    from gpu_pipeline import Pipeline, lattice
+    from test_oracle import _energy_force
+    print("case: SNAP BCC 2000 disordered, kernel level", flush=True)
+    beta = list(decks.snap_beta(seed=2024, sigma=0.05))
+    a = 3.1803
+    x = lattice("bcc", 10, a) + np.random.default_rng(3).normal(0, 0.1, (2000, 3))
+    box = (10 * a,) * 3
+    pl = Pipeline()
+    pl.load(x, box, mass=183.84)
+    pl.build(4.73442 + 1.0, newton=1)
+    pl.snap_setup(beta)
+    f64 = pl.snap_force(emd.EMD_FP64)
+    e64 = pl.snap_energy()
+    f32 = pl.snap_force(emd.EMD_MIXED)
+    pl.close()
+    E, F = _energy_force(oracle, x % (10 * a), 10 * a, "snap", snap=oracle.snap_params(beta),
+                         cut=5.73442)
+    print("force rel err", rel_err(f64, F), "energy", e64, E)
+    assert rel_err(f64, F) <= 1e-11
+    assert abs(e64 - E) <= 1e-11 * abs(E)
+    df = f32 - f64
+    rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+    print("mixed SNAP force RMS error (eV/A)", rms, "force RMS", np.sqrt(np.mean(np.sum(f64 ** 2, 1))))
+    assert rms <= 1e-4
