@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--snap-steps", type=int, default=100)
+    ap.add_argument("--no-snap", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device (profiler calibration)")
     return ap.parse_args()
 
 
@@ -79,6 +83,84 @@ def force_error(emd, decks, tmpdir, n):
         res[prec] = app.arrays()
         app.close()
     df = res["mixed"]["f"] - res["fp64"]["f"]
+    return float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+
+
+def snap_files(emd, decks, tmpdir):
+    """W-like SNAP potential: beta_1..55 ~ N(0, 0.05^2) (seed 2024), beta_0 set
+    so that the perfect BCC lattice (a = 3.1803 A) has E/atom = -8.9 eV"""
+    beta = decks.snap_beta(seed=2024, sigma=0.05)
+    param = decks.write(os.path.join(tmpdir, "W.snapparam"), decks.SNAP_PARAM)
+    coeff = decks.write(os.path.join(tmpdir, "W.snapcoeff"), decks.snap_coeff_text(beta))
+    deck = decks.write(os.path.join(tmpdir, "in.snap.fit"),
+                       decks.snap_deck(n=10, steps=0, coeff=coeff, param=param))
+    app = emd.App(["-il", deck, "--neigh-type", "CSR"]).init()
+    pe0 = app.thermo()[1]
+    app.close()
+    beta[0] = -8.9 - pe0
+    decks.write(coeff, decks.snap_coeff_text(beta))
+    return coeff, param, beta
+
+
+def snap_run(emd, decks, tmpdir, coeff, param, precision, warmup, steps):
+    deck = decks.write(os.path.join(tmpdir, f"in.snap.{precision}"),
+                       decks.snap_deck(n=10, steps=steps, coeff=coeff, param=param))
+    app = emd.App(["-il", deck, "--neigh-type", "CSR", "--precision", precision]).init()
+    app.set_phase_sync(False)
+    L = emd.hip_lib()
+    ctx = app.ctx
+    if warmup:
+        app.run(warmup)
+    L.emd_synchronize(ctx)
+    L.emd_profile_enable(ctx, 1)
+    L.emd_profile_reset(ctx)
+    t0 = time.perf_counter()
+    app.run(steps)
+    L.emd_synchronize(ctx)
+    wall = time.perf_counter() - t0
+    buf = C.create_string_buffer(1 << 16)
+    L.emd_profile_report(ctx, buf, len(buf))
+    prof = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.split()
+        prof[name] = (int(cnt), float(ms))
+    natoms = app.natoms
+    e_per_atom = app.thermo()[2]
+    app.close()
+    return natoms, wall, prof, e_per_atom
+
+
+def snap_force_error(emd, beta):
+    """FP32 U/Y/dU force RMS error against FP64 on a thermally displaced BCC
+    configuration (0.1 A Gaussian displacements, 2000 atoms)"""
+    import numpy as np
+    from examinimd_amd.pipeline import Pipeline, lattice
+    a = 3.1803
+    x = lattice("bcc", 10, a) + np.random.default_rng(3).normal(0, 0.1, (2000, 3))
+    pl = Pipeline()
+    pl.load(x, (10 * a,) * 3, mass=183.84)
+    pl.build(4.73442 + 1.0, newton=1)
+    pl.snap_setup(list(beta))
+    f64 = pl.snap_force(emd.EMD_FP64)
+    f32 = pl.snap_force(emd.EMD_MIXED)
+    pl.close()
+    df = f32 - f64
+    return float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+
+
+def lj_force_error_disordered(emd):
+    """FP32 LJ force RMS error against FP64 on a displaced FCC lattice"""
+    import numpy as np
+    from examinimd_amd.pipeline import Pipeline, lattice
+    a = (4 / 0.8442) ** (1 / 3)
+    x = lattice("fcc", 20, a) + np.random.default_rng(5).normal(0, 0.05, (32000, 3))
+    pl = Pipeline()
+    pl.load(x, (20 * a,) * 3)
+    pl.build(2.8)
+    f64 = pl.lj(emd.EMD_FP64)
+    f32 = pl.lj(emd.EMD_MIXED)
+    pl.close()
+    df = f32 - f64
     return float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
 
 
@@ -133,6 +215,21 @@ def main():
     avg_ms = mixed["kernel_ms"] / max(mixed["kernel_launches"], 1)
     achieved = bytes_per_atom * natoms / (avg_ms * 1e-3) / 1e9
     ferr = force_error(emd, decks, tmpdir, 20)
+    ferr_dis = lj_force_error_disordered(emd)
+
+    # SNAP 2000 atoms (BCC W), FP64 and mixed
+    snap = {}
+    if not a.no_snap:
+        coeff, param, beta = snap_files(emd, decks, tmpdir)
+        for prec in ("fp64", "mixed"):
+            sn, sw, sp, se = snap_run(emd, decks, tmpdir, coeff, param, prec, 10, a.snap_steps)
+            snap[prec] = {"value": sn * a.snap_steps / sw, "ms_per_step": 1e3 * sw / a.snap_steps,
+                          "kernels_ms_per_step": {k: v[1] / a.snap_steps for k, v in sp.items()},
+                          "etot_per_atom_end": se}
+        snap["mixed_over_fp64"] = snap["mixed"]["value"] / snap["fp64"]["value"]
+        snap["mixed_force_rms_error_eV_per_A"] = snap_force_error(emd, beta)
+        snap["workload"] = (f"SNAP W BCC 10^3 (2000 atoms), twojmax 8, rc 4.73442, skin 1.0, "
+                            f"rebuild every 10, dt 0.5 fs, {a.snap_steps} steps")
 
     line = {
         "metric": "atom-steps/s on 1 MI355X, LJ 4M & SNAP 2k atoms: mixed vs FP64 at fixed error",
@@ -154,6 +251,8 @@ def main():
                  "dtype": "f64"},
         "mixed_over_fp64": value / value_fp64,
         "mixed_force_rms_error_vs_fp64": ferr,
+        "mixed_force_rms_error_vs_fp64_displaced_lattice": ferr_dis,
+        "snap2k": snap,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": mixed["kernel"], "avg_ms": avg_ms,
@@ -163,6 +262,18 @@ def main():
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.cpu_steps, a.cpu_threads)
     print(json.dumps(line), flush=True)
+    if a.calib_copy_mib:
+        # a device-to-device copy of known size, for scaling profiler byte counters
+        ctx = emd.Context()
+        L = emd.hip_lib()
+        nbytes = a.calib_copy_mib << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        ctx.check(L.emd_malloc(ctx.h, C.byref(src), nbytes), "malloc")
+        ctx.check(L.emd_malloc(ctx.h, C.byref(dst), nbytes), "malloc")
+        ctx.check(L.emd_memcpy_d2d(ctx.h, dst, src, nbytes), "copy")
+        ctx.sync()
+        L.emd_free(ctx.h, src)
+        L.emd_free(ctx.h, dst)
 
 
 if __name__ == "__main__":

@@ -1,4 +1,4 @@
-"""Kernel-level driver for GPU tests: one force evaluation of an arbitrary
+"""Kernel-level driver: one force evaluation of an arbitrary
 configuration through the C ABI (wrap, bin + sort, halo, bin, neighbour
 list, force, reverse fold), results returned in input order."""
 import ctypes as C

@@ -157,7 +157,7 @@ def test_lj_mixed_force_error(tmp_path, oracle):
 def test_lj_kernel_level_disordered(oracle):
     """displaced FCC (0.05 sigma): FP64 force against the oracle through the
     kernel-level C ABI; FP32 pair arithmetic error against FP64"""
-    from gpu_pipeline import Pipeline, lattice
+    from examinimd_amd.pipeline import Pipeline, lattice
     from test_oracle import _energy_force
     print("case: LJ 32k disordered, kernel level", flush=True)
     a = (4 / 0.8442) ** (1 / 3)

@@ -111,7 +111,7 @@ def test_snap_2d_list(tmp_path, oracle):
 def test_snap_kernel_level_disordered(oracle):
     """thermally displaced BCC W (0.1 A): GPU FP64 force and energy against
     the oracle, and the FP32 path's force RMS error against FP64"""
-    from gpu_pipeline import Pipeline, lattice
+    from examinimd_amd.pipeline import Pipeline, lattice
     from test_oracle import _energy_force
     print("case: SNAP BCC 2000 disordered, kernel level", flush=True)
     beta = list(decks.snap_beta(seed=2024, sigma=0.05))
