@@ -216,6 +216,14 @@ def main():
     achieved = bytes_per_atom * natoms / (avg_ms * 1e-3) / 1e9
     ferr = force_error(emd, decks, tmpdir, 20)
     ferr_dis = lj_force_error_disordered(emd)
+    # HBM bytes per launch of the same kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes (scaled by a 256 MiB copy calibration)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "lj4m_traffic.json")
+    if n == 100 and os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t.get("kernel") == mixed["kernel"] and t.get("atoms") == natoms:
+            traffic = t["fetch_bytes"] + t["write_bytes"]
 
     # SNAP 2000 atoms (BCC W), FP64 and mixed
     snap = {}
@@ -254,7 +262,8 @@ def main():
         "mixed_force_rms_error_vs_fp64_displaced_lattice": ferr_dis,
         "snap2k": snap,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": bytes_per_atom * natoms,
                      "kernel": mixed["kernel"], "avg_ms": avg_ms,
                      "algorithmic_bytes_per_atom": bytes_per_atom},
         "kernels_ms_per_step": {k: v[1] / a.steps for k, v in mixed["prof"].items()},
