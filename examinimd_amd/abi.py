@@ -98,6 +98,10 @@ HIP_PROTOS = {
     "emd_snap_set_coeff": (C.c_int, [vp, vp, dptr]),
     "emd_snap_force": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, C.c_int]),
     "emd_snap_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, P(C.c_double), dptr]),
+    "emd_snap_adjoint_beta": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdHalo), vp, dptr,
+                                        C.c_double, C.c_int, dptr]),
+    "emd_lj_adjoint": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), C.c_double, C.c_double,
+                                 C.c_double, dptr, C.c_double, C.c_int, dptr]),
 }
 
 APP_PROTOS = {

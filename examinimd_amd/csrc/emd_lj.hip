@@ -361,6 +361,93 @@ extern "C" emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd
   return finish_dd(ctx, part, nb, pe);
 }
 
+// ------------------------------------------------ adjoint d/d(eps, sigma)
+// For a cotangent g = dL/dF over the local atoms and gE = dL/dE:
+//   dL/dtheta = sum_i sum_{j in N(i)} g_i . r_ij dfpair/dtheta + gE dE/dtheta
+// with fpair = r^-2 r^-6 (48 eps s^12 r^-6 - 24 eps s^6) (full list: atom i's
+// force is its own row) and the shifted pair energy counted half per row.
+// Pair arithmetic in T, accumulation in FP64.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_lj_adjoint(const double *__restrict__ x, const int *__restrict__ row,
+             const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
+             double eps, double sigma, double cutsq, const double *__restrict__ g,
+             double *__restrict__ out) {
+  __shared__ double s_red[4][4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
+    const double g0 = g[3 * (size_t)i], g1 = g[3 * (size_t)i + 1], g2 = g[3 * (size_t)i + 2];
+    const T s2 = (T)(sigma * sigma), s6 = s2 * s2 * s2, s12 = s6 * s6;
+    const T s5 = s6 / (T)sigma, s11 = s12 / (T)sigma;
+    const T e = (T)eps;
+    const T rc2inv = (T)(1.0 / cutsq), rc6 = rc2inv * rc2inv * rc2inv;
+    const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+    const int cnt = num[i];
+    for (int jj = 0; jj < cnt; jj++) {
+      const int j = entries[start + jj];
+      const double dx = x_i - x[3 * (size_t)j];
+      const double dy = y_i - x[3 * (size_t)j + 1];
+      const double dz = z_i - x[3 * (size_t)j + 2];
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      if (rsq < cutsq) {
+        const T r2inv = (T)1 / (T)rsq;
+        const T r6inv = r2inv * r2inv * r2inv;
+        const T dfde = r6inv * ((T)48 * s12 * r6inv - (T)24 * s6) * r2inv;
+        const T dfds = r6inv * ((T)576 * e * s11 * r6inv - (T)144 * e * s5) * r2inv;
+        const double gdx = g0 * dx + g1 * dy + g2 * dz;
+        a[0] += gdx * (double)dfde;
+        a[1] += gdx * (double)dfds;
+        a[2] += (double)((T)2 * ((s12 * r6inv * r6inv - s6 * r6inv) - (s12 * rc6 * rc6 - s6 * rc6)));
+        a[3] += (double)((T)2 * e * (((T)12 * s11 * r6inv * r6inv - (T)6 * s5 * r6inv) -
+                                     ((T)12 * s11 * rc6 * rc6 - (T)6 * s5 * rc6)));
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int o = EMD_WAVE / 2; o > 0; o >>= 1) a[k] += __shfl_xor(a[k], o, EMD_WAVE);
+  if (__lane_id() == 0)
+    for (int k = 0; k < 4; k++) s_red[threadIdx.x / EMD_WAVE][k] = a[k];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const double v = (s_red[0][threadIdx.x] + s_red[1][threadIdx.x]) +
+                     (s_red[2][threadIdx.x] + s_red[3][threadIdx.x]);
+    atomicAdd(&out[threadIdx.x], v);
+  }
+}
+
+__global__ void k_zero4(double *p) {
+  if (threadIdx.x < 4) p[threadIdx.x] = 0.0;
+}
+
+extern "C" emd_status emd_lj_adjoint(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                     double eps, double sigma, double cut, const double *dLdF,
+                                     double dLdE, int precision, double *out) {
+  if (!ctx || !s || !nl || !dLdF || !out) return EMD_ERR_ARG;
+  EMD_ARG(ctx, eps > 0 && sigma > 0 && cut > 0, "eps, sigma and cut must be positive");
+  out[0] = out[1] = 0.0;
+  const int n = s->n_local;
+  if (n <= 0) return EMD_OK;
+  double *acc = ctx->d_dsmall;
+  EMD_LAUNCH(ctx, "lj_adjoint", k_zero4, 1, 64, 0, acc);
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  if (precision == EMD_MIXED)
+    EMD_LAUNCH(ctx, "lj_adjoint", k_lj_adjoint<float>, emd_blocks(n, 256), 256, 0, s->x, row,
+               nl->num, nl->entries, nl->maxneighs, n, eps, sigma, cut * cut, dLdF, acc);
+  else
+    EMD_LAUNCH(ctx, "lj_adjoint", k_lj_adjoint<double>, emd_blocks(n, 256), 256, 0, s->x, row,
+               nl->num, nl->entries, nl->maxneighs, n, eps, sigma, cut * cut, dLdF, acc);
+  EMD_HIP(ctx, hipMemcpyAsync(ctx->h_dsmall, acc, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  out[0] = ctx->h_dsmall[0] + dLdE * ctx->h_dsmall[2];
+  out[1] = ctx->h_dsmall[1] + dLdE * ctx->h_dsmall[3];
+  return EMD_OK;
+}
+
 // ---------------------------------------------------------------- NVE
 // IntegratorNVE InitialIntegrateFunctor, src/integrator_nve.cpp:65-74
 template <bool MIXED>

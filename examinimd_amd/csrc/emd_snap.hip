@@ -66,6 +66,12 @@ struct etab_entry {
   double beta;
 };
 
+struct stab_entry {
+  int a, b, cgoff, tid;
+};
+
+constexpr int SNAP_MAXT = 128;               // (a,b,J) triples for twojmax 8: 125
+
 __device__ __forceinline__ int u_idx(const snap_dev_tables &t, int j, int ma, int mb) {
   return t.ustart[j] + ma * (j + 1) + mb;
 }
@@ -546,6 +552,12 @@ struct emd_snap {
   size_t cap_u = 0, cap_y = 0, cap_ui = 0, cap_yi = 0;
   double *eatom = nullptr;
   int cap_e = 0;
+  // adjoint: every (a,b,J) triple grouped by J, with its id
+  std::map<std::tuple<int, int, int>, int> tid;
+  std::vector<int> sjstart;
+  stab_entry *stab = nullptr;
+  double *w_r = nullptr, *w_i = nullptr, *sbuf = nullptr;
+  size_t cap_w = 0, cap_wi = 0, cap_s = 0;
 };
 
 static double host_factorial(int n) {
@@ -673,6 +685,28 @@ extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, em
           for (int k2 = 0; k2 <= j2; k2++) sn->cg_h.push_back(host_cg(j1, j2, j, k1, k2));
       }
   sn->ncoeff = (int)sn->idxj.size() / 3;
+  {
+    // all (a,b,J) triples grouped by J for the adjoint contraction
+    std::vector<stab_entry> st;
+    for (int J = 0; J <= twojmax; J++) {
+      sn->sjstart.push_back((int)st.size());
+      for (auto &kv : sn->cgoff) {
+        int a, b, JJ;
+        std::tie(a, b, JJ) = kv.first;
+        if (JJ != J) continue;
+        const int id = (int)sn->tid.size();
+        sn->tid[kv.first] = id;
+        st.push_back(stab_entry{a, b, kv.second, id});
+      }
+    }
+    sn->sjstart.push_back((int)st.size());
+    if (st.size() > (size_t)SNAP_MAXT) {
+      delete sn;
+      return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: too many (a,b,J) triples");
+    }
+    EMD_HIP(ctx, hipMalloc(&sn->stab, st.size() * sizeof(stab_entry)));
+    EMD_HIP(ctx, hipMemcpy(sn->stab, st.data(), st.size() * sizeof(stab_entry), hipMemcpyHostToDevice));
+  }
   if (p->ncoeff != sn->ncoeff) {
     delete sn;
     return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: ncoeff %d but twojmax %d has %d",
@@ -697,7 +731,8 @@ extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, em
 extern "C" emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn) {
   if (!sn) return EMD_OK;
   if (ctx) hipStreamSynchronize(ctx->stream);
-  void *ptrs[] = {sn->cg, sn->ytab, sn->etab, sn->nin, sn->nj, sn->nr,
+  void *ptrs[] = {sn->cg, sn->ytab, sn->etab, sn->nin, sn->nj, sn->nr, sn->stab, sn->w_r,
+                  sn->w_i, sn->sbuf,
                   sn->u_r, sn->u_i, sn->y_r, sn->y_i, sn->eatom};
   for (void *q : ptrs)
     if (q) hipFree(q);
@@ -801,5 +836,288 @@ extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const e
     sum = tt;
   }
   *energy = sum + c;
+  return EMD_OK;
+}
+
+// ====================================================== adjoint (dL/dbeta)
+// F is affine in beta: F_ij = sum_J sum' Re(conj(dU_J) Y_J(beta)) + repulsion.
+// For a cotangent g = dL/dF, sum_i g_i . dF_i/dbeta_k
+//   = sum_i sum_J sum' Re(conj(W^i_J) dY^i_J/dbeta_k),
+//   W^i_J = sum_{j in N(i)} w(ma,mb) sum_d (g_i - g_j)_d dU_J/dr_d
+// (ghost j carries the cotangent of its owner), and dY_J/dbeta_k is the
+// three-role combination of Z_{abJ}; so per triple (a,b,J) only
+// S_{abJ} = sum_i sum' Re(conj(W^i_J) Z^i_{abJ}) is needed.
+
+// one U/dU layer step for the lane's entry (same recursion as k_snap_fij)
+template <class T>
+__device__ __forceinline__ void du_step(const snap_dev_tables &t, const T *P, int j, int ma, int mb,
+                                        T a_r, T a_i, T b_r, T b_i, const T *da_r, const T *da_i,
+                                        const T *db_r, const T *db_i, T &ur, T &ui, T *dr, T *di) {
+  ur = ui = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) dr[k] = di[k] = 0;
+  if (ma < j) {
+    const T rp = (T)t.rootpq[(j - ma) * (SNAP_JD + 1) + (j - mb)];
+    const int s = ma * j + mb;
+    const T pr = P[s], pi = P[64 + s];
+    ur += rp * (a_r * pr + a_i * pi);
+    ui += rp * (a_r * pi - a_i * pr);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const T dpr = P[(2 + 2 * k) * 64 + s], dpi = P[(3 + 2 * k) * 64 + s];
+      dr[k] += rp * (da_r[k] * pr + da_i[k] * pi + a_r * dpr + a_i * dpi);
+      di[k] += rp * (da_r[k] * pi - da_i[k] * pr + a_r * dpi - a_i * dpr);
+    }
+  }
+  if (ma > 0) {
+    const T rp = (T)t.rootpq[ma * (SNAP_JD + 1) + (j - mb)];
+    const int s = (ma - 1) * j + mb;
+    const T pr = P[s], pi = P[64 + s];
+    ur += -rp * (b_r * pr + b_i * pi);
+    ui += -rp * (b_r * pi - b_i * pr);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const T dpr = P[(2 + 2 * k) * 64 + s], dpi = P[(3 + 2 * k) * 64 + s];
+      dr[k] += -rp * (db_r[k] * pr + db_i[k] * pi + b_r * dpr + b_i * dpi);
+      di[k] += -rp * (db_r[k] * pi - db_i[k] * pr + b_r * dpi - b_i * dpr);
+    }
+  }
+}
+
+// W^i for every atom: one wavefront per atom, neighbours in list order,
+// FP64 accumulators (T is the arithmetic of the recursion)
+template <class T>
+__global__ void __launch_bounds__(64)
+k_snap_w(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
+         const double *__restrict__ nr, int maxn, int n, const int *__restrict__ root,
+         const double *__restrict__ g, double *__restrict__ w_r, double *__restrict__ w_i) {
+  __shared__ T s_u[2][8][64];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const int l = threadIdx.x;
+  double acc_r[SNAP_JD], acc_i[SNAP_JD];
+#pragma unroll
+  for (int j = 0; j < SNAP_JD; j++) acc_r[j] = acc_i[j] = 0.0;
+  const double gi0 = g[3 * (size_t)i], gi1 = g[3 * (size_t)i + 1], gi2 = g[3 * (size_t)i + 2];
+  const int cnt = nin[i];
+  for (int jj = 0; jj < cnt; jj++) {
+    const size_t o = (size_t)i * maxn + jj;
+    const int jn = nj[o];
+    const int own_j = jn < n ? jn : root[jn - n];
+    const T gd[3] = {(T)(gi0 - g[3 * (size_t)own_j]), (T)(gi1 - g[3 * (size_t)own_j + 1]),
+                     (T)(gi2 - g[3 * (size_t)own_j + 2])};
+    const T x = (T)nr[3 * o], y = (T)nr[3 * o + 1], z = (T)nr[3 * o + 2];
+    const T rsq = x * x + y * y + z * z;
+    const T r = sqrt(rsq);
+    const T rscale0 = (T)t.rfac0 * (T)SNAP_PI / ((T)t.rcut - (T)t.rmin0);
+    const T theta0 = (r - (T)t.rmin0) * rscale0;
+    const T z0 = r * cos(theta0) / sin(theta0);
+    const T dz0dr = z0 / r - (r * rscale0) * (rsq + z0 * z0) / rsq;
+    const T rinv = (T)1 / r;
+    const T u3[3] = {x * rinv, y * rinv, z * rinv};
+    const T r0inv = (T)1 / sqrt(r * r + z0 * z0);
+    const T a_r = z0 * r0inv, a_i = -z * r0inv, b_r = y * r0inv, b_i = -x * r0inv;
+    const T dr0invdr = -(r0inv * r0inv * r0inv) * (r + z0 * dz0dr);
+    T da_r[3], da_i[3], db_r[3], db_i[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const T dr0 = dr0invdr * u3[k], dz = dz0dr * u3[k];
+      da_r[k] = dz * r0inv + z0 * dr0;
+      da_i[k] = -z * dr0;
+      db_r[k] = y * dr0;
+      db_i[k] = -x * dr0;
+    }
+    da_i[2] += -r0inv;
+    db_i[0] += -r0inv;
+    db_r[1] += r0inv;
+    T sfac, dsfac;
+    sfac_dsfac(t, r, sfac, dsfac);
+    sfac *= (T)t.wj;
+    dsfac *= (T)t.wj;
+    const T gu = gd[0] * u3[0] + gd[1] * u3[1] + gd[2] * u3[2];
+    if (l == 0) {
+      s_u[0][0][0] = 1;
+      s_u[0][1][0] = 0;
+      for (int k = 0; k < 3; k++) { s_u[0][2 + 2 * k][0] = 0; s_u[0][3 + 2 * k][0] = 0; }
+      acc_r[0] += 0.5 * (double)(dsfac * gu);
+    }
+    __syncthreads();
+    for (int j = 1; j <= t.twojmax; j++) {
+      const int p = (j - 1) & 1, c = j & 1;
+      int ma, mb;
+      const bool own = l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb);
+      T ur = 0, ui = 0, dr[3] = {0, 0, 0}, di[3] = {0, 0, 0};
+      if (own)
+        du_step<T>(t, &s_u[p][0][0], j, ma, mb, a_r, a_i, b_r, b_i, da_r, da_i, db_r, db_i, ur,
+                   ui, dr, di);
+      __syncthreads();
+      if (own && j < t.twojmax) {
+        T *C = &s_u[c][0][0];
+        const int s = ma * (j + 1) + mb;
+        C[s] = ur; C[64 + s] = ui;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + s] = dr[k]; C[(3 + 2 * k) * 64 + s] = di[k]; }
+        if (2 * mb != j) {
+          const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
+          const int m = (j - ma) * (j + 1) + (j - mb);
+          C[m] = sg * ur; C[64 + m] = -sg * ui;
+#pragma unroll
+          for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + m] = sg * dr[k]; C[(3 + 2 * k) * 64 + m] = -sg * di[k]; }
+        }
+      }
+      if (own) {
+        double wgt = 1.0;
+        if (2 * mb == j) wgt = (ma < mb) ? 1.0 : ((ma == mb) ? 0.5 : 0.0);
+        const T wr = dsfac * ur * gu + sfac * (gd[0] * dr[0] + gd[1] * dr[1] + gd[2] * dr[2]);
+        const T wi = dsfac * ui * gu + sfac * (gd[0] * di[0] + gd[1] * di[1] + gd[2] * di[2]);
+        acc_r[j] += wgt * (double)wr;
+        acc_i[j] += wgt * (double)wi;
+      }
+      __syncthreads();
+    }
+  }
+  for (int j = 0; j <= t.twojmax; j++) {
+    int ma, mb;
+    if (l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb)) {
+      const int e = t.ystart[j] + mb * (j + 1) + ma;
+      w_r[(size_t)i * t.ny + e] = acc_r[j];
+      w_i[(size_t)i * t.ny + e] = acc_i[j];
+    }
+  }
+}
+
+// S_{abJ} += sum' Re(conj(W^i_J) Z^i_{abJ}) over atoms (FP64 accumulation)
+template <class T>
+__global__ void __launch_bounds__(192)
+k_snap_s(snap_dev_tables t, const double *__restrict__ cg, const stab_entry *__restrict__ stab,
+         const int *__restrict__ sjstart, int ntrip, int n, const T *__restrict__ utot_r,
+         const T *__restrict__ utot_i, const double *__restrict__ w_r,
+         const double *__restrict__ w_i, double *__restrict__ S) {
+  __shared__ T su_r[SNAP_NU], su_i[SNAP_NU];
+  __shared__ double sS[SNAP_MAXT];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  for (int k = threadIdx.x; k < t.nu; k += blockDim.x) {
+    su_r[k] = utot_r[(size_t)i * t.nu + k];
+    su_i[k] = utot_i[(size_t)i * t.nu + k];
+  }
+  for (int k = threadIdx.x; k < ntrip; k += blockDim.x) sS[k] = 0.0;
+  __syncthreads();
+  const int e = threadIdx.x;
+  if (e < t.ny) {
+    int J, ma, mb;
+    y_entry(t, e, J, ma, mb);
+    const double wr = w_r[(size_t)i * t.ny + e], wi = w_i[(size_t)i * t.ny + e];
+    if (wr != 0.0 || wi != 0.0) {
+      for (int q = sjstart[J]; q < sjstart[J + 1]; q++) {
+        const stab_entry te = stab[q];
+        const cplx<T> z = z_entry<T>(t, cg, su_r, su_i, te.a, te.b, J, te.cgoff, ma, mb);
+        atomicAdd(&sS[te.tid], wr * (double)z.r + wi * (double)z.i);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < ntrip; k += blockDim.x) atomicAdd(&S[k], sS[k]);
+}
+
+// sum over atoms of every bispectrum component B_k (dE/dbeta_k), FP64
+__global__ void __launch_bounds__(256)
+k_snap_bsum(snap_dev_tables t, const double *__restrict__ cg, const etab_entry *__restrict__ etab,
+            int ne, int bzero, int n, const double *__restrict__ utot_r,
+            const double *__restrict__ utot_i, double *__restrict__ bsum) {
+  __shared__ double su_r[SNAP_NU], su_i[SNAP_NU];
+  __shared__ double sB[SNAP_MAXT];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  for (int k = threadIdx.x; k < t.nu; k += blockDim.x) {
+    su_r[k] = utot_r[(size_t)i * t.nu + k];
+    su_i[k] = utot_i[(size_t)i * t.nu + k];
+  }
+  for (int k = threadIdx.x; k < ne; k += blockDim.x) sB[k] = 0.0;
+  __syncthreads();
+  for (int k = 0; k < ne; k++) {
+    const etab_entry te = etab[k];
+    const int J = te.j;
+    const int nyj = t.ystart[J + 1] - t.ystart[J];
+    for (int q = threadIdx.x; q < nyj; q += blockDim.x) {
+      const int mb = q / (J + 1), ma = q - mb * (J + 1);
+      double wgt = 1.0;
+      if (2 * mb == J) wgt = (ma < mb) ? 1.0 : ((ma == mb) ? 0.5 : 0.0);
+      if (wgt == 0.0) continue;
+      const cplx<double> z = z_entry<double>(t, cg, su_r, su_i, te.j1, te.j2, J, te.cgoff, ma, mb);
+      const int u = t.ustart[J] + ma * (J + 1) + mb;
+      atomicAdd(&sB[k], 2.0 * wgt * (su_r[u] * z.r + su_i[u] * z.i));
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < ne; k += blockDim.x) {
+    double b = sB[k];
+    if (bzero) b -= t.wself * t.wself * t.wself * (etab[k].j + 1);
+    atomicAdd(&bsum[k], b);
+  }
+}
+
+__global__ void k_zero_d(double *p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0.0;
+}
+
+template <class T>
+static emd_status snap_adjoint_impl(emd_ctx *ctx, const emd_system *s, const emd_halo *h,
+                                    emd_snap *sn, const double *g, double *S) {
+  const int n = s->n_local;
+  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n,
+             (T *)sn->u_r, (T *)sn->u_i);
+  EMD_LAUNCH(ctx, "snap_adjoint_w", k_snap_w<T>, n, 64, 0, sn->t, sn->nin, sn->nj, sn->nr,
+             sn->cap_maxn, n, h ? h->root : nullptr, g, sn->w_r, sn->w_i);
+  const int ntrip = (int)sn->tid.size();
+  EMD_LAUNCH(ctx, "snap_adjoint_s", k_snap_s<T>, n, 192, 0, sn->t, sn->cg, sn->stab,
+             (const int *)(sn->sbuf + SNAP_MAXT + 64), ntrip, n, (const T *)sn->u_r,
+             (const T *)sn->u_i, sn->w_r, sn->w_i, S);
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                            const emd_halo *h, emd_snap *sn, const double *dLdF,
+                                            double dLdE, int precision, double *dLdbeta) {
+  if (!ctx || !s || !nl || !sn || !dLdF || !dLdbeta) return EMD_ERR_ARG;
+  const int n = s->n_local;
+  const int ntrip = (int)sn->tid.size();
+  for (int k = 0; k <= sn->ncoeff; k++) dLdbeta[k] = 0.0;
+  if (n <= 0) return EMD_OK;
+  EMD_ARG(ctx, s->n_ghost == 0 || (h && h->root && h->n_ghost == s->n_ghost),
+          "ghost atoms need the halo (owner of each ghost)");
+  EMD_TRY(snap_neigh(ctx, s, nl, sn));
+  EMD_TRY(emd_ensure(ctx, (void **)&sn->w_r, &sn->cap_w, (size_t)n * sn->t.ny * sizeof(double)));
+  EMD_TRY(emd_ensure(ctx, (void **)&sn->w_i, &sn->cap_wi, (size_t)n * sn->t.ny * sizeof(double)));
+  // sbuf: S[SNAP_MAXT] | Bsum[64] | sjstart ints
+  EMD_TRY(emd_ensure(ctx, (void **)&sn->sbuf, &sn->cap_s, (SNAP_MAXT + 64 + 64) * sizeof(double)));
+  double *S = sn->sbuf, *B = sn->sbuf + SNAP_MAXT;
+  EMD_HIP(ctx, hipMemcpyAsync(sn->sbuf + SNAP_MAXT + 64, sn->sjstart.data(),
+                              sn->sjstart.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  EMD_LAUNCH(ctx, "snap_adjoint_s", k_zero_d, 1, 256, 0, sn->sbuf, SNAP_MAXT + 64);
+  if (precision == EMD_MIXED) EMD_TRY(snap_adjoint_impl<float>(ctx, s, h, sn, dLdF, S));
+  else EMD_TRY(snap_adjoint_impl<double>(ctx, s, h, sn, dLdF, S));
+  // dE/dbeta_k = sum_i B_k^i in FP64
+  if (dLdE != 0.0) {
+    EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<double>, n, 64, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn,
+               n, (double *)sn->u_r, (double *)sn->u_i);
+    EMD_LAUNCH(ctx, "snap_adjoint_bsum", k_snap_bsum, n, 256, 0, sn->t, sn->cg, sn->etab,
+               sn->ncoeff, sn->bzeroflag, n, (const double *)sn->u_r, (const double *)sn->u_i, B);
+  }
+  std::vector<double> hS(SNAP_MAXT + 64);
+  EMD_HIP(ctx, hipMemcpyAsync(hS.data(), sn->sbuf, hS.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dLdbeta[0] = dLdE * n;
+  for (int k = 0; k < sn->ncoeff; k++) {
+    const int j1 = sn->idxj[3 * k], j2 = sn->idxj[3 * k + 1], j = sn->idxj[3 * k + 2];
+    const double s1 = hS[sn->tid.at(std::make_tuple(std::max(j1, j2), std::min(j1, j2), j))];
+    const double s2 = hS[sn->tid.at(std::make_tuple(std::max(j, j2), std::min(j, j2), j1))];
+    const double s3 = hS[sn->tid.at(std::make_tuple(std::max(j1, j), std::min(j1, j), j2))];
+    dLdbeta[k + 1] = 2.0 * s1 + 2.0 * (j + 1) / (j1 + 1.0) * s2 + 2.0 * (j + 1) / (j2 + 1.0) * s3 +
+                     dLdE * hS[SNAP_MAXT + k];
+  }
+  (void)ntrip;
   return EMD_OK;
 }

@@ -98,6 +98,55 @@ class Pipeline:
                                         C.byref(e), None), "snap_energy")
         return e.value
 
+    def snap_set_coeff(self, beta):
+        self._beta = np.ascontiguousarray(beta, dtype=np.float64)
+        self.chk(self.L.emd_snap_set_coeff(self.ctx.h, self.snap, _d(self._beta)), "set_coeff")
+
+    def _cotangent(self, gF):
+        """upload dL/dF given in input order into the device atom order"""
+        n = self.s.n_local
+        ids = np.zeros(n, np.int32)
+        self.chk(self.L.emd_system_download(self.ctx.h, C.byref(self.s), n, None, None, None,
+                                            None, _i(ids), None), "ids")
+        g = np.ascontiguousarray(np.asarray(gF, dtype=np.float64)[ids - 1])
+        if getattr(self, "_gdev", None) is None or self._gcap < g.nbytes:
+            if getattr(self, "_gdev", None) is not None:
+                self.L.emd_free(self.ctx.h, self._gdev)
+            self._gdev = C.c_void_p()
+            self.chk(self.L.emd_malloc(self.ctx.h, C.byref(self._gdev), g.nbytes), "malloc")
+            self._gcap = g.nbytes
+        self.chk(self.L.emd_memcpy_h2d(self.ctx.h, self._gdev, g.ctypes.data_as(C.c_void_p),
+                                       g.nbytes), "h2d")
+        return C.cast(self._gdev, P(C.c_double))
+
+    def snap_adjoint(self, gF, gE, precision=EMD_FP64):
+        """sum_i gF_i . dF_i/dbeta + gE dE/dbeta (host array, ncoeff+1)"""
+        out = np.zeros(len(self._beta))
+        g = self._cotangent(gF)
+        self.chk(self.L.emd_snap_adjoint_beta(self.ctx.h, C.byref(self.s), C.byref(self.nl),
+                                              C.byref(self.h), self.snap, g, float(gE),
+                                              precision, _d(out)), "snap_adjoint")
+        return out
+
+    def lj_adjoint(self, gF, gE, eps=1.0, sigma=1.0, rc=2.5, precision=EMD_FP64):
+        """(dL/deps, dL/dsigma) for cotangents gF (input order), gE"""
+        out = np.zeros(2)
+        g = self._cotangent(gF)
+        self.chk(self.L.emd_lj_adjoint(self.ctx.h, C.byref(self.s), C.byref(self.nl), eps, sigma,
+                                       rc, g, float(gE), precision, _d(out)), "lj_adjoint")
+        return out
+
+    def lj_energy(self, eps=1.0, sigma=1.0, rc=2.5):
+        p = EmdLJParams()
+        p.ntypes = 1
+        p.lj1[0][0] = 48.0 * eps * sigma ** 12
+        p.lj2[0][0] = 24.0 * eps * sigma ** 6
+        p.cutsq[0][0] = rc * rc
+        e = C.c_double()
+        self.chk(self.L.emd_lj_energy(self.ctx.h, C.byref(self.s), C.byref(self.nl), C.byref(p), 0,
+                                      C.byref(e)), "lj_energy")
+        return e.value
+
     def forces(self, precision):
         n = self.s.n_local
         ids = np.zeros(n, np.int32)
@@ -122,6 +171,9 @@ class Pipeline:
         if self.snap:
             L.emd_snap_destroy(c, self.snap)
             self.snap = None
+        if getattr(self, "_gdev", None) is not None:
+            L.emd_free(c, self._gdev)
+            self._gdev = None
         L.emd_neigh_free(c, C.byref(self.nl))
         L.emd_bins_free(c, C.byref(self.b))
         L.emd_halo_free(c, C.byref(self.h))

@@ -212,6 +212,24 @@ emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, emd_
 emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
                            emd_snap *sn, double *energy, double *eatom);
 
+/* ------------------------------------------- force-matching gradients */
+/* No counterpart in the reference (it has no parameter gradients).  Given
+ * the cotangent dLdF (DEVICE, double[n_local][3], dL/dF of the local atoms
+ * after the ghost fold) and dLdE = dL/dE of the total energy, these return
+ *   sum_i dLdF_i . dF_i/dtheta + dLdE . dE/dtheta
+ * on the HOST.  precision EMD_MIXED: FP32 pair/recursion arithmetic with
+ * FP64 accumulation; EMD_FP64 throughout.  Positions and neighbour list are
+ * those of the last force evaluation. */
+/* SNAP: theta = beta_0..beta_ncoeff (dLdbeta has ncoeff+1 entries); the halo
+ * maps ghost neighbours to their owners (may be NULL without ghosts) */
+emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                 const emd_halo *h, emd_snap *sn, const double *dLdF,
+                                 double dLdE, int precision, double *dLdbeta);
+/* LJ (one type, full list): theta = (eps, sigma); out[2] */
+emd_status emd_lj_adjoint(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl, double eps,
+                          double sigma, double cut, const double *dLdF, double dLdE,
+                          int precision, double *out);
+
 #ifdef __cplusplus
 }
 #endif

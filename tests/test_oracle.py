@@ -287,6 +287,52 @@ def test_lj_force_is_minus_energy_gradient(oracle):
         assert abs(fd - F[i, d]) < 1e-6 * max(1.0, abs(F[i, d]))
 
 
+def test_lj_parameter_gradient_closed_form(oracle):
+    """d/d(eps, sigma) of a force-matching loss: the oracle's central
+    differences against the closed-form derivative of the shifted 12-6 pair
+    force and energy summed over all minimum-image pairs (what the
+    hand-written LJ adjoint kernel evaluates)"""
+    rng = np.random.default_rng(7)
+    a = (4 / 0.8442) ** (1 / 3)
+    basis = np.array([(0, 0, 0), (0.5, 0.5, 0), (0.5, 0, 0.5), (0, 0.5, 0.5)])
+    g = np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    box = 4 * a
+    x = (((g[:, None, :] + basis[None]) * a).reshape(-1, 3) + rng.normal(0, 0.05, (256, 3))) % box
+    n, rc = len(x), 2.5
+    Et, Ft = _energy_force(oracle, x, box, "lj")
+    th0 = np.array([1.07, 0.97])
+
+    def loss(th):
+        E, F = _energy_force(oracle, x, box, "lj", eps=th[0], sig=th[1], rc=rc)
+        return ((F - Ft) ** 2).sum() + 100 * ((E - Et) / n) ** 2, E, F
+
+    _, E0, F0 = loss(th0)
+    gF, gE = 2 * (F0 - Ft), 200 * (E0 - Et) / n ** 2
+    d = x[:, None, :] - x[None, :, :]
+    d -= box * np.round(d / box)
+    r2 = (d * d).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    m = r2 < rc * rc
+    eps, sig = th0
+    r2i = np.where(m, 1 / r2, 0.0)
+    r6i = r2i ** 3
+    c6 = rc ** -6
+    dfde = r6i * (48 * sig ** 12 * r6i - 24 * sig ** 6) * r2i
+    dfds = r6i * (576 * eps * sig ** 11 * r6i - 144 * eps * sig ** 5) * r2i
+    gdx = (gF[:, None, :] * d).sum(-1)
+    dEde = 2 * np.where(m, sig ** 12 * (r6i ** 2 - c6 ** 2) - sig ** 6 * (r6i - c6), 0).sum()
+    dEds = 2 * eps * np.where(m, 12 * sig ** 11 * (r6i ** 2 - c6 ** 2)
+                              - 6 * sig ** 5 * (r6i - c6), 0).sum()
+    exact = np.array([(gdx * dfde).sum() + gE * dEde, (gdx * dfds).sum() + gE * dEds])
+    h = 1e-6
+    for k in range(2):
+        tp, tm = th0.copy(), th0.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (loss(tp)[0] - loss(tm)[0]) / (2 * h)
+        assert abs(fd - exact[k]) <= 1e-7 * abs(exact).max(), (k, fd, exact[k])
+
+
 # ------------------------------------------------------------------- NVE
 def test_nve_reversibility_and_energy(oracle):
     m = oracle.MD(oracle.make_config(n=(8, 8, 8), every=10))
