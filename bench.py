@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--snap-steps", type=int, default=100)
     ap.add_argument("--no-snap", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--loop-steps", type=int, default=100)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (profiler calibration)")
     return ap.parse_args()
@@ -183,6 +186,45 @@ def cpu_baseline(n, steps, threads):
                       f"{natoms}-atom workload, {steps} steps incl. 1 rebuild, {dt:.1f} s"}
 
 
+def batch_leg(emd, steps, loop_steps):
+    """256 uneven periodic LJ replicas (N log-uniform in [256, 8192] rounded
+    to FCC n^3 cells, rho in [0.70, 0.95], T* in [0.8, 2.0]), mixed
+    precision: batched launches with graph-replayed rebuild periods and an
+    autotuned force tile, against the same ensemble run replica after
+    replica through the per-system C ABI"""
+    import numpy as np
+    from examinimd_amd import batch
+    from examinimd_amd.abi import EMD_MIXED
+    e = batch.ensemble(nrep=256, seed=4)
+    b = batch.Batch(e["natoms"], e["box"], precision=EMD_MIXED)
+    b.upload(e["x"], e["v"])
+    b.setup()
+    lanes, block, tune_ms = b.autotune(trial_steps=10)
+    b.run(20)                      # warm-up, captures the period graph
+    ke0, pe0 = b.thermo()
+    t0 = time.perf_counter()
+    b.run(steps)                   # returns after a stream synchronisation
+    wall = time.perf_counter() - t0
+    ke1, pe1 = b.thermo()
+    total = b.total
+    b.close()
+    sec, _, _ = batch.replica_loop(e["natoms"], e["box"], e["x"], e["v"], loop_steps,
+                                   precision=EMD_MIXED)
+    value = total * steps / wall
+    loop_value = total * loop_steps / sec
+    return {"value": value, "unit": "atom-steps/s", "ms_per_step": 1e3 * wall / steps,
+            "steps": steps, "replicas": 256, "atoms": total,
+            "atoms_min_max": [int(e["natoms"].min()), int(e["natoms"].max())],
+            "tile": {"lanes": lanes, "block": block, "tuned_ms_per_step": tune_ms},
+            "max_energy_change_per_atom": float(np.max(np.abs((ke1 + pe1) - (ke0 + pe0))
+                                                       / e["natoms"])),
+            "replica_loop": {"value": loop_value, "ms_per_step": 1e3 * sec / loop_steps,
+                             "steps": loop_steps},
+            "batched_over_loop": value / loop_value,
+            "workload": "256 uneven periodic LJ replicas, FP32 forces / FP64 state, rc 2.5, "
+                        "skin 0.3, rebuild every 20, dt 0.005"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -239,6 +281,8 @@ def main():
         snap["workload"] = (f"SNAP W BCC 10^3 (2000 atoms), twojmax 8, rc 4.73442, skin 1.0, "
                             f"rebuild every 10, dt 0.5 fs, {a.snap_steps} steps")
 
+    batched = None if a.no_batch else batch_leg(emd, a.batch_steps, a.loop_steps)
+
     line = {
         "metric": "atom-steps/s on 1 MI355X, LJ 4M & SNAP 2k atoms: mixed vs FP64 at fixed error",
         "value": value,
@@ -261,6 +305,7 @@ def main():
         "mixed_force_rms_error_vs_fp64": ferr,
         "mixed_force_rms_error_vs_fp64_displaced_lattice": ferr_dis,
         "snap2k": snap,
+        "batched": batched,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_atom * natoms,

@@ -52,6 +52,13 @@ class EmdSnapParams(C.Structure):
                 ("coeff", dptr)]
 
 
+class EmdBatchDesc(C.Structure):
+    _fields_ = [("nrep", C.c_int), ("natoms", iptr), ("box", dptr), ("mass", dptr),
+                ("eps", C.c_double), ("sigma", C.c_double), ("cut", C.c_double),
+                ("skin", C.c_double), ("dt", C.c_double), ("rebuild_every", C.c_int),
+                ("precision", C.c_int)]
+
+
 HIP_PROTOS = {
     "emd_create": (C.c_int, [C.c_int, P(vp)]),
     "emd_destroy": (C.c_int, [vp]),
@@ -102,6 +109,17 @@ HIP_PROTOS = {
                                         C.c_double, C.c_int, dptr]),
     "emd_lj_adjoint": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), C.c_double, C.c_double,
                                  C.c_double, dptr, C.c_double, C.c_int, dptr]),
+    "emd_batch_create": (C.c_int, [vp, P(EmdBatchDesc), P(vp)]),
+    "emd_batch_destroy": (C.c_int, [vp, vp]),
+    "emd_batch_upload": (C.c_int, [vp, vp, dptr, dptr]),
+    "emd_batch_download": (C.c_int, [vp, vp, dptr, dptr, dptr]),
+    "emd_batch_setup": (C.c_int, [vp, vp]),
+    "emd_batch_run": (C.c_int, [vp, vp, C.c_int, C.c_int]),
+    "emd_batch_thermo": (C.c_int, [vp, vp, dptr, dptr]),
+    "emd_batch_set_tile": (C.c_int, [vp, vp, C.c_int, C.c_int]),
+    "emd_batch_autotune": (C.c_int, [vp, vp, C.c_int, iptr, iptr, P(C.c_double)]),
+    "emd_batch_info": (C.c_int, [vp, vp, P(C.c_long), iptr, iptr]),
+    "emd_batch_neigh": (C.c_int, [vp, vp, iptr, P(C.c_uint)]),
 }
 
 APP_PROTOS = {
@@ -123,6 +141,8 @@ APP_PROTOS = {
     "emd_app_neigh": (P(EmdNeigh), [vp]),
     "emd_app_halo": (P(EmdHalo), [vp]),
     "emd_app_precision": (C.c_int, [vp]),
+    "emd_replica_loop_run": (C.c_int, [vp, P(EmdBatchDesc), dptr, dptr, C.c_int, P(C.c_double),
+                                       dptr, dptr]),
 }
 
 

@@ -263,5 +263,50 @@ struct emd_scan_array {
   }
 };
 
+// ------------------------------------------------------ compensated sums
+// error-free transformation a + b = s + e (Knuth TwoSum)
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// block reduction of (hi, lo) pairs (blockDim.x <= 1024); the block's
+// double-double total lands in out[2*blockIdx.x + {0,1}]
+__device__ inline void block_dd_sum(double hi, double lo, double *out) {
+#pragma clang fp contract(off)
+  __shared__ double s_hi[16], s_lo[16];
+#pragma unroll
+  for (int o = EMD_WAVE / 2; o > 0; o >>= 1) {
+    const double oh = __shfl_xor(hi, o, EMD_WAVE), ol = __shfl_xor(lo, o, EMD_WAVE);
+    double s, e;
+    two_sum(hi, oh, s, e);
+    hi = s;
+    lo = lo + ol + e;
+  }
+  const int w = threadIdx.x / EMD_WAVE;
+  if (__lane_id() == 0) { s_hi[w] = hi; s_lo[w] = lo; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double H = 0.0, Lo = 0.0;
+    for (int k = 0; k < (int)(blockDim.x / EMD_WAVE); k++) {
+      double s, e;
+      two_sum(H, s_hi[k], s, e);
+      H = s;
+      Lo += s_lo[k] + e;
+    }
+    out[2 * blockIdx.x] = H;
+    out[2 * blockIdx.x + 1] = Lo;
+  }
+}
+
+template <int K, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = K / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, K);
+  return v;
+}
+
 // ------------------------------------------------------------ misc helpers
 inline int emd_blocks(long n, int threads) { return (int)((n + threads - 1) / threads); }

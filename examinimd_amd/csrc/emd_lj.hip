@@ -33,13 +33,6 @@ extern "C" emd_status emd_force_zero(emd_ctx *ctx, emd_system *s, int precision)
   return EMD_OK;
 }
 
-template <int K, class T>
-__device__ __forceinline__ T group_sum(T v) {
-#pragma unroll
-  for (int o = K / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, K);
-  return v;
-}
-
 // ForceLJNeigh TagFullNeigh, src/force_types/force_lj_neigh_impl.h:158-206.
 // K lanes per atom; each lane takes every K-th entry of the row, U entries
 // per round with their index loads and position gathers issued together
@@ -254,41 +247,6 @@ extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh 
   }
 }
 
-// ------------------------------------------------------ compensated sums
-// error-free transformation a + b = s + e (Knuth TwoSum)
-__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
-#pragma clang fp contract(off)
-  s = a + b;
-  const double bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
-
-// block reduction of (hi, lo) pairs; results land in out[2*blockIdx.x + {0,1}]
-__device__ void block_dd_sum(double hi, double lo, double *out) {
-  __shared__ double s_hi[16], s_lo[16];
-#pragma unroll
-  for (int o = EMD_WAVE / 2; o > 0; o >>= 1) {
-    const double oh = __shfl_xor(hi, o, EMD_WAVE), ol = __shfl_xor(lo, o, EMD_WAVE);
-    double s, e;
-    two_sum(hi, oh, s, e);
-    hi = s;
-    lo = lo + ol + e;
-  }
-  const int w = threadIdx.x / EMD_WAVE;
-  if (__lane_id() == 0) { s_hi[w] = hi; s_lo[w] = lo; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double H = 0.0, Lo = 0.0;
-    for (int k = 0; k < (int)(blockDim.x / EMD_WAVE); k++) {
-      double s, e;
-      two_sum(H, s_hi[k], s, e);
-      H = s;
-      Lo += s_lo[k] + e;
-    }
-    out[2 * blockIdx.x] = H;
-    out[2 * blockIdx.x + 1] = Lo;
-  }
-}
 
 // ForceLJNeigh Tag{Full,Half}NeighPE, force_lj_neigh_impl.h:256-343
 __global__ void __launch_bounds__(256)

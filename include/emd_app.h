@@ -46,6 +46,18 @@ const emd_neigh  *emd_app_neigh(emd_app *app);
 const emd_halo   *emd_app_halo(emd_app *app);
 int               emd_app_precision(const emd_app *app);
 
+/* An ensemble run the conventional way, without batching: every replica of
+ * `d` (see emd_batch_desc) is its own system through the per-system C ABI
+ * (exchange, binning, halo, CSR list, LJ force, NVE: the reference's step
+ * src/examinimd.cpp:176-294), replicas visited one after another each step,
+ * one launch sequence per replica.  x, v: [total][3] host, replica-major.
+ * Runs the set-up force, then nsteps timed steps; *seconds = wall time of
+ * those steps.  x_out / v_out (may be NULL) receive the final state in the
+ * input order of each replica.  The reference point for emd_batch_run. */
+int emd_replica_loop_run(emd_ctx *ctx, const emd_batch_desc *d, const double *x,
+                         const double *v, int nsteps, double *seconds, double *x_out,
+                         double *v_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -230,6 +230,66 @@ emd_status emd_lj_adjoint(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl
                           double sigma, double cut, const double *dLdF, double dLdE,
                           int precision, double *out);
 
+/* -------------------------------------------------- batched replicas */
+/* No counterpart in the reference: many independent periodic LJ systems
+ * ("replicas", uneven sizes) advanced together.  Per replica the step is
+ * the reference's serial-comm LJ step (src/examinimd.cpp:176-294:
+ * IntegratorNVE initial/final, CommSerial exchange + periodic images,
+ * full CSR neighbour list rebuilt every rebuild_every steps, ForceLJNeigh
+ * full-list force with strict rsq < rc^2 and shifted energy).  Periodic
+ * images are carried as 2-bit-per-axis codes in the neighbour entries
+ * instead of ghost atoms; an image position is x_j +/- L per axis, as a
+ * ghost's is.  Atoms of all replicas live in one flat array (replica r
+ * owns [start_r, start_r + natoms_r)), so every kernel is one launch for
+ * the whole batch; a step is one fused kernel (force, final kick, next
+ * half-kick and drift into a second position buffer), and the rebuild
+ * period is captured once as a HIP graph and replayed.  Single atom type
+ * per replica, one shared 12-6 potential (eps, sigma, cut). */
+typedef struct emd_batch emd_batch;
+
+typedef struct {
+  int nrep;
+  const int *natoms;    /* [nrep] atoms per replica (host) */
+  const double *box;    /* [nrep][3] periodic box lengths, origin 0 (host) */
+  const double *mass;   /* [nrep] atom mass per replica (host) */
+  double eps, sigma, cut;
+  double skin;          /* neighbour cutoff = cut + skin */
+  double dt;
+  int rebuild_every;    /* rebuild when step % rebuild_every == 0 */
+  int precision;        /* EMD_FP64 or EMD_MIXED (FP32 pair math and forces) */
+} emd_batch_desc;
+
+/* every box length must be >= cut + skin; total atoms < 2^26 */
+emd_status emd_batch_create(emd_ctx *ctx, const emd_batch_desc *d, emd_batch **out);
+emd_status emd_batch_destroy(emd_ctx *ctx, emd_batch *b);
+/* host arrays [total][3], replica-major; resets the step counter to 0 */
+emd_status emd_batch_upload(emd_ctx *ctx, emd_batch *b, const double *x, const double *v);
+/* any pointer may be NULL; f is the last force (converted from FP32 in mixed mode) */
+emd_status emd_batch_download(emd_ctx *ctx, emd_batch *b, double *x, double *v, double *f);
+/* wrap + neighbour build + force at the current positions (the reference's
+ * set-up force before step 1) */
+emd_status emd_batch_setup(emd_ctx *ctx, emd_batch *b);
+/* advance nsteps; use_graph != 0 replays the captured rebuild period.
+ * Returns EMD_ERR_OVERFLOW if a neighbour row outgrew its capacity during
+ * the run (results are then invalid; call emd_batch_setup again, which
+ * re-sizes the rows) */
+emd_status emd_batch_run(emd_ctx *ctx, emd_batch *b, int nsteps, int use_graph);
+/* per replica: ke[r] = 0.5 sum m v^2, pe[r] = shifted LJ energy (FP64,
+ * compensated); either may be NULL */
+emd_status emd_batch_thermo(emd_ctx *ctx, emd_batch *b, double *ke, double *pe);
+/* force-kernel tile: lanes per atom (1,2,4,8,16) and threads per block
+ * (64..1024, multiple of 64) */
+emd_status emd_batch_set_tile(emd_ctx *ctx, emd_batch *b, int lanes, int block);
+/* times every candidate tile over trial_steps steps (state restored
+ * afterwards) and keeps the fastest; outputs may be NULL */
+emd_status emd_batch_autotune(emd_ctx *ctx, emd_batch *b, int trial_steps, int *lanes,
+                              int *block, double *ms_per_step);
+/* step counter, largest row so far, row capacity */
+emd_status emd_batch_info(emd_ctx *ctx, emd_batch *b, long *step, int *max_row, int *row_cap);
+/* neighbour rows: num[total], entries[total][row_cap] as (j | code << 26),
+ * code bits 2d..2d+1 = image of axis d (0 none, 1 -L, 2 +L); host arrays */
+emd_status emd_batch_neigh(emd_ctx *ctx, emd_batch *b, int *num, unsigned *entries);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,3 +22,68 @@ def sorted_rows(row, ent, key=None):
             r = key[r]
         out.append(np.sort(r))
     return out
+
+
+def oracle_lj_md(oracle, x, v, box, nsteps, dt=0.005, eps=1.0, sig=1.0, rc=2.5, skin=0.3,
+                 every=20, mass=1.0):
+    """the reference's serial-comm LJ loop (src/examinimd.cpp:176-294) on the
+    CPU oracle for one periodic box, from given positions and velocities:
+    set-up force, then per step NVE initial, rebuild (step % every == 0) or
+    halo update, full-list force, NVE final.  Returns x, v, f of the local
+    atoms in input order."""
+    import ctypes as C
+    L = oracle.lib()
+    s = oracle.System()
+    L.or_system_init(C.byref(s))
+    n = len(x)
+    L.or_system_grow(C.byref(s), n)
+    s.N_local = s.N = n
+    s.ntypes = 1
+    s.mass[0] = mass
+    s.dt, s.mvv2e, s.boltz = dt, 1.0, 1.0
+    for d in range(3):
+        s.domain[d] = s.hi[d] = box[d]
+        s.lo[d] = 0.0
+    np.ctypeslib.as_array(s.x, shape=(3 * n,))[:] = np.asarray(x, dtype=np.float64).ravel()
+    np.ctypeslib.as_array(s.v, shape=(3 * n,))[:] = np.asarray(v, dtype=np.float64).ravel()
+    np.ctypeslib.as_array(s.id, shape=(n,))[:] = np.arange(1, n + 1)
+    np.ctypeslib.as_array(s.type, shape=(n,))[:] = 0
+    cut = rc + skin
+
+    def build():
+        h, b, c = oracle.Halo(), oracle.Bins(), oracle.CSR()
+        L.or_comm_exchange(C.byref(s))
+        L.or_binning(C.byref(s), C.byref(b), cut, cut, cut, 1, 1, 0, 1)
+        L.or_comm_exchange_halo(C.byref(s), C.byref(h), cut)
+        L.or_binning(C.byref(s), C.byref(b), cut, cut, cut, 1, 1, 1, 0)
+        L.or_neigh_csr(C.byref(s), C.byref(b), cut, 0, 0, C.byref(c))
+        return h, b, c
+
+    def force(c):
+        np.ctypeslib.as_array(s.f, shape=(3 * s.N_max,))[:] = 0
+        L.or_lj_force(C.byref(s), C.byref(c), eps, sig, rc, 0)
+
+    h, b, c = build()
+    force(c)
+    for step in range(1, nsteps + 1):
+        L.or_nve_initial(C.byref(s))
+        if step % every == 0:
+            L.or_halo_free(C.byref(h))
+            L.or_bins_free(C.byref(b))
+            L.or_csr_free(C.byref(c))
+            h, b, c = build()
+        else:
+            L.or_comm_update_halo(C.byref(s), C.byref(h))
+        force(c)
+        L.or_nve_final(C.byref(s))
+    ids = oracle.arr(s.id, n, np.int32) - 1
+    out = []
+    for p in (s.x, s.v, s.f):
+        a = np.zeros((n, 3))
+        a[ids] = oracle.arr(p, 3 * n).reshape(n, 3)
+        out.append(a)
+    L.or_system_free(C.byref(s))
+    L.or_halo_free(C.byref(h))
+    L.or_bins_free(C.byref(b))
+    L.or_csr_free(C.byref(c))
+    return tuple(out)
