@@ -76,6 +76,32 @@ def lj_run(emd, decks, tmpdir, n, precision, warmup, steps):
     return app, wall, prof, nlist
 
 
+def melt_force_error(emd, app):
+    """force RMS error of the mixed path on its own trajectory: the FP32
+    forces of its last step against FP64 forces re-evaluated at the same
+    positions with the same neighbour list (trajectories of the two paths
+    diverge, so errors are only comparable on identical positions)"""
+    import numpy as np
+    from examinimd_amd.abi import EMD_FP64, EmdLJParams
+    L, ctx = emd.hip_lib(), app.ctx
+    f32 = app.arrays()["f"]
+    s = app.L.emd_app_system(app.h)
+    nl = app.L.emd_app_neigh(app.h)
+    p = EmdLJParams()
+    p.ntypes = 1
+    p.lj1[0][0], p.lj2[0][0], p.cutsq[0][0] = 48.0, 24.0, 6.25
+    emd.check(ctx, L.emd_force_zero(ctx, s, EMD_FP64), "force_zero")
+    emd.check(ctx, L.emd_lj_force(ctx, s, nl, C.byref(p), EMD_FP64, 0), "lj_force")
+    n = app.nlocal
+    f64 = np.zeros((n, 3))
+    emd.check(ctx, L.emd_system_download(ctx, s, n, None, None,
+                                         f64.ctypes.data_as(C.POINTER(C.c_double)), None, None,
+                                         None), "download")
+    df = f32 - f64
+    return {"rms_error": float(np.sqrt(np.mean(np.sum(df * df, axis=1)))),
+            "force_rms": float(np.sqrt(np.mean(np.sum(f64 * f64, axis=1))))}
+
+
 def force_error(emd, decks, tmpdir, n):
     """force RMS error of the mixed path against FP64 on identical positions"""
     import numpy as np
@@ -159,7 +185,7 @@ def lj_force_error_disordered(emd):
     x = lattice("fcc", 20, a) + np.random.default_rng(5).normal(0, 0.05, (32000, 3))
     pl = Pipeline()
     pl.load(x, (20 * a,) * 3)
-    pl.build(2.8)
+    pl.build(2.8, tile=True)
     f64 = pl.lj(emd.EMD_FP64)
     f32 = pl.lj(emd.EMD_MIXED)
     pl.close()
@@ -243,7 +269,10 @@ def main():
         kname = "lj_force_mixed" if prec == "mixed" else "lj_force_fp64"
         cnt, ms = prof.get(kname, (0, 0.0))
         out[prec] = {"wall": wall, "natoms": natoms, "prof": prof, "nlist": nlist,
-                     "kernel": kname, "kernel_launches": cnt, "kernel_ms": ms}
+                     "kernel": kname, "kernel_launches": cnt, "kernel_ms": ms,
+                     "thermo_end": list(app.thermo())}
+        if prec == "mixed":
+            out[prec]["melt_error"] = melt_force_error(emd, app)
         app.close()
 
     mixed, fp64 = out["mixed"], out["fp64"]
@@ -303,6 +332,9 @@ def main():
                  "dtype": "f64"},
         "mixed_over_fp64": value / value_fp64,
         "mixed_force_rms_error_vs_fp64": ferr,
+        "mixed_force_rms_error_on_melt": mixed["melt_error"],
+        "thermo_end": {"fp64": fp64["thermo_end"], "mixed": mixed["thermo_end"],
+                       "fields": ["T", "PE/N", "E/N"]},
         "mixed_force_rms_error_vs_fp64_displaced_lattice": ferr_dis,
         "snap2k": snap,
         "batched": batched,

@@ -37,7 +37,9 @@ class EmdBins(C.Structure):
 class EmdNeigh(C.Structure):
     _fields_ = [("layout", C.c_int), ("n", C.c_int), ("total", C.c_long),
                 ("maxneighs", C.c_int), ("row", iptr), ("num", iptr), ("entries", iptr),
-                ("cap_entries", C.c_long), ("cap_rows", C.c_int)]
+                ("cap_entries", C.c_long), ("cap_rows", C.c_int), ("want_slot", C.c_int),
+                ("slot", P(C.c_ushort)), ("cap_slot", C.c_long), ("bins", C.c_void_p),
+                ("visit", iptr), ("cap_visit", C.c_int)]
 
 
 class EmdLJParams(C.Structure):
@@ -97,6 +99,7 @@ HIP_PROTOS = {
     "emd_lj_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int,
                                 P(C.c_double)]),
     "emd_lj_set_lanes": (C.c_int, [vp, C.c_int, C.c_int]),
+    "emd_lj_set_tile": (C.c_int, [vp, C.c_int, C.c_int]),
     "emd_nve_initial": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_double, C.c_int]),
     "emd_nve_final": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_int]),
     "emd_reduce_mv2": (C.c_int, [vp, P(EmdSystem), P(C.c_double)]),

@@ -258,20 +258,18 @@ k_bstep(const double *__restrict__ x, double *__restrict__ xn, double *__restric
   const unsigned *rowp = entries + (size_t)i * cap;
   const int cnt = num[i];
   T fxi = 0, fyi = 0, fzi = 0;
-  auto pair = [&](unsigned e, double xj, double yj, double zj) {
+  auto pair = [&](bool ok, unsigned e, double xj, double yj, double zj) {
     const unsigned c = e >> B_CODE_SHIFT;
     const double dx = x_i - b_image(xj, c & 3u, Lx);
     const double dy = y_i - b_image(yj, (c >> 2) & 3u, Ly);
     const double dz = z_i - b_image(zj, (c >> 4) & 3u, Lz);
     const double rsq = dx * dx + dy * dy + dz * dz;
-    if (rsq < cutsq) {
+    if (ok && rsq < cutsq) {
       if (MIXED) {
-        const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
-        const float r6inv = r2inv * r2inv * r2inv;
-        const float fpair = (r6inv * ((float)lj1 * r6inv - (float)lj2)) * r2inv;
-        fxi += (float)dx * fpair;
-        fyi += (float)dy * fpair;
-        fzi += (float)dz * fpair;
+        const float fpair = emd_lj_fpair_mixed(rsq, (float)lj1, (float)lj2);
+        fxi = __builtin_fmaf((float)dx, fpair, fxi);
+        fyi = __builtin_fmaf((float)dy, fpair, fyi);
+        fzi = __builtin_fmaf((float)dz, fpair, fzi);
       } else {
         const double r2inv = 1.0 / rsq;
         const double r6inv = r2inv * r2inv * r2inv;
@@ -282,12 +280,17 @@ k_bstep(const double *__restrict__ x, double *__restrict__ xn, double *__restric
       }
     }
   };
-  int jj = sub;
-  for (; jj + (U - 1) * K < cnt; jj += U * K) {
+  // U entries per lane per memory round (index loads, then gathers, then
+  // arithmetic); slots past the row end re-read its last entry, masked
+  for (int jj = sub; jj < cnt; jj += U * K) {
     unsigned e[U];
+    bool ok[U];
     double xj[U], yj[U], zj[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) e[u] = __builtin_nontemporal_load(rowp + jj + u * K);
+    for (int u = 0; u < U; u++) {
+      ok[u] = jj + u * K < cnt;
+      e[u] = __builtin_nontemporal_load(rowp + (ok[u] ? jj + u * K : cnt - 1));
+    }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const size_t j = e[u] & B_JMASK;
@@ -296,12 +299,7 @@ k_bstep(const double *__restrict__ x, double *__restrict__ xn, double *__restric
       zj[u] = x[3 * j + 2];
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) pair(e[u], xj[u], yj[u], zj[u]);
-  }
-  for (; jj < cnt; jj += K) {
-    const unsigned e = __builtin_nontemporal_load(rowp + jj);
-    const size_t j = e & B_JMASK;
-    pair(e, x[3 * j], x[3 * j + 1], x[3 * j + 2]);
+    for (int u = 0; u < U; u++) pair(ok[u], e[u], xj[u], yj[u], zj[u]);
   }
   if (K > 1) {
     fxi = group_sum<K>(fxi);
@@ -437,11 +435,11 @@ emd_status launch_step_k(emd_ctx *ctx, emd_batch *b, int mode) {
 template <bool MIXED>
 emd_status launch_step_p(emd_ctx *ctx, emd_batch *b, int mode) {
   switch (b->lanes) {
-    case 1: return launch_step_k<1, 2, MIXED>(ctx, b, mode);
+    case 1: return launch_step_k<1, 4, MIXED>(ctx, b, mode);
     case 2: return launch_step_k<2, 4, MIXED>(ctx, b, mode);
     case 4: return launch_step_k<4, 4, MIXED>(ctx, b, mode);
-    case 8: return launch_step_k<8, 2, MIXED>(ctx, b, mode);
-    default: return launch_step_k<16, 2, MIXED>(ctx, b, mode);
+    case 8: return launch_step_k<8, 5, MIXED>(ctx, b, mode);
+    default: return launch_step_k<16, 5, MIXED>(ctx, b, mode);
   }
 }
 

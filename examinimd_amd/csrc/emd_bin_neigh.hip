@@ -211,18 +211,14 @@ constexpr int NB_THREADS = 256;
 constexpr int NB_WAVES = NB_THREADS / EMD_WAVE;
 constexpr int NB_MAXC = 1024;  // candidates staged per LDS chunk
 
-struct neigh_geom {
-  int nbx, nby, nbz, nhalo;  // full grid incl. halo
-  int nix, niy, niz;         // interior bins
-};
-
 // NeighborCSR Tag{Count,Fill}Neigh{Full,Half}, src/neighbor_types/neighbor_csr.h:176-358
 template <bool HALF, bool FILL>
 __global__ void __launch_bounds__(NB_THREADS)
 k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
         const int *__restrict__ binoffsets, const int *__restrict__ permute, neigh_geom g,
         int n_local, double cutsq, int newton, int *__restrict__ num, const int *__restrict__ row,
-        int pitch, int *__restrict__ entries, int *__restrict__ cursor, int *__restrict__ maxrow) {
+        int pitch, int *__restrict__ entries, unsigned short *__restrict__ slot,
+        int *__restrict__ cursor, int *__restrict__ maxrow) {
 #pragma clang fp contract(off)
   __shared__ double s_x[NB_MAXC], s_y[NB_MAXC], s_z[NB_MAXC];
   __shared__ int s_j[NB_MAXC];
@@ -293,7 +289,11 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
           }
         }
         const unsigned long long m = __ballot(keep);
-        if (FILL && keep) entries[base + run + __popcll(m & lt)] = j;
+        if (FILL && keep) {
+          const size_t e = base + run + __popcll(m & lt);
+          entries[e] = j;
+          if (slot) slot[e] = (unsigned short)(cs + c);
+        }
         run += __popcll(m);
       }
       if (lane == 0) {
@@ -341,6 +341,47 @@ struct row_offsets_op {
   }
 };
 
+// Force sweep order: interior bin columns (bx, by; all interior bz, which
+// are consecutive bins and so one contiguous permute range of local atoms)
+// visited in T x T tiles of columns, tiles and the columns inside a tile in
+// row-major order.  Slot s of the enumeration is (tile, column in tile);
+// slots outside the grid are empty.
+__device__ __forceinline__ int visit_column(const neigh_geom &g, int T, int s, int *first_bin) {
+  const int nty = (g.niy + T - 1) / T;
+  const int tile = s / (T * T), in = s % (T * T);
+  const int cx = (tile / nty) * T + in / T, cy = (tile % nty) * T + in % T;
+  if (cx >= g.nix || cy >= g.niy) return 0;
+  *first_bin = ((cx + g.nhalo) * g.nby + (cy + g.nhalo)) * g.nbz + g.nhalo;
+  return 1;
+}
+
+struct visit_len_op {
+  neigh_geom g;
+  int T;
+  const int *bincount, *binoffsets;
+  int *start;  // [nslots + 1] exclusive prefix of the column lengths
+  __device__ int value(int s) const {
+    int b0;
+    if (!visit_column(g, T, s, &b0)) return 0;
+    const int b1 = b0 + g.niz - 1;
+    return binoffsets[b1] + bincount[b1] - binoffsets[b0];
+  }
+  __device__ void emit(int s, int excl, int) const { start[s] = excl; }
+};
+
+// one block per column slot: copy the column's local atoms into the order
+__global__ void __launch_bounds__(256)
+k_visit_fill(neigh_geom g, int T, const int *__restrict__ binoffsets,
+             const int *__restrict__ bincount, const int *__restrict__ permute,
+             const int *__restrict__ start, int *__restrict__ visit) {
+  int b0;
+  if (!visit_column(g, T, blockIdx.x, &b0)) return;
+  const int b1 = b0 + g.niz - 1;
+  const int o = binoffsets[b0], len = binoffsets[b1] + bincount[b1] - o;
+  const int dst = start[blockIdx.x];
+  for (int k = threadIdx.x; k < len; k += blockDim.x) visit[dst + k] = permute[o + k];
+}
+
 extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const emd_bins *b,
                                       double neigh_cut, int half, int newton, int layout,
                                       emd_neigh *nl) {
@@ -355,6 +396,8 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
   const long nib = (long)g.nix * g.niy * g.niz;
   EMD_ARG(ctx, b->nhalo >= 1, "neighbour build needs a halo of at least one bin");
   const double cutsq = neigh_cut * neigh_cut;
+  unsigned short *slot = nullptr;
+  nl->bins = b;
 
   int rcap = nl->cap_rows;
   EMD_TRY(emd_ensure_int(ctx, &nl->num, &nl->cap_rows, n + 1));
@@ -369,7 +412,7 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
 #define NB_ARGS(F)                                                                          \
   s->x, b->bincount, b->binoffsets, b->permute, g, n, cutsq, newton, nl->num,              \
       (F) ? (layout == EMD_NEIGH_CSR ? nl->row : nullptr) : nullptr, nl->maxneighs,        \
-      nl->entries, cursor, ctx->d_small + 4
+      nl->entries, (F) ? slot : nullptr, cursor, ctx->d_small + 4
   if (nib > 0) {
     if (half) EMD_LAUNCH(ctx, "neigh_count", (k_neigh<true, false>), nib, NB_THREADS, 0, NB_ARGS(0));
     else EMD_LAUNCH(ctx, "neigh_count", (k_neigh<false, false>), nib, NB_THREADS, 0, NB_ARGS(0));
@@ -410,11 +453,55 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
                            want);
     nl->cap_entries = want;
   }
+  if (nl->want_slot && !half && ctx->lj_tile) {
+    if (need > nl->cap_slot || !nl->slot) {
+      if (nl->slot) {
+        EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        EMD_HIP(ctx, hipFree(nl->slot));
+      }
+      nl->slot = nullptr;
+      nl->cap_slot = 0;
+      const long want = need + need / 8 + 64;
+      if (hipMalloc(&nl->slot, (size_t)want * sizeof(unsigned short)) != hipSuccess)
+        return emd_set_error(ctx, EMD_ERR_ALLOC, "neighbour slots: hipMalloc of %ld failed",
+                             want);
+      nl->cap_slot = want;
+    }
+    slot = nl->slot;
+  } else if (nl->slot) {
+    // a list built without the tile view carries none
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_HIP(ctx, hipFree(nl->slot));
+    nl->slot = nullptr;
+    nl->cap_slot = 0;
+  }
   if (nib > 0) {
     if (half) EMD_LAUNCH(ctx, "neigh_fill", (k_neigh<true, true>), nib, NB_THREADS, 0, NB_ARGS(1));
     else EMD_LAUNCH(ctx, "neigh_fill", (k_neigh<false, true>), nib, NB_THREADS, 0, NB_ARGS(1));
   }
 #undef NB_ARGS
+  // sweep order of full lists (every local atom sits in an interior bin)
+  const int T = ctx->visit_tile;
+  if (!half && T > 0 && n > 0 && nib > 0) {
+    EMD_TRY(emd_ensure_int(ctx, &nl->visit, &nl->cap_visit, n));
+    const int nslots = ((g.nix + T - 1) / T) * ((g.niy + T - 1) / T) * T * T;
+    EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)(nslots + 1) * sizeof(int)));
+    int *start = (int *)ctx->d_tmp2;
+    visit_len_op op{g, T, b->bincount, b->binoffsets, start};
+    int placed = 0;
+    EMD_TRY(emd_scan(ctx, op, nslots, "neigh_visit", &placed));
+    if (placed != n)
+      return emd_set_error(ctx, EMD_ERR_STATE,
+                           "sweep order covers %d of %d local atoms (atoms outside interior bins)",
+                           placed, n);
+    EMD_LAUNCH(ctx, "neigh_visit", k_visit_fill, nslots, 256, 0, g, T, b->binoffsets,
+               b->bincount, b->permute, start, nl->visit);
+  } else if (nl->visit) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_HIP(ctx, hipFree(nl->visit));
+    nl->visit = nullptr;
+    nl->cap_visit = 0;
+  }
   return EMD_OK;
 }
 
@@ -424,7 +511,14 @@ extern "C" emd_status emd_neigh_free(emd_ctx *ctx, emd_neigh *nl) {
   if (nl->row) hipFree(nl->row);
   if (nl->num) hipFree(nl->num);
   if (nl->entries) hipFree(nl->entries);
+  if (nl->slot) hipFree(nl->slot);
+  if (nl->visit) hipFree(nl->visit);
+  nl->visit = nullptr;
+  nl->cap_visit = 0;
   nl->row = nl->num = nl->entries = nullptr;
+  nl->slot = nullptr;
+  nl->cap_slot = 0;
+  nl->bins = nullptr;
   nl->cap_entries = 0;
   nl->cap_rows = 0;
   return EMD_OK;

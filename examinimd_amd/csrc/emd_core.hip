@@ -73,6 +73,23 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
     const int k = ctx->lj_lanes[p];
     if (!(k == 0 || k == 1 || k == 2 || k == 4 || k == 8 || k == 16)) ctx->lj_lanes[p] = 0;
   }
+  // bin-tile LJ kernel: EMD_LJ_TILE=0 switches it off, EMD_LJ_TILE_LANES
+  // picks its lanes per atom (4, 8 or 16)
+  if (const char *e = getenv("EMD_LJ_TILE")) ctx->lj_tile = atoi(e) != 0;
+  // column-tile width of the force sweep order built with full lists
+  if (const char *e = getenv("EMD_VISIT_TILE")) {
+    const int t = atoi(e);
+    if (t >= 0 && t <= 64) ctx->visit_tile = t;
+  }
+  // entries per lane and memory round of the 16-lane CSR LJ kernel (2..6)
+  if (const char *e = getenv("EMD_LJ_UNROLL")) {
+    const int u = atoi(e);
+    if (u >= 2 && u <= 6) ctx->lj_unroll = u;
+  }
+  if (const char *e = getenv("EMD_LJ_TILE_LANES")) {
+    const int k = atoi(e);
+    if (k == 4 || k == 8 || k == 16) ctx->lj_tile_lanes = k;
+  }
   *out = ctx;
   return EMD_OK;
 }

@@ -57,6 +57,10 @@ struct emd_ctx {
 
   // tuning
   int lj_lanes[2] = {0, 0};
+  int lj_tile = 0;        // use the bin-tile LJ kernel when the list carries slots
+  int lj_tile_lanes = 8;  // lanes per atom of the bin-tile kernel
+  int lj_unroll = 5;      // entries per lane per round, 16-lane CSR kernel
+  int visit_tile = 8;     // column-tile width of the force sweep order (0 = index order)
 };
 
 emd_status emd_set_error(emd_ctx *ctx, emd_status code, const char *fmt, ...);
@@ -301,12 +305,104 @@ __device__ inline void block_dd_sum(double hi, double lo, double *out) {
   }
 }
 
+// LJ pair force factor of the mixed path.  fpair ~ r^-14 multiplies the
+// relative rounding of r^2 sevenfold, so 1/r^2 comes from the hardware FP64
+// reciprocal refined by one Newton step (squares its error, far below FP32
+// rounding) and r^-6 is formed in FP64; the force product and accumulation
+// are FP32.  On an LJ melt (rho 0.8442, T* ~0.74, force RMS 24) this keeps
+// the force RMS error near 3e-6, where rounding r^2 to FP32 first gives
+// 1.05e-5.
+__device__ __forceinline__ float emd_lj_fpair_mixed(double rsq, float lj1, float lj2) {
+  const double r0 = __builtin_amdgcn_rcp(rsq);
+  const double e = __builtin_fma(-rsq, r0, 1.0);
+  const double r2 = __builtin_fma(r0, e, r0);
+  const double r6 = r2 * r2 * r2;
+  const float r6f = (float)r6, r2f = (float)r2;
+  return (r6f * (lj1 * r6f - lj2)) * r2f;
+}
+
+// Sum over groups of K consecutive lanes (K = 1, 2, 4, 8, 16; groups aligned
+// to K) with DPP lane permutations inside 16-lane rows, no LDS traffic:
+// quad_perm [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2), row_half_mirror
+// (lane l <-> 7-l within 8, joining the two quads) and row_mirror (l <-> 15-l,
+// joining the two half-rows).  Every lane of a group ends with the group sum.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_any(float v) { return dpp_f32<CTRL>(v); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_any(double v) { return dpp_f64<CTRL>(v); }
+
 template <int K, class T>
 __device__ __forceinline__ T group_sum(T v) {
-#pragma unroll
-  for (int o = K / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, K);
+  static_assert(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, "group of 1..16 lanes");
+  if (K >= 2) v += dpp_any<0xB1>(v);   // quad_perm [1,0,3,2]
+  if (K >= 4) v += dpp_any<0x4E>(v);   // quad_perm [2,3,0,1]
+  if (K >= 8) v += dpp_any<0x141>(v);  // row_half_mirror
+  if (K >= 16) v += dpp_any<0x140>(v); // row_mirror
   return v;
 }
 
 // ------------------------------------------------------------ misc helpers
 inline int emd_blocks(long n, int threads) { return (int)((n + threads - 1) / threads); }
+
+// bin grid of a neighbour build: interior bins are the non-halo ones, one
+// block per interior bin; the 27-bin stencil order is dx, dy, dz = -1..1
+// with dz fastest (stencil slot 13 is the bin itself)
+struct neigh_geom {
+  int nbx, nby, nbz, nhalo;  // full grid incl. halo
+  int nix, niy, niz;         // interior bins
+};
+
+inline neigh_geom emd_neigh_geom(const emd_bins *b) {
+  neigh_geom g;
+  g.nbx = b->nbinx; g.nby = b->nbiny; g.nbz = b->nbinz; g.nhalo = b->nhalo;
+  g.nix = b->nbinx - 2 * b->nhalo; g.niy = b->nbiny - 2 * b->nhalo;
+  g.niz = b->nbinz - 2 * b->nhalo;
+  return g;
+}
+
+// the interior bin of block lr and its 27 stencil bins' offsets and
+// exclusive count prefix (s_pre[27] = candidates of the bin); call with
+// all threads of the block, blockDim.x >= 27
+__device__ __forceinline__ int emd_stencil_setup(const neigh_geom &g, int lr,
+                                                 const int *__restrict__ bincount,
+                                                 const int *__restrict__ binoffsets, int *s_off,
+                                                 int *s_pre) {
+  const int bx = lr / (g.niy * g.niz) + g.nhalo;
+  const int by = (lr / g.niz) % g.niy + g.nhalo;
+  const int bz = lr % g.niz + g.nhalo;
+  const int tid = threadIdx.x;
+  if (tid < 27) {
+    const int ddx = tid / 9 - 1, ddy = (tid / 3) % 3 - 1, ddz = tid % 3 - 1;
+    const int b = ((bx + ddx) * g.nby + (by + ddy)) * g.nbz + (bz + ddz);
+    s_off[tid] = binoffsets[b];
+    s_pre[tid + 1] = bincount[b];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    s_pre[0] = 0;
+    for (int k = 1; k <= 27; k++) s_pre[k] += s_pre[k - 1];
+  }
+  __syncthreads();
+  return (bx * g.nby + by) * g.nbz + bz;
+}
+
+// stencil slot k of candidate c: s_pre[k] <= c < s_pre[k+1] (binary search)
+__device__ __forceinline__ int emd_stencil_slot(const int *s_pre, int c) {
+  int lo = 0, hi = 27;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_pre[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}

@@ -34,10 +34,13 @@ extern "C" emd_status emd_force_zero(emd_ctx *ctx, emd_system *s, int precision)
 }
 
 // ForceLJNeigh TagFullNeigh, src/force_types/force_lj_neigh_impl.h:158-206.
-// K lanes per atom; each lane takes every K-th entry of the row, U entries
-// per round with their index loads and position gathers issued together
-// (independent memory requests in flight), row indices loaded non-temporally
-// (read once, kept out of L2 so positions stay resident).
+// K lanes per atom; each lane takes every K-th entry of the row.  The kernel
+// is bound by memory latency (index load, then the dependent position
+// gather), so a lane issues the index loads of U entries, then their U
+// gathers, before any arithmetic: one round trip covers U*K entries (16 x 5
+// holds a typical 75-entry row).  Slots past the row end re-read its last
+// entry and are masked.  Row indices are loaded non-temporally (read once,
+// kept out of L2 so positions stay resident).
 template <bool MIXED>
 struct lj_acc {
   typedef double T;
@@ -52,20 +55,22 @@ __global__ void __launch_bounds__(LJ_THREADS)
 k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
           const int *__restrict__ row, const int *__restrict__ num,
           const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
-          double cutsq_0, const lj_table *__restrict__ tab, double *__restrict__ f,
-          float *__restrict__ f32) {
+          double cutsq_0, const lj_table *__restrict__ tab, const int *__restrict__ visit,
+          double *__restrict__ f, float *__restrict__ f32) {
 #pragma clang fp contract(off)
   typedef typename lj_acc<MIXED>::T T;
   const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
   const int gid = tile * LJ_THREADS + threadIdx.x;
-  const int i = gid / K, sub = gid % K;
-  if (i >= n) return;
+  const int a = gid / K, sub = gid % K;
+  if (a >= n) return;
+  // atoms in the list's sweep order (column tiles) when it has one
+  const int i = visit ? visit[a] : a;
   const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
   const int type_i = ONE_TYPE ? 0 : type[i];
   const int *rowp = entries + (row ? (size_t)row[i] : (size_t)i * (size_t)pitch);
   const int cnt = num[i];
   T fxi = 0, fyi = 0, fzi = 0;
-  auto pair = [&](int j, double xj, double yj, double zj) {
+  auto pair = [&](bool ok, int j, double xj, double yj, double zj) {
     const double dx = x_i - xj;
     const double dy = y_i - yj;
     const double dz = z_i - zj;
@@ -75,14 +80,12 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
       const int t = type_i * LJ_MAXT + type[j];
       cutsq = tab->cutsq[t]; lj1 = tab->lj1[t]; lj2 = tab->lj2[t];
     }
-    if (rsq < cutsq) {
+    if (ok && rsq < cutsq) {
       if (MIXED) {
-        const float r2inv = __builtin_amdgcn_rcpf((float)rsq);
-        const float r6inv = r2inv * r2inv * r2inv;
-        const float fpair = (r6inv * ((float)lj1 * r6inv - (float)lj2)) * r2inv;
-        fxi += (float)dx * fpair;
-        fyi += (float)dy * fpair;
-        fzi += (float)dz * fpair;
+        const float fpair = emd_lj_fpair_mixed(rsq, (float)lj1, (float)lj2);
+        fxi = __builtin_fmaf((float)dx, fpair, fxi);
+        fyi = __builtin_fmaf((float)dy, fpair, fyi);
+        fzi = __builtin_fmaf((float)dz, fpair, fzi);
       } else {
         const double r2inv = 1.0 / rsq;
         const double r6inv = r2inv * r2inv * r2inv;
@@ -93,12 +96,15 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
       }
     }
   };
-  int jj = sub;
-  for (; jj + (U - 1) * K < cnt; jj += U * K) {
+  for (int jj = sub; jj < cnt; jj += U * K) {
     int j[U];
+    bool ok[U];
     double xj[U], yj[U], zj[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) j[u] = __builtin_nontemporal_load(rowp + jj + u * K);
+    for (int u = 0; u < U; u++) {
+      ok[u] = jj + u * K < cnt;
+      j[u] = __builtin_nontemporal_load(rowp + (ok[u] ? jj + u * K : cnt - 1));
+    }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       xj[u] = x[3 * (size_t)j[u]];
@@ -106,11 +112,7 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
       zj[u] = x[3 * (size_t)j[u] + 2];
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) pair(j[u], xj[u], yj[u], zj[u]);
-  }
-  for (; jj < cnt; jj += K) {
-    const int j = __builtin_nontemporal_load(rowp + jj);
-    pair(j, x[3 * (size_t)j], x[3 * (size_t)j + 1], x[3 * (size_t)j + 2]);
+    for (int u = 0; u < U; u++) pair(ok[u], j[u], xj[u], yj[u], zj[u]);
   }
   if (K > 1) {
     fxi = group_sum<K>(fxi);
@@ -126,6 +128,97 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
       f[3 * (size_t)i] += fxi;
       f[3 * (size_t)i + 1] += fyi;
       f[3 * (size_t)i + 2] += fzi;
+    }
+  }
+}
+
+// Bin-tile form of the same full-list force (one type).  One block per
+// interior bin: the block copies the positions of the bin's 27-bin
+// candidate list (the order the neighbour build enumerated them in) into
+// LDS once, then every row entry's slot addresses its neighbour's position
+// there, so the only global traffic per pair is the 2-byte slot.  The lanes
+// of an atom take the same entries in the same order as k_lj_full with the
+// same K, and reduce with the same shuffles: the FP64 result is bitwise that
+// of k_lj_full<K>.
+template <int K, int U, bool MIXED>
+__global__ void __launch_bounds__(LJ_THREADS)
+k_lj_tile(const double *__restrict__ x, const int *__restrict__ bincount,
+          const int *__restrict__ binoffsets, const int *__restrict__ permute, neigh_geom g,
+          const int *__restrict__ row, const int *__restrict__ num,
+          const unsigned short *__restrict__ slot, int pitch, int n_local, int tcap, double lj1,
+          double lj2, double cutsq, double *__restrict__ f, float *__restrict__ f32) {
+#pragma clang fp contract(off)
+  typedef typename lj_acc<MIXED>::T T;
+  extern __shared__ double s_cand[];
+  double *s_x = s_cand, *s_y = s_cand + tcap, *s_z = s_cand + 2 * tcap;
+  __shared__ int s_off[27], s_pre[28];
+  const int lr = emd_xcd_tile(blockIdx.x, gridDim.x);
+  emd_stencil_setup(g, lr, bincount, binoffsets, s_off, s_pre);
+  const int ncand = s_pre[27];
+  for (int c = threadIdx.x; c < ncand; c += LJ_THREADS) {
+    const int k = emd_stencil_slot(s_pre, c);
+    const size_t j = permute[s_off[k] + (c - s_pre[k])];
+    s_x[c] = x[3 * j];
+    s_y[c] = x[3 * j + 1];
+    s_z[c] = x[3 * j + 2];
+  }
+  __syncthreads();
+  const int c0 = s_pre[13], c_cnt = s_pre[14] - s_pre[13], c_off = s_off[13];
+  const int sub = threadIdx.x % K;
+  for (int a = threadIdx.x / K; a < c_cnt; a += LJ_THREADS / K) {
+    const int i = permute[c_off + a];
+    if (i >= n_local) continue;
+    const double x_i = s_x[c0 + a], y_i = s_y[c0 + a], z_i = s_z[c0 + a];
+    const unsigned short *rowp = slot + (row ? (size_t)row[i] : (size_t)i * (size_t)pitch);
+    const int cnt = num[i];
+    T fxi = 0, fyi = 0, fzi = 0;
+    auto pair = [&](bool ok, int c) {
+      const double dx = x_i - s_x[c];
+      const double dy = y_i - s_y[c];
+      const double dz = z_i - s_z[c];
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      if (ok && rsq < cutsq) {
+        if (MIXED) {
+          const float fpair = emd_lj_fpair_mixed(rsq, (float)lj1, (float)lj2);
+          fxi = __builtin_fmaf((float)dx, fpair, fxi);
+          fyi = __builtin_fmaf((float)dy, fpair, fyi);
+          fzi = __builtin_fmaf((float)dz, fpair, fzi);
+        } else {
+          const double r2inv = 1.0 / rsq;
+          const double r6inv = r2inv * r2inv * r2inv;
+          const double fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
+          fxi += dx * fpair;
+          fyi += dy * fpair;
+          fzi += dz * fpair;
+        }
+      }
+    };
+    for (int jj = sub; jj < cnt; jj += U * K) {
+      int c[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        ok[u] = jj + u * K < cnt;
+        c[u] = __builtin_nontemporal_load(rowp + (ok[u] ? jj + u * K : cnt - 1));
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) pair(ok[u], c[u]);
+    }
+    if (K > 1) {
+      fxi = group_sum<K>(fxi);
+      fyi = group_sum<K>(fyi);
+      fzi = group_sum<K>(fzi);
+    }
+    if (sub == 0) {
+      if (MIXED) {
+        f32[3 * (size_t)i] += (float)fxi;
+        f32[3 * (size_t)i + 1] += (float)fyi;
+        f32[3 * (size_t)i + 2] += (float)fzi;
+      } else {
+        f[3 * (size_t)i] += fxi;
+        f[3 * (size_t)i + 1] += fyi;
+        f[3 * (size_t)i + 2] += fzi;
+      }
     }
   }
 }
@@ -188,6 +281,15 @@ static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p, lj_table **
 // lanes per atom and gathers per round (measured on MI355X, LJ 4M atoms)
 static int default_lanes(int precision) { return precision == EMD_MIXED ? 16 : 16; }
 
+extern "C" emd_status emd_lj_set_tile(emd_ctx *ctx, int on, int lanes) {
+  if (!ctx) return EMD_ERR_ARG;
+  if (!(lanes == 0 || lanes == 4 || lanes == 8 || lanes == 16))
+    return emd_set_error(ctx, EMD_ERR_ARG, "bin-tile lanes must be 0, 4, 8 or 16");
+  ctx->lj_tile = on != 0;
+  if (lanes) ctx->lj_tile_lanes = lanes;
+  return EMD_OK;
+}
+
 extern "C" emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes) {
   if (!ctx || (precision != EMD_FP64 && precision != EMD_MIXED)) return EMD_ERR_ARG;
   if (!(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16))
@@ -205,7 +307,7 @@ static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
   const bool one = p->ntypes == 1;
 #define LJ_FULL_ARGS                                                                        \
   s->x, s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],   \
-      p->cutsq[0][0], tab, s->f, s->f32
+      p->cutsq[0][0], tab, nl->visit, s->f, s->f32
   if (precision == EMD_MIXED) {
     if (one) EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, true, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
     else EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, false, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
@@ -217,12 +319,59 @@ static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
   return EMD_OK;
 }
 
+// LDS budget of the bin-tile kernel: three FP64 coordinates per candidate
+// in 60 KiB (the static stencil tables take the rest of the 64 KiB)
+constexpr int LJ_TILE_MAX_CAND = (60 * 1024) / (3 * 8);
+
+template <int K, int U>
+static emd_status launch_tile(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                              const emd_lj_params *p, int precision, int tcap) {
+  const emd_bins *b = nl->bins;
+  const neigh_geom g = emd_neigh_geom(b);
+  const long nib = (long)g.nix * g.niy * g.niz;
+  const size_t shmem = 3 * (size_t)tcap * sizeof(double);
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+#define LJ_TILE_ARGS                                                                          \
+  s->x, b->bincount, b->binoffsets, b->permute, g, row, nl->num, nl->slot, nl->maxneighs,     \
+      s->n_local, tcap, p->lj1[0][0], p->lj2[0][0], p->cutsq[0][0], s->f, s->f32
+  if (precision == EMD_MIXED)
+    EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_tile<K, U, true>), nib, LJ_THREADS, shmem, LJ_TILE_ARGS);
+  else
+    EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_tile<K, U, false>), nib, LJ_THREADS, shmem, LJ_TILE_ARGS);
+#undef LJ_TILE_ARGS
+  return EMD_OK;
+}
+
+// the bin-tile kernel applies to one-type full lists that carry slots, when
+// the largest candidate list (27 x the fullest bin) fits the LDS budget
+static int tile_capacity(const emd_ctx *ctx, const emd_neigh *nl, const emd_lj_params *p,
+                         int half) {
+  if (!ctx->lj_tile || half || p->ntypes != 1 || !nl->slot || !nl->bins) return 0;
+  const emd_bins *b = nl->bins;
+  if (b->nhalo < 1 || b->nbinx - 2 * b->nhalo < 1 || b->nbiny - 2 * b->nhalo < 1 ||
+      b->nbinz - 2 * b->nhalo < 1)
+    return 0;
+  const long t = (27l * (b->max_count > 0 ? b->max_count : 1) + 63) / 64 * 64;
+  return t <= LJ_TILE_MAX_CAND ? (int)t : 0;
+}
+
 extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                                    const emd_lj_params *p, int precision, int half) {
   if (!ctx || !s || !nl || !p) return EMD_ERR_ARG;
   EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
   EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
   if (s->n_local <= 0) return EMD_OK;
+  // an explicit request for 1 or 2 lanes per atom keeps the CSR kernel
+  // (1 lane reproduces the reference's serial sum bit for bit)
+  const int kreq = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
+  const int tcap = (kreq == 0 || kreq >= 4) ? tile_capacity(ctx, nl, p, half) : 0;
+  if (tcap) {
+    switch (kreq ? kreq : ctx->lj_tile_lanes) {
+      case 4: return launch_tile<4, 4>(ctx, s, nl, p, precision, tcap);
+      case 16: return launch_tile<16, 2>(ctx, s, nl, p, precision, tcap);
+      default: return launch_tile<8, 2>(ctx, s, nl, p, precision, tcap);
+    }
+  }
   lj_table *tab = nullptr;
   if (p->ntypes > 1 || half) EMD_TRY(upload_table(ctx, p, &tab));
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
@@ -239,11 +388,18 @@ extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh 
   int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
   if (K == 0) K = default_lanes(precision);
   switch (K) {
-    case 1: return launch_full<1, 1>(ctx, s, nl, p, precision, tab);
+    case 1: return launch_full<1, 4>(ctx, s, nl, p, precision, tab);
     case 2: return launch_full<2, 4>(ctx, s, nl, p, precision, tab);
     case 4: return launch_full<4, 4>(ctx, s, nl, p, precision, tab);
-    case 8: return launch_full<8, 2>(ctx, s, nl, p, precision, tab);
-    default: return launch_full<16, 2>(ctx, s, nl, p, precision, tab);
+    case 8: return launch_full<8, 5>(ctx, s, nl, p, precision, tab);
+    default:
+      switch (ctx->lj_unroll) {
+        case 2: return launch_full<16, 2>(ctx, s, nl, p, precision, tab);
+        case 3: return launch_full<16, 3>(ctx, s, nl, p, precision, tab);
+        case 4: return launch_full<16, 4>(ctx, s, nl, p, precision, tab);
+        case 6: return launch_full<16, 6>(ctx, s, nl, p, precision, tab);
+        default: return launch_full<16, 5>(ctx, s, nl, p, precision, tab);
+      }
   }
 }
 

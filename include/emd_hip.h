@@ -149,6 +149,23 @@ typedef struct {
   int *entries;     /* CSR: [total]; 2D: [n][maxneighs] row-major */
   long cap_entries;
   int cap_rows;
+  /* bin-tile view, full lists only.  Set want_slot = 1 before
+   * emd_neigh_build to get, beside every entry, its position in the
+   * candidate list of the row atom's bin (the 27 stencil bins'
+   * atoms concatenated in stencil order, each bin in permute order).  A
+   * force kernel can then stage a bin's candidates in LDS once and read
+   * neighbour positions from there.  bins points at the binning the list was
+   * built from; it must stay unchanged while the list is used. */
+  int want_slot;
+  unsigned short *slot;   /* device, same shape as entries; NULL if not built */
+  long cap_slot;
+  const emd_bins *bins;
+  /* sweep order for force kernels over the rows of a full list: every local
+   * atom once, bin columns (all z) taken in square tiles of columns, so the
+   * neighbourhood of the atoms in flight stays cache resident.  Built when
+   * the context's tile width is > 0 (default 8); NULL otherwise. */
+  int *visit;             /* device [n] */
+  int cap_visit;
 } emd_neigh;
 
 /* NeighborCSR::create_neigh_list (src/neighbor_types/neighbor_csr.h:370-431)
@@ -173,8 +190,12 @@ emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
 /* ForceLJNeigh::compute_energy, force_lj_neigh_impl.h:128-156 (compensated) */
 emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
                          const emd_lj_params *p, int half, double *pe);
-/* lanes per atom of the LJ force kernel (0 = autotuned default) */
+/* lanes per atom of the LJ force kernel (0 = built-in default); 1 or 2
+ * selects the CSR-gather kernel even where the bin-tile one applies */
 emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes);
+/* bin-tile LJ kernel (one-type full lists built with want_slot): on/off and
+ * its lanes per atom (4, 8, 16; 0 keeps the current value) */
+emd_status emd_lj_set_tile(emd_ctx *ctx, int on, int lanes);
 
 /* ---------------------------------------------------------------- NVE */
 /* IntegratorNVE::initial_integrate / final_integrate,

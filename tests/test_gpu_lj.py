@@ -175,3 +175,69 @@ def test_lj_kernel_level_disordered(oracle):
     rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
     print("mixed LJ force RMS error", rms, "force RMS", np.sqrt(np.mean(np.sum(f64 ** 2, 1))))
     assert rms <= 1e-5
+
+
+def _dev(ctx, ptr, n, dtype):
+    import ctypes as C
+    out = np.zeros(n, dtype)
+    if n:
+        ctx.check(emd.hip_lib().emd_memcpy_d2h(ctx.h, out.ctypes.data_as(C.c_void_p),
+                                                C.cast(ptr, C.c_void_p), out.nbytes), "d2h")
+    return out
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_lj_bin_tile_kernel(oracle, lanes):
+    """bin-tile LJ kernel: its slot view names exactly the CSR neighbours, its
+    FP64 force is bitwise the CSR kernel's at the same lanes per atom, and
+    within 1e-13 of the oracle; FP32 pair arithmetic within 1e-5 RMS at 8
+    and 16 lanes per atom (with 4 lanes each lane accumulates ~20 pair
+    terms in FP32 and the error roughly doubles; 4 lanes is not a default)"""
+    from examinimd_amd.pipeline import Pipeline, lattice
+    from test_oracle import _energy_force
+    print(f"case: LJ bin-tile kernel, 13500 atoms disordered, {lanes} lanes", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    nc = 15
+    x = lattice("fcc", nc, a) + np.random.default_rng(6).normal(0, 0.05, (4 * nc ** 3, 3))
+    box = (nc * a,) * 3
+    pl = Pipeline()
+    pl.load(x, box)
+    pl.build(2.8, tile=True)
+    ctx, L, nl, b = pl.ctx, pl.L, pl.nl, pl.b
+    # slot view against the CSR entries
+    n = pl.s.n_local
+    row = _dev(ctx, nl.row, n + 1, np.int32)
+    ent = _dev(ctx, nl.entries, int(row[-1]), np.int32)
+    slot = _dev(ctx, nl.slot, int(row[-1]), np.uint16)
+    cnt = _dev(ctx, b.bincount, b.nbins, np.int32)
+    off = _dev(ctx, b.binoffsets, b.nbins, np.int32)
+    perm = _dev(ctx, b.permute, b.n_range, np.int32)
+    # the force sweep order visits every local atom once
+    visit = _dev(ctx, nl.visit, n, np.int32)
+    assert np.array_equal(np.sort(visit), np.arange(n))
+    nb = (b.nbinx, b.nbiny, b.nbinz)
+    for cb in range(b.nbins):
+        bx, by, bz = cb // (nb[1] * nb[2]), (cb // nb[2]) % nb[1], cb % nb[2]
+        if min(bx, by, bz) < 1 or bx >= nb[0] - 1 or by >= nb[1] - 1 or bz >= nb[2] - 1:
+            continue
+        cand = np.concatenate([perm[off[s]:off[s] + cnt[s]] for s in
+                               [((bx + dx) * nb[1] + by + dy) * nb[2] + bz + dz
+                                for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)]])
+        for i in perm[off[cb]:off[cb] + cnt[cb]]:
+            if i < n:
+                assert np.array_equal(cand[slot[row[i]:row[i + 1]]], ent[row[i]:row[i + 1]])
+    out = {}
+    for tile in (True, False):
+        ctx.check(L.emd_lj_set_tile(ctx.h, int(tile), lanes if tile else 0), "set_tile")
+        for prec in (emd.EMD_FP64, emd.EMD_MIXED):
+            ctx.check(L.emd_lj_set_lanes(ctx.h, prec, 0 if tile else lanes), "set_lanes")
+            out[tile, prec] = pl.lj(prec)
+    pl.close()
+    assert np.array_equal(out[True, emd.EMD_FP64], out[False, emd.EMD_FP64])
+    assert np.array_equal(out[True, emd.EMD_MIXED], out[False, emd.EMD_MIXED])
+    E, F = _energy_force(oracle, x % box[0], box[0], "lj")
+    assert rel_err(out[True, emd.EMD_FP64], F) <= 1e-13
+    df = out[True, emd.EMD_MIXED] - out[True, emd.EMD_FP64]
+    rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+    print("bin-tile mixed force RMS error", rms)
+    assert rms <= (1e-5 if lanes >= 8 else 3e-5)
