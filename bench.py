@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6     # MI355X vector FP64
+FP32_PEAK_TFLOPS = 157.3    # MI355X vector FP32 (published, packed)
 
 
 def parse():
@@ -306,6 +307,20 @@ def main():
                           "kernels_ms_per_step": {k: v[1] / a.snap_steps for k, v in sp.items()},
                           "etot_per_atom_end": se}
         snap["mixed_over_fp64"] = snap["mixed"]["value"] / snap["fp64"]["value"]
+        # VALU roofline of the SNAP kernels: loop-bound FLOPs of the Y form
+        # (examinimd_amd/flopcount.py; 26 neighbours inside the cutoff of BCC
+        # W at a = 3.1803) over the summed SNAP kernel time per step
+        from examinimd_amd.flopcount import snap_flops
+        fl = snap_flops(8, 26)["total"]
+        for prec, peak in (("fp64", FP64_PEAK_TFLOPS), ("mixed", FP32_PEAK_TFLOPS)):
+            kms = sum(v for k, v in snap[prec]["kernels_ms_per_step"].items()
+                      if k.startswith("snap"))
+            ach = fl * 2000 / (kms * 1e-3) / 1e12 if kms > 0 else None
+            snap[prec]["roofline"] = {"bound": "valu", "achieved": ach, "peak": peak,
+                                      "unit": "TFLOP/s",
+                                      "frac": ach / peak if ach else None,
+                                      "flops_per_atom_step": fl,
+                                      "kernel_ms_per_step": kms}
         snap["mixed_force_rms_error_eV_per_A"] = snap_force_error(emd, beta)
         snap["workload"] = (f"SNAP W BCC 10^3 (2000 atoms), twojmax 8, rc 4.73442, skin 1.0, "
                             f"rebuild every 10, dt 0.5 fs, {a.snap_steps} steps")

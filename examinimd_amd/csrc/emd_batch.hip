@@ -58,6 +58,7 @@ struct emd_batch {
   float *f32 = nullptr;
   int *binof = nullptr, *bincount = nullptr, *binstart = nullptr, *bincur = nullptr;
   int *binatoms = nullptr, *num = nullptr;
+  int *binrep = nullptr;   // replica of every bin
   unsigned *entries = nullptr;
   size_t entries_cap = 0;  // bytes
   int cap = 0;
@@ -158,79 +159,114 @@ __global__ void k_bsort(const int *__restrict__ start, const int *__restrict__ c
   }
 }
 
-// one thread per atom: 27 stencil bins (each with its image code), atoms of
-// a bin in ascending order, keep rsq <= cutneighsq except the atom itself
-__global__ void __launch_bounds__(256)
-k_bneigh(const double *__restrict__ x, const int *__restrict__ rep,
-         const rep_info *__restrict__ info, const int *__restrict__ binof,
-         const int *__restrict__ binstart, const int *__restrict__ bincount,
-         const int *__restrict__ binatoms, int n, double cutneighsq, int cap,
-         int *__restrict__ num, unsigned *__restrict__ entries, int *__restrict__ flags) {
-#pragma clang fp contract(off)
-  __shared__ int s_max[256 / EMD_WAVE];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int cnt = 0;
-  if (i < n) {
-    const rep_info R = info[rep[i]];
-    const double xi = x[3 * (size_t)i], yi = x[3 * (size_t)i + 1], zi = x[3 * (size_t)i + 2];
-    const int local = binof[i] - R.binbase;
-    const int bz = local % R.nb[2], by = (local / R.nb[2]) % R.nb[1],
-              bx = local / (R.nb[2] * R.nb[1]);
-    unsigned *row = entries + (size_t)i * cap;
-    for (int ox = -1; ox <= 1; ox++) {
-      int cx = bx + ox, kx = 0;
-      if (cx < 0) { cx += R.nb[0]; kx = 1; }
-      if (cx >= R.nb[0]) { cx -= R.nb[0]; kx = 2; }
-      for (int oy = -1; oy <= 1; oy++) {
-        int cy = by + oy, ky = 0;
-        if (cy < 0) { cy += R.nb[1]; ky = 1; }
-        if (cy >= R.nb[1]) { cy -= R.nb[1]; ky = 2; }
-        for (int oz = -1; oz <= 1; oz++) {
-          int cz = bz + oz, kz = 0;
-          if (cz < 0) { cz += R.nb[2]; kz = 1; }
-          if (cz >= R.nb[2]) { cz -= R.nb[2]; kz = 2; }
-          const unsigned code = (unsigned)(kx | (ky << 2) | (kz << 4));
-          const double sx = kx == 0 ? 0.0 : (kx == 1 ? -R.L[0] : R.L[0]);
-          const double sy = ky == 0 ? 0.0 : (ky == 1 ? -R.L[1] : R.L[1]);
-          const double sz = kz == 0 ? 0.0 : (kz == 1 ? -R.L[2] : R.L[2]);
-          const int g = R.binbase + (cx * R.nb[1] + cy) * R.nb[2] + cz;
-          const int s = binstart[g], c = bincount[g];
-          for (int k = 0; k < c; k++) {
-            const int j = binatoms[s + k];
-            const double xj = kx ? x[3 * (size_t)j] + sx : x[3 * (size_t)j];
-            const double yj = ky ? x[3 * (size_t)j + 1] + sy : x[3 * (size_t)j + 1];
-            const double zj = kz ? x[3 * (size_t)j + 2] + sz : x[3 * (size_t)j + 2];
-            const double dx = xi - xj, dy = yi - yj, dz = zi - zj;
-            const double rsq = dx * dx + dy * dy + dz * dz;
-            if (rsq <= cutneighsq && !(j == i && code == 0u)) {
-              if (cnt < cap) row[cnt] = (unsigned)j | (code << B_CODE_SHIFT);
-              cnt++;
-            }
-          }
-        }
-      }
-    }
-    num[i] = cnt < cap ? cnt : cap;
-  }
-  // block max of the row lengths, one atomic per block
-  int m = cnt;
-#pragma unroll
-  for (int o = EMD_WAVE / 2; o > 0; o >>= 1) {
-    const int t = __shfl_xor(m, o, EMD_WAVE);
-    m = t > m ? t : m;
-  }
-  if (__lane_id() == 0) s_max[threadIdx.x / EMD_WAVE] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int mm = 0;
-    for (int w = 0; w < (int)(blockDim.x / EMD_WAVE); w++) mm = s_max[w] > mm ? s_max[w] : mm;
-    atomicMax(&flags[0], mm);
-  }
-}
-
 // image position of an entry on one axis (the ghost copy's arithmetic)
 __device__ __forceinline__ double b_image(double xj, unsigned c, double L) {
   return c == 0u ? xj : xj + (c == 1u ? -L : L);
+}
+
+constexpr int BN_THREADS = 256;
+constexpr int BN_MAXC = 1024;  // candidates staged per LDS chunk
+
+// one block per bin (the plain neighbour build's scheme): the 27 stencil
+// bins of the bin's replica, each with its periodic image code, are staged in
+// LDS as candidates (stencil order, each bin in ascending atom order, image
+// positions as the force kernel forms them); each wavefront takes atoms of
+// the bin and tests 64 candidates per step, compacting hits with ballot and
+// popcount, so rows come out in stencil-then-bin order.  Keeps
+// rsq <= cutneighsq except the atom itself (image code 0).
+__global__ void __launch_bounds__(BN_THREADS)
+k_bneigh(const double *__restrict__ x, const rep_info *__restrict__ info,
+         const int *__restrict__ binrep, const int *__restrict__ binstart,
+         const int *__restrict__ bincount, const int *__restrict__ binatoms, double cutneighsq,
+         int cap, int *__restrict__ num, unsigned *__restrict__ entries,
+         int *__restrict__ flags) {
+#pragma clang fp contract(off)
+  __shared__ double s_x[BN_MAXC], s_y[BN_MAXC], s_z[BN_MAXC];
+  __shared__ unsigned s_e[BN_MAXC];
+  __shared__ int s_start[27], s_pre[28];
+  __shared__ unsigned s_code[27];
+  __shared__ int s_max[BN_THREADS / EMD_WAVE];
+  const int g = emd_xcd_tile(blockIdx.x, gridDim.x);
+  const int c_cnt = bincount[g];
+  if (c_cnt == 0) return;  // uniform over the block
+  const rep_info R = info[binrep[g]];
+  const int local = g - R.binbase;
+  const int bz = local % R.nb[2], by = (local / R.nb[2]) % R.nb[1],
+            bx = local / (R.nb[2] * R.nb[1]);
+  const int tid = threadIdx.x, wave = tid / EMD_WAVE, lane = tid % EMD_WAVE;
+  if (tid < 27) {
+    int cx = bx + tid / 9 - 1, cy = by + (tid / 3) % 3 - 1, cz = bz + tid % 3 - 1;
+    unsigned kx = 0, ky = 0, kz = 0;
+    if (cx < 0) { cx += R.nb[0]; kx = 1; }
+    if (cx >= R.nb[0]) { cx -= R.nb[0]; kx = 2; }
+    if (cy < 0) { cy += R.nb[1]; ky = 1; }
+    if (cy >= R.nb[1]) { cy -= R.nb[1]; ky = 2; }
+    if (cz < 0) { cz += R.nb[2]; kz = 1; }
+    if (cz >= R.nb[2]) { cz -= R.nb[2]; kz = 2; }
+    const int gb = R.binbase + (cx * R.nb[1] + cy) * R.nb[2] + cz;
+    s_start[tid] = binstart[gb];
+    s_pre[tid + 1] = bincount[gb];
+    s_code[tid] = kx | (ky << 2) | (kz << 4);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    s_pre[0] = 0;
+    for (int k = 1; k <= 27; k++) s_pre[k] += s_pre[k - 1];
+  }
+  __syncthreads();
+  const int T = s_pre[27];
+  const int c_start = binstart[g];
+  const unsigned long long lt = emd_lanemask_lt();
+  int mymax = 0;
+  for (int cs = 0; cs < T; cs += BN_MAXC) {
+    const int cn = (T - cs) < BN_MAXC ? (T - cs) : BN_MAXC;
+    for (int c = tid; c < cn; c += BN_THREADS) {
+      const int cc = cs + c;
+      const int k = emd_stencil_slot(s_pre, cc);
+      const int j = binatoms[s_start[k] + (cc - s_pre[k])];
+      const unsigned code = s_code[k];
+      s_x[c] = b_image(x[3 * (size_t)j], code & 3u, R.L[0]);
+      s_y[c] = b_image(x[3 * (size_t)j + 1], (code >> 2) & 3u, R.L[1]);
+      s_z[c] = b_image(x[3 * (size_t)j + 2], (code >> 4) & 3u, R.L[2]);
+      s_e[c] = (unsigned)j | (code << B_CODE_SHIFT);
+    }
+    __syncthreads();
+    const bool last = cs + BN_MAXC >= T;
+    for (int bi = wave; bi < c_cnt; bi += BN_THREADS / EMD_WAVE) {
+      const int i = binatoms[c_start + bi];
+      const double xi = x[3 * (size_t)i], yi = x[3 * (size_t)i + 1], zi = x[3 * (size_t)i + 2];
+      unsigned *row = entries + (size_t)i * cap;
+      int run = cs > 0 ? num[i] : 0;
+      for (int c0 = 0; c0 < cn; c0 += EMD_WAVE) {
+        const int c = c0 + lane;
+        bool keep = false;
+        unsigned e = 0u;
+        if (c < cn) {
+          e = s_e[c];
+          const double dx = xi - s_x[c], dy = yi - s_y[c], dz = zi - s_z[c];
+          const double rsq = dx * dx + dy * dy + dz * dz;
+          keep = rsq <= cutneighsq && !((int)(e & B_JMASK) == i && (e >> B_CODE_SHIFT) == 0u);
+        }
+        const unsigned long long m = __ballot(keep);
+        if (keep) {
+          const int pos = run + __popcll(m & lt);
+          if (pos < cap) row[pos] = e;
+        }
+        run += __popcll(m);
+      }
+      if (lane == 0) num[i] = last ? (run < cap ? run : cap) : run;
+      if (last) mymax = run > mymax ? run : mymax;
+    }
+    __syncthreads();
+  }
+  // block max of the row lengths, one atomic per block
+  if (lane == 0) s_max[wave] = mymax;
+  __syncthreads();
+  if (tid == 0) {
+    int mm = 0;
+    for (int w = 0; w < BN_THREADS / EMD_WAVE; w++) mm = s_max[w] > mm ? s_max[w] : mm;
+    atomicMax(&flags[0], mm);
+  }
 }
 
 // Force on atom i from its row (K lanes per atom, U gathers per round) and,
@@ -473,9 +509,9 @@ emd_status rebuild(emd_ctx *ctx, emd_batch *b) {
              b->binatoms, n);
   EMD_LAUNCH(ctx, "batch_bin_sort", k_bsort, emd_blocks(nb, 256), 256, 0, b->binstart,
              b->bincount, b->binatoms, nb);
-  EMD_LAUNCH(ctx, "batch_neigh", k_bneigh, emd_blocks(n, 256), 256, 0, x, b->rep, b->info,
-             b->binof, b->binstart, b->bincount, b->binatoms, n, b->cutneighsq, b->cap, b->num,
-             b->entries, b->flags);
+  EMD_LAUNCH(ctx, "batch_neigh", k_bneigh, nb, BN_THREADS, 0, x, b->info, b->binrep,
+             b->binstart, b->bincount, b->binatoms, b->cutneighsq, b->cap, b->num, b->entries,
+             b->flags);
   return EMD_OK;
 }
 
@@ -629,6 +665,7 @@ extern "C" emd_status emd_batch_create(emd_ctx *ctx, const emd_batch_desc *d, em
   if (st == EMD_OK) st = dalloc(ctx, &b->binstart, nbins + 1);
   if (st == EMD_OK) st = dalloc(ctx, &b->bincur, nbins);
   if (st == EMD_OK) st = dalloc(ctx, &b->binatoms, total);
+  if (st == EMD_OK) st = dalloc(ctx, &b->binrep, nbins);
   if (st == EMD_OK) st = dalloc(ctx, &b->num, total);
   if (st == EMD_OK) st = dalloc(ctx, &b->flags, 16);
   if (st == EMD_OK) st = dalloc(ctx, &b->part, 2 * (size_t)d->nrep + d->nrep);
@@ -640,6 +677,13 @@ extern "C" emd_status emd_batch_create(emd_ctx *ctx, const emd_batch_desc *d, em
   EMD_HIP(ctx, hipMemcpy(b->info, b->h_info.data(), d->nrep * sizeof(rep_info),
                          hipMemcpyHostToDevice));
   EMD_HIP(ctx, hipMemcpy(b->rep, rep.data(), total * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> binrep(nbins);
+  for (int r = 0; r < d->nrep; r++) {
+    const rep_info &R = b->h_info[r];
+    const int nbr = R.nb[0] * R.nb[1] * R.nb[2];
+    for (int k = 0; k < nbr; k++) binrep[R.binbase + k] = r;
+  }
+  EMD_HIP(ctx, hipMemcpy(b->binrep, binrep.data(), nbins * sizeof(int), hipMemcpyHostToDevice));
   // masses live after the 2*nrep reduction partials
   EMD_HIP(ctx, hipMemcpy(b->part + 2 * d->nrep, mass.data(), d->nrep * sizeof(double),
                          hipMemcpyHostToDevice));
@@ -655,7 +699,7 @@ extern "C" emd_status emd_batch_destroy(emd_ctx *ctx, emd_batch *b) {
   drop_graphs(b);
   void *bufs[] = {b->info, b->rep, b->x[0], b->x[1], b->v, b->f, b->f32, b->binof, b->bincount,
                   b->binstart, b->bincur, b->binatoms, b->num, b->entries, b->flags, b->part,
-                  b->save};
+                  b->save, b->binrep};
   for (void *p : bufs)
     if (p) hipFree(p);
   delete b;

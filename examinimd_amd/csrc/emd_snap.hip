@@ -190,22 +190,37 @@ __device__ __forceinline__ cplx<T> u_step(const snap_dev_tables &t, const T *pr,
   return u;
 }
 
-// U_tot of every atom (SNA::compute_ui, sna_impl.hpp:152-190): one wavefront
-// per atom, lanes over the half-plane entries of a layer
+// ordering of LDS traffic among the lanes of one wavefront (no block barrier)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// U_tot of every atom (SNA::compute_ui, sna_impl.hpp:152-190): one block of
+// UI_WAVES wavefronts per atom, wavefront w takes neighbours w, w+UI_WAVES, ...
+// (lanes over the half-plane entries of a layer, the previous layer in the
+// wavefront's own LDS buffer), then the partial sums of the wavefronts are
+// added in wavefront order (deterministic)
+constexpr int UI_WAVES = 4;
+
 template <class T>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * UI_WAVES)
 k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restrict__ nr,
           int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
-  __shared__ T s_r[2][SNAP_JD * SNAP_JD], s_i[2][SNAP_JD * SNAP_JD];
+  __shared__ T s_r[UI_WAVES][2][SNAP_JD * SNAP_JD], s_i[UI_WAVES][2][SNAP_JD * SNAP_JD];
+  __shared__ T s_pr[UI_WAVES][SNAP_NY], s_pi[UI_WAVES][SNAP_NY];
   const int i = blockIdx.x;
   if (i >= n) return;
-  const int l = threadIdx.x;
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int twojmax = t.twojmax;
   T acc_r[SNAP_JD], acc_i[SNAP_JD];
 #pragma unroll
   for (int j = 0; j < SNAP_JD; j++) acc_r[j] = acc_i[j] = 0;
   const int cnt = nin[i];
-  for (int jj = 0; jj < cnt; jj++) {
+  T (*br)[SNAP_JD * SNAP_JD] = s_r[w];
+  T (*bi)[SNAP_JD * SNAP_JD] = s_i[w];
+  for (int jj = w; jj < cnt; jj += UI_WAVES) {
     const size_t o = (size_t)i * maxn + jj;
     const T x = (T)nr[3 * o], y = (T)nr[3 * o + 1], z = (T)nr[3 * o + 2];
     const T r = sqrt(x * x + y * y + z * z);
@@ -218,38 +233,53 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
     sfac *= (T)t.wj;
     // layer 0
     if (l == 0) {
-      s_r[0][0] = 1;
-      s_i[0][0] = 0;
+      br[0][0] = 1;
+      bi[0][0] = 0;
       acc_r[0] += sfac;
     }
-    __syncthreads();
+    wave_sync();
     for (int j = 1; j <= twojmax; j++) {
       const int p = (j - 1) & 1, c = j & 1;
       int ma, mb;
       const bool own = l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb);
       cplx<T> u{0, 0};
       if (own) {
-        u = u_step<T>(t, s_r[p], s_i[p], j, ma, mb, a_r, a_i, b_r, b_i);
+        u = u_step<T>(t, br[p], bi[p], j, ma, mb, a_r, a_i, b_r, b_i);
         // full layer j: own entry and, off the middle column, its mirror
-        s_r[c][ma * (j + 1) + mb] = u.r;
-        s_i[c][ma * (j + 1) + mb] = u.i;
+        br[c][ma * (j + 1) + mb] = u.r;
+        bi[c][ma * (j + 1) + mb] = u.i;
         if (2 * mb != j) {
           const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
-          s_r[c][(j - ma) * (j + 1) + (j - mb)] = sg * u.r;
-          s_i[c][(j - ma) * (j + 1) + (j - mb)] = -sg * u.i;
+          br[c][(j - ma) * (j + 1) + (j - mb)] = sg * u.r;
+          bi[c][(j - ma) * (j + 1) + (j - mb)] = -sg * u.i;
         }
         acc_r[j] += sfac * u.r;
         acc_i[j] += sfac * u.i;
       }
-      __syncthreads();
+      wave_sync();
     }
   }
-  // U_tot = wself * I + accumulated sum; write own entries and mirrors
+  // partial sums of every wavefront, half-plane index ystart[j] + l
+  for (int j = 0; j <= twojmax; j++) {
+    int ma, mb;
+    if (l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb)) {
+      s_pr[w][t.ystart[j] + l] = acc_r[j];
+      s_pi[w][t.ystart[j] + l] = acc_i[j];
+    }
+  }
+  __syncthreads();
+  if (w != 0) return;
+  // U_tot = wself * I + sum over wavefronts; own entries and mirrors
   T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
   for (int j = 0; j <= twojmax; j++) {
     int ma, mb;
     if (l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb)) {
-      T vr = acc_r[j], vi = acc_i[j];
+      T vr = 0, vi = 0;
+#pragma unroll
+      for (int q = 0; q < UI_WAVES; q++) {
+        vr += s_pr[q][t.ystart[j] + l];
+        vi += s_pi[q][t.ystart[j] + l];
+      }
       if (ma == mb) vr += (T)t.wself;
       ur[t.ustart[j] + ma * (j + 1) + mb] = vr;
       ui[t.ustart[j] + ma * (j + 1) + mb] = vi;
@@ -708,9 +738,10 @@ extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, em
     EMD_HIP(ctx, hipMemcpy(sn->stab, st.data(), st.size() * sizeof(stab_entry), hipMemcpyHostToDevice));
   }
   if (p->ncoeff != sn->ncoeff) {
-    delete sn;
+    const int want = sn->ncoeff;
+    emd_snap_destroy(ctx, sn);
     return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: ncoeff %d but twojmax %d has %d",
-                         p->ncoeff, twojmax, (int)(sn ? 0 : 0) + p->ncoeff);
+                         p->ncoeff, twojmax, want);
   }
   sn->beta.assign(p->coeff, p->coeff + sn->ncoeff + 1);
   if (hipMalloc(&sn->cg, sn->cg_h.size() * sizeof(double)) != hipSuccess) {
@@ -781,7 +812,7 @@ static emd_status snap_neigh(emd_ctx *ctx, const emd_system *s, const emd_neigh 
 template <class T>
 static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   T *ur = (T *)sn->u_r, *ui = (T *)sn->u_i, *yr = (T *)sn->y_r, *yi = (T *)sn->y_i;
-  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n, ur, ui);
+  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n, ur, ui);
   EMD_LAUNCH(ctx, "snap_y", k_snap_y<T>, n, 192, 0, sn->t, sn->cg, sn->ytab, n, ur, ui, yr, yi);
   return EMD_OK;
 }
@@ -815,7 +846,7 @@ extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const e
   *energy = 0.0;
   if (n <= 0) return EMD_OK;
   EMD_TRY(snap_neigh(ctx, s, nl, sn));
-  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<double>, n, 64, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n,
+  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<double>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n,
              (double *)sn->u_r, (double *)sn->u_i);
   double *ea = eatom;
   if (!ea) {
@@ -1066,7 +1097,7 @@ template <class T>
 static emd_status snap_adjoint_impl(emd_ctx *ctx, const emd_system *s, const emd_halo *h,
                                     emd_snap *sn, const double *g, double *S) {
   const int n = s->n_local;
-  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n,
+  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n,
              (T *)sn->u_r, (T *)sn->u_i);
   EMD_LAUNCH(ctx, "snap_adjoint_w", k_snap_w<T>, n, 64, 0, sn->t, sn->nin, sn->nj, sn->nr,
              sn->cap_maxn, n, h ? h->root : nullptr, g, sn->w_r, sn->w_i);
@@ -1100,7 +1131,7 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
   else EMD_TRY(snap_adjoint_impl<double>(ctx, s, h, sn, dLdF, S));
   // dE/dbeta_k = sum_i B_k^i in FP64
   if (dLdE != 0.0) {
-    EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<double>, n, 64, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn,
+    EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<double>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn,
                n, (double *)sn->u_r, (double *)sn->u_i);
     EMD_LAUNCH(ctx, "snap_adjoint_bsum", k_snap_bsum, n, 256, 0, sn->t, sn->cg, sn->etab,
                sn->ncoeff, sn->bzeroflag, n, (const double *)sn->u_r, (const double *)sn->u_i, B);
