@@ -42,12 +42,16 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--loop-steps", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-forcematch", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (profiler calibration)")
     return ap.parse_args()
 
 
-def lj_run(emd, decks, tmpdir, n, precision, warmup, steps):
+def lj_run(emd, decks, tmpdir, n, precision, warmup, steps, chunk=100):
+    """the timed LJ run, in chunks of `chunk` steps (a multiple of the
+    rebuild interval, so the rebuild schedule is that of one run); E/N is
+    sampled between chunks outside the timed region"""
     deck = decks.write(os.path.join(tmpdir, f"in.lj.{precision}"),
                        decks.lj_deck(n=n, steps=steps))
     app = emd.App(["-il", deck, "--neigh-type", "CSR", "--comm-type", "SERIAL",
@@ -58,12 +62,22 @@ def lj_run(emd, decks, tmpdir, n, precision, warmup, steps):
     if warmup:
         app.run(warmup)
     L.emd_synchronize(ctx)
+    samples = [(0, app.thermo()[2])]
     L.emd_profile_enable(ctx, 1)
     L.emd_profile_reset(ctx)
-    t0 = time.perf_counter()
-    app.run(steps)
-    L.emd_synchronize(ctx)
-    wall = time.perf_counter() - t0
+    wall, done = 0.0, 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        L.emd_synchronize(ctx)
+        t0 = time.perf_counter()
+        app.run(k)
+        L.emd_synchronize(ctx)
+        wall += time.perf_counter() - t0
+        done += k
+        L.emd_profile_enable(ctx, 0)
+        samples.append((done, app.thermo()[2]))
+        L.emd_profile_enable(ctx, 1)
+    app.energy_samples = samples
     prof = {}
     buf = C.create_string_buffer(1 << 16)
     L.emd_profile_report(ctx, buf, len(buf))
@@ -75,6 +89,17 @@ def lj_run(emd, decks, tmpdir, n, precision, warmup, steps):
     natoms = app.natoms
     nlist = float(len(ent)) / natoms
     return app, wall, prof, nlist
+
+
+def energy_drift(samples, dt):
+    """least-squares slope of E/N against time (reduced units per tau) and
+    the largest excursion from the first sample"""
+    import numpy as np
+    st = np.array([s for s, _ in samples], dtype=float) * dt
+    e = np.array([v for _, v in samples], dtype=float)
+    slope = float(np.polyfit(st, e, 1)[0]) if len(e) > 1 else 0.0
+    return {"slope_per_tau": slope, "max_excursion": float(np.abs(e - e[0]).max()),
+            "samples": len(e), "e0": float(e[0])}
 
 
 def melt_force_error(emd, app):
@@ -252,6 +277,53 @@ def batch_leg(emd, steps, loop_steps):
                         "skin 0.3, rebuild every 20, dt 0.005"}
 
 
+def forcematch_leg(emd, repeats=3):
+    """force-matching gradients: 64 SNAP configurations (BCC W, 128..1024
+    atoms) w.r.t. the 56 beta, and 32 LJ boxes (500..4000 atoms) w.r.t.
+    (eps, sigma); mixed-precision adjoint timed, checked against the FP64
+    adjoint and a central difference of the FP64 loss"""
+    import numpy as np
+    from examinimd_amd import forcematch as fm
+    from examinimd_amd.abi import EMD_FP64, EMD_MIXED
+    out = {}
+    bt = fm.snap_beta_true()
+    b = fm.snap_beta_perturbed(bt)
+    cfgs = fm.snap_set()
+    s = fm.SnapForceMatch(cfgs, bt)
+    s.evaluate(b, EMD_MIXED)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        L32, g32 = s.evaluate(b, EMD_MIXED)
+    wall = (time.perf_counter() - t0) / repeats
+    L64, g64 = s.evaluate(b, EMD_FP64)
+    h, k = 1e-6, 29
+    bp, bm = b.copy(), b.copy()
+    bp[k] += h
+    bm[k] -= h
+    fd = (s.evaluate(bp, EMD_FP64, grad=False)[0] - s.evaluate(bm, EMD_FP64, grad=False)[0]) / (2 * h)
+    natoms = sum(len(x) for x, _ in cfgs)
+    s.close()
+    out["snap"] = {"configs": len(cfgs), "atoms": natoms, "ms_per_gradient": 1e3 * wall,
+                   "atom_gradients_per_s": natoms / wall, "loss": L32,
+                   "grad_rel_err_mixed_vs_fp64": float(np.abs(g32 - g64).max() / np.abs(g64).max()),
+                   "fd_check": {"component": k, "fd": fd, "adjoint_fp64": float(g64[k])}}
+    cfgs = fm.lj_set()
+    lj = fm.LJForceMatch(cfgs)
+    th = np.array([1.04, 0.98])
+    lj.evaluate(th, EMD_MIXED)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        L32, g32 = lj.evaluate(th, EMD_MIXED)
+    wall = (time.perf_counter() - t0) / repeats
+    L64, g64 = lj.evaluate(th, EMD_FP64)
+    natoms = sum(len(x) for x, _ in cfgs)
+    lj.close()
+    out["lj"] = {"configs": len(cfgs), "atoms": natoms, "ms_per_gradient": 1e3 * wall,
+                 "atom_gradients_per_s": natoms / wall,
+                 "grad_rel_err_mixed_vs_fp64": float(np.abs(g32 - g64).max() / np.abs(g64).max())}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -271,7 +343,8 @@ def main():
         cnt, ms = prof.get(kname, (0, 0.0))
         out[prec] = {"wall": wall, "natoms": natoms, "prof": prof, "nlist": nlist,
                      "kernel": kname, "kernel_launches": cnt, "kernel_ms": ms,
-                     "thermo_end": list(app.thermo())}
+                     "thermo_end": list(app.thermo()),
+                     "drift": energy_drift(app.energy_samples, 0.005)}
         if prec == "mixed":
             out[prec]["melt_error"] = melt_force_error(emd, app)
         app.close()
@@ -326,6 +399,7 @@ def main():
                             f"rebuild every 10, dt 0.5 fs, {a.snap_steps} steps")
 
     batched = None if a.no_batch else batch_leg(emd, a.batch_steps, a.loop_steps)
+    gradients = None if a.no_forcematch else forcematch_leg(emd)
 
     line = {
         "metric": "atom-steps/s on 1 MI355X, LJ 4M & SNAP 2k atoms: mixed vs FP64 at fixed error",
@@ -350,9 +424,12 @@ def main():
         "mixed_force_rms_error_on_melt": mixed["melt_error"],
         "thermo_end": {"fp64": fp64["thermo_end"], "mixed": mixed["thermo_end"],
                        "fields": ["T", "PE/N", "E/N"]},
+        "energy_drift": {"fp64": fp64["drift"], "mixed": mixed["drift"],
+                         "unit": "E/N per tau (LJ reduced units), sampled every 100 steps"},
         "mixed_force_rms_error_vs_fp64_displaced_lattice": ferr_dis,
         "snap2k": snap,
         "batched": batched,
+        "force_matching": gradients,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_atom * natoms,

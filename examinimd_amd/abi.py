@@ -39,7 +39,7 @@ class EmdNeigh(C.Structure):
                 ("maxneighs", C.c_int), ("row", iptr), ("num", iptr), ("entries", iptr),
                 ("cap_entries", C.c_long), ("cap_rows", C.c_int), ("want_slot", C.c_int),
                 ("slot", P(C.c_ushort)), ("cap_slot", C.c_long), ("bins", C.c_void_p),
-                ("visit", iptr), ("cap_visit", C.c_int)]
+                ("visit", iptr), ("cap_visit", C.c_int), ("n_visit", C.c_int)]
 
 
 class EmdLJParams(C.Structure):

@@ -218,7 +218,7 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
         const int *__restrict__ binoffsets, const int *__restrict__ permute, neigh_geom g,
         int n_local, double cutsq, int newton, int *__restrict__ num, const int *__restrict__ row,
         int pitch, int *__restrict__ entries, unsigned short *__restrict__ slot,
-        int *__restrict__ cursor, int *__restrict__ maxrow) {
+        int *__restrict__ cursor, int *__restrict__ maxrow, int cap, int write_num) {
 #pragma clang fp contract(off)
   __shared__ double s_x[NB_MAXC], s_y[NB_MAXC], s_z[NB_MAXC];
   __shared__ int s_j[NB_MAXC];
@@ -290,15 +290,19 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
         }
         const unsigned long long m = __ballot(keep);
         if (FILL && keep) {
-          const size_t e = base + run + __popcll(m & lt);
-          entries[e] = j;
-          if (slot) slot[e] = (unsigned short)(cs + c);
+          const int pos = run + __popcll(m & lt);
+          if (pos < cap) {
+            const size_t e = base + pos;
+            entries[e] = j;
+            if (slot) slot[e] = (unsigned short)(cs + c);
+          }
         }
         run += __popcll(m);
       }
       if (lane == 0) {
         if (FILL) {
           if (!last) cursor[i] = run;
+          else if (write_num) num[i] = run;
         } else {
           num[i] = run;
         }
@@ -330,6 +334,22 @@ __global__ void __launch_bounds__(256) k_max_int(const int *__restrict__ a, int 
   }
 }
 
+// padded rows (pitch P) -> CSR rows, one wavefront per row
+__global__ void __launch_bounds__(256)
+k_compact_rows(const int *__restrict__ tmp, const unsigned short *__restrict__ tslot, int P,
+               const int *__restrict__ num, const int *__restrict__ row, int n,
+               int *__restrict__ entries, unsigned short *__restrict__ slot) {
+  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / EMD_WAVE);
+  const int lane = threadIdx.x % EMD_WAVE;
+  if (r >= n) return;
+  const int c = num[r];
+  const size_t src = (size_t)r * P, dst = (size_t)row[r];
+  for (int k = lane; k < c; k += EMD_WAVE) {
+    entries[dst + k] = tmp[src + k];
+    if (slot) slot[dst + k] = tslot[src + k];
+  }
+}
+
 struct row_offsets_op {
   const int *num;
   int *row;
@@ -342,10 +362,11 @@ struct row_offsets_op {
 };
 
 // Force sweep order: interior bin columns (bx, by; all interior bz, which
-// are consecutive bins and so one contiguous permute range of local atoms)
-// visited in T x T tiles of columns, tiles and the columns inside a tile in
-// row-major order.  Slot s of the enumeration is (tile, column in tile);
-// slots outside the grid are empty.
+// are consecutive bins and so one contiguous permute range) visited in T x T
+// tiles of columns, tiles and the columns inside a tile in row-major order.
+// Slot s of the enumeration is (tile, column in tile); slots outside the
+// grid are empty.  Interior bins hold every local atom and, through the
+// cell/1000 margin of the bin bounds, possibly a few ghosts.
 __device__ __forceinline__ int visit_column(const neigh_geom &g, int T, int s, int *first_bin) {
   const int nty = (g.niy + T - 1) / T;
   const int tile = s / (T * T), in = s % (T * T);
@@ -369,7 +390,7 @@ struct visit_len_op {
   __device__ void emit(int s, int excl, int) const { start[s] = excl; }
 };
 
-// one block per column slot: copy the column's local atoms into the order
+// one block per column slot: copy the column's atoms into the order
 __global__ void __launch_bounds__(256)
 k_visit_fill(neigh_geom g, int T, const int *__restrict__ binoffsets,
              const int *__restrict__ bincount, const int *__restrict__ permute,
@@ -396,7 +417,6 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
   const long nib = (long)g.nix * g.niy * g.niz;
   EMD_ARG(ctx, b->nhalo >= 1, "neighbour build needs a halo of at least one bin");
   const double cutsq = neigh_cut * neigh_cut;
-  unsigned short *slot = nullptr;
   nl->bins = b;
 
   int rcap = nl->cap_rows;
@@ -408,52 +428,48 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
   int *cursor = (int *)ctx->d_tmp2;
   if (n > 0) EMD_LAUNCH(ctx, "neigh_count", k_fill_int, emd_blocks(n, 256), 256, 0, nl->num, n, 0);
   EMD_LAUNCH(ctx, "neigh_count", k_fill_int, 1, 64, 0, ctx->d_small + 4, 1, 0);
+  const bool with_slot = nl->want_slot && !half && ctx->lj_tile;
+  if (!with_slot && nl->slot) {
+    // a list built without the tile view carries none
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_HIP(ctx, hipFree(nl->slot));
+    nl->slot = nullptr;
+    nl->cap_slot = 0;
+  }
 
-#define NB_ARGS(F)                                                                          \
-  s->x, b->bincount, b->binoffsets, b->permute, g, n, cutsq, newton, nl->num,              \
-      (F) ? (layout == EMD_NEIGH_CSR ? nl->row : nullptr) : nullptr, nl->maxneighs,        \
-      nl->entries, (F) ? slot : nullptr, cursor, ctx->d_small + 4
-  if (nib > 0) {
-    if (half) EMD_LAUNCH(ctx, "neigh_count", (k_neigh<true, false>), nib, NB_THREADS, 0, NB_ARGS(0));
-    else EMD_LAUNCH(ctx, "neigh_count", (k_neigh<false, false>), nib, NB_THREADS, 0, NB_ARGS(0));
-  }
-  // row offsets (CSR) and the longest row
-  if (n > 0) {
-    const int nbm = emd_blocks(n, 256) < 512 ? emd_blocks(n, 256) : 512;
-    EMD_LAUNCH(ctx, "neigh_offsets", k_max_int, nbm, 256, 0, nl->num, n, ctx->d_small + 4);
-  }
-  int total = 0;
-  row_offsets_op op{nl->num, nl->row, n};
-  if (n > 0) EMD_TRY(emd_scan(ctx, op, n, "neigh_offsets", &total));
-  else total = 0;
-  EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 4, ctx->d_small + 4, sizeof(int),
-                              hipMemcpyDeviceToHost, ctx->stream));
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const int maxrow = ctx->h_small[4];
-  long need;
-  if (layout == EMD_NEIGH_CSR) {
-    nl->maxneighs = maxrow;
-    need = total;
-  } else {
-    // Neighbor2D grows maxneighs x1.2 until every row fits (neighbor_2d.h:304-330)
+  // k_neigh launch: fill?, CSR row offsets (or null: rows at i * pitch),
+  // pitch, entries, slots, row capacity, record full row lengths in num?
+#define NB_LAUNCH(name, F, ROW, PITCH, ENT, SLOT, CAP, WNUM)                                  \
+  do {                                                                                       \
+    if (nib > 0) {                                                                           \
+      if (half)                                                                              \
+        EMD_LAUNCH(ctx, name, (k_neigh<true, F>), nib, NB_THREADS, 0, s->x, b->bincount,     \
+                   b->binoffsets, b->permute, g, n, cutsq, newton, nl->num, ROW, PITCH, ENT, \
+                   SLOT, cursor, ctx->d_small + 4, CAP, WNUM);                               \
+      else                                                                                   \
+        EMD_LAUNCH(ctx, name, (k_neigh<false, F>), nib, NB_THREADS, 0, s->x, b->bincount,    \
+                   b->binoffsets, b->permute, g, n, cutsq, newton, nl->num, ROW, PITCH, ENT, \
+                   SLOT, cursor, ctx->d_small + 4, CAP, WNUM);                               \
+    }                                                                                        \
+  } while (0)
+
+  // One pass when a row capacity is known: the fill writes rows padded to
+  // that capacity and records every row's full length.  2D: the rows are the
+  // list (pitch maxneighs, grown x1.2 and refilled while a row does not fit,
+  // Neighbor2D, neighbor_2d.h:304-330).  CSR: padded rows go to scratch
+  // sized from the previous build's longest row, then are compacted; if a
+  // row did not fit, the exact CSR fill runs with the counts just made.  The
+  // first CSR build counts, scans and fills (NeighborCSR's two passes).
+  int P = 0;
+  if (layout == EMD_NEIGH_2D) {
     if (nl->maxneighs <= 0) nl->maxneighs = 16;
-    while (maxrow > nl->maxneighs) nl->maxneighs = (int)(maxrow * 1.2);
-    need = (long)n * nl->maxneighs;
+    P = nl->maxneighs;
+  } else if (nl->maxneighs > 0) {
+    P = ((nl->maxneighs + nl->maxneighs / 8 + 16) + 15) / 16 * 16;
   }
-  nl->total = total;
-  if (need > nl->cap_entries || !nl->entries) {
-    if (nl->entries) {
-      EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      EMD_HIP(ctx, hipFree(nl->entries));
-    }
-    nl->entries = nullptr;
-    long want = need + need / 8 + 64;
-    if (hipMalloc(&nl->entries, (size_t)want * sizeof(int)) != hipSuccess)
-      return emd_set_error(ctx, EMD_ERR_ALLOC, "neighbour list: hipMalloc of %ld ints failed",
-                           want);
-    nl->cap_entries = want;
-  }
-  if (nl->want_slot && !half && ctx->lj_tile) {
+  int *pad = nullptr;
+  unsigned short *pad_slot = nullptr;
+  auto ensure_slot = [&](long need) -> emd_status {
     if (need > nl->cap_slot || !nl->slot) {
       if (nl->slot) {
         EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -467,35 +483,92 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
                              want);
       nl->cap_slot = want;
     }
-    slot = nl->slot;
-  } else if (nl->slot) {
-    // a list built without the tile view carries none
-    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    EMD_HIP(ctx, hipFree(nl->slot));
-    nl->slot = nullptr;
-    nl->cap_slot = 0;
+    return EMD_OK;
+  };
+  auto ensure_entries = [&](long need) -> emd_status {
+    if (need > nl->cap_entries || !nl->entries) {
+      if (nl->entries) {
+        EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        EMD_HIP(ctx, hipFree(nl->entries));
+      }
+      nl->entries = nullptr;
+      const long want = need + need / 8 + 64;
+      if (hipMalloc(&nl->entries, (size_t)want * sizeof(int)) != hipSuccess)
+        return emd_set_error(ctx, EMD_ERR_ALLOC, "neighbour list: hipMalloc of %ld ints failed",
+                             want);
+      nl->cap_entries = want;
+    }
+    if (with_slot) EMD_TRY(ensure_slot(need));
+    return EMD_OK;
+  };
+  if (P > 0 && n > 0) {
+    if (layout == EMD_NEIGH_2D) {
+      EMD_TRY(ensure_entries((long)n * P));
+      pad = nl->entries;
+      pad_slot = with_slot ? nl->slot : nullptr;
+    } else {
+      const size_t ib = (size_t)n * P * sizeof(int), sb = with_slot ? (size_t)n * P * 2 : 0;
+      EMD_TRY(emd_ensure(ctx, &ctx->d_tmp, &ctx->tmp_cap, ib + sb + 256));
+      pad = (int *)ctx->d_tmp;
+      pad_slot = with_slot ? (unsigned short *)((char *)ctx->d_tmp + ((ib + 255) / 256) * 256)
+                           : nullptr;
+    }
+    NB_LAUNCH("neigh_fill", true, nullptr, P, pad, pad_slot, P, 1);
   }
-  if (nib > 0) {
-    if (half) EMD_LAUNCH(ctx, "neigh_fill", (k_neigh<true, true>), nib, NB_THREADS, 0, NB_ARGS(1));
-    else EMD_LAUNCH(ctx, "neigh_fill", (k_neigh<false, true>), nib, NB_THREADS, 0, NB_ARGS(1));
+  if (P == 0 || n == 0)
+    NB_LAUNCH("neigh_count", false, nullptr, 0, nullptr, nullptr, 0, 0);
+  // row offsets and the longest row
+  if (n > 0) {
+    const int nbm = emd_blocks(n, 256) < 512 ? emd_blocks(n, 256) : 512;
+    EMD_LAUNCH(ctx, "neigh_offsets", k_max_int, nbm, 256, 0, nl->num, n, ctx->d_small + 4);
   }
-#undef NB_ARGS
+  int total = 0;
+  row_offsets_op op{nl->num, nl->row, n};
+  if (n > 0) EMD_TRY(emd_scan(ctx, op, n, "neigh_offsets", &total));
+  EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 4, ctx->d_small + 4, sizeof(int),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int maxrow = ctx->h_small[4];
+  nl->total = total;
+  if (layout == EMD_NEIGH_2D) {
+    const bool fits = P > 0 && maxrow <= P;
+    if (!fits) {
+      while (maxrow > nl->maxneighs) nl->maxneighs = (int)(maxrow * 1.2);
+      EMD_TRY(ensure_entries((long)n * nl->maxneighs));
+      NB_LAUNCH("neigh_fill", true, nullptr, nl->maxneighs, nl->entries,
+                with_slot ? nl->slot : nullptr, nl->maxneighs, 0);
+    }
+  } else {
+    EMD_TRY(ensure_entries(total));
+    if (P > 0 && maxrow <= P) {
+      if (n > 0)
+        EMD_LAUNCH(ctx, "neigh_compact", k_compact_rows, emd_blocks((long)n * EMD_WAVE, 256), 256,
+                   0, pad, pad_slot, P, nl->num, nl->row, n, nl->entries,
+                   with_slot ? nl->slot : nullptr);
+    } else {
+      NB_LAUNCH("neigh_fill", true, nl->row, 0, nl->entries, with_slot ? nl->slot : nullptr,
+                0x7fffffff, 0);
+    }
+    nl->maxneighs = maxrow;
+  }
+#undef NB_LAUNCH
   // sweep order of full lists (every local atom sits in an interior bin)
   const int T = ctx->visit_tile;
+  nl->n_visit = 0;
   if (!half && T > 0 && n > 0 && nib > 0) {
-    EMD_TRY(emd_ensure_int(ctx, &nl->visit, &nl->cap_visit, n));
     const int nslots = ((g.nix + T - 1) / T) * ((g.niy + T - 1) / T) * T * T;
     EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)(nslots + 1) * sizeof(int)));
     int *start = (int *)ctx->d_tmp2;
     visit_len_op op{g, T, b->bincount, b->binoffsets, start};
     int placed = 0;
     EMD_TRY(emd_scan(ctx, op, nslots, "neigh_visit", &placed));
-    if (placed != n)
+    if (placed < n)
       return emd_set_error(ctx, EMD_ERR_STATE,
-                           "sweep order covers %d of %d local atoms (atoms outside interior bins)",
-                           placed, n);
+                           "interior bins hold %d atoms but there are %d local atoms", placed, n);
+    EMD_TRY(emd_ensure_int(ctx, &nl->visit, &nl->cap_visit, placed));
     EMD_LAUNCH(ctx, "neigh_visit", k_visit_fill, nslots, 256, 0, g, T, b->binoffsets,
                b->bincount, b->permute, start, nl->visit);
+    nl->n_visit = placed;
   } else if (nl->visit) {
     EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     EMD_HIP(ctx, hipFree(nl->visit));

@@ -56,15 +56,17 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
           const int *__restrict__ row, const int *__restrict__ num,
           const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
           double cutsq_0, const lj_table *__restrict__ tab, const int *__restrict__ visit,
-          double *__restrict__ f, float *__restrict__ f32) {
+          int n_visit, double *__restrict__ f, float *__restrict__ f32) {
 #pragma clang fp contract(off)
   typedef typename lj_acc<MIXED>::T T;
   const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
   const int gid = tile * LJ_THREADS + threadIdx.x;
   const int a = gid / K, sub = gid % K;
-  if (a >= n) return;
-  // atoms in the list's sweep order (column tiles) when it has one
+  // atoms in the list's sweep order (column tiles) when it has one; the
+  // order may name a few ghosts, which have no row
+  if (a >= (visit ? n_visit : n)) return;
   const int i = visit ? visit[a] : a;
+  if (i >= n) return;
   const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
   const int type_i = ONE_TYPE ? 0 : type[i];
   const int *rowp = entries + (row ? (size_t)row[i] : (size_t)i * (size_t)pitch);
@@ -302,12 +304,12 @@ template <int K, int U>
 static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                               const emd_lj_params *p, int precision, const lj_table *tab) {
   const int n = s->n_local;
-  const int nb = emd_blocks((long)n * K, LJ_THREADS);
+  const int nb = emd_blocks((long)(nl->visit ? nl->n_visit : n) * K, LJ_THREADS);
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   const bool one = p->ntypes == 1;
 #define LJ_FULL_ARGS                                                                        \
   s->x, s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],   \
-      p->cutsq[0][0], tab, nl->visit, s->f, s->f32
+      p->cutsq[0][0], tab, nl->visit, nl->n_visit, s->f, s->f32
   if (precision == EMD_MIXED) {
     if (one) EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, true, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
     else EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, false, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);

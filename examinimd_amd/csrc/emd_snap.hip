@@ -18,15 +18,17 @@
 // Kernels (T = double, or float for the mixed path):
 //   k_snap_neigh  one wavefront per atom: compacts the in-cutoff neighbours
 //                 (FP64 distance test, list order kept) with ballots
-//   k_snap_ui     one wavefront per atom: Cayley-Klein recursion layer by
-//                 layer (previous layer in LDS, one lane per half-plane
-//                 entry), accumulates sfac*w*U, writes U_tot (285 complex)
-//   k_snap_y      one block per atom: U_tot staged in LDS, one thread per
-//                 half-plane entry of Y (155), loops over the (a,b) pairs
-//                 of its J with Clebsch-Gordan weights from a compact table
+//   k_snap_ui     four wavefronts per atom, each over every fourth
+//                 neighbour: Cayley-Klein recursion layer by layer (previous
+//                 layer in the wavefront's LDS buffer, one lane per
+//                 half-plane entry), sums sfac*w*U; the wavefronts' sums
+//                 are added in order and U_tot (285 complex) written
+//   k_snap_y      one block per atom: U_tot staged in LDS, two lanes per
+//                 half-plane entry of Y (155), each over every other (a,b)
+//                 pair of its J, Clebsch-Gordan weights from a compact table
 //   k_snap_fij    one wavefront per (atom, neighbour): U and dU recursion
-//                 (LDS layer buffers), contraction with Y_J, wavefront
-//                 reduction, atomic scatter to f_i / f_j
+//                 (LDS layer buffers), contraction with Y_J (staged in LDS
+//                 per block), wavefront reduction, atomic scatter to f_i / f_j
 //   k_snap_energy one block per atom: B_k from U_tot and Z, energy
 #include <cmath>
 #include <map>
@@ -214,9 +216,11 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
   if (i >= n) return;
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int twojmax = t.twojmax;
-  T acc_r[SNAP_JD], acc_i[SNAP_JD];
-#pragma unroll
-  for (int j = 0; j < SNAP_JD; j++) acc_r[j] = acc_i[j] = 0;
+  // this wavefront's sums over its neighbours, half-plane index
+  // ystart[j] + l (only lane l ever touches entry l of a layer)
+  T *acc_r = s_pr[w], *acc_i = s_pi[w];
+  for (int k = l; k < t.ny; k += 64) acc_r[k] = acc_i[k] = 0;
+  wave_sync();
   const int cnt = nin[i];
   T (*br)[SNAP_JD * SNAP_JD] = s_r[w];
   T (*bi)[SNAP_JD * SNAP_JD] = s_i[w];
@@ -253,18 +257,10 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
           br[c][(j - ma) * (j + 1) + (j - mb)] = sg * u.r;
           bi[c][(j - ma) * (j + 1) + (j - mb)] = -sg * u.i;
         }
-        acc_r[j] += sfac * u.r;
-        acc_i[j] += sfac * u.i;
+        acc_r[t.ystart[j] + l] += sfac * u.r;
+        acc_i[t.ystart[j] + l] += sfac * u.i;
       }
       wave_sync();
-    }
-  }
-  // partial sums of every wavefront, half-plane index ystart[j] + l
-  for (int j = 0; j <= twojmax; j++) {
-    int ma, mb;
-    if (l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb)) {
-      s_pr[w][t.ystart[j] + l] = acc_r[j];
-      s_pi[w][t.ystart[j] + l] = acc_i[j];
     }
   }
   __syncthreads();
@@ -330,8 +326,13 @@ __device__ __forceinline__ void y_entry(const snap_dev_tables &t, int e, int &J,
   ma = k - mb * (J + 1);
 }
 
+// Y of one atom per block: two adjacent lanes per half-plane entry, each
+// summing every other (a, b) pair of the entry's layer; the even lane adds
+// its partner's partial to its own (fixed order)
+constexpr int Y_THREADS = 320;  // >= 2 * SNAP_NY, whole wavefronts
+
 template <class T>
-__global__ void __launch_bounds__(192)
+__global__ void __launch_bounds__(Y_THREADS)
 k_snap_y(snap_dev_tables t, const double *__restrict__ cg, const ytab_entry *__restrict__ ytab,
          int n, const T *__restrict__ utot_r, const T *__restrict__ utot_i, T *__restrict__ y_r,
          T *__restrict__ y_i) {
@@ -343,19 +344,23 @@ k_snap_y(snap_dev_tables t, const double *__restrict__ cg, const ytab_entry *__r
     su_i[k] = utot_i[(size_t)i * t.nu + k];
   }
   __syncthreads();
-  const int e = threadIdx.x;
-  if (e >= t.ny) return;
-  int J, ma, mb;
-  y_entry(t, e, J, ma, mb);
+  const int e = threadIdx.x >> 1, h = threadIdx.x & 1;
   T yr = 0, yi = 0;
-  for (int q = t.yjstart[J]; q < t.yjstart[J + 1]; q++) {
-    const ytab_entry te = ytab[q];
-    const cplx<T> z = z_entry<T>(t, cg, su_r, su_i, te.a, te.b, J, te.cgoff, ma, mb);
-    yr += (T)te.coef * z.r;
-    yi += (T)te.coef * z.i;
+  if (e < t.ny) {
+    int J, ma, mb;
+    y_entry(t, e, J, ma, mb);
+    for (int q = t.yjstart[J] + h; q < t.yjstart[J + 1]; q += 2) {
+      const ytab_entry te = ytab[q];
+      const cplx<T> z = z_entry<T>(t, cg, su_r, su_i, te.a, te.b, J, te.cgoff, ma, mb);
+      yr += (T)te.coef * z.r;
+      yi += (T)te.coef * z.i;
+    }
   }
-  y_r[(size_t)i * t.ny + e] = yr;
-  y_i[(size_t)i * t.ny + e] = yi;
+  const T or_ = dpp_any<0xB1>(yr), oi = dpp_any<0xB1>(yi);  // partner lane (xor 1)
+  if (e < t.ny && h == 0) {
+    y_r[(size_t)i * t.ny + e] = yr + or_;
+    y_i[(size_t)i * t.ny + e] = yi + oi;
+  }
 }
 
 // F_ij for one (atom, neighbour) pair per wavefront (SNA::compute_duidrj +
@@ -369,10 +374,20 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
            const T *__restrict__ y_i, TF *__restrict__ f) {
   // previous layer (j-1 <= 7): U and dU (3 components), 64 entries each
   __shared__ T s_u[FIJ_WAVES][2][8][64];
+  __shared__ T s_y[2][SNAP_NY];
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int i = blockIdx.x;
+  if (i >= n) return;
+  const int cnt_i = nin[i];
+  if (blockIdx.y * FIJ_WAVES >= cnt_i) return;  // no neighbour for this block
+  // Y of atom i, read by every layer of every wavefront of the block
+  for (int k = threadIdx.x; k < t.ny; k += blockDim.x) {
+    s_y[0][k] = y_r[(size_t)i * t.ny + k];
+    s_y[1][k] = y_i[(size_t)i * t.ny + k];
+  }
+  __syncthreads();
   const int jj = blockIdx.y * FIJ_WAVES + w;
-  if (i >= n || jj >= nin[i]) return;
+  if (jj >= cnt_i) return;
   const size_t o = (size_t)i * maxn + jj;
   const double xd = nr[3 * o], yd = nr[3 * o + 1], zd = nr[3 * o + 2];
   const T x = (T)xd, y = (T)yd, z = (T)zd;
@@ -406,7 +421,7 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   dsfac *= (T)t.wj;
 
   T (*buf)[8][64] = s_u[w];
-  const T *yr = y_r + (size_t)i * t.ny, *yi = y_i + (size_t)i * t.ny;
+  const T *yr = s_y[0], *yi = s_y[1];
   T fx = 0, fy = 0, fz = 0;
   // layer 0: U = 1, dU = 0; its dU_tot = dsfac * u; contraction with Y_0
   if (l == 0) {
@@ -813,7 +828,7 @@ template <class T>
 static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   T *ur = (T *)sn->u_r, *ui = (T *)sn->u_i, *yr = (T *)sn->y_r, *yi = (T *)sn->y_i;
   EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n, ur, ui);
-  EMD_LAUNCH(ctx, "snap_y", k_snap_y<T>, n, 192, 0, sn->t, sn->cg, sn->ytab, n, ur, ui, yr, yi);
+  EMD_LAUNCH(ctx, "snap_y", k_snap_y<T>, n, Y_THREADS, 0, sn->t, sn->cg, sn->ytab, n, ur, ui, yr, yi);
   return EMD_OK;
 }
 

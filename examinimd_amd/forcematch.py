@@ -1,0 +1,137 @@
+"""Force-matching loss and gradient over sets of configurations through the
+hand-written adjoint kernels (emd_snap_adjoint_beta, emd_lj_adjoint).
+
+Per configuration c with N_c atoms:
+    L_c = sum_i |F_i(theta) - F*_i|^2 + 100 ((E(theta) - E*) / N_c)^2
+with targets F*, E* from the plain FP64 path at a 'true' parameter set.
+The gradient is sum_c [sum_i g_i . dF_i/dtheta + g_E dE/dtheta] with
+g = 2 (F - F*) and g_E = 200 (E - E*) / N_c^2, evaluated by the adjoint
+kernels (FP32 arithmetic with FP64 accumulation in mixed precision).
+
+The workloads are the seeded sets of the force-matching benchmark:
+  snap_set(): 64 BCC tungsten configurations, a = 3.1803 A, 2 n^3 atoms with
+              n in {4..8} (128..1024 atoms), Gaussian displacements 0.05 A;
+  lj_set():   32 FCC LJ boxes at density 0.8442, 4 n^3 atoms with n in
+              {5..10} (500..4000 atoms), displacements 0.05 sigma.
+Each configuration keeps its own device state (positions, neighbour list)
+for the lifetime of the set; an evaluation uploads nothing but parameters.
+"""
+import numpy as np
+
+from examinimd_amd import decks
+from examinimd_amd.abi import EMD_FP64
+from examinimd_amd.pipeline import Pipeline, lattice
+
+A_W = 3.1803
+W_MASS = 183.84
+SNAP_CUT = 4.73442           # rcutfac * 2 * radelem (0.5)
+LJ_A = (4 / 0.8442) ** (1 / 3)
+
+
+def snap_set(count=64, seed=17, sizes=(4, 5, 6, 7, 8), disp=0.05):
+    """[(positions, box)] of uneven BCC W configurations"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(count):
+        n = int(sizes[c % len(sizes)])
+        x = lattice("bcc", n, A_W) + rng.normal(0, disp, (2 * n ** 3, 3))
+        out.append((x, (n * A_W,) * 3))
+    return out
+
+
+def lj_set(count=32, seed=23, sizes=(5, 6, 7, 8, 9, 10), disp=0.05):
+    """[(positions, box)] of uneven FCC LJ boxes"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(count):
+        n = int(sizes[c % len(sizes)])
+        x = lattice("fcc", n, LJ_A) + rng.normal(0, disp, (4 * n ** 3, 3))
+        out.append((x, (n * LJ_A,) * 3))
+    return out
+
+
+class SnapForceMatch:
+    """loss(beta) and dloss/dbeta over SNAP configurations"""
+
+    def __init__(self, configs, beta_true, ctx=None):
+        self.pls = []
+        self.targets = []
+        for x, box in configs:
+            pl = Pipeline(ctx)
+            ctx = pl.ctx
+            pl.load(x, box, mass=W_MASS)
+            # static configurations: the list needs no skin; newton on for
+            # the ghost force fold
+            pl.build(SNAP_CUT, newton=1)
+            pl.snap_setup(list(beta_true))
+            F = pl.snap_force(EMD_FP64)
+            E = pl.snap_energy()
+            self.pls.append(pl)
+            self.targets.append((F, E, len(x)))
+        self.ctx = ctx
+
+    def evaluate(self, beta, precision, grad=True):
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        loss = 0.0
+        g = np.zeros(len(beta))
+        for pl, (Ft, Et, n) in zip(self.pls, self.targets):
+            pl.snap_set_coeff(beta)
+            F = pl.snap_force(precision)
+            E = pl.snap_energy()
+            dF = F - Ft
+            loss += float((dF * dF).sum()) + 100.0 * ((E - Et) / n) ** 2
+            if grad:
+                g += pl.snap_adjoint(2.0 * dF, 200.0 * (E - Et) / n ** 2, precision)
+        return loss, g
+
+    def close(self):
+        for pl in self.pls:
+            pl.close()
+        self.pls = []
+
+
+class LJForceMatch:
+    """loss(eps, sigma) and its gradient over LJ boxes (cutoff 2.5)"""
+
+    def __init__(self, configs, true=(1.0, 1.0), ctx=None, rc=2.5):
+        self.rc = rc
+        self.pls = []
+        self.targets = []
+        for x, box in configs:
+            pl = Pipeline(ctx)
+            ctx = pl.ctx
+            pl.load(x, box)
+            pl.build(rc)
+            F = pl.lj(EMD_FP64, eps=true[0], sigma=true[1], rc=rc)
+            E = pl.lj_energy(true[0], true[1], rc)
+            self.pls.append(pl)
+            self.targets.append((F, E, len(x)))
+        self.ctx = ctx
+
+    def evaluate(self, theta, precision, grad=True):
+        eps, sig = (float(t) for t in theta)
+        loss = 0.0
+        g = np.zeros(2)
+        for pl, (Ft, Et, n) in zip(self.pls, self.targets):
+            F = pl.lj(precision, eps=eps, sigma=sig, rc=self.rc)
+            E = pl.lj_energy(eps, sig, self.rc)
+            dF = F - Ft
+            loss += float((dF * dF).sum()) + 100.0 * ((E - Et) / n) ** 2
+            if grad:
+                g += pl.lj_adjoint(2.0 * dF, 200.0 * (E - Et) / n ** 2, eps, sig, self.rc,
+                                   precision)
+        return loss, g
+
+    def close(self):
+        for pl in self.pls:
+            pl.close()
+        self.pls = []
+
+
+def snap_beta_true(seed=2024):
+    return decks.snap_beta(seed=seed, sigma=0.05)
+
+
+def snap_beta_perturbed(beta_true, seed=99):
+    rng = np.random.default_rng(seed)
+    return beta_true + np.concatenate([[0.0], rng.normal(0, 0.01, len(beta_true) - 1)])

@@ -160,12 +160,16 @@ typedef struct {
   unsigned short *slot;   /* device, same shape as entries; NULL if not built */
   long cap_slot;
   const emd_bins *bins;
-  /* sweep order for force kernels over the rows of a full list: every local
-   * atom once, bin columns (all z) taken in square tiles of columns, so the
-   * neighbourhood of the atoms in flight stays cache resident.  Built when
-   * the context's tile width is > 0 (default 8); NULL otherwise. */
-  int *visit;             /* device [n] */
+  /* sweep order for force kernels over the rows of a full list: the atoms
+   * of the interior bins, bin columns (all z) taken in square tiles of
+   * columns, so the neighbourhood of the atoms in flight stays cache
+   * resident.  Every local atom appears once; so may the few ghosts that
+   * the bins' cell/1000 margin places inside interior bins (index >=
+   * n_local, to be skipped).  Built when the context's tile width is > 0
+   * (EMD_VISIT_TILE; default 0, measured no faster at LJ 4M); NULL otherwise. */
+  int *visit;             /* device [n_visit] */
   int cap_visit;
+  int n_visit;
 } emd_neigh;
 
 /* NeighborCSR::create_neigh_list (src/neighbor_types/neighbor_csr.h:370-431)

@@ -87,3 +87,34 @@ def oracle_lj_md(oracle, x, v, box, nsteps, dt=0.005, eps=1.0, sig=1.0, rc=2.5, 
     L.or_bins_free(C.byref(b))
     L.or_csr_free(C.byref(c))
     return tuple(out)
+
+
+def oracle_csr_from(oracle, x, ids, box, cut=2.8):
+    """the oracle's full CSR list (exchange, binning sort, halo, binning,
+    NeighborCSR) for local positions given in the caller's order"""
+    import ctypes as C
+    L = oracle.lib()
+    s = oracle.System()
+    L.or_system_init(C.byref(s))
+    n = len(x)
+    L.or_system_grow(C.byref(s), n)
+    s.N_local = s.N = n
+    s.ntypes = 1
+    s.mass[0] = 1.0
+    for d in range(3):
+        s.domain[d] = s.hi[d] = box[d]
+        s.lo[d] = 0.0
+    np.ctypeslib.as_array(s.x, shape=(3 * n,))[:] = np.asarray(x, dtype=np.float64).ravel()
+    np.ctypeslib.as_array(s.id, shape=(n,))[:] = ids
+    h, b, c = oracle.Halo(), oracle.Bins(), oracle.CSR()
+    L.or_comm_exchange(C.byref(s))
+    L.or_binning(C.byref(s), C.byref(b), cut, cut, cut, 1, 1, 0, 1)
+    L.or_comm_exchange_halo(C.byref(s), C.byref(h), cut)
+    L.or_binning(C.byref(s), C.byref(b), cut, cut, cut, 1, 1, 1, 0)
+    L.or_neigh_csr(C.byref(s), C.byref(b), cut, 0, 0, C.byref(c))
+    row, ent = oracle.csr_arrays(c)
+    out_ids = oracle.arr(s.id, n, np.int32)
+    for f, o in ((L.or_system_free, s), (L.or_halo_free, h), (L.or_bins_free, b),
+                 (L.or_csr_free, c)):
+        f(C.byref(o))
+    return row, ent, out_ids

@@ -200,7 +200,11 @@ def test_lj_bin_tile_kernel(oracle, lanes):
     nc = 15
     x = lattice("fcc", nc, a) + np.random.default_rng(6).normal(0, 0.05, (4 * nc ** 3, 3))
     box = (nc * a,) * 3
-    pl = Pipeline()
+    os.environ["EMD_VISIT_TILE"] = "8"     # also build the column-tiled sweep order
+    try:
+        pl = Pipeline()
+    finally:
+        os.environ.pop("EMD_VISIT_TILE", None)
     pl.load(x, box)
     pl.build(2.8, tile=True)
     ctx, L, nl, b = pl.ctx, pl.L, pl.nl, pl.b
@@ -212,9 +216,10 @@ def test_lj_bin_tile_kernel(oracle, lanes):
     cnt = _dev(ctx, b.bincount, b.nbins, np.int32)
     off = _dev(ctx, b.binoffsets, b.nbins, np.int32)
     perm = _dev(ctx, b.permute, b.n_range, np.int32)
-    # the force sweep order visits every local atom once
-    visit = _dev(ctx, nl.visit, n, np.int32)
-    assert np.array_equal(np.sort(visit), np.arange(n))
+    # the force sweep order visits every local atom once (plus any ghost
+    # that the bins' margin puts in an interior bin)
+    visit = _dev(ctx, nl.visit, nl.n_visit, np.int32)
+    assert np.array_equal(np.sort(visit[visit < n]), np.arange(n))
     nb = (b.nbinx, b.nbiny, b.nbinz)
     for cb in range(b.nbins):
         bx, by, bz = cb // (nb[1] * nb[2]), (cb // nb[2]) % nb[1], cb % nb[2]
@@ -241,3 +246,69 @@ def test_lj_bin_tile_kernel(oracle, lanes):
     rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
     print("bin-tile mixed force RMS error", rms)
     assert rms <= (1e-5 if lanes >= 8 else 3e-5)
+
+
+@pytest.mark.parametrize("layout", [emd.EMD_NEIGH_CSR, emd.EMD_NEIGH_2D])
+def test_neighbour_rebuild_capacity_fallback(layout):
+    """a rebuild whose rows outgrow the previous build's capacity (one-pass
+    fill into padded rows overflows) gives the same list as a fresh build"""
+    import ctypes as C
+    from examinimd_amd.abi import EmdNeigh
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print(f"case: neighbour capacity fallback, layout {layout}", flush=True)
+    rng = np.random.default_rng(12)
+
+    def system(rho):
+        a = (4 / rho) ** (1 / 3)
+        return lattice("fcc", 10, a) + rng.normal(0, 0.03, (4000, 3)), (10 * a,) * 3
+
+    def rows(pl, nl):
+        n = pl.s.n_local
+        num = _dev(pl.ctx, nl.num, n, np.int32)
+        if layout == emd.EMD_NEIGH_CSR:
+            row = _dev(pl.ctx, nl.row, n + 1, np.int32)
+            ent = _dev(pl.ctx, nl.entries, int(row[-1]), np.int32)
+            return [ent[row[i]:row[i] + num[i]] for i in range(n)]
+        ent = _dev(pl.ctx, nl.entries, n * nl.maxneighs, np.int32).reshape(n, nl.maxneighs)
+        return [ent[i, :num[i]] for i in range(n)]
+
+    def build(pl, nl, x, box):
+        pl.load(x, box)
+        L, c, s = pl.L, pl.ctx.h, C.byref(pl.s)
+        pl.chk(L.emd_comm_exchange(c, s), "exchange")
+        pl.chk(L.emd_binning(c, s, 2.8, 2.8, 2.8, 1, 1, 0, 1, C.byref(pl.b)), "bin")
+        pl.chk(L.emd_comm_exchange_halo(c, s, 2.8, C.byref(pl.h)), "halo")
+        pl.chk(L.emd_binning(c, s, 2.8, 2.8, 2.8, 1, 1, 1, 0, C.byref(pl.b)), "bin")
+        pl.chk(L.emd_neigh_build(c, s, C.byref(pl.b), 2.8, 0, 0, layout, C.byref(nl)), "neigh")
+
+    x1, b1 = system(0.45)
+    x2, b2 = system(0.95)
+    pl = Pipeline()
+    nl = EmdNeigh()
+    build(pl, nl, x1, b1)          # sparse: short rows set the capacity
+    build(pl, nl, x2, b2)          # dense: rows ~2x longer than that capacity
+    got = rows(pl, nl)
+    fresh = EmdNeigh()
+    build(pl, fresh, x2, b2)       # first build of a new list (exact passes)
+    want = rows(pl, fresh)
+    assert len(got) == len(want)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    for l in (nl, fresh):
+        pl.L.emd_neigh_free(pl.ctx.h, C.byref(l))
+    pl.close()
+
+
+def test_lj_mixed_neighbour_identity_at_every_rebuild(tmp_path, oracle):
+    """mixed-precision trajectory: at each of 5 rebuilds the GPU list equals
+    the oracle's list built from the same (GPU) positions, entry for entry"""
+    from helpers import oracle_csr_from
+    print("case: LJ 32k mixed, neighbour identity at 5 rebuilds", flush=True)
+    app = gpu_app(tmp_path, 100, precision="mixed")
+    box = (20 * (4 / 0.8442) ** (1 / 3),) * 3
+    for r in range(5):
+        app.run(20)                       # step 20 of each chunk rebuilds
+        a = app.arrays()
+        row, num, ent = app.neigh()
+        orow, oent, oid = oracle_csr_from(oracle, a["x"], a["id"], box)
+        assert np.array_equal(oid, a["id"]), r       # same order after the oracle's sort
+        assert np.array_equal(row, orow) and np.array_equal(ent, oent), r
