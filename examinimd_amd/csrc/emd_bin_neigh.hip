@@ -334,18 +334,19 @@ __global__ void __launch_bounds__(256) k_max_int(const int *__restrict__ a, int 
   }
 }
 
-// padded rows (pitch P) -> CSR rows, one wavefront per row
+// padded rows (pitch P) -> CSR rows, 16 lanes per row
+constexpr int COMPACT_LANES = 16;
 __global__ void __launch_bounds__(256)
 k_compact_rows(const int *__restrict__ tmp, const unsigned short *__restrict__ tslot, int P,
                const int *__restrict__ num, const int *__restrict__ row, int n,
                int *__restrict__ entries, unsigned short *__restrict__ slot) {
-  const int r = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / EMD_WAVE);
-  const int lane = threadIdx.x % EMD_WAVE;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(t / COMPACT_LANES), lane = (int)(t % COMPACT_LANES);
   if (r >= n) return;
   const int c = num[r];
   const size_t src = (size_t)r * P, dst = (size_t)row[r];
-  for (int k = lane; k < c; k += EMD_WAVE) {
-    entries[dst + k] = tmp[src + k];
+  for (int k = lane; k < c; k += COMPACT_LANES) {
+    entries[dst + k] = __builtin_nontemporal_load(tmp + src + k);
     if (slot) slot[dst + k] = tslot[src + k];
   }
 }
@@ -542,7 +543,7 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
     EMD_TRY(ensure_entries(total));
     if (P > 0 && maxrow <= P) {
       if (n > 0)
-        EMD_LAUNCH(ctx, "neigh_compact", k_compact_rows, emd_blocks((long)n * EMD_WAVE, 256), 256,
+        EMD_LAUNCH(ctx, "neigh_compact", k_compact_rows, emd_blocks((long)n * COMPACT_LANES, 256), 256,
                    0, pad, pad_slot, P, nl->num, nl->row, n, nl->entries,
                    with_slot ? nl->slot : nullptr);
     } else {

@@ -134,6 +134,159 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
   }
 }
 
+// Software-pipelined form of k_lj_full (one type, full CSR or 2D rows).
+// Each wavefront walks groups of 64/K atoms through its XCD's contiguous
+// share of the atoms (blocks are dealt to the 8 XCDs round-robin; the share
+// only affects cache locality, every group is done exactly once).  While a
+// wavefront gathers positions for group t and computes it, the index loads
+// of group t+1 and the row offsets of group t+2 are already in flight, so
+// the latency of the HBM index stream overlaps arithmetic.  The first U*K
+// entries of a row take the pipelined round, longer rows finish in a plain
+// loop.  Per lane the entries and their order are those of k_lj_full<K,U>
+// and the reduction is the same: the FP64 result is bitwise equal.
+constexpr int PIPE_WAVES = 4;
+
+template <int K, int U, bool MIXED, bool CSR>
+__global__ void __launch_bounds__(64 * PIPE_WAVES)
+k_lj_pipe(const double *__restrict__ x, const int *__restrict__ row, const int *__restrict__ num,
+          const int *__restrict__ entries, int pitch, int n, double lj1, double lj2,
+          double cutsq, int groups_per_xcd, int waves_per_xcd, double *__restrict__ f,
+          float *__restrict__ f32) {
+#pragma clang fp contract(off)
+  typedef typename lj_acc<MIXED>::T T;
+  constexpr int G = 64 / K;  // atoms per wavefront group
+  const int lane = threadIdx.x % 64, sub = lane % K, at = lane / K;
+  const int xcd = blockIdx.x % 8;
+  const int g_lo = xcd * groups_per_xcd;
+  const int g_hi = min((n + G - 1) / G, g_lo + groups_per_xcd);
+  int g = g_lo + (blockIdx.x / 8) * PIPE_WAVES + threadIdx.x / 64;
+  if (g >= g_hi) return;  // uniform over the wavefront
+  // All loads below are unconditional, from addresses kept in range by
+  // clamping (padding atoms of the last group read atom n-1, slots past a
+  // row's end re-read its start, which lies inside the entries allocation),
+  // and invalid results are discarded by selects, so no branch separates a
+  // load from the loads issued after it.
+  auto row_info = [&](int gg, long &base, int &cnt) {
+    const int i = gg * G + at;
+    const int ic = i < n ? i : n - 1;
+    base = CSR ? (long)row[ic] : (long)ic * pitch;
+    const int c = num[ic];
+    cnt = i < n ? c : 0;
+  };
+  auto load_idx = [&](int gg, long base, int cnt, int *j, bool *ok) {
+    const int i = gg * G + at;
+    const int ic = i < n ? i : n - 1;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = sub + u * K;
+      ok[u] = e < cnt;
+      const int v = __builtin_nontemporal_load(entries + base + (ok[u] ? e : 0));
+      j[u] = ok[u] ? v : ic;
+    }
+  };
+  // groups past the end are replaced by the current one for the prefetches
+  // (their results go unused), keeping every load unconditional
+  long b0, b1;
+  int c0, c1;
+  row_info(g, b0, c0);
+  int j0[U];
+  bool ok0[U];
+  load_idx(g, b0, c0, j0, ok0);
+  int g1 = g + waves_per_xcd;
+  row_info(g1 < g_hi ? g1 : g, b1, c1);
+  while (true) {
+    const int i = g * G + at;
+    const int ic = i < n ? i : n - 1;
+    // gathers of group g, its own position and current force
+    double xj[U], yj[U], zj[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      xj[u] = x[3 * (size_t)j0[u]];
+      yj[u] = x[3 * (size_t)j0[u] + 1];
+      zj[u] = x[3 * (size_t)j0[u] + 2];
+    }
+    const double x_i = x[3 * (size_t)ic], y_i = x[3 * (size_t)ic + 1], z_i = x[3 * (size_t)ic + 2];
+    T f0x, f0y, f0z;
+    if (MIXED) {
+      f0x = f32[3 * (size_t)ic];
+      f0y = f32[3 * (size_t)ic + 1];
+      f0z = f32[3 * (size_t)ic + 2];
+    } else {
+      f0x = f[3 * (size_t)ic];
+      f0y = f[3 * (size_t)ic + 1];
+      f0z = f[3 * (size_t)ic + 2];
+    }
+    // keep the prefetches below behind this group's loads in the memory
+    // queue (waits on vector loads retire in issue order)
+    __builtin_amdgcn_sched_barrier(0);
+    // next group's indices and the row offsets of the one after
+    int j1[U];
+    bool ok1[U];
+    const int g2 = g1 + waves_per_xcd;
+    long b2;
+    int c2;
+    load_idx(g1 < g_hi ? g1 : g, b1, c1, j1, ok1);
+    row_info(g2 < g_hi ? g2 : g, b2, c2);
+    T fxi = 0, fyi = 0, fzi = 0;
+    auto pair = [&](bool ok, double xj_, double yj_, double zj_) {
+      const double dx = x_i - xj_;
+      const double dy = y_i - yj_;
+      const double dz = z_i - zj_;
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      if (ok && rsq < cutsq) {
+        if (MIXED) {
+          const float fpair = emd_lj_fpair_mixed(rsq, (float)lj1, (float)lj2);
+          fxi = __builtin_fmaf((float)dx, fpair, fxi);
+          fyi = __builtin_fmaf((float)dy, fpair, fyi);
+          fzi = __builtin_fmaf((float)dz, fpair, fzi);
+        } else {
+          const double r2inv = 1.0 / rsq;
+          const double r6inv = r2inv * r2inv * r2inv;
+          const double fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
+          fxi += dx * fpair;
+          fyi += dy * fpair;
+          fzi += dz * fpair;
+        }
+      }
+    };
+#pragma unroll
+    for (int u = 0; u < U; u++) pair(ok0[u], xj[u], yj[u], zj[u]);
+    // rows longer than one round
+    for (int jj = sub + U * K; jj < c0; jj += K) {
+      const int j = __builtin_nontemporal_load(entries + b0 + jj);
+      pair(true, x[3 * (size_t)j], x[3 * (size_t)j + 1], x[3 * (size_t)j + 2]);
+    }
+    if (K > 1) {
+      fxi = group_sum<K>(fxi);
+      fyi = group_sum<K>(fyi);
+      fzi = group_sum<K>(fzi);
+    }
+    if (sub == 0 && i < n) {
+      if (MIXED) {
+        f32[3 * (size_t)i] = f0x + (float)fxi;
+        f32[3 * (size_t)i + 1] = f0y + (float)fyi;
+        f32[3 * (size_t)i + 2] = f0z + (float)fzi;
+      } else {
+        f[3 * (size_t)i] = f0x + fxi;
+        f[3 * (size_t)i + 1] = f0y + fyi;
+        f[3 * (size_t)i + 2] = f0z + fzi;
+      }
+    }
+    if (g1 >= g_hi) break;
+    g = g1;
+    g1 = g2;
+    b0 = b1;
+    c0 = c1;
+    b1 = b2;
+    c1 = c2;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      j0[u] = j1[u];
+      ok0[u] = ok1[u];
+    }
+  }
+}
+
 // Bin-tile form of the same full-list force (one type).  One block per
 // interior bin: the block copies the positions of the bin's 27-bin
 // candidate list (the order the neighbour build enumerated them in) into
@@ -301,6 +454,33 @@ extern "C" emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes) {
 }
 
 template <int K, int U>
+static emd_status launch_pipe(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                              const emd_lj_params *p, int precision) {
+  const int n = s->n_local;
+  constexpr int G = 64 / K;
+  const int ngroups = (n + G - 1) / G;
+  const int gpx = (ngroups + 7) / 8;
+  int bpx = ctx->lj_pipe_blocks;  // blocks per XCD
+  const int need = (gpx + PIPE_WAVES - 1) / PIPE_WAVES;
+  if (bpx > need) bpx = need;
+  if (bpx < 1) bpx = 1;
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+#define LJ_PIPE_ARGS                                                                        \
+  s->x, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],            \
+      p->cutsq[0][0], gpx, bpx * PIPE_WAVES, s->f, s->f32
+  const bool csr = nl->layout == EMD_NEIGH_CSR;
+  if (precision == EMD_MIXED) {
+    if (csr) EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_pipe<K, U, true, true>), 8 * bpx, 64 * PIPE_WAVES, 0, LJ_PIPE_ARGS);
+    else EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_pipe<K, U, true, false>), 8 * bpx, 64 * PIPE_WAVES, 0, LJ_PIPE_ARGS);
+  } else {
+    if (csr) EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_pipe<K, U, false, true>), 8 * bpx, 64 * PIPE_WAVES, 0, LJ_PIPE_ARGS);
+    else EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_pipe<K, U, false, false>), 8 * bpx, 64 * PIPE_WAVES, 0, LJ_PIPE_ARGS);
+  }
+#undef LJ_PIPE_ARGS
+  return EMD_OK;
+}
+
+template <int K, int U>
 static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                               const emd_lj_params *p, int precision, const lj_table *tab) {
   const int n = s->n_local;
@@ -389,6 +569,18 @@ extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh 
   }
   int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
   if (K == 0) K = default_lanes(precision);
+  if (ctx->lj_pipe_blocks > 0 && p->ntypes == 1 && K >= 4) {
+    switch (K) {
+      case 4: return launch_pipe<4, 4>(ctx, s, nl, p, precision);
+      case 8: return launch_pipe<8, 5>(ctx, s, nl, p, precision);
+      default:
+        switch (ctx->lj_unroll) {
+          case 4: return launch_pipe<16, 4>(ctx, s, nl, p, precision);
+          case 6: return launch_pipe<16, 6>(ctx, s, nl, p, precision);
+          default: return launch_pipe<16, 5>(ctx, s, nl, p, precision);
+        }
+    }
+  }
   switch (K) {
     case 1: return launch_full<1, 4>(ctx, s, nl, p, precision, tab);
     case 2: return launch_full<2, 4>(ctx, s, nl, p, precision, tab);

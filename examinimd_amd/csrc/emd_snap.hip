@@ -363,32 +363,43 @@ k_snap_y(snap_dev_tables t, const double *__restrict__ cg, const ytab_entry *__r
   }
 }
 
-// F_ij for one (atom, neighbour) pair per wavefront (SNA::compute_duidrj +
-// compute_duarray, sna_impl.hpp:290-321 / 728-893, then the Y contraction)
-constexpr int FIJ_WAVES = 4;
+// F_ij for two (atom, neighbour) pairs per wavefront, 32 lanes each
+// (SNA::compute_duidrj + compute_duarray, sna_impl.hpp:290-321 / 728-893,
+// then the Y contraction): the lanes of a half-wavefront take the half-plane
+// entries of a layer 32 at a time (one step up to layer 7, two for the 45
+// entries of layer 8), so about twice as many lanes do useful work as with
+// one pair per wavefront.  A half-wavefront without a neighbour computes on
+// its partner's and discards the result (no lane leaves before the wave
+// barriers).
+constexpr int FIJ_WAVES = 2;                      // wavefronts per block
+constexpr int FIJ_NB = 2 * FIJ_WAVES;             // neighbours per block
 
 template <class T, class TF>
 __global__ void __launch_bounds__(64 * FIJ_WAVES)
 k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
            const double *__restrict__ nr, int maxn, int n, const T *__restrict__ y_r,
            const T *__restrict__ y_i, TF *__restrict__ f) {
-  // previous layer (j-1 <= 7): U and dU (3 components), 64 entries each
-  __shared__ T s_u[FIJ_WAVES][2][8][64];
+  // previous layer (j-1 <= 7) of each neighbour: U and dU (3 components),
+  // 64 entries each, double buffered
+  __shared__ T s_u[FIJ_NB][2][8][64];
   __shared__ T s_y[2][SNAP_NY];
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int hs = l / 32, hl = l % 32;
   const int i = blockIdx.x;
   if (i >= n) return;
   const int cnt_i = nin[i];
-  if (blockIdx.y * FIJ_WAVES >= cnt_i) return;  // no neighbour for this block
-  // Y of atom i, read by every layer of every wavefront of the block
+  if (blockIdx.y * FIJ_NB >= cnt_i) return;  // no neighbour for this block
+  // Y of atom i, read by every layer of every neighbour of the block
   for (int k = threadIdx.x; k < t.ny; k += blockDim.x) {
     s_y[0][k] = y_r[(size_t)i * t.ny + k];
     s_y[1][k] = y_i[(size_t)i * t.ny + k];
   }
   __syncthreads();
-  const int jj = blockIdx.y * FIJ_WAVES + w;
-  if (jj >= cnt_i) return;
-  const size_t o = (size_t)i * maxn + jj;
+  const int jj0 = blockIdx.y * FIJ_NB + 2 * w;  // first half-wavefront's neighbour
+  if (jj0 >= cnt_i) return;                     // uniform over the wavefront
+  const int jj = jj0 + hs;
+  const bool act = jj < cnt_i;
+  const size_t o = (size_t)i * maxn + (act ? jj : jj0);
   const double xd = nr[3 * o], yd = nr[3 * o + 1], zd = nr[3 * o + 2];
   const T x = (T)xd, y = (T)yd, z = (T)zd;
   const T rsq = x * x + y * y + z * z;
@@ -420,11 +431,11 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   sfac *= (T)t.wj;
   dsfac *= (T)t.wj;
 
-  T (*buf)[8][64] = s_u[w];
+  T (*buf)[8][64] = s_u[2 * w + hs];
   const T *yr = s_y[0], *yi = s_y[1];
   T fx = 0, fy = 0, fz = 0;
   // layer 0: U = 1, dU = 0; its dU_tot = dsfac * u; contraction with Y_0
-  if (l == 0) {
+  if (hl == 0) {
     buf[0][0][0] = 1; buf[0][1][0] = 0;
     for (int k = 0; k < 3; k++) { buf[0][2 + 2 * k][0] = 0; buf[0][3 + 2 * k][0] = 0; }
     const T w0 = (T)0.5;  // J = 0: the single entry is the middle-column diagonal
@@ -433,83 +444,84 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
     fy += c * u3[1];
     fz += c * u3[2];
   }
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  wave_sync();
   for (int j = 1; j <= t.twojmax; j++) {
     const int p = (j - 1) & 1, c = j & 1;
-    int ma, mb;
-    const bool own = l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb);
-    T ur = 0, ui = 0, dr[3] = {0, 0, 0}, di[3] = {0, 0, 0};
-    if (own) {
-      const T *P = &buf[p][0][0];
-      // P[q*64 + idx]: q = 0 U_r, 1 U_i, 2+2k dU_r[k], 3+2k dU_i[k]
-      if (ma < j) {
-        const T rp = (T)t.rootpq[(j - ma) * (SNAP_JD + 1) + (j - mb)];
-        const int s = ma * j + mb;
-        const T pr = P[s], pi = P[64 + s];
-        ur += rp * (a_r * pr + a_i * pi);
-        ui += rp * (a_r * pi - a_i * pr);
+    const int ne = (j + 1) * (j / 2 + 1);
+    for (int s0 = 0; s0 < ne; s0 += 32) {
+      const int e = s0 + hl;
+      int ma = 0, mb = 0;
+      const bool own = e < ne && lane_entry(e, j, ma, mb);
+      T ur = 0, ui = 0, dr[3] = {0, 0, 0}, di[3] = {0, 0, 0};
+      if (own) {
+        const T *P = &buf[p][0][0];
+        // P[q*64 + idx]: q = 0 U_r, 1 U_i, 2+2k dU_r[k], 3+2k dU_i[k]
+        if (ma < j) {
+          const T rp = (T)t.rootpq[(j - ma) * (SNAP_JD + 1) + (j - mb)];
+          const int sidx = ma * j + mb;
+          const T pr = P[sidx], pi = P[64 + sidx];
+          ur += rp * (a_r * pr + a_i * pi);
+          ui += rp * (a_r * pi - a_i * pr);
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const T dpr = P[(2 + 2 * k) * 64 + s], dpi = P[(3 + 2 * k) * 64 + s];
-          dr[k] += rp * (da_r[k] * pr + da_i[k] * pi + a_r * dpr + a_i * dpi);
-          di[k] += rp * (da_r[k] * pi - da_i[k] * pr + a_r * dpi - a_i * dpr);
+          for (int k = 0; k < 3; k++) {
+            const T dpr = P[(2 + 2 * k) * 64 + sidx], dpi = P[(3 + 2 * k) * 64 + sidx];
+            dr[k] += rp * (da_r[k] * pr + da_i[k] * pi + a_r * dpr + a_i * dpi);
+            di[k] += rp * (da_r[k] * pi - da_i[k] * pr + a_r * dpi - a_i * dpr);
+          }
+        }
+        if (ma > 0) {
+          const T rp = (T)t.rootpq[ma * (SNAP_JD + 1) + (j - mb)];
+          const int sidx = (ma - 1) * j + mb;
+          const T pr = P[sidx], pi = P[64 + sidx];
+          ur += -rp * (b_r * pr + b_i * pi);
+          ui += -rp * (b_r * pi - b_i * pr);
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            const T dpr = P[(2 + 2 * k) * 64 + sidx], dpi = P[(3 + 2 * k) * 64 + sidx];
+            dr[k] += -rp * (db_r[k] * pr + db_i[k] * pi + b_r * dpr + b_i * dpi);
+            di[k] += -rp * (db_r[k] * pi - db_i[k] * pr + b_r * dpi - b_i * dpr);
+          }
         }
       }
-      if (ma > 0) {
-        const T rp = (T)t.rootpq[ma * (SNAP_JD + 1) + (j - mb)];
-        const int s = (ma - 1) * j + mb;
-        const T pr = P[s], pi = P[64 + s];
-        ur += -rp * (b_r * pr + b_i * pi);
-        ui += -rp * (b_r * pi - b_i * pr);
+      if (own && j < t.twojmax) {
+        T *C = &buf[c][0][0];
+        const int sidx = ma * (j + 1) + mb;
+        C[sidx] = ur; C[64 + sidx] = ui;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const T dpr = P[(2 + 2 * k) * 64 + s], dpi = P[(3 + 2 * k) * 64 + s];
-          dr[k] += -rp * (db_r[k] * pr + db_i[k] * pi + b_r * dpr + b_i * dpi);
-          di[k] += -rp * (db_r[k] * pi - db_i[k] * pr + b_r * dpi - b_i * dpr);
+        for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + sidx] = dr[k]; C[(3 + 2 * k) * 64 + sidx] = di[k]; }
+        if (2 * mb != j) {
+          const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
+          const int m = (j - ma) * (j + 1) + (j - mb);
+          C[m] = sg * ur; C[64 + m] = -sg * ui;
+#pragma unroll
+          for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + m] = sg * dr[k]; C[(3 + 2 * k) * 64 + m] = -sg * di[k]; }
+        }
+      }
+      if (own) {
+        // half-plane weight of (ma, mb) in layer j
+        T wgt = 1;
+        if (2 * mb == j) wgt = (ma < mb) ? (T)1 : ((ma == mb) ? (T)0.5 : (T)0);
+        if (wgt != (T)0) {
+          const int ey = t.ystart[j] + mb * (j + 1) + ma;
+          const T Yr = wgt * yr[ey], Yi = wgt * yi[ey];
+          const T uy = ur * Yr + ui * Yi;   // Re(conj(U) Y) (switching-derivative term)
+          const T g[3] = {dr[0] * Yr + di[0] * Yi, dr[1] * Yr + di[1] * Yi, dr[2] * Yr + di[2] * Yi};
+          fx += dsfac * uy * u3[0] + sfac * g[0];
+          fy += dsfac * uy * u3[1] + sfac * g[1];
+          fz += dsfac * uy * u3[2] + sfac * g[2];
         }
       }
     }
-    __builtin_amdgcn_wave_barrier();
-    if (own && j < t.twojmax) {
-      T *C = &buf[c][0][0];
-      const int s = ma * (j + 1) + mb;
-      C[s] = ur; C[64 + s] = ui;
-#pragma unroll
-      for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + s] = dr[k]; C[(3 + 2 * k) * 64 + s] = di[k]; }
-      if (2 * mb != j) {
-        const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
-        const int m = (j - ma) * (j + 1) + (j - mb);
-        C[m] = sg * ur; C[64 + m] = -sg * ui;
-#pragma unroll
-        for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + m] = sg * dr[k]; C[(3 + 2 * k) * 64 + m] = -sg * di[k]; }
-      }
-    }
-    if (own) {
-      // half-plane weight of (ma, mb) in layer j
-      T wgt = 1;
-      if (2 * mb == j) wgt = (ma < mb) ? (T)1 : ((ma == mb) ? (T)0.5 : (T)0);
-      if (wgt != (T)0) {
-        const int e = t.ystart[j] + mb * (j + 1) + ma;
-        const T Yr = wgt * yr[e], Yi = wgt * yi[e];
-        const T uy = ur * Yr + ui * Yi;   // Re(conj(U) Y) (switching-derivative term)
-        const T g[3] = {dr[0] * Yr + di[0] * Yi, dr[1] * Yr + di[1] * Yi, dr[2] * Yr + di[2] * Yi};
-        fx += dsfac * uy * u3[0] + sfac * g[0];
-        fy += dsfac * uy * u3[1] + sfac * g[1];
-        fz += dsfac * uy * u3[2] + sfac * g[2];
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
   }
-  // wavefront sum of the force components
+  // half-wavefront sum of the force components
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    fx += __shfl_xor(fx, off, 64);
-    fy += __shfl_xor(fy, off, 64);
-    fz += __shfl_xor(fz, off, 64);
+  for (int off = 16; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off, 32);
+    fy += __shfl_xor(fy, off, 32);
+    fz += __shfl_xor(fz, off, 32);
   }
-  if (l == 0) {
+  if (hl == 0 && act) {
     // ad-hoc repulsion of the reference, force_snap_neigh_impl.h:705-711
     const double rsqd = xd * xd + yd * yd + zd * zd;
     const double fdivr = -1.5e6 / pow(rsqd, 7.0);
@@ -839,7 +851,7 @@ extern "C" emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neig
   const int n = s->n_local;
   if (n <= 0) return EMD_OK;
   EMD_TRY(snap_neigh(ctx, s, nl, sn));
-  const dim3 grid(n, (sn->cap_maxn + FIJ_WAVES - 1) / FIJ_WAVES);
+  const dim3 grid(n, (sn->cap_maxn + FIJ_NB - 1) / FIJ_NB);
   if (precision == EMD_MIXED) {
     EMD_TRY(snap_ui_y<float>(ctx, sn, n));
     EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<float, float>), grid, 64 * FIJ_WAVES, 0, sn->t, sn->nin,

@@ -312,3 +312,26 @@ def test_lj_mixed_neighbour_identity_at_every_rebuild(tmp_path, oracle):
         orow, oent, oid = oracle_csr_from(oracle, a["x"], a["id"], box)
         assert np.array_equal(oid, a["id"]), r       # same order after the oracle's sort
         assert np.array_equal(row, orow) and np.array_equal(ent, oent), r
+
+
+def test_lj_pipelined_kernel_bitwise(oracle):
+    """the software-pipelined CSR kernel (EMD_LJ_PIPE_BLOCKS) gives bitwise
+    the forces of k_lj_full at the same lanes per atom, both precisions"""
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print("case: LJ pipelined kernel vs CSR kernel, 13500 atoms", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    nc = 15
+    x = lattice("fcc", nc, a) + np.random.default_rng(9).normal(0, 0.05, (4 * nc ** 3, 3))
+    out = {}
+    for blocks in ("0", "16"):
+        os.environ["EMD_LJ_PIPE_BLOCKS"] = blocks
+        try:
+            pl = Pipeline()
+        finally:
+            os.environ.pop("EMD_LJ_PIPE_BLOCKS", None)
+        pl.load(x, (nc * a,) * 3)
+        pl.build(2.8)
+        out[blocks] = (pl.lj(emd.EMD_FP64), pl.lj(emd.EMD_MIXED))
+        pl.close()
+    assert np.array_equal(out["0"][0], out["16"][0])
+    assert np.array_equal(out["0"][1], out["16"][1])
