@@ -43,18 +43,52 @@ def parse():
     ap.add_argument("--loop-steps", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-forcematch", action="store_true")
+    ap.add_argument("--neigh", default="2D", choices=["CSR", "2D"],
+                    help="neighbour list layout of the LJ runs: Neighbor2D (the reference's "
+                         "default, src/input.cpp:132) or NeighborCSR")
+    ap.add_argument("--no-alt-layout", action="store_true",
+                    help="skip the mixed LJ run with the other neighbour list layout")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (profiler calibration)")
     return ap.parse_args()
 
 
-def lj_run(emd, decks, tmpdir, n, precision, warmup, steps, chunk=100):
+def _profile_table(L, ctx):
+    buf = C.create_string_buffer(1 << 16)
+    L.emd_profile_report(ctx, buf, len(buf))
+    prof = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.split()
+        prof[name] = (int(cnt), float(ms))
+    return prof
+
+
+def _breakdown(app, L, ctx, steps):
+    """per-kernel milliseconds per step over `steps` further steps with every
+    launch timed (outside the timed region: the event records add gaps)"""
+    if steps <= 0:
+        return {}
+    L.emd_profile_select(ctx, None)
+    L.emd_profile_enable(ctx, 1)
+    L.emd_profile_reset(ctx)
+    app.run(steps)
+    L.emd_synchronize(ctx)
+    prof = _profile_table(L, ctx)
+    L.emd_profile_enable(ctx, 0)
+    return {k: v[1] / steps for k, v in prof.items()}
+
+
+def lj_run(emd, decks, tmpdir, n, precision, warmup, steps, chunk=100, breakdown_steps=40,
+           neigh="CSR"):
     """the timed LJ run, in chunks of `chunk` steps (a multiple of the
     rebuild interval, so the rebuild schedule is that of one run); E/N is
-    sampled between chunks outside the timed region"""
+    sampled between chunks outside the timed region.  Inside the timed
+    region only the force kernel carries HIP events (its mean duration is
+    the roofline's denominator); a per-kernel breakdown follows over
+    `breakdown_steps` untimed steps."""
     deck = decks.write(os.path.join(tmpdir, f"in.lj.{precision}"),
                        decks.lj_deck(n=n, steps=steps))
-    app = emd.App(["-il", deck, "--neigh-type", "CSR", "--comm-type", "SERIAL",
+    app = emd.App(["-il", deck, "--neigh-type", neigh, "--comm-type", "SERIAL",
                    "--precision", precision]).init()
     app.set_phase_sync(False)
     L = emd.hip_lib()
@@ -63,6 +97,8 @@ def lj_run(emd, decks, tmpdir, n, precision, warmup, steps, chunk=100):
         app.run(warmup)
     L.emd_synchronize(ctx)
     samples = [(0, app.thermo()[2])]
+    kname = "lj_force_mixed" if precision == "mixed" else "lj_force_fp64"
+    L.emd_profile_select(ctx, kname.encode())
     L.emd_profile_enable(ctx, 1)
     L.emd_profile_reset(ctx)
     wall, done = 0.0, 0
@@ -78,16 +114,14 @@ def lj_run(emd, decks, tmpdir, n, precision, warmup, steps, chunk=100):
         samples.append((done, app.thermo()[2]))
         L.emd_profile_enable(ctx, 1)
     app.energy_samples = samples
-    prof = {}
-    buf = C.create_string_buffer(1 << 16)
-    L.emd_profile_report(ctx, buf, len(buf))
-    for line in buf.value.decode().splitlines():
-        name, cnt, ms = line.split()
-        prof[name] = (int(cnt), float(ms))
+    prof = _profile_table(L, ctx)
     L.emd_profile_enable(ctx, 0)
-    row, num, ent = app.neigh()
+    app.thermo_end = list(app.thermo())
+    nl = app.L.emd_app_neigh(app.h).contents
+    num = emd.download_ints(ctx, nl.num, nl.n)
     natoms = app.natoms
-    nlist = float(len(ent)) / natoms
+    nlist = float(num.sum()) / natoms
+    app.breakdown = _breakdown(app, L, ctx, breakdown_steps)
     return app, wall, prof, nlist
 
 
@@ -157,9 +191,12 @@ def snap_files(emd, decks, tmpdir):
     return coeff, param, beta
 
 
-def snap_run(emd, decks, tmpdir, coeff, param, precision, warmup, steps):
+def snap_run(emd, decks, tmpdir, coeff, param, precision, warmup, steps, breakdown_steps=20):
+    """timed SNAP run; events only on the three SNAP kernels (the roofline's
+    denominator), then an untimed per-kernel breakdown"""
     deck = decks.write(os.path.join(tmpdir, f"in.snap.{precision}"),
-                       decks.snap_deck(n=10, steps=steps, coeff=coeff, param=param))
+                       decks.snap_deck(n=10, steps=steps + breakdown_steps, coeff=coeff,
+                                       param=param))
     app = emd.App(["-il", deck, "--neigh-type", "CSR", "--precision", precision]).init()
     app.set_phase_sync(False)
     L = emd.hip_lib()
@@ -167,22 +204,20 @@ def snap_run(emd, decks, tmpdir, coeff, param, precision, warmup, steps):
     if warmup:
         app.run(warmup)
     L.emd_synchronize(ctx)
+    L.emd_profile_select(ctx, b"snap_ui,snap_y,snap_fij")
     L.emd_profile_enable(ctx, 1)
     L.emd_profile_reset(ctx)
     t0 = time.perf_counter()
     app.run(steps)
     L.emd_synchronize(ctx)
     wall = time.perf_counter() - t0
-    buf = C.create_string_buffer(1 << 16)
-    L.emd_profile_report(ctx, buf, len(buf))
-    prof = {}
-    for line in buf.value.decode().splitlines():
-        name, cnt, ms = line.split()
-        prof[name] = (int(cnt), float(ms))
+    prof = _profile_table(L, ctx)
+    L.emd_profile_enable(ctx, 0)
     natoms = app.natoms
     e_per_atom = app.thermo()[2]
+    breakdown = _breakdown(app, L, ctx, breakdown_steps)
     app.close()
-    return natoms, wall, prof, e_per_atom
+    return natoms, wall, prof, e_per_atom, breakdown
 
 
 def snap_force_error(emd, beta):
@@ -337,16 +372,29 @@ def main():
     tmpdir = tempfile.mkdtemp(prefix="emd_bench_")
     out = {}
     for prec in ("fp64", "mixed"):
-        app, wall, prof, nlist = lj_run(emd, decks, tmpdir, n, prec, a.warmup, a.steps)
+        app, wall, prof, nlist = lj_run(emd, decks, tmpdir, n, prec, a.warmup, a.steps,
+                                        neigh=a.neigh)
         natoms = app.natoms
         kname = "lj_force_mixed" if prec == "mixed" else "lj_force_fp64"
         cnt, ms = prof.get(kname, (0, 0.0))
         out[prec] = {"wall": wall, "natoms": natoms, "prof": prof, "nlist": nlist,
                      "kernel": kname, "kernel_launches": cnt, "kernel_ms": ms,
-                     "thermo_end": list(app.thermo()),
+                     "thermo_end": app.thermo_end, "breakdown": app.breakdown,
                      "drift": energy_drift(app.energy_samples, 0.005)}
         if prec == "mixed":
             out[prec]["melt_error"] = melt_force_error(emd, app)
+        app.close()
+
+    # the mixed run again with the other list layout (same workload)
+    alt = None
+    if not a.no_alt_layout:
+        lay = "CSR" if a.neigh == "2D" else "2D"
+        app, wall, prof, nlist = lj_run(emd, decks, tmpdir, n, "mixed", a.warmup, a.steps,
+                                        neigh=lay, breakdown_steps=0)
+        cnt, ms = prof.get("lj_force_mixed", (0, 0.0))
+        alt = {"neigh": lay, "value": app.natoms * a.steps / wall,
+               "ms_per_step": 1e3 * wall / a.steps,
+               "lj_force_mixed_avg_ms": ms / max(cnt, 1)}
         app.close()
 
     mixed, fp64 = out["mixed"], out["fp64"]
@@ -354,9 +402,10 @@ def main():
     value = natoms * a.steps / mixed["wall"]
     value_fp64 = natoms * a.steps / fp64["wall"]
     # roofline of the dominant kernel (LJ force, mixed): algorithmic bytes per
-    # atom = 4*nlist (CSR entries) + 8 (row offset, length) + 24 (x_i)
-    # + 24 (each x_j fetched once) + 24 (f32 read-modify-write)
-    bytes_per_atom = 4.0 * mixed["nlist"] + 8 + 24 + 24 + 24
+    # atom = 4*nlist (row entries) + row bookkeeping (2D: length 4 B; CSR:
+    # offset and length 8 B) + 24 (x_i) + 24 (each x_j fetched once) + 12
+    # (the f32 force, stored once)
+    bytes_per_atom = 4.0 * mixed["nlist"] + (4 if a.neigh == "2D" else 8) + 24 + 24 + 12
     avg_ms = mixed["kernel_ms"] / max(mixed["kernel_launches"], 1)
     achieved = bytes_per_atom * natoms / (avg_ms * 1e-3) / 1e9
     ferr = force_error(emd, decks, tmpdir, 20)
@@ -367,7 +416,8 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "lj4m_traffic.json")
     if n == 100 and os.path.exists(tpath):
         t = json.load(open(tpath))
-        if t.get("kernel") == mixed["kernel"] and t.get("atoms") == natoms:
+        if (t.get("kernel") == mixed["kernel"] and t.get("atoms") == natoms
+                and t.get("neigh", "CSR") == a.neigh):
             traffic = t["fetch_bytes"] + t["write_bytes"]
 
     # SNAP 2000 atoms (BCC W), FP64 and mixed
@@ -375,9 +425,12 @@ def main():
     if not a.no_snap:
         coeff, param, beta = snap_files(emd, decks, tmpdir)
         for prec in ("fp64", "mixed"):
-            sn, sw, sp, se = snap_run(emd, decks, tmpdir, coeff, param, prec, 10, a.snap_steps)
+            sn, sw, sp, se, sb = snap_run(emd, decks, tmpdir, coeff, param, prec, 10,
+                                          a.snap_steps)
             snap[prec] = {"value": sn * a.snap_steps / sw, "ms_per_step": 1e3 * sw / a.snap_steps,
-                          "kernels_ms_per_step": {k: v[1] / a.snap_steps for k, v in sp.items()},
+                          "snap_kernels_ms_per_step": {k: v[1] / a.snap_steps
+                                                       for k, v in sp.items()},
+                          "kernels_ms_per_step_untimed_breakdown": sb,
                           "etot_per_atom_end": se}
         snap["mixed_over_fp64"] = snap["mixed"]["value"] / snap["fp64"]["value"]
         # VALU roofline of the SNAP kernels: loop-bound FLOPs of the Y form
@@ -386,8 +439,7 @@ def main():
         from examinimd_amd.flopcount import snap_flops
         fl = snap_flops(8, 26)["total"]
         for prec, peak in (("fp64", FP64_PEAK_TFLOPS), ("mixed", FP32_PEAK_TFLOPS)):
-            kms = sum(v for k, v in snap[prec]["kernels_ms_per_step"].items()
-                      if k.startswith("snap"))
+            kms = sum(snap[prec]["snap_kernels_ms_per_step"].values())
             ach = fl * 2000 / (kms * 1e-3) / 1e12 if kms > 0 else None
             snap[prec]["roofline"] = {"bound": "valu", "achieved": ach, "peak": peak,
                                       "unit": "TFLOP/s",
@@ -416,7 +468,9 @@ def main():
         "data": "synthetic FCC lattice, seeded Gaussian velocities",
         "config": {"workload": f"LJ melt {natoms} atoms (FCC {n}^3, rho 0.8442, rc 2.5, "
                                f"skin 0.3, rebuild every 20), mixed precision",
-                   "atoms": natoms, "neighbors_per_atom": round(mixed["nlist"], 3)},
+                   "atoms": natoms, "neighbors_per_atom": round(mixed["nlist"], 3),
+                   "neigh": a.neigh},
+        "other_layout": alt,
         "fp64": {"value": value_fp64, "ms_per_step": 1e3 * fp64["wall"] / a.steps,
                  "dtype": "f64"},
         "mixed_over_fp64": value / value_fp64,
@@ -435,7 +489,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_atom * natoms,
                      "kernel": mixed["kernel"], "avg_ms": avg_ms,
                      "algorithmic_bytes_per_atom": bytes_per_atom},
-        "kernels_ms_per_step": {k: v[1] / a.steps for k, v in mixed["prof"].items()},
+        "kernels_ms_per_step_untimed_breakdown": mixed["breakdown"],
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.cpu_steps, a.cpu_threads)

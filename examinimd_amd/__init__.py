@@ -115,6 +115,9 @@ class App:
     def set_phase_sync(self, on):
         self.L.emd_app_set_phase_sync(self.h, 1 if on else 0)
 
+    def set_fused_integrate(self, on):
+        self.L.emd_app_set_fused_integrate(self.h, 1 if on else 0)
+
     @property
     def nsteps(self):
         return self.L.emd_app_nsteps(self.h)

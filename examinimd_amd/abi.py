@@ -75,6 +75,7 @@ HIP_PROTOS = {
     "emd_memcpy_d2d": (C.c_int, [vp, vp, vp, C.c_size_t]),
     "emd_profile_enable": (C.c_int, [vp, C.c_int]),
     "emd_profile_reset": (C.c_int, [vp]),
+    "emd_profile_select": (C.c_int, [vp, C.c_char_p]),
     "emd_profile_get": (C.c_int, [vp, C.c_char_p, P(C.c_double), P(C.c_long)]),
     "emd_profile_report": (C.c_size_t, [vp, C.c_char_p, C.c_size_t]),
     "emd_system_grow": (C.c_int, [vp, P(EmdSystem), C.c_int]),
@@ -96,12 +97,14 @@ HIP_PROTOS = {
     "emd_neigh_free": (C.c_int, [vp, P(EmdNeigh)]),
     "emd_force_zero": (C.c_int, [vp, P(EmdSystem), C.c_int]),
     "emd_lj_force": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int, C.c_int]),
+    "emd_lj_force_store": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int]),
     "emd_lj_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int,
                                 P(C.c_double)]),
     "emd_lj_set_lanes": (C.c_int, [vp, C.c_int, C.c_int]),
     "emd_lj_set_tile": (C.c_int, [vp, C.c_int, C.c_int]),
     "emd_nve_initial": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_double, C.c_int]),
     "emd_nve_final": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_int]),
+    "emd_nve_final_initial": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_double, C.c_int]),
     "emd_reduce_mv2": (C.c_int, [vp, P(EmdSystem), P(C.c_double)]),
     "emd_snap_create": (C.c_int, [vp, P(EmdSnapParams), P(vp)]),
     "emd_snap_destroy": (C.c_int, [vp, vp]),
@@ -130,6 +133,7 @@ APP_PROTOS = {
     "emd_app_init": (C.c_int, [vp]),
     "emd_app_run": (C.c_int, [vp, C.c_int]),
     "emd_app_set_phase_sync": (C.c_int, [vp, C.c_int]),
+    "emd_app_set_fused_integrate": (C.c_int, [vp, C.c_int]),
     "emd_app_destroy": (C.c_int, [vp]),
     "emd_app_nsteps": (C.c_int, [vp]),
     "emd_app_natoms": (C.c_int, [vp]),

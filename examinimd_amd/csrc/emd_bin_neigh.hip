@@ -1,10 +1,11 @@
 // emd_bin_neigh.hip -- cell binning (counting sort) and neighbour lists.
 //
 // Binning: bin id per atom (Kokkos BinOp3D arithmetic: int(mul*(x-min)),
-// mul = nbin/(max-min), x-major flattening), atomic histogram, exclusive
-// scan -> offsets, atomic scatter, then a per-bin insertion sort so that the
-// permutation lists each bin's atoms in ascending index (the order the
-// reference produces on its serial backend; deterministic here).
+// mul = nbin/(max-min), x-major flattening), then a stable LSD radix sort of
+// (bin id, index) pairs over the bin id's bits (rocPRIM), so the permutation
+// lists each bin's atoms in ascending index (the order the reference produces
+// on its serial backend; deterministic here); counts and offsets are the
+// bounds of each bin's run in the sorted keys.
 //
 // Neighbour build: one 256-thread block per interior bin.  The block stages
 // the 27 stencil bins' candidates (index + FP64 position) in LDS in stencil
@@ -14,13 +15,16 @@
 // with an exclusive scan between them, as NeighborCSR does.
 #include "emd_internal.hpp"
 
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
 __global__ void k_max_int(const int *__restrict__ a, int n, int *__restrict__ out);
 
 // ------------------------------------------------------------------ binning
-__global__ void k_bin_assign(const double *__restrict__ x, int r0, int n, double minx,
-                             double miny, double minz, double mulx, double muly, double mulz,
-                             int nbx, int nby, int nbz, int *__restrict__ binid,
-                             int *__restrict__ count, int *__restrict__ err) {
+// BinOp3D::bin (Kokkos_Sort.hpp) as used by binning_kksort.cpp:105-113
+__global__ void k_bin_key(const double *__restrict__ x, int r0, int n, double minx, double miny,
+                          double minz, double mulx, double muly, double mulz, int nbx, int nby,
+                          int nbz, unsigned *__restrict__ key, int *__restrict__ err) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double *xi = x + 3 * (size_t)(r0 + i);
@@ -33,45 +37,126 @@ __global__ void k_bin_assign(const double *__restrict__ x, int r0, int n, double
     by = by < 0 ? 0 : (by >= nby ? nby - 1 : by);
     bz = bz < 0 ? 0 : (bz >= nbz ? nbz - 1 : bz);
   }
-  const int b = (bx * nby + by) * nbz + bz;
-  binid[i] = b;
-  atomicAdd(&count[b], 1);
+  key[i] = (unsigned)((bx * nby + by) * nbz + bz);
 }
 
-struct bin_offsets_op {
-  const int *count;
-  int *offset;
-  int *cursor;
-  int *maxc;
-  __device__ int value(int b) const { return count[b]; }
-  __device__ void emit(int b, int excl, int v) const {
-    offset[b] = excl;
-    cursor[b] = excl;
+// first position in the sorted keys with key >= b
+__device__ inline int lower_bound_key(const unsigned *__restrict__ k, int n, unsigned b) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (k[mid] < b) lo = mid + 1;
+    else hi = mid;
   }
-};
-
-__global__ void k_bin_fill(const int *__restrict__ binid, int n, int *__restrict__ cursor,
-                           int *__restrict__ permute) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int slot = atomicAdd(&cursor[binid[i]], 1);
-  permute[slot] = i;
+  return lo;
 }
 
-__global__ void k_bin_sort(const int *__restrict__ count, const int *__restrict__ offset,
-                           int *__restrict__ permute, int nbins) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
+// bincount / binoffsets from the sorted keys: each bin's run [lb(b), lb(b+1))
+__global__ void k_bin_bounds(const unsigned *__restrict__ skey, int n, int nbins,
+                             int *__restrict__ count, int *__restrict__ offset) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbins) return;
-  int *p = permute + offset[b];
-  const int c = count[b];
-  for (int k = 1; k < c; k++) {
-    const int v = p[k];
-    int m = k - 1;
-    while (m >= 0 && p[m] > v) {
-      p[m + 1] = p[m];
-      m--;
+  const int lo = lower_bound_key(skey, n, (unsigned)b);
+  const int hi = lower_bound_key(skey, n, (unsigned)b + 1u);
+  offset[b] = lo;
+  count[b] = hi - lo;
+}
+
+// Small ranges (n <= BIN_ONE_MAX_N atoms, nbins <= BIN_ONE_MAX_BINS) in one
+// workgroup launch: bin ids, an LDS histogram, the exclusive scan of the
+// counts, an LDS scatter, then an insertion sort of each bin's slice of the
+// LDS permutation (ascending index: the radix path's order), the largest
+// count and the out-of-grid flag -- one launch instead of the key, sort,
+// bounds and max kernels.
+constexpr int BIN_ONE_THREADS = 1024;
+constexpr int BIN_ONE_MAX_BINS = 4096;
+constexpr int BIN_ONE_MAX_N = 7168;  // LDS: 2 x 16 KiB bin tables + 28 KiB permutation
+
+__global__ void __launch_bounds__(BIN_ONE_THREADS)
+k_bin_one(const double *__restrict__ x, int r0, int n, double minx, double miny, double minz,
+          double mulx, double muly, double mulz, int nbx, int nby, int nbz, int nbins,
+          int *__restrict__ bincount, int *__restrict__ binoffsets, int *__restrict__ permute,
+          int *__restrict__ small) {
+  __shared__ int s_cnt[BIN_ONE_MAX_BINS];  // counts, then scatter cursors
+  __shared__ int s_off[BIN_ONE_MAX_BINS];
+  __shared__ int s_perm[BIN_ONE_MAX_N];
+  __shared__ int s_w[BIN_ONE_THREADS / EMD_WAVE];
+  __shared__ int s_err, s_max;
+  const int tid = threadIdx.x, wid = tid / EMD_WAVE, lane = tid % EMD_WAVE;
+  for (int b = tid; b < nbins; b += BIN_ONE_THREADS) s_cnt[b] = 0;
+  if (tid == 0) { s_err = 0; s_max = 0; }
+  __syncthreads();
+  // pass 1: histogram (the bin id is recomputed in pass 2, not stored)
+  auto bin_of = [&](int i, bool &bad) {
+    const double *xi = x + 3 * (size_t)(r0 + i);
+    int bx = (int)(mulx * (xi[0] - minx));
+    int by = (int)(muly * (xi[1] - miny));
+    int bz = (int)(mulz * (xi[2] - minz));
+    bad = bx < 0 || bx >= nbx || by < 0 || by >= nby || bz < 0 || bz >= nbz;
+    bx = bx < 0 ? 0 : (bx >= nbx ? nbx - 1 : bx);
+    by = by < 0 ? 0 : (by >= nby ? nby - 1 : by);
+    bz = bz < 0 ? 0 : (bz >= nbz ? nbz - 1 : bz);
+    return (bx * nby + by) * nbz + bz;
+  };
+  for (int i = tid; i < n; i += BIN_ONE_THREADS) {
+    bool bad;
+    const int b = bin_of(i, bad);
+    if (bad) s_err = 1;
+    atomicAdd(&s_cnt[b], 1);
+  }
+  __syncthreads();
+  // exclusive scan of the counts, 1024 bins per round
+  int carry = 0, mymax = 0;
+  for (int base = 0; base < nbins; base += BIN_ONE_THREADS) {
+    const int b = base + tid;
+    const int v = b < nbins ? s_cnt[b] : 0;
+    mymax = v > mymax ? v : mymax;
+    const int inc = emd_wave_incl_scan(v);
+    if (lane == EMD_WAVE - 1) s_w[wid] = inc;
+    __syncthreads();
+    int woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < BIN_ONE_THREADS / EMD_WAVE; w++) {
+      const int sw = s_w[w];
+      woff += (w < wid) ? sw : 0;
+      tot += sw;
     }
-    p[m + 1] = v;
+    if (b < nbins) {
+      const int off = carry + woff + inc - v;
+      bincount[b] = v;
+      binoffsets[b] = off;
+      s_off[b] = off;
+      s_cnt[b] = off;  // scatter cursor
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  atomicMax(&s_max, mymax);
+  // pass 2: scatter (any order), then sort each bin's slice
+  for (int i = tid; i < n; i += BIN_ONE_THREADS) {
+    bool bad;
+    const int b = bin_of(i, bad);
+    s_perm[atomicAdd(&s_cnt[b], 1)] = i;
+  }
+  __syncthreads();
+  for (int b = tid; b < nbins; b += BIN_ONE_THREADS) {
+    int *p = s_perm + s_off[b];
+    const int c = s_cnt[b] - s_off[b];
+    for (int k = 1; k < c; k++) {
+      const int v = p[k];
+      int m = k - 1;
+      while (m >= 0 && p[m] > v) {
+        p[m + 1] = p[m];
+        m--;
+      }
+      p[m + 1] = v;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += BIN_ONE_THREADS) permute[i] = s_perm[i];
+  if (tid == 0) {
+    small[1] = s_max;
+    small[2] = s_err;
   }
 }
 
@@ -100,6 +185,53 @@ __global__ void k_bin_gather(const int *__restrict__ perm, int r0, int n,
 __global__ void k_fill_int(int *p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// the sorted arrays back into the system's range, one launch for all six
+__global__ void k_bin_copy_back(int r0, int n, const double *__restrict__ tx,
+                                const double *__restrict__ tv, const double *__restrict__ tf,
+                                const double *__restrict__ tq, const int *__restrict__ tt,
+                                const int *__restrict__ ti, double *__restrict__ x,
+                                double *__restrict__ v, double *__restrict__ f,
+                                double *__restrict__ q, int *__restrict__ type,
+                                int *__restrict__ id) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t s3 = 3 * (size_t)i, d3 = 3 * ((size_t)r0 + i);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    x[d3 + d] = tx[s3 + d];
+    v[d3 + d] = tv[s3 + d];
+    f[d3 + d] = tf[s3 + d];
+  }
+  q[r0 + i] = tq[i];
+  type[r0 + i] = tt[i];
+  id[r0 + i] = ti[i];
+}
+
+// common tail: largest count and out-of-grid flag to the host (the one sync
+// of a binning), then BinSort::sort of the range when asked
+static emd_status bin_finish(emd_ctx *ctx, emd_system *s, emd_bins *b, int r0, int n, int sort) {
+  EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, 2 * sizeof(int),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  b->max_count = ctx->h_small[1];
+  if (ctx->h_small[2])
+    return emd_set_error(ctx, EMD_ERR_OVERFLOW,
+                         "emd_binning: an atom lies outside the bin grid (lost atom?)");
+  if (sort && n > 0) {
+    size_t bytes = (size_t)n * (72 + 8 + 8);
+    EMD_TRY(emd_ensure(ctx, &ctx->d_tmp, &ctx->tmp_cap, bytes));
+    char *t = (char *)ctx->d_tmp;
+    double *tx = (double *)t, *tv = tx + 3 * (size_t)n, *tf = tv + 3 * (size_t)n,
+           *tq = tf + 3 * (size_t)n;
+    int *tt = (int *)(tq + n), *ti = tt + n;
+    EMD_LAUNCH(ctx, "binning_sort", k_bin_gather, emd_blocks(n, 256), 256, 0, b->permute, r0, n,
+               s->x, s->v, s->f, s->q, s->type, s->id, tx, tv, tf, tq, tt, ti);
+    EMD_LAUNCH(ctx, "binning_sort", k_bin_copy_back, emd_blocks(n, 256), 256, 0, r0, n, tx, tv,
+               tf, tq, tt, ti, s->x, s->v, s->f, s->q, s->type, s->id);
+  }
+  return EMD_OK;
 }
 
 extern "C" emd_status emd_binning(emd_ctx *ctx, emd_system *s, double dx_in, double dy_in,
@@ -144,55 +276,43 @@ extern "C" emd_status emd_binning(emd_ctx *ctx, emd_system *s, double dx_in, dou
   EMD_TRY(emd_ensure_int(ctx, &b->permute, &b->cap_range, n));
   EMD_TRY(emd_ensure_int(ctx, &b->binid, &cap_r2, n));
   b->cap_range = b->cap_range < cap_r2 ? b->cap_range : cap_r2;
-  // cursor scratch
-  EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)nbt * sizeof(int)));
-  int *cursor = (int *)ctx->d_tmp2;
+  if (ctx->bin_one && n <= BIN_ONE_MAX_N && nbt <= BIN_ONE_MAX_BINS) {
+    EMD_LAUNCH(ctx, "binning", k_bin_one, 1, BIN_ONE_THREADS, 0, s->x, r0, n, mn[0], mn[1], mn[2],
+               mul[0], mul[1], mul[2], nb[0], nb[1], nb[2], (int)nbt, b->bincount,
+               b->binoffsets, b->permute, ctx->d_small);
+    return bin_finish(ctx, s, b, r0, n, sort);
+  }
+  // scratch: unsorted keys (binid), sorted keys, radix-sort workspace
+  unsigned *key = reinterpret_cast<unsigned *>(b->binid);
+  int kbits = 1;
+  while (kbits < 31 && (1l << kbits) < nbt) kbits++;
+  size_t sort_bytes = 0;
+  const rocprim::counting_iterator<int> iota(0);
+  EMD_HIP(ctx, rocprim::radix_sort_pairs(nullptr, sort_bytes, key, (unsigned *)nullptr, iota,
+                                         b->permute, n > 0 ? n : 1, 0u, (unsigned)kbits,
+                                         ctx->stream));
+  const size_t skey_bytes = ((size_t)(n > 0 ? n : 1) * sizeof(unsigned) + 255) & ~(size_t)255;
+  EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, skey_bytes + sort_bytes));
+  unsigned *skey = reinterpret_cast<unsigned *>(ctx->d_tmp2);
+  void *sort_tmp = static_cast<char *>(ctx->d_tmp2) + skey_bytes;
 
-  EMD_LAUNCH(ctx, "binning", k_fill_int, emd_blocks(nbt, 256), 256, 0, b->bincount, (int)nbt, 0);
   EMD_LAUNCH(ctx, "binning", k_fill_int, 1, 64, 0, ctx->d_small + 1, 2, 0);
-  if (n > 0)
-    EMD_LAUNCH(ctx, "binning", k_bin_assign, emd_blocks(n, 256), 256, 0, s->x, r0, n, mn[0], mn[1],
-               mn[2], mul[0], mul[1], mul[2], nb[0], nb[1], nb[2], b->binid, b->bincount,
-               ctx->d_small + 2);
-  bin_offsets_op op{b->bincount, b->binoffsets, cursor, ctx->d_small + 1};
-  EMD_TRY(emd_scan(ctx, op, (int)nbt, "binning", nullptr));
+  if (n > 0) {
+    EMD_LAUNCH(ctx, "binning", k_bin_key, emd_blocks(n, 256), 256, 0, s->x, r0, n, mn[0], mn[1],
+               mn[2], mul[0], mul[1], mul[2], nb[0], nb[1], nb[2], key, ctx->d_small + 2);
+    emd_prof_rec rec;
+    emd_prof_begin(ctx, "binning_sort_keys", &rec);
+    EMD_HIP(ctx, rocprim::radix_sort_pairs(sort_tmp, sort_bytes, key, skey, iota, b->permute, n,
+                                           0u, (unsigned)kbits, ctx->stream));
+    emd_prof_end(ctx, &rec);
+  }
+  EMD_LAUNCH(ctx, "binning", k_bin_bounds, emd_blocks(nbt, 256), 256, 0, skey, n, (int)nbt,
+             b->bincount, b->binoffsets);
   {
     const int nbm = emd_blocks(nbt, 256) < 512 ? emd_blocks(nbt, 256) : 512;
     EMD_LAUNCH(ctx, "binning", k_max_int, nbm, 256, 0, b->bincount, (int)nbt, ctx->d_small + 1);
   }
-  if (n > 0) {
-    EMD_LAUNCH(ctx, "binning", k_bin_fill, emd_blocks(n, 256), 256, 0, b->binid, n, cursor,
-               b->permute);
-    EMD_LAUNCH(ctx, "binning", k_bin_sort, emd_blocks(nbt, 128), 128, 0, b->bincount,
-               b->binoffsets, b->permute, (int)nbt);
-  }
-  EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, 2 * sizeof(int),
-                              hipMemcpyDeviceToHost, ctx->stream));
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  b->max_count = ctx->h_small[1];
-  if (ctx->h_small[2])
-    return emd_set_error(ctx, EMD_ERR_OVERFLOW,
-                         "emd_binning: an atom lies outside the bin grid (lost atom?)");
-
-  if (sort && n > 0) {
-    size_t bytes = (size_t)n * (72 + 8 + 8);
-    EMD_TRY(emd_ensure(ctx, &ctx->d_tmp, &ctx->tmp_cap, bytes));
-    char *t = (char *)ctx->d_tmp;
-    double *tx = (double *)t, *tv = tx + 3 * (size_t)n, *tf = tv + 3 * (size_t)n,
-           *tq = tf + 3 * (size_t)n;
-    int *tt = (int *)(tq + n), *ti = tt + n;
-    EMD_LAUNCH(ctx, "binning_sort", k_bin_gather, emd_blocks(n, 256), 256, 0, b->permute, r0, n,
-               s->x, s->v, s->f, s->q, s->type, s->id, tx, tv, tf, tq, tt, ti);
-    hipStream_t st = ctx->stream;
-    const size_t o3 = 3 * (size_t)r0, n3 = 3 * (size_t)n * sizeof(double);
-    EMD_HIP(ctx, hipMemcpyAsync(s->x + o3, tx, n3, hipMemcpyDeviceToDevice, st));
-    EMD_HIP(ctx, hipMemcpyAsync(s->v + o3, tv, n3, hipMemcpyDeviceToDevice, st));
-    EMD_HIP(ctx, hipMemcpyAsync(s->f + o3, tf, n3, hipMemcpyDeviceToDevice, st));
-    EMD_HIP(ctx, hipMemcpyAsync(s->q + r0, tq, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
-    EMD_HIP(ctx, hipMemcpyAsync(s->type + r0, tt, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
-    EMD_HIP(ctx, hipMemcpyAsync(s->id + r0, ti, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
-  }
-  return EMD_OK;
+  return bin_finish(ctx, s, b, r0, n, sort);
 }
 
 extern "C" emd_status emd_bins_free(emd_ctx *ctx, emd_bins *b) {
@@ -209,7 +329,7 @@ extern "C" emd_status emd_bins_free(emd_ctx *ctx, emd_bins *b) {
 // ---------------------------------------------------------------- neighbours
 constexpr int NB_THREADS = 256;
 constexpr int NB_WAVES = NB_THREADS / EMD_WAVE;
-constexpr int NB_MAXC = 1024;  // candidates staged per LDS chunk
+constexpr int NB_MAXC = 640;  // candidates staged per LDS chunk (27 bins of an LJ melt: ~500)
 
 // NeighborCSR Tag{Count,Fill}Neigh{Full,Half}, src/neighbor_types/neighbor_csr.h:176-358
 template <bool HALF, bool FILL>
@@ -270,34 +390,45 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
       if (cs > 0) run = FILL ? cursor[i] : num[i];
       size_t base = 0;
       if (FILL) base = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
-      for (int c0 = 0; c0 < cn; c0 += EMD_WAVE) {
-        const int c = c0 + lane;
-        bool keep = false;
-        int j = -1;
-        if (c < cn) {
-          j = s_j[c];
-          const double x_j = s_x[c], y_j = s_y[c], z_j = s_z[c];
-          const double dx = x_i - x_j, dy = y_i - y_j, dz = z_i - z_j;
-          const double rsq = dx * dx + dy * dy + dz * dz;
-          if (HALF) {
-            const bool drop =
-                ((j == i) || (j < n_local || newton)) &&
-                !((x_j > x_i) || ((x_j == x_i) && ((y_j > y_i) || ((y_j == y_i) && (z_j > z_i)))));
-            keep = !drop && (rsq <= cutsq);
-          } else {
-            keep = (rsq <= cutsq) && (i != j);
-          }
+      // candidate test of position c of the staged list (the reference's
+      // predicate, neighbor_csr.h:206 full / 290 half)
+      auto test = [&](int c, int &j) -> bool {
+        if (c >= cn) return false;
+        j = s_j[c];
+        const double x_j = s_x[c], y_j = s_y[c], z_j = s_z[c];
+        const double dx = x_i - x_j, dy = y_i - y_j, dz = z_i - z_j;
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        if (HALF) {
+          const bool drop =
+              ((j == i) || (j < n_local || newton)) &&
+              !((x_j > x_i) || ((x_j == x_i) && ((y_j > y_i) || ((y_j == y_i) && (z_j > z_i)))));
+          return !drop && (rsq <= cutsq);
         }
-        const unsigned long long m = __ballot(keep);
+        return (rsq <= cutsq) && (i != j);
+      };
+      auto put = [&](bool keep, unsigned long long m, int at, int c, int j) {
         if (FILL && keep) {
-          const int pos = run + __popcll(m & lt);
+          const int pos = at + __popcll(m & lt);
           if (pos < cap) {
             const size_t e = base + pos;
             entries[e] = j;
             if (slot) slot[e] = (unsigned short)(cs + c);
           }
         }
-        run += __popcll(m);
+      };
+      // two 64-candidate groups per iteration: independent LDS reads and
+      // distance tests in flight together; entries keep candidate order
+      for (int c0 = 0; c0 < cn; c0 += 2 * EMD_WAVE) {
+        const int ca = c0 + lane, cb = c0 + EMD_WAVE + lane;
+        int ja = -1, jb = -1;
+        const bool ka = test(ca, ja);
+        const bool kb = test(cb, jb);
+        const unsigned long long ma = __ballot(ka);
+        const unsigned long long mb = __ballot(kb);
+        put(ka, ma, run, ca, ja);
+        const int runb = run + __popcll(ma);
+        put(kb, mb, runb, cb, jb);
+        run = runb + __popcll(mb);
       }
       if (lane == 0) {
         if (FILL) {

@@ -72,10 +72,124 @@ struct halo_phase_op {
   }
 };
 
+// All six exchange_halo phases in one launch of one workgroup, for small
+// systems where the per-phase scans are launch- and sync-bound.  Each phase
+// is the same stable compaction as the scan path (ascending source index,
+// halo_phase_op's predicate and emit), 1024 candidates per round; a device
+// fence and a barrier separate phases, whose candidates include the ghosts
+// the previous phases wrote.  Capacities (system arrays, pack lists, the
+// root/image tables) are checked per ghost: a phase that does not fit is
+// still counted but not written, and the kernel stops there, reporting the
+// phase; the host grows and runs it again.
+constexpr int HALO_ONE_THREADS = 1024;
+constexpr int HALO_ONE_MAX = 16384;  // local + ghost atoms; above, the per-phase scans
+
+struct halo_one_args {
+  halo_phase_op op[6];
+  int cap_pack[6];
+  int n_local, n_max, root_cap;
+  int *out;  // [0..5] ghosts per phase, [6] 1 + the phase that did not fit (0: all fit)
+};
+
+__global__ void __launch_bounds__(HALO_ONE_THREADS) k_halo_one(halo_one_args a) {
+  __shared__ int s_w[HALO_ONE_THREADS / EMD_WAVE];
+  const int tid = threadIdx.x, wid = tid / EMD_WAVE, lane = tid % EMD_WAVE;
+  int n_ghost = 0, prev = 0, failed = 0;
+  for (int phase = 0; phase < 6 && !failed; phase++) {
+    halo_phase_op op = a.op[phase];
+    const int ncand = a.n_local + n_ghost - ((phase & 1) ? prev : 0);
+    op.dst0 = a.n_local + n_ghost;
+    const int room = min(min(a.n_max - op.dst0, a.cap_pack[phase]), a.root_cap - n_ghost);
+    int k = 0;
+    for (int base = 0; base < ncand; base += HALO_ONE_THREADS) {
+      const int i = base + tid;
+      const int v = (i < ncand) ? op.value(i) : 0;
+      const int inc = emd_wave_incl_scan(v);
+      if (lane == EMD_WAVE - 1) s_w[wid] = inc;
+      __syncthreads();
+      int woff = 0, tot = 0;
+#pragma unroll
+      for (int w = 0; w < HALO_ONE_THREADS / EMD_WAVE; w++) {
+        const int sw = s_w[w];
+        woff += (w < wid) ? sw : 0;
+        tot += sw;
+      }
+      const int pos = k + woff + inc - v;
+      if (v && pos < room) op.emit(i, pos, v);
+      k += tot;
+      __syncthreads();  // s_w is rewritten by the next round
+    }
+    if (tid == 0) a.out[phase] = k;
+    if (k > room) failed = phase + 1;
+    prev = k;
+    n_ghost += k;
+    __threadfence();   // this phase's ghosts are the next phase's candidates
+    __syncthreads();
+  }
+  if (tid == 0) a.out[6] = failed;
+}
+
+static emd_status exchange_halo_one(emd_ctx *ctx, emd_system *s, double comm_depth,
+                                    emd_halo *h) {
+  const int n_local = s->n_local;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    halo_one_args a;
+    for (int phase = 0; phase < 6; phase++) {
+      const int dim = phase / 2, hi_side = (phase % 2 == 0);
+      halo_phase_op &op = a.op[phase];
+      op.dim = dim;
+      op.hi_side = hi_side;
+      op.thr = hi_side ? s->hi[dim] - comm_depth : s->lo[dim] + comm_depth;
+      op.shift = hi_side ? -s->domain[dim] : s->domain[dim];
+      op.dst0 = 0;  // set per phase by the kernel
+      op.n_local = n_local;
+      op.code = (hi_side ? 1 : 2) << (2 * dim);
+      op.x = s->x; op.vel = s->v; op.q = s->q; op.type = s->type; op.id = s->id;
+      op.pack = h->pack[phase];
+      op.root = h->root;
+      op.image = h->image;
+      a.cap_pack[phase] = h->pack[phase] ? h->cap[phase] : 0;
+    }
+    a.n_local = n_local;
+    a.n_max = s->n_max;
+    a.root_cap = (h->root && h->image) ? h->root_cap : 0;
+    a.out = ctx->d_small + 8;
+    EMD_LAUNCH(ctx, "comm_exchange_halo", k_halo_one, 1, HALO_ONE_THREADS, 0, a);
+    EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 8, ctx->d_small + 8, 7 * sizeof(int),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int *cnt = ctx->h_small + 8;
+    const int failed = cnt[6];
+    int n_ghost = 0;
+    for (int p = 0; p < (failed ? failed : 6); p++) n_ghost += cnt[p];
+    if (!failed) {
+      for (int p = 0; p < 6; p++) h->num_ghost[p] = cnt[p];
+      s->n_ghost = n_ghost;
+      h->n_ghost = n_ghost;
+      return EMD_OK;
+    }
+    // grow what the failed phase needs, with room for the phases after it
+    const int fp = failed - 1;
+    long want = (long)n_local + 2l * n_ghost + 64;
+    if (want > s->n_max) EMD_TRY(emd_system_grow(ctx, s, (int)(want + want / 10)));
+    EMD_TRY(emd_ensure_int(ctx, &h->pack[fp], &h->cap[fp], cnt[fp] + cnt[fp] / 8 + 64));
+    for (int p = fp + 1; p < 6; p++)
+      EMD_TRY(emd_ensure_int(ctx, &h->pack[p], &h->cap[p], cnt[fp] + cnt[fp] / 2 + 64));
+    int rcap = h->root ? h->root_cap : 0, icap = h->image ? h->root_cap : 0;
+    const long rwant = 2l * n_ghost + 64;
+    EMD_TRY(emd_grow_keep_int(ctx, &h->root, &rcap, rwant, 0));
+    EMD_TRY(emd_grow_keep_int(ctx, &h->image, &icap, rwant, 0));
+    h->root_cap = rcap < icap ? rcap : icap;
+  }
+  return emd_set_error(ctx, EMD_ERR_STATE, "emd_comm_exchange_halo: capacities did not settle");
+}
+
 extern "C" emd_status emd_comm_exchange_halo(emd_ctx *ctx, emd_system *s, double comm_depth,
                                              emd_halo *h) {
   if (!ctx || !s || !h) return EMD_ERR_ARG;
   const int n_local = s->n_local;
+  if (ctx->halo_one && (long)n_local + h->n_ghost <= HALO_ONE_MAX && n_local <= HALO_ONE_MAX / 2)
+    return exchange_halo_one(ctx, s, comm_depth, h);
   int n_ghost = 0;
   for (int phase = 0; phase < 6; phase++) {
     const int dim = phase / 2;
@@ -163,12 +277,62 @@ __global__ void k_halo_fold(T *__restrict__ f, const int *__restrict__ pack, int
   f[3 * (size_t)i + 2] += fz;
 }
 
+// all six phases in one launch of one workgroup, phases in reverse order with
+// a workgroup barrier between them (a phase reads ghost forces that later
+// phases folded into it); per atom the same additions in the same order as
+// the per-phase launches.  Within a phase the destinations pack[p][k] are
+// distinct atoms below the phase's ghost range, so no two lanes write one
+// address and no lane reads what another writes.
+struct fold_phases {
+  const int *pack[6];
+  int num[6];
+  int off[6];
+};
+constexpr int FOLD_ONE_THREADS = 1024;
+constexpr int FOLD_ONE_MAX = 16384;  // ghosts; above this the per-phase launches are faster
+
+template <class T>
+__global__ void __launch_bounds__(FOLD_ONE_THREADS)
+k_halo_fold_one(T *__restrict__ f, fold_phases ph) {
+  for (int p = 5; p >= 0; p--) {
+    const int n = ph.num[p], off = ph.off[p];
+    const int *__restrict__ pack = ph.pack[p];
+    for (int ii = threadIdx.x; ii < n; ii += FOLD_ONE_THREADS) {
+      const int i = pack[ii];
+      const size_t g = 3 * (size_t)(off + ii);
+      const T fx = f[g], fy = f[g + 1], fz = f[g + 2];
+      f[3 * (size_t)i] += fx;
+      f[3 * (size_t)i + 1] += fy;
+      f[3 * (size_t)i + 2] += fz;
+    }
+    __syncthreads();
+  }
+}
+
 extern "C" emd_status emd_comm_update_force(emd_ctx *ctx, emd_system *s, const emd_halo *h,
                                             int precision) {
   if (!ctx || !s || !h) return EMD_ERR_ARG;
   int off[6];
   off[0] = s->n_local;
   for (int p = 1; p < 6; p++) off[p] = off[p - 1] + h->num_ghost[p - 1];
+  long total = 0;
+  for (int p = 0; p < 6; p++) total += h->num_ghost[p] > 0 ? h->num_ghost[p] : 0;
+  if (total == 0) return EMD_OK;
+  if (total <= FOLD_ONE_MAX && ctx->fold_one) {
+    fold_phases ph;
+    for (int p = 0; p < 6; p++) {
+      ph.pack[p] = h->pack[p];
+      ph.num[p] = h->num_ghost[p] > 0 ? h->num_ghost[p] : 0;
+      ph.off[p] = off[p];
+    }
+    if (precision == EMD_MIXED)
+      EMD_LAUNCH(ctx, "comm_update_force", k_halo_fold_one<float>, 1, FOLD_ONE_THREADS, 0,
+                 s->f32, ph);
+    else
+      EMD_LAUNCH(ctx, "comm_update_force", k_halo_fold_one<double>, 1, FOLD_ONE_THREADS, 0, s->f,
+                 ph);
+    return EMD_OK;
+  }
   for (int p = 5; p >= 0; p--) {
     const int n = h->num_ghost[p];
     if (n <= 0) continue;

@@ -81,6 +81,15 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
     const int t = atoi(e);
     if (t >= 0 && t <= 64) ctx->visit_tile = t;
   }
+  // ghost force fold of a small halo (<= 16384 ghosts) in one launch (1) or
+  // one launch per phase (0)
+  if (const char *e = getenv("EMD_FOLD_ONE")) ctx->fold_one = atoi(e) != 0;
+  // exchange_halo of a small system (<= 16384 local + ghost atoms) in one
+  // launch (1) or as six scans with a host sync each (0)
+  if (const char *e = getenv("EMD_HALO_ONE")) ctx->halo_one = atoi(e) != 0;
+  // binning of a small range (<= 7168 atoms, <= 4096 bins) in one launch
+  if (const char *e = getenv("EMD_BIN_ONE")) ctx->bin_one = atoi(e) != 0;
+  if (const char *e = getenv("EMD_SNAP_YFLAT")) ctx->snap_yflat = atoi(e) != 0;
   // pipelined LJ kernel: blocks per XCD (0, the default, keeps k_lj_full;
   // measured slower at LJ 4M: 0.83 vs 0.74 ms for 192 blocks per XCD)
   if (const char *e = getenv("EMD_LJ_PIPE_BLOCKS")) {
@@ -227,6 +236,11 @@ static hipEvent_t prof_event(emd_ctx *ctx) {
 void emd_prof_begin(emd_ctx *ctx, const char *name, emd_prof_rec *rec) {
   rec->name = -1;
   if (!ctx->profile) return;
+  if (!ctx->prof_select.empty()) {
+    bool hit = false;
+    for (const auto &sel : ctx->prof_select) hit = hit || sel == name;
+    if (!hit) return;
+  }
   rec->name = prof_id(ctx, name);
   rec->start = prof_event(ctx);
   rec->stop = prof_event(ctx);
@@ -258,6 +272,24 @@ extern "C" emd_status emd_profile_enable(emd_ctx *ctx, int on) {
   if (!ctx) return EMD_ERR_ARG;
   prof_collect(ctx);
   ctx->profile = on != 0;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_profile_select(emd_ctx *ctx, const char *names) {
+  if (!ctx) return EMD_ERR_ARG;
+  prof_collect(ctx);
+  ctx->prof_select.clear();
+  if (!names) return EMD_OK;
+  std::string cur;
+  for (const char *p = names;; p++) {
+    if (*p == ',' || *p == 0) {
+      if (!cur.empty()) ctx->prof_select.push_back(cur);
+      cur.clear();
+      if (*p == 0) break;
+    } else if (*p != ' ') {
+      cur += *p;
+    }
+  }
   return EMD_OK;
 }
 

@@ -56,7 +56,7 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
           const int *__restrict__ row, const int *__restrict__ num,
           const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
           double cutsq_0, const lj_table *__restrict__ tab, const int *__restrict__ visit,
-          int n_visit, double *__restrict__ f, float *__restrict__ f32) {
+          int n_visit, int store, double *__restrict__ f, float *__restrict__ f32) {
 #pragma clang fp contract(off)
   typedef typename lj_acc<MIXED>::T T;
   const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
@@ -122,14 +122,16 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
     fzi = group_sum<K>(fzi);
   }
   if (sub == 0) {
+    // store: the row is the atom's whole force (full list), written without
+    // reading the zeroed array first (0 + v == v)
     if (MIXED) {
-      f32[3 * (size_t)i] += (float)fxi;
-      f32[3 * (size_t)i + 1] += (float)fyi;
-      f32[3 * (size_t)i + 2] += (float)fzi;
+      float *o = f32 + 3 * (size_t)i;
+      if (store) { o[0] = (float)fxi; o[1] = (float)fyi; o[2] = (float)fzi; }
+      else { o[0] += (float)fxi; o[1] += (float)fyi; o[2] += (float)fzi; }
     } else {
-      f[3 * (size_t)i] += fxi;
-      f[3 * (size_t)i + 1] += fyi;
-      f[3 * (size_t)i + 2] += fzi;
+      double *o = f + 3 * (size_t)i;
+      if (store) { o[0] = fxi; o[1] = fyi; o[2] = fzi; }
+      else { o[0] += fxi; o[1] += fyi; o[2] += fzi; }
     }
   }
 }
@@ -482,14 +484,15 @@ static emd_status launch_pipe(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
 
 template <int K, int U>
 static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
-                              const emd_lj_params *p, int precision, const lj_table *tab) {
+                              const emd_lj_params *p, int precision, const lj_table *tab,
+                              int store) {
   const int n = s->n_local;
   const int nb = emd_blocks((long)(nl->visit ? nl->n_visit : n) * K, LJ_THREADS);
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   const bool one = p->ntypes == 1;
 #define LJ_FULL_ARGS                                                                        \
   s->x, s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],   \
-      p->cutsq[0][0], tab, nl->visit, nl->n_visit, s->f, s->f32
+      p->cutsq[0][0], tab, nl->visit, nl->n_visit, store, s->f, s->f32
   if (precision == EMD_MIXED) {
     if (one) EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, true, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
     else EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, false, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
@@ -537,12 +540,24 @@ static int tile_capacity(const emd_ctx *ctx, const emd_neigh *nl, const emd_lj_p
   return t <= LJ_TILE_MAX_CAND ? (int)t : 0;
 }
 
-extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
-                                   const emd_lj_params *p, int precision, int half) {
+static emd_status lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                           const emd_lj_params *p, int precision, int half, int store) {
   if (!ctx || !s || !nl || !p) return EMD_ERR_ARG;
   EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
   EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
+  EMD_ARG(ctx, !(store && half), "storing forces needs a full list");
   if (s->n_local <= 0) return EMD_OK;
+  if (store) {
+    // kernels without a store form accumulate into zeroed local forces
+    const int kr = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
+    const int kk = kr ? kr : default_lanes(precision);
+    const bool tile = (kr == 0 || kr >= 4) && tile_capacity(ctx, nl, p, 0) > 0;
+    const bool pipe = ctx->lj_pipe_blocks > 0 && p->ntypes == 1 && kk >= 4;
+    if (tile || pipe) {
+      EMD_TRY(emd_force_zero(ctx, s, precision));
+      store = 0;
+    }
+  }
   // an explicit request for 1 or 2 lanes per atom keeps the CSR kernel
   // (1 lane reproduces the reference's serial sum bit for bit)
   const int kreq = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
@@ -582,19 +597,29 @@ extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh 
     }
   }
   switch (K) {
-    case 1: return launch_full<1, 4>(ctx, s, nl, p, precision, tab);
-    case 2: return launch_full<2, 4>(ctx, s, nl, p, precision, tab);
-    case 4: return launch_full<4, 4>(ctx, s, nl, p, precision, tab);
-    case 8: return launch_full<8, 5>(ctx, s, nl, p, precision, tab);
+    case 1: return launch_full<1, 4>(ctx, s, nl, p, precision, tab, store);
+    case 2: return launch_full<2, 4>(ctx, s, nl, p, precision, tab, store);
+    case 4: return launch_full<4, 4>(ctx, s, nl, p, precision, tab, store);
+    case 8: return launch_full<8, 5>(ctx, s, nl, p, precision, tab, store);
     default:
       switch (ctx->lj_unroll) {
-        case 2: return launch_full<16, 2>(ctx, s, nl, p, precision, tab);
-        case 3: return launch_full<16, 3>(ctx, s, nl, p, precision, tab);
-        case 4: return launch_full<16, 4>(ctx, s, nl, p, precision, tab);
-        case 6: return launch_full<16, 6>(ctx, s, nl, p, precision, tab);
-        default: return launch_full<16, 5>(ctx, s, nl, p, precision, tab);
+        case 2: return launch_full<16, 2>(ctx, s, nl, p, precision, tab, store);
+        case 3: return launch_full<16, 3>(ctx, s, nl, p, precision, tab, store);
+        case 4: return launch_full<16, 4>(ctx, s, nl, p, precision, tab, store);
+        case 6: return launch_full<16, 6>(ctx, s, nl, p, precision, tab, store);
+        default: return launch_full<16, 5>(ctx, s, nl, p, precision, tab, store);
       }
   }
+}
+
+extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                   const emd_lj_params *p, int precision, int half) {
+  return lj_force(ctx, s, nl, p, precision, half, 0);
+}
+
+extern "C" emd_status emd_lj_force_store(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                         const emd_lj_params *p, int precision) {
+  return lj_force(ctx, s, nl, p, precision, 0, 1);
 }
 
 
@@ -794,6 +819,48 @@ __global__ void k_nve_final(double *__restrict__ v, const double *__restrict__ f
   v[o] += dtfm * f0;
   v[o + 1] += dtfm * f1;
   v[o + 2] += dtfm * f2;
+}
+
+// FinalIntegrateFunctor of one step followed by InitialIntegrateFunctor of
+// the next (src/integrator_nve.cpp:104-112 then 65-74) in one pass over the
+// atoms: the same two velocity half-kicks and drift, in the same order, with
+// one read of x, v, f and one write of x, v instead of two kernels
+template <bool MIXED>
+__global__ void k_nve_final_initial(double *__restrict__ x, double *__restrict__ v,
+                                    const double *__restrict__ f, const float *__restrict__ f32,
+                                    const int *__restrict__ type,
+                                    const double *__restrict__ mass, int n, double dtf,
+                                    double dtv) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dtfm = dtf / mass[type[i]];
+  const size_t o = 3 * (size_t)i;
+  const double f0 = MIXED ? (double)f32[o] : f[o];
+  const double f1 = MIXED ? (double)f32[o + 1] : f[o + 1];
+  const double f2 = MIXED ? (double)f32[o + 2] : f[o + 2];
+  double v0 = v[o] + dtfm * f0, v1 = v[o + 1] + dtfm * f1, v2 = v[o + 2] + dtfm * f2;
+  v0 = v0 + dtfm * f0;
+  v1 = v1 + dtfm * f1;
+  v2 = v2 + dtfm * f2;
+  v[o] = v0; v[o + 1] = v1; v[o + 2] = v2;
+  x[o] += dtv * v0;
+  x[o + 1] += dtv * v1;
+  x[o + 2] += dtv * v2;
+}
+
+extern "C" emd_status emd_nve_final_initial(emd_ctx *ctx, emd_system *s, double dtf, double dtv,
+                                            int precision) {
+  if (!ctx || !s) return EMD_ERR_ARG;
+  const int n = s->n_local;
+  if (n <= 0) return EMD_OK;
+  if (precision == EMD_MIXED)
+    EMD_LAUNCH(ctx, "nve_final_initial", k_nve_final_initial<true>, emd_blocks(n, 256), 256, 0,
+               s->x, s->v, s->f, s->f32, s->type, s->mass, n, dtf, dtv);
+  else
+    EMD_LAUNCH(ctx, "nve_final_initial", k_nve_final_initial<false>, emd_blocks(n, 256), 256, 0,
+               s->x, s->v, s->f, s->f32, s->type, s->mass, n, dtf, dtv);
+  return EMD_OK;
 }
 
 extern "C" emd_status emd_nve_initial(emd_ctx *ctx, emd_system *s, double dtf, double dtv,

@@ -30,8 +30,12 @@
 //                 (LDS layer buffers), contraction with Y_J (staged in LDS
 //                 per block), wavefront reduction, atomic scatter to f_i / f_j
 //   k_snap_energy one block per atom: B_k from U_tot and Z, energy
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <map>
+#include <queue>
+#include <set>
 #include <tuple>
 #include <vector>
 
@@ -363,6 +367,90 @@ k_snap_y(snap_dev_tables t, const double *__restrict__ cg, const ytab_entry *__r
   }
 }
 
+// Y of one atom per block from a flat term stream.  Every half-plane entry
+// e = (J, ma, mb) is the sum over its (a, b) keys and (ma1, mb1) of
+//   w * U_a(ma1, mb1) * U_b(ma2, mb2),  w = coef(a,b,J) cg(ma1,ma2) cg(mb1,mb2)
+// (the double sum of SNA::compute_zi, sna_impl.hpp:221-274, weighted as in
+// compute_dbidrj's three roles).  The host lists these products (zero
+// Clebsch-Gordan weights dropped), cuts each entry's list into chunks of at
+// most YF_CHUNK terms, deals the chunks to the block's threads by longest
+// first onto the least loaded, and interleaves the threads' terms so that
+// step s of every thread is one coalesced load.  A term's code carries its
+// two U offsets, its chunk and a last-of-chunk bit; a thread stores a
+// chunk's sum in LDS when it ends, and each entry adds its chunks in order
+// (fixed order: deterministic).
+constexpr int YF_THREADS = 256;
+constexpr int YF_CHUNK = 48;
+constexpr int YF_UNROLL = 4;
+
+template <class T>
+struct alignas(2 * sizeof(T)) cpair {
+  T r, i;
+};
+
+template <class T>
+__global__ void __launch_bounds__(YF_THREADS)
+k_snap_yflat(int nu, int ny, int steps, const uint32_t *__restrict__ code,
+             const T *__restrict__ w, const int *__restrict__ cstart, int n,
+             const T *__restrict__ utot_r, const T *__restrict__ utot_i, T *__restrict__ y_r,
+             T *__restrict__ y_i) {
+  extern __shared__ unsigned char yf_smem[];
+  cpair<T> *su = reinterpret_cast<cpair<T> *>(yf_smem);
+  cpair<T> *part = su + nu;
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  for (int k = threadIdx.x; k < nu; k += YF_THREADS)
+    su[k] = cpair<T>{utot_r[(size_t)i * nu + k], utot_i[(size_t)i * nu + k]};
+  __syncthreads();
+  T ar = 0, ai = 0;
+  // steps is a multiple of YF_UNROLL (the host pads with zero terms); the
+  // loads of YF_UNROLL steps are issued before their arithmetic
+  for (int st = 0; st < steps; st += YF_UNROLL) {
+    uint32_t c[YF_UNROLL];
+    T ww[YF_UNROLL];
+#pragma unroll
+    for (int u = 0; u < YF_UNROLL; u++) {
+      const size_t slot = (size_t)(st + u) * YF_THREADS + threadIdx.x;
+      c[u] = code[slot];
+      ww[u] = w[slot];
+    }
+#pragma unroll
+    for (int u = 0; u < YF_UNROLL; u++) {
+      const cpair<T> u1 = su[c[u] & 511u], u2 = su[(c[u] >> 9) & 511u];
+      const T pr = u1.r * u2.r - u1.i * u2.i;
+      const T pi = u1.r * u2.i + u1.i * u2.r;
+      ar += ww[u] * pr;
+      ai += ww[u] * pi;
+      if (c[u] & (1u << 29)) {
+        part[(c[u] >> 18) & 2047u] = cpair<T>{ar, ai};
+        ar = 0;
+        ai = 0;
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ny; e += YF_THREADS) {
+    T yr = 0, yi = 0;
+    for (int c = cstart[e]; c < cstart[e + 1]; c++) {
+      yr += part[c].r;
+      yi += part[c].i;
+    }
+    y_r[(size_t)i * ny + e] = yr;
+    y_i[(size_t)i * ny + e] = yi;
+  }
+}
+
+// term weights for the current coefficients: w = coef(key) * cg product
+__global__ void k_yf_weights(int nslots, const unsigned short *__restrict__ key,
+                             const double *__restrict__ cgw, const double *__restrict__ kcoef,
+                             double *__restrict__ w64, float *__restrict__ w32) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const double v = kcoef[key[s]] * cgw[s];
+  w64[s] = v;
+  w32[s] = (float)v;
+}
+
 // F_ij for two (atom, neighbour) pairs per wavefront, 32 lanes each
 // (SNA::compute_duidrj + compute_duarray, sna_impl.hpp:290-321 / 728-893,
 // then the Y contraction): the lanes of a half-wavefront take the half-plane
@@ -609,6 +697,15 @@ struct emd_snap {
   size_t cap_u = 0, cap_y = 0, cap_ui = 0, cap_yi = 0;
   double *eatom = nullptr;
   int cap_e = 0;
+  // flat Y term stream (k_snap_yflat): codes, key per slot, cg products,
+  // weights for the current coefficients, chunk starts per entry
+  uint32_t *yf_code = nullptr;
+  unsigned short *yf_key = nullptr;
+  double *yf_cgw = nullptr, *yf_w64 = nullptr, *yf_kcoef = nullptr;
+  float *yf_w32 = nullptr;
+  int *yf_cstart = nullptr;
+  int yf_steps = 0, yf_nchunks = 0;
+  std::vector<std::tuple<int, int, int>> yf_keys;  // (a, b, J), sorted
   // adjoint: every (a,b,J) triple grouped by J, with its id
   std::map<std::tuple<int, int, int>, int> tid;
   std::vector<int> sjstart;
@@ -646,6 +743,119 @@ static double host_cg(int j1, int j2, int j, int k1, int k2) {
                                 host_factorial((j + cc2) / 2) * host_factorial((j - cc2) / 2) *
                                 (j + 1));
   return sum * delta * norm;
+}
+
+// the beta-independent part of the flat Y stream (see k_snap_yflat)
+static emd_status snap_build_yflat(emd_ctx *ctx, emd_snap *sn) {
+  const snap_dev_tables &t = sn->t;
+  const int nk = (int)sn->idxj.size() / 3;
+  std::set<std::tuple<int, int, int>> keyset;
+  for (int k = 0; k < nk; k++) {
+    const int j1 = sn->idxj[3 * k], j2 = sn->idxj[3 * k + 1], j = sn->idxj[3 * k + 2];
+    keyset.insert(std::make_tuple(std::max(j1, j2), std::min(j1, j2), j));
+    keyset.insert(std::make_tuple(std::max(j, j2), std::min(j, j2), j1));
+    keyset.insert(std::make_tuple(std::max(j1, j), std::min(j1, j), j2));
+  }
+  sn->yf_keys.assign(keyset.begin(), keyset.end());
+  struct term {
+    int i1, i2, key;
+    double cgw;
+  };
+  std::vector<std::vector<term>> eterms(t.ny);
+  for (int J = 0; J <= t.twojmax; J++)
+    for (int mb = 0; 2 * mb <= J; mb++)
+      for (int ma = 0; ma <= J; ma++) {
+        const int e = t.ystart[J] + mb * (J + 1) + ma;
+        for (size_t q = 0; q < sn->yf_keys.size(); q++) {
+          int a, b, JJ;
+          std::tie(a, b, JJ) = sn->yf_keys[q];
+          if (JJ != J) continue;
+          const int cgoff = sn->cgoff.at(sn->yf_keys[q]);
+          const int ma1lo = std::max(0, (2 * ma - J - b + a) / 2);
+          const int ma1hi = std::min(a, (2 * ma - J + b + a) / 2);
+          const int mb1lo = std::max(0, (2 * mb - J - b + a) / 2);
+          const int mb1hi = std::min(a, (2 * mb - J + b + a) / 2);
+          for (int ma1 = ma1lo; ma1 <= ma1hi; ma1++) {
+            const int ma2 = (2 * ma - J - (2 * ma1 - a) + b) / 2;
+            const double cma = sn->cg_h[cgoff + ma1 * (b + 1) + ma2];
+            for (int mb1 = mb1lo; mb1 <= mb1hi; mb1++) {
+              const int mb2 = (2 * mb - J - (2 * mb1 - a) + b) / 2;
+              const double cw = cma * sn->cg_h[cgoff + mb1 * (b + 1) + mb2];
+              if (cw == 0.0) continue;
+              eterms[e].push_back(term{t.ustart[a] + ma1 * (a + 1) + mb1,
+                                       t.ustart[b] + ma2 * (b + 1) + mb2, (int)q, cw});
+            }
+          }
+        }
+      }
+  // chunks in entry order
+  std::vector<int> cstart(t.ny + 1, 0), csize;
+  std::vector<std::pair<int, int>> crange;  // (entry, first term)
+  for (int e = 0; e < t.ny; e++) {
+    cstart[e] = (int)csize.size();
+    for (int f = 0; f < (int)eterms[e].size(); f += YF_CHUNK) {
+      csize.push_back(std::min(YF_CHUNK, (int)eterms[e].size() - f));
+      crange.push_back(std::make_pair(e, f));
+    }
+  }
+  cstart[t.ny] = (int)csize.size();
+  const int nchunks = (int)csize.size();
+  if (nchunks > 2048 || t.nu > 512)
+    return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: Y stream too large");
+  // longest chunk first onto the least loaded thread
+  std::vector<int> order(nchunks);
+  for (int c = 0; c < nchunks; c++) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return csize[x] > csize[y]; });
+  typedef std::pair<int, int> lt;  // (load, thread)
+  std::priority_queue<lt, std::vector<lt>, std::greater<lt>> pq;
+  for (int th = 0; th < YF_THREADS; th++) pq.push(lt(0, th));
+  std::vector<std::vector<int>> tchunks(YF_THREADS);
+  for (int c : order) {
+    lt top = pq.top();
+    pq.pop();
+    tchunks[top.second].push_back(c);
+    pq.push(lt(top.first + csize[c], top.second));
+  }
+  int steps = 0;
+  for (int th = 0; th < YF_THREADS; th++) {
+    std::sort(tchunks[th].begin(), tchunks[th].end());
+    int l = 0;
+    for (int c : tchunks[th]) l += csize[c];
+    steps = std::max(steps, l);
+  }
+  steps = (steps + YF_UNROLL - 1) / YF_UNROLL * YF_UNROLL;
+  const size_t nslots = (size_t)steps * YF_THREADS;
+  std::vector<uint32_t> code(nslots, 0u);
+  std::vector<unsigned short> key(nslots, 0);
+  std::vector<double> cgw(nslots, 0.0);
+  for (int th = 0; th < YF_THREADS; th++) {
+    int st = 0;
+    for (int c : tchunks[th]) {
+      const std::vector<term> &tl = eterms[crange[c].first];
+      for (int k = 0; k < csize[c]; k++, st++) {
+        const term &tm = tl[crange[c].second + k];
+        const size_t slot = (size_t)st * YF_THREADS + th;
+        code[slot] = (uint32_t)tm.i1 | ((uint32_t)tm.i2 << 9) | ((uint32_t)c << 18) |
+                     (k == csize[c] - 1 ? (1u << 29) : 0u);
+        key[slot] = (unsigned short)tm.key;
+        cgw[slot] = tm.cgw;
+      }
+    }
+  }
+  sn->yf_steps = steps;
+  sn->yf_nchunks = nchunks;
+  EMD_HIP(ctx, hipMalloc(&sn->yf_code, nslots * sizeof(uint32_t)));
+  EMD_HIP(ctx, hipMalloc(&sn->yf_key, nslots * sizeof(unsigned short)));
+  EMD_HIP(ctx, hipMalloc(&sn->yf_cgw, nslots * sizeof(double)));
+  EMD_HIP(ctx, hipMalloc(&sn->yf_w64, nslots * sizeof(double)));
+  EMD_HIP(ctx, hipMalloc(&sn->yf_w32, nslots * sizeof(float)));
+  EMD_HIP(ctx, hipMalloc(&sn->yf_cstart, (t.ny + 1) * sizeof(int)));
+  EMD_HIP(ctx, hipMalloc(&sn->yf_kcoef, std::max<size_t>(1, sn->yf_keys.size()) * sizeof(double)));
+  EMD_HIP(ctx, hipMemcpy(sn->yf_code, code.data(), nslots * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EMD_HIP(ctx, hipMemcpy(sn->yf_key, key.data(), nslots * sizeof(unsigned short), hipMemcpyHostToDevice));
+  EMD_HIP(ctx, hipMemcpy(sn->yf_cgw, cgw.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
+  EMD_HIP(ctx, hipMemcpy(sn->yf_cstart, cstart.data(), (t.ny + 1) * sizeof(int), hipMemcpyHostToDevice));
+  return EMD_OK;
 }
 
 static emd_status snap_upload_coeff(emd_ctx *ctx, emd_snap *sn) {
@@ -692,6 +902,20 @@ static emd_status snap_upload_coeff(emd_ctx *ctx, emd_snap *sn) {
     EMD_HIP(ctx, hipMemcpy(sn->ytab, yt.data(), yt.size() * sizeof(ytab_entry), hipMemcpyHostToDevice));
   if (!et.empty())
     EMD_HIP(ctx, hipMemcpy(sn->etab, et.data(), et.size() * sizeof(etab_entry), hipMemcpyHostToDevice));
+  if (sn->yf_code) {
+    std::vector<double> kc(sn->yf_keys.size(), 0.0);
+    for (size_t q = 0; q < kc.size(); q++) {
+      auto it = coef.find(sn->yf_keys[q]);
+      if (it != coef.end()) kc[q] = it->second;
+    }
+    if (!kc.empty())
+      EMD_HIP(ctx, hipMemcpy(sn->yf_kcoef, kc.data(), kc.size() * sizeof(double),
+                             hipMemcpyHostToDevice));
+    const int nslots = sn->yf_steps * YF_THREADS;
+    if (nslots > 0)
+      EMD_LAUNCH(ctx, "snap_coeff", k_yf_weights, emd_blocks(nslots, 256), 256, 0, nslots,
+                 sn->yf_key, sn->yf_cgw, sn->yf_kcoef, sn->yf_w64, sn->yf_w32);
+  }
   return EMD_OK;
 }
 
@@ -777,7 +1001,8 @@ extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, em
   }
   EMD_HIP(ctx, hipMemcpy(sn->cg, sn->cg_h.data(), sn->cg_h.size() * sizeof(double),
                          hipMemcpyHostToDevice));
-  emd_status st = snap_upload_coeff(ctx, sn);
+  emd_status st = snap_build_yflat(ctx, sn);
+  if (st == EMD_OK) st = snap_upload_coeff(ctx, sn);
   if (st != EMD_OK) {
     emd_snap_destroy(ctx, sn);
     return st;
@@ -791,7 +1016,9 @@ extern "C" emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn) {
   if (ctx) hipStreamSynchronize(ctx->stream);
   void *ptrs[] = {sn->cg, sn->ytab, sn->etab, sn->nin, sn->nj, sn->nr, sn->stab, sn->w_r,
                   sn->w_i, sn->sbuf,
-                  sn->u_r, sn->u_i, sn->y_r, sn->y_i, sn->eatom};
+                  sn->u_r, sn->u_i, sn->y_r, sn->y_i, sn->eatom,
+                  sn->yf_code, sn->yf_key, sn->yf_cgw, sn->yf_w64, sn->yf_w32, sn->yf_cstart,
+                  sn->yf_kcoef};
   for (void *q : ptrs)
     if (q) hipFree(q);
   delete sn;
@@ -840,7 +1067,15 @@ template <class T>
 static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   T *ur = (T *)sn->u_r, *ui = (T *)sn->u_i, *yr = (T *)sn->y_r, *yi = (T *)sn->y_i;
   EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n, ur, ui);
-  EMD_LAUNCH(ctx, "snap_y", k_snap_y<T>, n, Y_THREADS, 0, sn->t, sn->cg, sn->ytab, n, ur, ui, yr, yi);
+  if (ctx->snap_yflat) {
+    const T *w = std::is_same<T, float>::value ? (const T *)sn->yf_w32 : (const T *)sn->yf_w64;
+    const size_t shmem = (size_t)(sn->t.nu + sn->yf_nchunks) * 2 * sizeof(T);
+    EMD_LAUNCH(ctx, "snap_y", k_snap_yflat<T>, n, YF_THREADS, shmem, sn->t.nu, sn->t.ny,
+               sn->yf_steps, sn->yf_code, w, sn->yf_cstart, n, ur, ui, yr, yi);
+  } else {
+    EMD_LAUNCH(ctx, "snap_y", k_snap_y<T>, n, Y_THREADS, 0, sn->t, sn->cg, sn->ytab, n, ur, ui,
+               yr, yi);
+  }
   return EMD_OK;
 }
 

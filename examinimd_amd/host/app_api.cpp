@@ -36,6 +36,12 @@ extern "C" int emd_app_run(emd_app* a, int nsteps) {
   return EMD_OK;
 }
 
+extern "C" int emd_app_set_fused_integrate(emd_app* a, int on) {
+  if (!a) return EMD_ERR_ARG;
+  a->md->fuse_integrate = on != 0;
+  return EMD_OK;
+}
+
 extern "C" int emd_app_set_phase_sync(emd_app* a, int on) {
   if (!a) return EMD_ERR_ARG;
   a->md->phase_sync = on != 0;

@@ -23,6 +23,9 @@ public:
   // phase timers: synchronise the device at phase boundaries so the
   // force/neigh/comm/other columns are device time (on by default)
   bool phase_sync;
+  // fuse final_integrate with the next step's initial_integrate where no
+  // thermo, dump or correctness check observes the step boundary
+  bool fuse_integrate;
   double time_total, time_force, time_neigh, time_comm, time_other;
   double last_thermo[3];
 

@@ -42,9 +42,15 @@ void ForceLJNeighHIP::init_coeff(int nargs, char** args) {
 }
 
 // ForceLJNeigh::compute, force_lj_neigh_impl.h:100-126
+// (full lists: the force is stored, which includes the zeroing)
 void ForceLJNeighHIP::compute(System* system, Binning*, Neighbor* neighbor) {
-  EMD_CALL(system->ctx, emd_lj_force(system->ctx, system->view(), neighbor->get_neigh_list(),
-                                     &params, system->precision, half_neigh ? 1 : 0));
+  if (!half_neigh)
+    EMD_CALL(system->ctx, emd_lj_force_store(system->ctx, system->view(),
+                                             neighbor->get_neigh_list(), &params,
+                                             system->precision));
+  else
+    EMD_CALL(system->ctx, emd_lj_force(system->ctx, system->view(), neighbor->get_neigh_list(),
+                                       &params, system->precision, 1));
 }
 
 // ForceLJNeigh::compute_energy, force_lj_neigh_impl.h:128-156

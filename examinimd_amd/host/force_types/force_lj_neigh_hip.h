@@ -29,6 +29,7 @@ public:
   void init_coeff(int nargs, char** args) override;
   void compute(System* system, Binning* binning, Neighbor* neighbor) override;
   T_F_FLOAT compute_energy(System* system, Binning* binning, Neighbor* neighbor) override;
+  bool stores_forces() const override { return !half_neigh; }
   const char* name() override { return half_neigh ? "ForceLJNeighHalf" : "ForceLJNeighFull"; }
   const emd_lj_params& lj_params() const { return params; }
 };

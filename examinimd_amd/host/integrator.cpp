@@ -6,6 +6,10 @@ Integrator::Integrator(System* s) : system(s), timestep_size(0) {}
 Integrator::~Integrator() {}
 void Integrator::initial_integrate() {}
 void Integrator::final_integrate() {}
+void Integrator::final_initial_integrate() {
+  final_integrate();
+  initial_integrate();
+}
 
 IntegratorNVEHIP::IntegratorNVEHIP(System* s) : Integrator(s) {
   dtv = system->dt;
@@ -18,4 +22,9 @@ void IntegratorNVEHIP::initial_integrate() {
 
 void IntegratorNVEHIP::final_integrate() {
   EMD_CALL(system->ctx, emd_nve_final(system->ctx, system->view(), dtf, system->precision));
+}
+
+void IntegratorNVEHIP::final_initial_integrate() {
+  EMD_CALL(system->ctx,
+           emd_nve_final_initial(system->ctx, system->view(), dtf, dtv, system->precision));
 }

@@ -12,6 +12,9 @@ public:
   T_V_FLOAT timestep_size;
   virtual void initial_integrate();
   virtual void final_integrate();
+  // final_integrate of one step then initial_integrate of the next, for a
+  // step boundary with nothing in between; a module may fuse the two
+  virtual void final_initial_integrate();
 };
 
 #include "modules_integrator.h"

@@ -11,5 +11,6 @@ public:
   IntegratorNVEHIP(System* s);
   void initial_integrate() override;
   void final_integrate() override;
+  void final_initial_integrate() override;
 };
 #endif

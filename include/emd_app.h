@@ -24,6 +24,9 @@ int      emd_app_init(emd_app *app);
 int      emd_app_run(emd_app *app, int nsteps);
 /* synchronise the device between phases (per-phase timers) or not */
 int      emd_app_set_phase_sync(emd_app *app, int on);
+/* fuse final_integrate with the next step's initial_integrate where no thermo,
+ * dump or correctness check falls between them (on by default; bitwise equal) */
+int      emd_app_set_fused_integrate(emd_app *app, int on);
 int      emd_app_destroy(emd_app *app);
 
 int      emd_app_nsteps(const emd_app *app);       /* the deck's `run` count */

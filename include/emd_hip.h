@@ -65,6 +65,10 @@ emd_status emd_memcpy_d2d(emd_ctx *ctx, void *dst, const void *src, size_t bytes
 /* per-kernel timing with HIP events recorded on the context stream */
 emd_status emd_profile_enable(emd_ctx *ctx, int on);
 emd_status emd_profile_reset(emd_ctx *ctx);
+/* restrict timing to the comma-separated kernel names in `names` (NULL or ""
+   times every launch); each timed launch adds two event records, so a
+   timed run that needs one kernel's duration selects only that kernel */
+emd_status emd_profile_select(emd_ctx *ctx, const char *names);
 /* total milliseconds and launch count of kernel `name` since the last reset */
 emd_status emd_profile_get(emd_ctx *ctx, const char *name, double *ms, long *launches);
 /* newline-separated "name launches total_ms" table; returns bytes needed */
@@ -191,6 +195,13 @@ emd_status emd_force_zero(emd_ctx *ctx, emd_system *s, int precision);
 /* ForceLJNeigh::compute, src/force_types/force_lj_neigh_impl.h:100-126 */
 emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                         const emd_lj_params *p, int precision, int half);
+/* deep_copy(f, 0) then ForceLJNeigh::compute on a full list
+ * (src/examinimd.cpp:232, force_lj_neigh_impl.h:100-126) as one pass: each
+ * local atom's force is written, not added, so f needs no zeroing and is not
+ * read; ghost forces are left as they are (a full list never writes them).
+ * Same values as emd_force_zero + emd_lj_force(half = 0). */
+emd_status emd_lj_force_store(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                              const emd_lj_params *p, int precision);
 /* ForceLJNeigh::compute_energy, force_lj_neigh_impl.h:128-156 (compensated) */
 emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
                          const emd_lj_params *p, int half, double *pe);
@@ -207,6 +218,12 @@ emd_status emd_lj_set_tile(emd_ctx *ctx, int on, int lanes);
 emd_status emd_nve_initial(emd_ctx *ctx, emd_system *s, double dtf, double dtv,
                            int precision);
 emd_status emd_nve_final(emd_ctx *ctx, emd_system *s, double dtf, int precision);
+/* final_integrate of step s then initial_integrate of step s+1 (the
+ * reference's ExaMiniMD::run calls them back to back when no thermo, dump or
+ * correctness check falls between, src/examinimd.cpp:249 then 196) in one
+ * launch; results bitwise equal to the two calls */
+emd_status emd_nve_final_initial(emd_ctx *ctx, emd_system *s, double dtf, double dtv,
+                                 int precision);
 
 /* Temperature / KinE reductions, src/property_temperature.cpp:43-62,
  * src/property_kine.cpp:43-61: returns sum_i m_i v_i^2 (compensated) */

@@ -335,3 +335,133 @@ def test_lj_pipelined_kernel_bitwise(oracle):
         pl.close()
     assert np.array_equal(out["0"][0], out["16"][0])
     assert np.array_equal(out["0"][1], out["16"][1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_step_boundary_integration_bitwise(tmp_path, precision):
+    """final_integrate + the next initial_integrate in one launch gives the
+    same bits as the two launches (60 steps: three rebuilds)"""
+    print(f"case: LJ 32k, 60 steps, fused vs split integration, {precision}", flush=True)
+    out = []
+    for fused in (True, False):
+        app = gpu_app(tmp_path, 60, precision=precision)
+        app.set_fused_integrate(fused)
+        app.run(60)
+        out.append(app.arrays())
+        app.close()
+    for k in ("x", "v", "f"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_neighbour_list_candidates_beyond_one_lds_stage(oracle):
+    """a cutoff of 4.0 sigma at rho 0.8442 gives ~1700 candidates per bin,
+    more than one LDS staging round of the neighbour kernel holds: rows that
+    continue across rounds equal the oracle's entry for entry"""
+    from examinimd_amd.abi import EmdNeigh  # noqa: F401
+    from examinimd_amd.pipeline import Pipeline, lattice
+    from helpers import oracle_csr_from
+    print("case: neighbour list, cut 4.0, 4000 atoms, multi-round staging", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    nc, cut = 10, 4.0
+    box = (nc * a,) * 3
+    x = lattice("fcc", nc, a) + np.random.default_rng(31).normal(0, 0.05, (4 * nc ** 3, 3))
+    pl = Pipeline()
+    pl.load(x, box)
+    pl.build(cut)
+    n = pl.s.n_local
+    xs = _dev(pl.ctx, pl.s.x, 3 * n, np.float64).reshape(n, 3)
+    ids = _dev(pl.ctx, pl.s.id, n, np.int32)
+    num = _dev(pl.ctx, pl.nl.num, n, np.int32)
+    row = _dev(pl.ctx, pl.nl.row, n + 1, np.int32)
+    ent = _dev(pl.ctx, pl.nl.entries, int(row[-1]), np.int32)
+    pl.close()
+    orow, oent, oid = oracle_csr_from(oracle, xs, ids, box, cut=cut)
+    assert np.array_equal(oid, ids)
+    assert int(num.max()) > 200
+    assert np.array_equal(row, orow) and np.array_equal(ent, oent)
+
+
+@pytest.mark.parametrize("layout", [emd.EMD_NEIGH_CSR, emd.EMD_NEIGH_2D])
+@pytest.mark.parametrize("lanes", [1, 16])
+def test_lj_force_store_equals_zero_then_accumulate(oracle, layout, lanes):
+    """emd_lj_force_store writes each local force without reading f: with f
+    poisoned (NaN) beforehand it gives the bits of force_zero + lj_force"""
+    import ctypes as C
+    from examinimd_amd.abi import EmdLJParams
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print(f"case: LJ store vs zero+accumulate, layout {layout}, {lanes} lanes", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    nc = 12
+    x = lattice("fcc", nc, a) + np.random.default_rng(5).normal(0, 0.05, (4 * nc ** 3, 3))
+    os.environ["EMD_LJ_LANES"] = str(lanes)
+    try:
+        pl = Pipeline()
+    finally:
+        os.environ.pop("EMD_LJ_LANES", None)
+    pl.load(x, (nc * a,) * 3)
+    L, c, s = pl.L, pl.ctx.h, C.byref(pl.s)
+    pl.chk(L.emd_comm_exchange(c, s), "exchange")
+    pl.chk(L.emd_binning(c, s, 2.8, 2.8, 2.8, 1, 1, 0, 1, C.byref(pl.b)), "bin")
+    pl.chk(L.emd_comm_exchange_halo(c, s, 2.8, C.byref(pl.h)), "halo")
+    pl.chk(L.emd_binning(c, s, 2.8, 2.8, 2.8, 1, 1, 1, 0, C.byref(pl.b)), "bin")
+    pl.chk(L.emd_neigh_build(c, s, C.byref(pl.b), 2.8, 0, 0, layout, C.byref(pl.nl)), "neigh")
+    p = EmdLJParams()
+    p.ntypes = 1
+    p.lj1[0][0], p.lj2[0][0], p.cutsq[0][0] = 48.0, 24.0, 6.25
+    n = pl.s.n_local
+    for prec in (emd.EMD_FP64, emd.EMD_MIXED):
+        pl.chk(L.emd_force_zero(c, s, prec), "zero")
+        pl.chk(L.emd_lj_force(c, s, C.byref(pl.nl), C.byref(p), prec, 0), "lj")
+        want = pl.forces(prec)
+        ntot = n + pl.s.n_ghost
+        if prec == emd.EMD_MIXED:
+            nan = np.full(3 * ntot, np.nan, np.float32)
+            dst = pl.s.f32
+        else:
+            nan = np.full(3 * ntot, np.nan, np.float64)
+            dst = pl.s.f
+        pl.chk(L.emd_memcpy_h2d(c, C.cast(dst, C.c_void_p), nan.ctypes.data_as(C.c_void_p),
+                                nan.nbytes), "poison")
+        pl.chk(L.emd_lj_force_store(c, s, C.byref(pl.nl), C.byref(p), prec), "lj store")
+        got = pl.forces(prec)
+        assert np.isfinite(got).all()
+        assert np.array_equal(got, want), prec
+    pl.close()
+
+
+@pytest.mark.parametrize("nc", [6, 9])
+def test_binning_one_launch_equals_radix_sort(monkeypatch, nc):
+    """binning of a small range in one workgroup launch gives the counts,
+    offsets, permutation and largest count of the radix-sort path, for the
+    sorting local binning and the local + ghost binning after the halo"""
+    import ctypes as C
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print(f"case: binning one launch vs radix sort, FCC {nc}^3", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    x = lattice("fcc", nc, a) + np.random.default_rng(47).normal(0, 0.1, (4 * nc ** 3, 3))
+    out = []
+    for one in ("1", "0"):
+        monkeypatch.setenv("EMD_BIN_ONE", one)
+        pl = Pipeline()
+        pl.load(x, (nc * a,) * 3)
+        L, c, s = pl.L, pl.ctx.h, C.byref(pl.s)
+        pl.chk(L.emd_comm_exchange(c, s), "exchange")
+        rec = {}
+        for tag, ghost, sort in (("local", 0, 1), ("all", 1, 0)):
+            if ghost:
+                pl.chk(L.emd_comm_exchange_halo(c, s, 2.8, C.byref(pl.h)), "halo")
+            pl.chk(L.emd_binning(c, s, 2.8, 2.8, 2.8, 1, 1, ghost, sort, C.byref(pl.b)), "bin")
+            nb, nr = pl.b.nbins, pl.b.n_range
+            rec[tag + "_count"] = _dev(pl.ctx, pl.b.bincount, nb, np.int32)
+            rec[tag + "_offset"] = _dev(pl.ctx, pl.b.binoffsets, nb, np.int32)
+            rec[tag + "_permute"] = _dev(pl.ctx, pl.b.permute, nr, np.int32)
+            rec[tag + "_max"] = np.array([pl.b.max_count])
+        n = pl.s.n_local
+        rec["x"] = _dev(pl.ctx, pl.s.x, 3 * n, np.float64)
+        rec["id"] = _dev(pl.ctx, pl.s.id, n, np.int32)
+        out.append(rec)
+        pl.close()
+    assert out[0]["all_count"].sum() == len(out[0]["all_permute"])
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+

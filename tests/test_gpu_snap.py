@@ -135,3 +135,138 @@ def test_snap_kernel_level_disordered(oracle):
     rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
     print("mixed SNAP force RMS error (eV/A)", rms, "force RMS", np.sqrt(np.mean(np.sum(f64 ** 2, 1))))
     assert rms <= 1e-4
+
+
+@pytest.mark.parametrize("nc,cut", [(10, 5.73442), (30, 5.73442)])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_ghost_force_fold_against_numpy(monkeypatch, nc, cut, mixed):
+    """CommSerial::update_force on seeded random forces: the one-workgroup
+    six-phase fold (small halos) and the per-phase launches both equal the
+    sequential fold f[pack_p] += f[ghosts_p], p = 5..0, done in numpy.
+    (Whole SNAP runs cannot be compared bit for bit: the force scatter onto
+    neighbours uses atomics.)  BCC 10^3 gives ~3 000 ghosts (single launch),
+    30^3 ~ 22 000 (per-phase launches)."""
+    import ctypes as C
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print(f"case: ghost force fold, BCC {nc}^3, cut {cut}, mixed={mixed}", flush=True)
+    a = 3.1803
+    x = lattice("bcc", nc, a) + np.random.default_rng(41).normal(0, 0.05, (2 * nc ** 3, 3))
+    prec = emd.EMD_MIXED if mixed else emd.EMD_FP64
+    dt = np.float32 if mixed else np.float64
+    results = []
+    for one in ("1", "0"):
+        monkeypatch.setenv("EMD_FOLD_ONE", one)
+        pl = Pipeline()
+        pl.load(x, (nc * a,) * 3, mass=183.84)
+        pl.build(cut, newton=1)
+        L, c = pl.L, pl.ctx.h
+        n, ng = pl.s.n_local, pl.s.n_ghost
+        counts = [pl.h.num_ghost[p] for p in range(6)]
+        assert sum(counts) == ng
+        packs = []
+        for p in range(6):
+            arr = np.zeros(counts[p], np.int32)
+            if counts[p]:
+                pl.chk(L.emd_memcpy_d2h(c, arr.ctypes.data_as(C.c_void_p),
+                                        C.cast(pl.h.pack[p], C.c_void_p), arr.nbytes), "pack")
+            packs.append(arr)
+        f0 = np.random.default_rng(7).normal(0, 1, 3 * (n + ng)).astype(dt)
+        dst = pl.s.f32 if mixed else pl.s.f
+        pl.chk(L.emd_memcpy_h2d(c, C.cast(dst, C.c_void_p), f0.ctypes.data_as(C.c_void_p),
+                                f0.nbytes), "upload f")
+        pl.chk(L.emd_comm_update_force(c, C.byref(pl.s), C.byref(pl.h), prec), "fold")
+        got = np.zeros(3 * n, dt)
+        pl.chk(L.emd_memcpy_d2h(c, got.ctypes.data_as(C.c_void_p), C.cast(dst, C.c_void_p),
+                                got.nbytes), "download f")
+        want = f0.reshape(-1, 3).copy()
+        off = [n]
+        for p in range(5):
+            off.append(off[-1] + counts[p])
+        for p in range(5, -1, -1):
+            if counts[p]:
+                assert len(np.unique(packs[p])) == counts[p]
+                want[packs[p]] += want[off[p]:off[p] + counts[p]]
+        assert np.array_equal(got.reshape(-1, 3), want[:n]), f"fold one={one}"
+        results.append((ng, got))
+        pl.close()
+    print("ghosts", results[0][0])
+    assert (results[0][0] <= 16384) == (nc == 10)
+    assert np.array_equal(results[0][1], results[1][1])
+
+
+@pytest.mark.parametrize("nc", [10, 6])
+def test_halo_exchange_one_launch_equals_scans(monkeypatch, nc):
+    """exchange_halo of a small system in one workgroup launch gives the
+    ghosts of the six per-phase scans: counts, pack lists, positions,
+    velocities, ids, roots and image codes, bit for bit (including the first
+    call, which starts with no capacity and grows)"""
+    import ctypes as C
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print(f"case: exchange_halo one launch vs scans, BCC {nc}^3", flush=True)
+    a = 3.1803
+    x = lattice("bcc", nc, a) + np.random.default_rng(43).normal(0, 0.08, (2 * nc ** 3, 3))
+    v = np.random.default_rng(44).normal(0, 1, x.shape)
+    out = []
+    for one in ("1", "0"):
+        monkeypatch.setenv("EMD_HALO_ONE", one)
+        pl = Pipeline()
+        pl.load(x, (nc * a,) * 3, mass=183.84, v=v)
+        L, c, s = pl.L, pl.ctx.h, C.byref(pl.s)
+        pl.chk(L.emd_comm_exchange(c, s), "exchange")
+        pl.chk(L.emd_binning(c, s, 5.73442, 5.73442, 5.73442, 1, 1, 0, 1, C.byref(pl.b)), "bin")
+        pl.chk(L.emd_comm_exchange_halo(c, s, 5.73442, C.byref(pl.h)), "halo")
+        n, ng = pl.s.n_local, pl.s.n_ghost
+        tot = n + ng
+        rec = {"counts": [pl.h.num_ghost[p] for p in range(6)], "n_ghost": ng}
+        xs = np.zeros(3 * tot)
+        vs = np.zeros(3 * tot)
+        ids = np.zeros(tot, np.int32)
+        pl.chk(L.emd_system_download(c, s, tot, xs.ctypes.data_as(C.POINTER(C.c_double)),
+                                     vs.ctypes.data_as(C.POINTER(C.c_double)), None, None,
+                                     ids.ctypes.data_as(C.POINTER(C.c_int)), None), "download")
+        rec["x"], rec["v"], rec["id"] = xs, vs, ids
+        for name, ptr, cnt in [("root", pl.h.root, ng), ("image", pl.h.image, ng)] + \
+                [(f"pack{p}", pl.h.pack[p], pl.h.num_ghost[p]) for p in range(6)]:
+            arr = np.zeros(cnt, np.int32)
+            if cnt:
+                pl.chk(L.emd_memcpy_d2h(c, arr.ctypes.data_as(C.c_void_p), C.cast(ptr, C.c_void_p),
+                                        arr.nbytes), name)
+            rec[name] = arr
+        # a second exchange on the grown capacities gives the same ghosts
+        pl.chk(L.emd_comm_exchange_halo(c, s, 5.73442, C.byref(pl.h)), "halo again")
+        assert pl.s.n_ghost == ng
+        out.append(rec)
+        pl.close()
+    assert out[0]["n_ghost"] > 0
+    for k in out[0]:
+        assert np.array_equal(np.asarray(out[0][k]), np.asarray(out[1][k])), k
+
+
+def test_snap_flat_y_stream_against_per_entry_kernel(monkeypatch):
+    """Y from the flat term stream against the per-entry Y kernel on a
+    displaced BCC W configuration: FP64 forces agree to 1e-12 relative (the
+    two sum the same products in a different order); the mixed path's error
+    against FP64 stays within its 1e-4 eV/A bound either way"""
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print("case: SNAP flat Y stream vs per-entry Y kernel, BCC 2000", flush=True)
+    a = 3.1803
+    x = lattice("bcc", 10, a) + np.random.default_rng(3).normal(0, 0.1, (2000, 3))
+    beta = decks.snap_beta(seed=2024, sigma=0.05)
+    out = {}
+    for flat in ("1", "0"):
+        monkeypatch.setenv("EMD_SNAP_YFLAT", flat)
+        pl = Pipeline()
+        pl.load(x, (10 * a,) * 3, mass=183.84)
+        pl.build(4.73442 + 1.0, newton=1)
+        pl.snap_setup(list(beta))
+        out[flat] = (pl.snap_force(emd.EMD_FP64), pl.snap_force(emd.EMD_MIXED))
+        pl.close()
+    f1, f0 = out["1"][0], out["0"][0]
+    rel = float(np.max(np.abs(f1 - f0)) / np.max(np.abs(f0)))
+    print("flat vs per-entry FP64 relative difference", rel)
+    assert rel <= 1e-12
+    for flat in ("1", "0"):
+        d = out[flat][1] - out[flat][0]
+        rms = float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+        print("mixed RMS error, flat =", flat, rms)
+        assert rms <= 1e-4
