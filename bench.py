@@ -192,8 +192,9 @@ def snap_files(emd, decks, tmpdir):
 
 
 def snap_run(emd, decks, tmpdir, coeff, param, precision, warmup, steps, breakdown_steps=20):
-    """timed SNAP run; events only on the three SNAP kernels (the roofline's
-    denominator), then an untimed per-kernel breakdown"""
+    """timed SNAP run with no events (at 2000 atoms an event pair is a
+    noticeable share of a kernel's dispatch gap), then an untimed per-kernel
+    breakdown whose SNAP kernel times feed the roofline"""
     deck = decks.write(os.path.join(tmpdir, f"in.snap.{precision}"),
                        decks.snap_deck(n=10, steps=steps + breakdown_steps, coeff=coeff,
                                        param=param))
@@ -204,20 +205,16 @@ def snap_run(emd, decks, tmpdir, coeff, param, precision, warmup, steps, breakdo
     if warmup:
         app.run(warmup)
     L.emd_synchronize(ctx)
-    L.emd_profile_select(ctx, b"snap_ui,snap_y,snap_fij")
-    L.emd_profile_enable(ctx, 1)
-    L.emd_profile_reset(ctx)
+    L.emd_profile_enable(ctx, 0)
     t0 = time.perf_counter()
     app.run(steps)
     L.emd_synchronize(ctx)
     wall = time.perf_counter() - t0
-    prof = _profile_table(L, ctx)
-    L.emd_profile_enable(ctx, 0)
     natoms = app.natoms
     e_per_atom = app.thermo()[2]
     breakdown = _breakdown(app, L, ctx, breakdown_steps)
     app.close()
-    return natoms, wall, prof, e_per_atom, breakdown
+    return natoms, wall, e_per_atom, breakdown
 
 
 def snap_force_error(emd, beta):
@@ -425,11 +422,10 @@ def main():
     if not a.no_snap:
         coeff, param, beta = snap_files(emd, decks, tmpdir)
         for prec in ("fp64", "mixed"):
-            sn, sw, sp, se, sb = snap_run(emd, decks, tmpdir, coeff, param, prec, 10,
-                                          a.snap_steps)
+            sn, sw, se, sb = snap_run(emd, decks, tmpdir, coeff, param, prec, 10, a.snap_steps)
             snap[prec] = {"value": sn * a.snap_steps / sw, "ms_per_step": 1e3 * sw / a.snap_steps,
-                          "snap_kernels_ms_per_step": {k: v[1] / a.snap_steps
-                                                       for k, v in sp.items()},
+                          "snap_kernels_ms_per_step": {k: v for k, v in sb.items()
+                                                       if k in ("snap_ui", "snap_y", "snap_fij")},
                           "kernels_ms_per_step_untimed_breakdown": sb,
                           "etot_per_atom_end": se}
         snap["mixed_over_fp64"] = snap["mixed"]["value"] / snap["fp64"]["value"]

@@ -329,7 +329,9 @@ extern "C" emd_status emd_bins_free(emd_ctx *ctx, emd_bins *b) {
 // ---------------------------------------------------------------- neighbours
 constexpr int NB_THREADS = 256;
 constexpr int NB_WAVES = NB_THREADS / EMD_WAVE;
-constexpr int NB_MAXC = 640;  // candidates staged per LDS chunk (27 bins of an LJ melt: ~500)
+constexpr int NB_MAXC = 640;  // candidates staged per LDS chunk (27 bins of an LJ melt: ~500);
+                               // a multiple of 128 (two 64-candidate groups per step)
+static_assert(NB_MAXC % (2 * EMD_WAVE) == 0, "staging holds whole iterations");
 
 // NeighborCSR Tag{Count,Fill}Neigh{Full,Half}, src/neighbor_types/neighbor_csr.h:176-358
 template <bool HALF, bool FILL>
@@ -390,21 +392,20 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
       if (cs > 0) run = FILL ? cursor[i] : num[i];
       size_t base = 0;
       if (FILL) base = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
-      // candidate test of position c of the staged list (the reference's
-      // predicate, neighbor_csr.h:206 full / 290 half)
-      auto test = [&](int c, int &j) -> bool {
-        if (c >= cn) return false;
-        j = s_j[c];
-        const double x_j = s_x[c], y_j = s_y[c], z_j = s_z[c];
+      // candidate test at staged position c (the reference's predicate,
+      // neighbor_csr.h:206 full / 290 half), branch-free: positions past the
+      // staged count cn read stale LDS (the arrays hold NB_MAXC entries, a
+      // multiple of the 128 positions of one iteration) and are masked
+      auto test = [&](int c, int j, double x_j, double y_j, double z_j) -> bool {
         const double dx = x_i - x_j, dy = y_i - y_j, dz = z_i - z_j;
         const double rsq = dx * dx + dy * dy + dz * dz;
         if (HALF) {
-          const bool drop =
-              ((j == i) || (j < n_local || newton)) &&
-              !((x_j > x_i) || ((x_j == x_i) && ((y_j > y_i) || ((y_j == y_i) && (z_j > z_i)))));
-          return !drop && (rsq <= cutsq);
+          const bool ahead =
+              (x_j > x_i) | ((x_j == x_i) & ((y_j > y_i) | ((y_j == y_i) & (z_j > z_i))));
+          const bool drop = ((j == i) | (j < n_local) | (newton != 0)) & !ahead;
+          return (c < cn) & !drop & (rsq <= cutsq);
         }
-        return (rsq <= cutsq) && (i != j);
+        return (c < cn) & (rsq <= cutsq) & (i != j);
       };
       auto put = [&](bool keep, unsigned long long m, int at, int c, int j) {
         if (FILL && keep) {
@@ -416,13 +417,15 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
           }
         }
       };
-      // two 64-candidate groups per iteration: independent LDS reads and
-      // distance tests in flight together; entries keep candidate order
+      // two 64-candidate groups per iteration: both groups' LDS reads are
+      // issued before either distance test; entries keep candidate order
       for (int c0 = 0; c0 < cn; c0 += 2 * EMD_WAVE) {
         const int ca = c0 + lane, cb = c0 + EMD_WAVE + lane;
-        int ja = -1, jb = -1;
-        const bool ka = test(ca, ja);
-        const bool kb = test(cb, jb);
+        const int ja = s_j[ca], jb = s_j[cb];
+        const double xa = s_x[ca], ya = s_y[ca], za = s_z[ca];
+        const double xb = s_x[cb], yb = s_y[cb], zb = s_z[cb];
+        const bool ka = test(ca, ja, xa, ya, za);
+        const bool kb = test(cb, jb, xb, yb, zb);
         const unsigned long long ma = __ballot(ka);
         const unsigned long long mb = __ballot(kb);
         put(ka, ma, run, ca, ja);

@@ -98,15 +98,8 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
       }
     }
   };
-  for (int jj = sub; jj < cnt; jj += U * K) {
-    int j[U];
-    bool ok[U];
+  auto gather_pairs = [&](int jj, const int (&j)[U], const bool (&ok)[U]) {
     double xj[U], yj[U], zj[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      ok[u] = jj + u * K < cnt;
-      j[u] = __builtin_nontemporal_load(rowp + (ok[u] ? jj + u * K : cnt - 1));
-    }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       xj[u] = x[3 * (size_t)j[u]];
@@ -115,6 +108,38 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
     }
 #pragma unroll
     for (int u = 0; u < U; u++) pair(ok[u], j[u], xj[u], yj[u], zj[u]);
+  };
+  int jj = sub;
+  if (!row) {
+    // 2D rows start at i * pitch, so the first round's index loads need not
+    // wait for the row length (loaded alongside them): slots are clamped to
+    // the pitch, and a slot past the row end (an index of an earlier build,
+    // or never written) is replaced by the atom itself before its position
+    // is gathered (x_i is in cache; the slot is masked by ok)
+    int j[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = jj + u * K;
+      j[u] = __builtin_nontemporal_load(rowp + (e < pitch ? e : pitch - 1));
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      ok[u] = jj + u * K < cnt;
+      j[u] = ok[u] ? j[u] : i;
+    }
+    gather_pairs(jj, j, ok);
+    jj += U * K;
+  }
+  for (; jj < cnt; jj += U * K) {
+    int j[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      ok[u] = jj + u * K < cnt;
+      j[u] = __builtin_nontemporal_load(rowp + (ok[u] ? jj + u * K : cnt - 1));
+    }
+    gather_pairs(jj, j, ok);
   }
   if (K > 1) {
     fxi = group_sum<K>(fxi);

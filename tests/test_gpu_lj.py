@@ -465,3 +465,18 @@ def test_binning_one_launch_equals_radix_sort(monkeypatch, nc):
     for k in out[0]:
         assert np.array_equal(out[0][k], out[1][k]), k
 
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lj32k_2d_equals_csr_default_lanes(tmp_path, precision):
+    """the 2D list's first round loads row slots before the row length
+    (clamped to the pitch and to existing atoms, masked by length): 100 steps
+    give the bits of the CSR run at the default 16 lanes"""
+    print(f"case: LJ 32k, 100 steps, 2D vs CSR, 16 lanes, {precision}", flush=True)
+    out = []
+    for neigh in ("2D", "CSR"):
+        app = gpu_app(tmp_path, STEPS, neigh=neigh, precision=precision)
+        app.run(STEPS)
+        out.append(app.arrays())
+        app.close()
+    for k in ("x", "v", "f"):
+        assert np.array_equal(out[0][k], out[1][k]), k
