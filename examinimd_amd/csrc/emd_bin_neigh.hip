@@ -374,8 +374,14 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
     const int cn = (T - cs) < NB_MAXC ? (T - cs) : NB_MAXC;
     for (int c = tid; c < cn; c += NB_THREADS) {
       const int cc = cs + c;
-      int k = 0;
-      while (s_pre[k + 1] <= cc) k++;
+      // stencil bin of candidate cc: the last k with s_pre[k] <= cc
+      int k = 0, hi = 27;
+#pragma unroll
+      for (int step = 0; step < 5; step++) {
+        const int mid = (k + hi) >> 1;
+        if (s_pre[mid] <= cc) k = mid;
+        else hi = mid;
+      }
       const int j = permute[s_off[k] + (cc - s_pre[k])];
       s_j[c] = j;
       s_x[c] = x[3 * (size_t)j];
@@ -385,9 +391,23 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
     __syncthreads();
     const bool last = (cs + NB_MAXC >= T);
     for (int bi = wave; bi < c_cnt; bi += NB_WAVES) {
-      const int i = permute[c_off + bi];
+      // the centre bin is stencil bin 13: its atoms are staged too, so the
+      // row atom's index and position come from LDS when in this round
+      const int pc = s_pre[13] + bi - cs;
+      int i;
+      double x_i, y_i, z_i;
+      if (pc >= 0 && pc < cn) {
+        i = s_j[pc];
+        x_i = s_x[pc];
+        y_i = s_y[pc];
+        z_i = s_z[pc];
+      } else {
+        i = permute[c_off + bi];
+        x_i = x[3 * (size_t)i];
+        y_i = x[3 * (size_t)i + 1];
+        z_i = x[3 * (size_t)i + 2];
+      }
       if (i >= n_local) continue;
-      const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
       int run = 0;
       if (cs > 0) run = FILL ? cursor[i] : num[i];
       size_t base = 0;

@@ -161,6 +161,50 @@ __device__ __forceinline__ bool lane_entry(int l, int j, int &ma, int &mb) {
   return 2 * mb <= j;
 }
 
+// per-layer lane table, built once per block in LDS: for layer j and
+// half-plane entry e (e = mb*(j+1) + ma, mb <= j/2) the indices (ma, mb), and
+// the rootpq table; the per-neighbour layer loop then does no integer
+// division and reads the recursion factors rootpq(j-ma, j-mb) and
+// rootpq(ma, j-mb) from LDS
+template <class T>
+struct layer_tab {
+  int mm[SNAP_JD][64];          // ma | mb << 8 | (entry exists) << 16
+  T rp[SNAP_JD * SNAP_JD];      // rootpq(p, q) at p * SNAP_JD + q
+  __device__ T rpa(int j, int ma, int mb) const { return rp[(j - ma) * SNAP_JD + (j - mb)]; }
+  __device__ T rpb(int j, int ma, int mb) const { return rp[ma * SNAP_JD + (j - mb)]; }
+};
+
+template <class T>
+__device__ __forceinline__ void layer_tab_fill(const snap_dev_tables &t, layer_tab<T> &L) {
+  for (int k = threadIdx.x; k < SNAP_JD * 64; k += blockDim.x) {
+    const int j = k / 64, e = k % 64;
+    const int mb = e / (j + 1), ma = e - mb * (j + 1);
+    const bool own = j <= t.twojmax && e < (j + 1) * (j / 2 + 1);
+    L.mm[j][e] = ma | (mb << 8) | (own ? 1 << 16 : 0);
+  }
+  for (int k = threadIdx.x; k < SNAP_JD * SNAP_JD; k += blockDim.x)
+    L.rp[k] = (T)t.rootpq[(k / SNAP_JD) * (SNAP_JD + 1) + (k % SNAP_JD)];
+}
+
+// one entry of layer j from the full layer j-1 (row-major ma*j+mb), with the
+// recursion factors from the layer table (see u_step)
+template <class T>
+__device__ __forceinline__ cplx<T> u_step_tab(const T *pr, const T *pi, int j, int ma, int mb,
+                                              T rpa, T rpb, T a_r, T a_i, T b_r, T b_i) {
+  cplx<T> u{0, 0};
+  if (ma < j) {
+    const T ur = pr[ma * j + mb], ui = pi[ma * j + mb];
+    u.r += rpa * (a_r * ur + a_i * ui);
+    u.i += rpa * (a_r * ui - a_i * ur);
+  }
+  if (ma > 0) {
+    const T ur = pr[(ma - 1) * j + mb], ui = pi[(ma - 1) * j + mb];
+    u.r += -rpb * (b_r * ur + b_i * ui);
+    u.i += -rpb * (b_r * ui - b_i * ur);
+  }
+  return u;
+}
+
 // Cayley-Klein parameters a = (z0 - i z)/r0, b = (y - i x)/r0
 template <class T>
 __device__ __forceinline__ void ck_params(T x, T y, T z, T z0, T r, T &a_r, T &a_i, T &b_r,
@@ -216,8 +260,11 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
           int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
   __shared__ T s_r[UI_WAVES][2][SNAP_JD * SNAP_JD], s_i[UI_WAVES][2][SNAP_JD * SNAP_JD];
   __shared__ T s_pr[UI_WAVES][SNAP_NY], s_pi[UI_WAVES][SNAP_NY];
+  __shared__ layer_tab<T> LT;
   const int i = blockIdx.x;
   if (i >= n) return;
+  layer_tab_fill(t, LT);
+  __syncthreads();
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int twojmax = t.twojmax;
   // this wavefront's sums over its neighbours, half-plane index
@@ -248,11 +295,13 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
     wave_sync();
     for (int j = 1; j <= twojmax; j++) {
       const int p = (j - 1) & 1, c = j & 1;
-      int ma, mb;
-      const bool own = l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb);
+      const int info = LT.mm[j][l];
+      const bool own = info >> 16;
+      const int ma = info & 255, mb = (info >> 8) & 255;
       cplx<T> u{0, 0};
       if (own) {
-        u = u_step<T>(t, br[p], bi[p], j, ma, mb, a_r, a_i, b_r, b_i);
+        u = u_step_tab<T>(br[p], bi[p], j, ma, mb, LT.rpa(j, ma, mb), LT.rpb(j, ma, mb), a_r,
+                          a_i, b_r, b_i);
         // full layer j: own entry and, off the middle column, its mirror
         br[c][ma * (j + 1) + mb] = u.r;
         bi[c][ma * (j + 1) + mb] = u.i;
@@ -272,8 +321,9 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
   // U_tot = wself * I + sum over wavefronts; own entries and mirrors
   T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
   for (int j = 0; j <= twojmax; j++) {
-    int ma, mb;
-    if (l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb)) {
+    const int info = LT.mm[j][l];
+    const int ma = info & 255, mb = (info >> 8) & 255;
+    if (info >> 16) {
       T vr = 0, vi = 0;
 #pragma unroll
       for (int q = 0; q < UI_WAVES; q++) {
@@ -461,16 +511,21 @@ __global__ void k_yf_weights(int nslots, const unsigned short *__restrict__ key,
 // barriers).
 constexpr int FIJ_WAVES = 2;                      // wavefronts per block
 constexpr int FIJ_NB = 2 * FIJ_WAVES;             // neighbours per block
+constexpr int FIJ_HP = 32;                        // half-plane entries of layer 7
 
 template <class T, class TF>
 __global__ void __launch_bounds__(64 * FIJ_WAVES)
 k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
            const double *__restrict__ nr, int maxn, int n, const T *__restrict__ y_r,
            const T *__restrict__ y_i, TF *__restrict__ f) {
-  // previous layer (j-1 <= 7) of each neighbour: U and dU (3 components),
-  // 64 entries each, double buffered
-  __shared__ T s_u[FIJ_NB][2][8][64];
+  // previous layer (j-1 <= 7) of each neighbour: U and dU (3 components) of
+  // its half plane (mb <= (j-1)/2, at mb*j + ma: at most 32 entries), double
+  // buffered; the middle column of an even layer reads its mirror image
+  // U(j-1-ma, j-1-mb) = (-1)^(ma+mb) conj U(ma, mb) (the same symmetry holds
+  // for dU), so no lane writes a mirror entry
+  __shared__ T s_u[FIJ_NB][2][8][FIJ_HP];
   __shared__ T s_y[2][SNAP_NY];
+  __shared__ layer_tab<T> LT;
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int hs = l / 32, hl = l % 32;
   const int i = blockIdx.x;
@@ -482,6 +537,7 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
     s_y[0][k] = y_r[(size_t)i * t.ny + k];
     s_y[1][k] = y_i[(size_t)i * t.ny + k];
   }
+  layer_tab_fill(t, LT);
   __syncthreads();
   const int jj0 = blockIdx.y * FIJ_NB + 2 * w;  // first half-wavefront's neighbour
   if (jj0 >= cnt_i) return;                     // uniform over the wavefront
@@ -519,7 +575,7 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   sfac *= (T)t.wj;
   dsfac *= (T)t.wj;
 
-  T (*buf)[8][64] = s_u[2 * w + hs];
+  T (*buf)[8][FIJ_HP] = s_u[2 * w + hs];
   const T *yr = s_y[0], *yi = s_y[1];
   T fx = 0, fy = 0, fz = 0;
   // layer 0: U = 1, dU = 0; its dU_tot = dsfac * u; contraction with Y_0
@@ -538,52 +594,54 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
     const int ne = (j + 1) * (j / 2 + 1);
     for (int s0 = 0; s0 < ne; s0 += 32) {
       const int e = s0 + hl;
-      int ma = 0, mb = 0;
-      const bool own = e < ne && lane_entry(e, j, ma, mb);
+      const int info = LT.mm[j][e];
+      const int ma = info & 255, mb = (info >> 8) & 255;
+      const bool own = info >> 16;
       T ur = 0, ui = 0, dr[3] = {0, 0, 0}, di[3] = {0, 0, 0};
       if (own) {
         const T *P = &buf[p][0][0];
-        // P[q*64 + idx]: q = 0 U_r, 1 U_i, 2+2k dU_r[k], 3+2k dU_i[k]
+        // P[q*FIJ_HP + idx]: q = 0 U_r, 1 U_i, 2+2k dU_r[k], 3+2k dU_i[k].
+        // Layer j-1 entries (ma, mb) and (ma-1, mb); on the middle column of
+        // an even layer (mb = j/2 > (j-1)/2) they are read as mirrors.
+        const bool mir = 2 * mb == j;
+        const T sgn = ((ma + mb) & 1) ? (T)-1 : (T)1;
+        const T s_r = mir ? sgn : (T)1, s_i = mir ? -sgn : (T)1;  // (ma, mb); (ma-1, mb) flips sgn
         if (ma < j) {
-          const T rp = (T)t.rootpq[(j - ma) * (SNAP_JD + 1) + (j - mb)];
-          const int sidx = ma * j + mb;
-          const T pr = P[sidx], pi = P[64 + sidx];
+          const T rp = LT.rpa(j, ma, mb);
+          const int sidx = mir ? (j - 1 - mb) * j + (j - 1 - ma) : mb * j + ma;
+          const T pr = s_r * P[sidx], pi = s_i * P[FIJ_HP + sidx];
           ur += rp * (a_r * pr + a_i * pi);
           ui += rp * (a_r * pi - a_i * pr);
 #pragma unroll
           for (int k = 0; k < 3; k++) {
-            const T dpr = P[(2 + 2 * k) * 64 + sidx], dpi = P[(3 + 2 * k) * 64 + sidx];
+            const T dpr = s_r * P[(2 + 2 * k) * FIJ_HP + sidx];
+            const T dpi = s_i * P[(3 + 2 * k) * FIJ_HP + sidx];
             dr[k] += rp * (da_r[k] * pr + da_i[k] * pi + a_r * dpr + a_i * dpi);
             di[k] += rp * (da_r[k] * pi - da_i[k] * pr + a_r * dpi - a_i * dpr);
           }
         }
         if (ma > 0) {
-          const T rp = (T)t.rootpq[ma * (SNAP_JD + 1) + (j - mb)];
-          const int sidx = (ma - 1) * j + mb;
-          const T pr = P[sidx], pi = P[64 + sidx];
+          const T rp = LT.rpb(j, ma, mb);
+          const int sidx = mir ? (j - 1 - mb) * j + (j - ma) : mb * j + ma - 1;
+          const T tr = mir ? -s_r : (T)1, ti = mir ? -s_i : (T)1;
+          const T pr = tr * P[sidx], pi = ti * P[FIJ_HP + sidx];
           ur += -rp * (b_r * pr + b_i * pi);
           ui += -rp * (b_r * pi - b_i * pr);
 #pragma unroll
           for (int k = 0; k < 3; k++) {
-            const T dpr = P[(2 + 2 * k) * 64 + sidx], dpi = P[(3 + 2 * k) * 64 + sidx];
+            const T dpr = tr * P[(2 + 2 * k) * FIJ_HP + sidx];
+            const T dpi = ti * P[(3 + 2 * k) * FIJ_HP + sidx];
             dr[k] += -rp * (db_r[k] * pr + db_i[k] * pi + b_r * dpr + b_i * dpi);
             di[k] += -rp * (db_r[k] * pi - db_i[k] * pr + b_r * dpi - b_i * dpr);
           }
         }
       }
       if (own && j < t.twojmax) {
+        // own half-plane entry only
         T *C = &buf[c][0][0];
-        const int sidx = ma * (j + 1) + mb;
-        C[sidx] = ur; C[64 + sidx] = ui;
+        C[e] = ur; C[FIJ_HP + e] = ui;
 #pragma unroll
-        for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + sidx] = dr[k]; C[(3 + 2 * k) * 64 + sidx] = di[k]; }
-        if (2 * mb != j) {
-          const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
-          const int m = (j - ma) * (j + 1) + (j - mb);
-          C[m] = sg * ur; C[64 + m] = -sg * ui;
-#pragma unroll
-          for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + m] = sg * dr[k]; C[(3 + 2 * k) * 64 + m] = -sg * di[k]; }
-        }
+        for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * FIJ_HP + e] = dr[k]; C[(3 + 2 * k) * FIJ_HP + e] = di[k]; }
       }
       if (own) {
         // half-plane weight of (ma, mb) in layer j
