@@ -342,6 +342,151 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
   }
 }
 
+// U_tot with 16-lane groups (twojmax 8): each wavefront runs four
+// neighbours side by side, one per 16-lane group, so a layer of h half-plane
+// entries takes ceil(h/16) steps for four neighbours instead of one step per
+// neighbour (13 steps per four neighbours instead of 32).  A group keeps
+// only the half plane of the previous layer in LDS and reads the middle
+// column of even layers as mirrors, U(J-ma, J-mb) = (-1)^(ma+mb) conj
+// U(ma, mb); each lane accumulates its own entries (fixed slots, fully
+// unrolled layers) over its group's neighbours in registers.  The sums are
+// then added over the four groups of a wavefront (xor tree) and over the
+// wavefronts in order: deterministic.
+constexpr int UG_LANES = 16;
+constexpr int UG_GROUPS = 64 / UG_LANES;           // per wavefront
+constexpr int UG_WAVES = 4;                        // per block
+constexpr int UG_HP = 32;                          // half-plane entries of layer 7
+
+__host__ __device__ constexpr int ug_hp(int j) { return (j + 1) * (j / 2 + 1); }
+__host__ __device__ constexpr int ug_steps(int j) { return (ug_hp(j) + UG_LANES - 1) / UG_LANES; }
+__host__ __device__ constexpr int ug_slot0(int j) { return j == 0 ? 0 : ug_slot0(j - 1) + ug_steps(j - 1); }
+constexpr int UG_SLOTS = ug_slot0(SNAP_MAXJ + 1);  // 14 for twojmax 8
+
+template <class T>
+__global__ void __launch_bounds__(64 * UG_WAVES)
+k_snap_ui_grp(snap_dev_tables t, const int *__restrict__ nin, const double *__restrict__ nr,
+              int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
+  constexpr int TJ = SNAP_MAXJ;
+  __shared__ T s_b[UG_WAVES * UG_GROUPS][2][2][UG_HP];  // [group][buffer][r/i][entry]
+  __shared__ T s_acc[UG_WAVES][2][SNAP_NY];
+  __shared__ layer_tab<T> LT;
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  layer_tab_fill(t, LT);
+  __syncthreads();
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int grp = l / UG_LANES, lg = l % UG_LANES;
+  T (*buf)[2][UG_HP] = s_b[w * UG_GROUPS + grp];
+  T acc_r[UG_SLOTS], acc_i[UG_SLOTS];
+#pragma unroll
+  for (int q = 0; q < UG_SLOTS; q++) acc_r[q] = acc_i[q] = 0;
+  const int cnt = nin[i];
+  for (int base = w * UG_GROUPS; base < cnt; base += UG_WAVES * UG_GROUPS) {
+    const int jj = base + grp;
+    const bool act = jj < cnt;
+    const size_t o = (size_t)i * maxn + (act ? jj : base);
+    const T x = (T)nr[3 * o], y = (T)nr[3 * o + 1], z = (T)nr[3 * o + 2];
+    const T r = sqrt(x * x + y * y + z * z);
+    const T theta0 = (r - (T)t.rmin0) * (T)t.rfac0 * (T)SNAP_PI / ((T)t.rcut - (T)t.rmin0);
+    const T z0 = r / tan(theta0);
+    T a_r, a_i, b_r, b_i;
+    ck_params(x, y, z, z0, r, a_r, a_i, b_r, b_i);
+    T sfac, dsfac;
+    sfac_dsfac(t, r, sfac, dsfac);
+    sfac = act ? sfac * (T)t.wj : (T)0;  // an idle group adds nothing
+    // layer 0: U = 1
+    if (lg == 0) {
+      buf[0][0][0] = 1;
+      buf[0][1][0] = 0;
+      acc_r[0] += sfac;
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 1; j <= TJ; j++) {
+      const int p = (j - 1) & 1, c = j & 1;
+#pragma unroll
+      for (int st = 0; st < ug_steps(j); st++) {
+        const int e = st * UG_LANES + lg;
+        const int info = LT.mm[j][e];
+        const bool own = e < ug_hp(j);
+        const int ma = info & 255, mb = (info >> 8) & 255;
+        T ur = 0, ui = 0;
+        if (own) {
+          const bool mir = 2 * mb == j;
+          const T sgn = ((ma + mb) & 1) ? (T)-1 : (T)1;
+          const T s_r = mir ? sgn : (T)1, s_i = mir ? -sgn : (T)1;
+          if (ma < j) {
+            const T rp = LT.rpa(j, ma, mb);
+            const int sidx = mir ? (j - 1 - mb) * j + (j - 1 - ma) : mb * j + ma;
+            const T pr = s_r * buf[p][0][sidx], pi = s_i * buf[p][1][sidx];
+            ur += rp * (a_r * pr + a_i * pi);
+            ui += rp * (a_r * pi - a_i * pr);
+          }
+          if (ma > 0) {
+            const T rp = LT.rpb(j, ma, mb);
+            const int sidx = mir ? (j - 1 - mb) * j + (j - ma) : mb * j + ma - 1;
+            const T tr = mir ? -s_r : (T)1, ti = mir ? -s_i : (T)1;
+            const T pr = tr * buf[p][0][sidx], pi = ti * buf[p][1][sidx];
+            ur += -rp * (b_r * pr + b_i * pi);
+            ui += -rp * (b_r * pi - b_i * pr);
+          }
+          if (j < TJ) {
+            buf[c][0][e] = ur;
+            buf[c][1][e] = ui;
+          }
+        }
+        acc_r[ug_slot0(j) + st] += sfac * ur;
+        acc_i[ug_slot0(j) + st] += sfac * ui;
+      }
+      wave_sync();
+    }
+  }
+  // sum over the four groups of the wavefront, then over the wavefronts
+#pragma unroll
+  for (int q = 0; q < UG_SLOTS; q++) {
+    acc_r[q] += __shfl_xor(acc_r[q], UG_LANES, 64);
+    acc_i[q] += __shfl_xor(acc_i[q], UG_LANES, 64);
+    acc_r[q] += __shfl_xor(acc_r[q], 2 * UG_LANES, 64);
+    acc_i[q] += __shfl_xor(acc_i[q], 2 * UG_LANES, 64);
+  }
+  if (grp == 0) {
+#pragma unroll
+    for (int j = 0; j <= TJ; j++)
+#pragma unroll
+      for (int st = 0; st < ug_steps(j); st++) {
+        const int e = st * UG_LANES + lg;
+        if (e < ug_hp(j)) {
+          s_acc[w][0][t.ystart[j] + e] = acc_r[ug_slot0(j) + st];
+          s_acc[w][1][t.ystart[j] + e] = acc_i[ug_slot0(j) + st];
+        }
+      }
+  }
+  __syncthreads();
+  if (w != 0) return;
+  // U_tot = wself * I + the wavefront sums in order; own entries and mirrors
+  T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
+  for (int j = 0; j <= TJ; j++) {
+    const int info = LT.mm[j][l];
+    const int ma = info & 255, mb = (info >> 8) & 255;
+    if (info >> 16) {
+      T vr = 0, vi = 0;
+#pragma unroll
+      for (int q = 0; q < UG_WAVES; q++) {
+        vr += s_acc[q][0][t.ystart[j] + l];
+        vi += s_acc[q][1][t.ystart[j] + l];
+      }
+      if (ma == mb) vr += (T)t.wself;
+      ur[t.ustart[j] + ma * (j + 1) + mb] = vr;
+      ui[t.ustart[j] + ma * (j + 1) + mb] = vi;
+      if (2 * mb != j) {
+        const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
+        ur[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = sg * vr;
+        ui[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = -sg * vi;
+      }
+    }
+  }
+}
+
 // Z_{abJ}(ma,mb) = sum_ma1 cg(ma1,ma2) sum_mb1 cg(mb1,mb2) U_a(ma1,mb1) U_b(ma2,mb2)
 // (SNA::compute_zi, sna_impl.hpp:221-274) from U_tot staged in LDS
 template <class T>
@@ -1121,10 +1266,23 @@ static emd_status snap_neigh(emd_ctx *ctx, const emd_system *s, const emd_neigh 
   return EMD_OK;
 }
 
+// U_tot of every atom: the grouped kernel for twojmax 8, else one neighbour
+// per wavefront
+template <class T>
+static emd_status launch_ui(emd_ctx *ctx, emd_snap *sn, int n, T *ur, T *ui) {
+  if (ctx->snap_ui_grp && sn->t.twojmax == SNAP_MAXJ)
+    EMD_LAUNCH(ctx, "snap_ui", k_snap_ui_grp<T>, n, 64 * UG_WAVES, 0, sn->t, sn->nin, sn->nr,
+               sn->cap_maxn, n, ur, ui);
+  else
+    EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr,
+               sn->cap_maxn, n, ur, ui);
+  return EMD_OK;
+}
+
 template <class T>
 static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   T *ur = (T *)sn->u_r, *ui = (T *)sn->u_i, *yr = (T *)sn->y_r, *yi = (T *)sn->y_i;
-  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n, ur, ui);
+  EMD_TRY(launch_ui<T>(ctx, sn, n, ur, ui));
   if (ctx->snap_yflat) {
     const T *w = std::is_same<T, float>::value ? (const T *)sn->yf_w32 : (const T *)sn->yf_w64;
     const size_t shmem = (size_t)(sn->t.nu + sn->yf_nchunks) * 2 * sizeof(T);
@@ -1166,8 +1324,7 @@ extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const e
   *energy = 0.0;
   if (n <= 0) return EMD_OK;
   EMD_TRY(snap_neigh(ctx, s, nl, sn));
-  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<double>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n,
-             (double *)sn->u_r, (double *)sn->u_i);
+  EMD_TRY(launch_ui<double>(ctx, sn, n, (double *)sn->u_r, (double *)sn->u_i));
   double *ea = eatom;
   if (!ea) {
     EMD_TRY(emd_ensure_int(ctx, (int **)&sn->eatom, &sn->cap_e, 2l * n));
@@ -1417,8 +1574,7 @@ template <class T>
 static emd_status snap_adjoint_impl(emd_ctx *ctx, const emd_system *s, const emd_halo *h,
                                     emd_snap *sn, const double *g, double *S) {
   const int n = s->n_local;
-  EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n,
-             (T *)sn->u_r, (T *)sn->u_i);
+  EMD_TRY(launch_ui<T>(ctx, sn, n, (T *)sn->u_r, (T *)sn->u_i));
   EMD_LAUNCH(ctx, "snap_adjoint_w", k_snap_w<T>, n, 64, 0, sn->t, sn->nin, sn->nj, sn->nr,
              sn->cap_maxn, n, h ? h->root : nullptr, g, sn->w_r, sn->w_i);
   const int ntrip = (int)sn->tid.size();
@@ -1451,8 +1607,7 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
   else EMD_TRY(snap_adjoint_impl<double>(ctx, s, h, sn, dLdF, S));
   // dE/dbeta_k = sum_i B_k^i in FP64
   if (dLdE != 0.0) {
-    EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<double>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn,
-               n, (double *)sn->u_r, (double *)sn->u_i);
+    EMD_TRY(launch_ui<double>(ctx, sn, n, (double *)sn->u_r, (double *)sn->u_i));
     EMD_LAUNCH(ctx, "snap_adjoint_bsum", k_snap_bsum, n, 256, 0, sn->t, sn->cg, sn->etab,
                sn->ncoeff, sn->bzeroflag, n, (const double *)sn->u_r, (const double *)sn->u_i, B);
   }
