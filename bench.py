@@ -28,13 +28,21 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X vector FP64
 FP32_PEAK_TFLOPS = 157.3    # MI355X vector FP32 (published, packed)
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """one line per finished leg on stderr (the JSON line stays alone on stdout)"""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="lj4m", choices=["lj4m", "lj32k"])
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--snap-steps", type=int, default=100)
@@ -381,6 +389,7 @@ def main():
         if prec == "mixed":
             out[prec]["melt_error"] = melt_force_error(emd, app)
         app.close()
+        progress(f"LJ {natoms} atoms {prec} ({a.neigh}): {natoms * a.steps / wall:.4g} atom-steps/s")
 
     # the mixed run again with the other list layout (same workload)
     alt = None
@@ -389,6 +398,7 @@ def main():
         app, wall, prof, nlist = lj_run(emd, decks, tmpdir, n, "mixed", a.warmup, a.steps,
                                         neigh=lay, breakdown_steps=0)
         cnt, ms = prof.get("lj_force_mixed", (0, 0.0))
+        progress(f"LJ mixed ({lay}): {app.natoms * a.steps / wall:.4g} atom-steps/s")
         alt = {"neigh": lay, "value": app.natoms * a.steps / wall,
                "ms_per_step": 1e3 * wall / a.steps,
                "lj_force_mixed_avg_ms": ms / max(cnt, 1)}
@@ -423,6 +433,7 @@ def main():
         coeff, param, beta = snap_files(emd, decks, tmpdir)
         for prec in ("fp64", "mixed"):
             sn, sw, se, sb = snap_run(emd, decks, tmpdir, coeff, param, prec, 10, a.snap_steps)
+            progress(f"SNAP {sn} atoms {prec}: {sn * a.snap_steps / sw:.4g} atom-steps/s")
             snap[prec] = {"value": sn * a.snap_steps / sw, "ms_per_step": 1e3 * sw / a.snap_steps,
                           "snap_kernels_ms_per_step": {k: v for k, v in sb.items()
                                                        if k in ("snap_ui", "snap_y", "snap_fij")},
@@ -447,7 +458,11 @@ def main():
                             f"rebuild every 10, dt 0.5 fs, {a.snap_steps} steps")
 
     batched = None if a.no_batch else batch_leg(emd, a.batch_steps, a.loop_steps)
+    if batched:
+        progress("batched replicas done")
     gradients = None if a.no_forcematch else forcematch_leg(emd)
+    if gradients:
+        progress("force-matching gradients done")
 
     line = {
         "metric": "atom-steps/s on 1 MI355X, LJ 4M & SNAP 2k atoms: mixed vs FP64 at fixed error",
@@ -489,6 +504,7 @@ def main():
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.cpu_steps, a.cpu_threads)
+        progress("CPU baseline done")
     print(json.dumps(line), flush=True)
     if a.calib_copy_mib:
         # a device-to-device copy of known size, for scaling profiler byte counters

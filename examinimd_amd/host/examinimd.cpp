@@ -90,6 +90,7 @@ void ExaMiniMD::init(int argc, char* argv[]) {
     printf("Using: %s %s %s %s\n", force->name(), neighbor->name(), comm->name(), binning->name());
 
   if (system->N == 0) input->create_lattice(comm);
+  if (input->restart_file) read_restart(input->restart_file);
 
   comm->exchange();
   T_F_FLOAT neigh_cutoff = input->force_cutoff + input->neighbor_skin;
@@ -247,6 +248,35 @@ void ExaMiniMD::run(int nsteps) {
              comm->num_processes(), nsteps, system->N);
     }
   }
+}
+
+// initial state from a binary dump (the layout dump_binary writes): the
+// deck still defines the box, types and potential; ids, types, charges,
+// positions and velocities of the N local atoms come from the file, in its
+// order.  The stored force is not used (init recomputes it).
+void ExaMiniMD::read_restart(const char* filename) {
+  FILE* fp = fopen(filename, "rb");
+  if (fp == NULL) {
+    char str[MAXPATHLEN + 64];
+    snprintf(str, sizeof(str), "Cannot open restart file %s", filename);
+    comm->error(str);
+  }
+  T_INT n = 0;
+  if (fread(&n, sizeof(T_INT), 1, fp) != 1 || n != system->N_local)
+    comm->error("Restart file atom count does not match the deck");
+  std::vector<int> id(n), type(n);
+  std::vector<double> q(n), x(3 * (size_t)n), v(3 * (size_t)n);
+  size_t got = fread(id.data(), sizeof(T_INT), n, fp) + fread(type.data(), sizeof(T_INT), n, fp) +
+               fread(q.data(), sizeof(T_FLOAT), n, fp) +
+               fread(x.data(), sizeof(T_X_FLOAT), 3 * (size_t)n, fp) +
+               fread(v.data(), sizeof(T_V_FLOAT), 3 * (size_t)n, fp);
+  fclose(fp);
+  if (got != (size_t)n * 9) comm->error("Short restart file");
+  for (int k = 0; k < n; k++)
+    if (type[k] < 0 || type[k] >= system->ntypes) comm->error("Restart file has an unknown atom type");
+  emd_ctx* ctx = system->ctx;
+  EMD_CALL(ctx, emd_system_upload(ctx, &system->d, n, x.data(), v.data(), type.data(), id.data(),
+                                  q.data()));
 }
 
 // dump layout (src/examinimd.cpp:296-346): int n, int id[n], int type[n],

@@ -35,6 +35,7 @@ public:
   void run(int nsteps);
   void thermo(double out[3]);
   void dump_binary(int step);
+  void read_restart(const char* filename);
   void check_correctness(int step);
   void print_performance();
   void shutdown();

@@ -130,6 +130,7 @@ Input::Input(System* p) : timestepflag(false), system(p) {
   dumpbinaryflag = false;
   correctnessflag = false;
   dumpbinary_path = reference_path = correctness_file = nullptr;
+  restart_file = nullptr;
   comm_newton = 0;
   precision = EMD_FP64;
   device = 0;
@@ -155,6 +156,8 @@ void Input::read_command_line_args(int argc, char* argv[]) {
         printf("  --precision [fp64|mixed]:   force arithmetic (mixed: FP32 pair math and\n");
         printf("                              accumulation, FP64 state)\n");
         printf("  --device [N]:               GPU ordinal\n");
+        printf("  --restart [FILE]:           start from a binary dump FILE (the --dumpbinary\n");
+        printf("                              format) instead of the deck's lattice\n");
       }
     } else if ((strcmp(argv[i], "-il") == 0) || (strcmp(argv[i], "--input-lammps") == 0)) {
       input_file = argv[++i];
@@ -189,6 +192,8 @@ void Input::read_command_line_args(int argc, char* argv[]) {
       }
     } else if ((strcmp(argv[i], "--device") == 0)) {
       device = atoi(argv[++i]);
+    } else if ((strcmp(argv[i], "--restart") == 0)) {
+      restart_file = argv[++i];
     } else if ((strstr(argv[i], "--kokkos-") == NULL)) {
       if (system->do_print) printf("ERROR: Unknown command line argument: %s\n", argv[i]);
       exit(1);

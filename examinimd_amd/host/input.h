@@ -68,6 +68,7 @@ public:
   int thermo_rate, dumpbinary_rate, correctness_rate;
   bool dumpbinaryflag, correctnessflag;
   char *dumpbinary_path, *reference_path, *correctness_file;
+  char *restart_file;  // --restart: initial state from a binary dump
 
   int precision;  // EMD_FP64 or EMD_MIXED
   int device;

@@ -480,3 +480,31 @@ def test_lj32k_2d_equals_csr_default_lanes(tmp_path, precision):
         app.close()
     for k in ("x", "v", "f"):
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_restart_from_binary_dump_continues_bitwise(tmp_path, precision):
+    """--restart from a --dumpbinary file written after step 40 (a rebuild
+    step) and 40 more steps give the bits of an uninterrupted 80-step run:
+    the dump holds the sorted, wrapped positions the rebuild used, so the
+    restarted run rebuilds the same list"""
+    print(f"case: LJ 32k restart at step 40 vs 80 steps, {precision}", flush=True)
+    deck = decks.write(str(tmp_path / "in.lj"), decks.lj_deck(n=N_CELLS, steps=80))
+    dump_dir = tmp_path / "dump"
+    dump_dir.mkdir()
+    base = ["-il", deck, "--neigh-type", "CSR", "--comm-type", "SERIAL", "--precision", precision]
+    a = emd.App(base + ["--dumpbinary", "40", str(dump_dir)]).init()
+    a.run(40)
+    a.close()
+    dump = dump_dir / "output.0000000040.000"
+    assert dump.exists()
+    b = emd.App(base).init()
+    b.run(80)
+    want = b.arrays()
+    b.close()
+    c = emd.App(base + ["--restart", str(dump)]).init()
+    c.run(40)
+    got = c.arrays()
+    c.close()
+    for k in ("id", "x", "v", "f"):
+        assert np.array_equal(got[k], want[k]), k
