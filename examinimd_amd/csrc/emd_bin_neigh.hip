@@ -214,7 +214,7 @@ __global__ void k_bin_copy_back(int r0, int n, const double *__restrict__ tx,
 static emd_status bin_finish(emd_ctx *ctx, emd_system *s, emd_bins *b, int r0, int n, int sort) {
   EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, 2 * sizeof(int),
                               hipMemcpyDeviceToHost, ctx->stream));
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  EMD_TRY(emd_wait(ctx));
   b->max_count = ctx->h_small[1];
   if (ctx->h_small[2])
     return emd_set_error(ctx, EMD_ERR_OVERFLOW,
@@ -682,7 +682,7 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
   if (n > 0) EMD_TRY(emd_scan(ctx, op, n, "neigh_offsets", &total));
   EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 4, ctx->d_small + 4, sizeof(int),
                               hipMemcpyDeviceToHost, ctx->stream));
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  EMD_TRY(emd_wait(ctx));
   const int maxrow = ctx->h_small[4];
   nl->total = total;
   if (layout == EMD_NEIGH_2D) {

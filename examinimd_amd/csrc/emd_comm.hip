@@ -83,12 +83,14 @@ struct halo_phase_op {
 // phase; the host grows and runs it again.
 constexpr int HALO_ONE_THREADS = 1024;
 constexpr int HALO_ONE_MAX = 16384;  // local + ghost atoms; above, the per-phase scans
+constexpr int HALO_ONE_ROUNDS = HALO_ONE_MAX / HALO_ONE_THREADS;
 
 struct halo_one_args {
   halo_phase_op op[6];
   int cap_pack[6];
   int n_local, n_max, root_cap;
-  int *out;  // [0..5] ghosts per phase, [6] 1 + the phase that did not fit (0: all fit)
+  int *out;  // [0..5] ghosts per phase, [6] 1 + the phase that did not fit (0: all fit;
+             // 7: the system is too large for one workgroup)
 };
 
 __global__ void __launch_bounds__(HALO_ONE_THREADS) k_halo_one(halo_one_args a) {
@@ -98,12 +100,23 @@ __global__ void __launch_bounds__(HALO_ONE_THREADS) k_halo_one(halo_one_args a) 
   for (int phase = 0; phase < 6 && !failed; phase++) {
     halo_phase_op op = a.op[phase];
     const int ncand = a.n_local + n_ghost - ((phase & 1) ? prev : 0);
+    if (ncand > HALO_ONE_MAX) {  // more candidates than one workgroup's rounds
+      failed = 7;
+      break;
+    }
     op.dst0 = a.n_local + n_ghost;
     const int room = min(min(a.n_max - op.dst0, a.cap_pack[phase]), a.root_cap - n_ghost);
+    // the predicates of all rounds first (their loads in flight together)
+    unsigned flags = 0;
+#pragma unroll
+    for (int r = 0; r < HALO_ONE_ROUNDS; r++) {
+      const int i = r * HALO_ONE_THREADS + tid;
+      if (i < ncand && op.value(i)) flags |= 1u << r;
+    }
     int k = 0;
-    for (int base = 0; base < ncand; base += HALO_ONE_THREADS) {
+    for (int base = 0, r = 0; base < ncand; base += HALO_ONE_THREADS, r++) {
       const int i = base + tid;
-      const int v = (i < ncand) ? op.value(i) : 0;
+      const int v = (flags >> r) & 1;
       const int inc = emd_wave_incl_scan(v);
       if (lane == EMD_WAVE - 1) s_w[wid] = inc;
       __syncthreads();
@@ -157,9 +170,10 @@ static emd_status exchange_halo_one(emd_ctx *ctx, emd_system *s, double comm_dep
     EMD_LAUNCH(ctx, "comm_exchange_halo", k_halo_one, 1, HALO_ONE_THREADS, 0, a);
     EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 8, ctx->d_small + 8, 7 * sizeof(int),
                                 hipMemcpyDeviceToHost, ctx->stream));
-    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_TRY(emd_wait(ctx));
     const int *cnt = ctx->h_small + 8;
     const int failed = cnt[6];
+    if (failed == 7) return EMD_ERR_OVERFLOW;  // the caller runs the per-phase scans
     int n_ghost = 0;
     for (int p = 0; p < (failed ? failed : 6); p++) n_ghost += cnt[p];
     if (!failed) {
@@ -188,8 +202,11 @@ extern "C" emd_status emd_comm_exchange_halo(emd_ctx *ctx, emd_system *s, double
                                              emd_halo *h) {
   if (!ctx || !s || !h) return EMD_ERR_ARG;
   const int n_local = s->n_local;
-  if (ctx->halo_one && (long)n_local + h->n_ghost <= HALO_ONE_MAX && n_local <= HALO_ONE_MAX / 2)
-    return exchange_halo_one(ctx, s, comm_depth, h);
+  if (ctx->halo_one && (long)n_local + h->n_ghost <= HALO_ONE_MAX && n_local <= HALO_ONE_MAX / 2) {
+    const emd_status st = exchange_halo_one(ctx, s, comm_depth, h);
+    if (st != EMD_ERR_OVERFLOW) return st;
+    // the halo outgrew one workgroup: the per-phase scans below
+  }
   int n_ghost = 0;
   for (int phase = 0; phase < 6; phase++) {
     const int dim = phase / 2;

@@ -84,6 +84,8 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
   // ghost force fold of a small halo (<= 16384 ghosts) in one launch (1) or
   // one launch per phase (0)
   if (const char *e = getenv("EMD_FOLD_ONE")) ctx->fold_one = atoi(e) != 0;
+  // host waits for counts: poll an event (1) or hipStreamSynchronize (0)
+  if (const char *e = getenv("EMD_SPIN_WAIT")) ctx->spin_wait = atoi(e) != 0;
   // exchange_halo of a small system (<= 16384 local + ghost atoms) in one
   // launch (1) or as six scans with a host sync each (0)
   if (const char *e = getenv("EMD_HALO_ONE")) ctx->halo_one = atoi(e) != 0;
@@ -97,10 +99,11 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
     const int b = atoi(e);
     if (b >= 0 && b <= 4096) ctx->lj_pipe_blocks = b;
   }
-  // entries per lane and memory round of the 16-lane CSR LJ kernel (2..6)
+  // entries per lane and memory round of the LJ kernel: 16 lanes 2..6,
+  // 8 lanes 5, 8 or 10
   if (const char *e = getenv("EMD_LJ_UNROLL")) {
     const int u = atoi(e);
-    if (u >= 2 && u <= 6) ctx->lj_unroll = u;
+    if ((u >= 2 && u <= 6) || u == 8 || u == 10) ctx->lj_unroll = u;
   }
   if (const char *e = getenv("EMD_LJ_TILE_LANES")) {
     const int k = atoi(e);
@@ -124,15 +127,29 @@ extern "C" emd_status emd_destroy(emd_ctx *ctx) {
   if (ctx->d_tmp) hipFree(ctx->d_tmp);
   if (ctx->d_tmp2) hipFree(ctx->d_tmp2);
   if (ctx->d_red) hipFree(ctx->d_red);
+  if (ctx->wait_ev) hipEventDestroy(ctx->wait_ev);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return EMD_OK;
 }
 
+emd_status emd_wait(emd_ctx *ctx) {
+  if (!ctx->spin_wait) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return EMD_OK;
+  }
+  if (!ctx->wait_ev) EMD_HIP(ctx, hipEventCreateWithFlags(&ctx->wait_ev, hipEventDisableTiming));
+  EMD_HIP(ctx, hipEventRecord(ctx->wait_ev, ctx->stream));
+  hipError_t e;
+  while ((e = hipEventQuery(ctx->wait_ev)) == hipErrorNotReady) {
+  }
+  EMD_HIP(ctx, e);
+  return EMD_OK;
+}
+
 extern "C" emd_status emd_synchronize(emd_ctx *ctx) {
   if (!ctx) return EMD_ERR_ARG;
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return EMD_OK;
+  return emd_wait(ctx);
 }
 
 extern "C" void *emd_get_stream(emd_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }

@@ -46,6 +46,11 @@ struct emd_ctx {
   double *d_red = nullptr;     // reduction partials
   size_t red_cap = 0;
 
+  // host waits: an event recorded on the stream and polled, so a wait for a
+  // count ends as soon as the copy lands (no blocking-sync wake-up latency)
+  hipEvent_t wait_ev = nullptr;
+  int spin_wait = 1;
+
   // kernel timing
   bool profile = false;
   std::vector<std::string> prof_names;
@@ -108,6 +113,8 @@ void emd_prof_end(emd_ctx *ctx, emd_prof_rec *rec);
 
 // grow a device buffer (contents not preserved)
 emd_status emd_ensure(emd_ctx *ctx, void **p, size_t *cap, size_t bytes);
+// wait until all work queued on the context's stream has completed
+emd_status emd_wait(emd_ctx *ctx);
 template <class T>
 inline emd_status emd_ensure_t(emd_ctx *ctx, T **p, size_t *cap, size_t count) {
   return emd_ensure(ctx, reinterpret_cast<void **>(p), cap, count * sizeof(T));
@@ -234,7 +241,7 @@ emd_status emd_scan_count(emd_ctx *ctx, const F &f, int n, const char *name, int
   if (total_host) {
     EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int), hipMemcpyDeviceToHost,
                                 ctx->stream));
-    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_TRY(emd_wait(ctx));
     *total_host = ctx->h_small[0];
   }
   return EMD_OK;
@@ -255,7 +262,7 @@ emd_status emd_scan(emd_ctx *ctx, const F &f, int n, const char *name, int *tota
   if (total_host) {
     EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int), hipMemcpyDeviceToHost,
                                 ctx->stream));
-    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    EMD_TRY(emd_wait(ctx));
     *total_host = ctx->h_small[0];
   }
   return EMD_OK;

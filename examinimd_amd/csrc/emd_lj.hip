@@ -625,7 +625,12 @@ static emd_status lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
     case 1: return launch_full<1, 4>(ctx, s, nl, p, precision, tab, store);
     case 2: return launch_full<2, 4>(ctx, s, nl, p, precision, tab, store);
     case 4: return launch_full<4, 4>(ctx, s, nl, p, precision, tab, store);
-    case 8: return launch_full<8, 5>(ctx, s, nl, p, precision, tab, store);
+    case 8:
+      switch (ctx->lj_unroll) {
+        case 8: return launch_full<8, 8>(ctx, s, nl, p, precision, tab, store);
+        case 10: return launch_full<8, 10>(ctx, s, nl, p, precision, tab, store);
+        default: return launch_full<8, 5>(ctx, s, nl, p, precision, tab, store);
+      }
     default:
       switch (ctx->lj_unroll) {
         case 2: return launch_full<16, 2>(ctx, s, nl, p, precision, tab, store);
@@ -688,7 +693,7 @@ static emd_status finish_dd(emd_ctx *ctx, double *part, int nb, double *out) {
   std::vector<double> h(2 * (size_t)nb);
   EMD_HIP(ctx, hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                               ctx->stream));
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  EMD_TRY(emd_wait(ctx));
   // Neumaier summation of the block (hi, lo) pairs
   double sum = 0.0, c = 0.0;
   for (size_t k = 0; k < h.size(); k++) {
@@ -800,7 +805,7 @@ extern "C" emd_status emd_lj_adjoint(emd_ctx *ctx, const emd_system *s, const em
                nl->num, nl->entries, nl->maxneighs, n, eps, sigma, cut * cut, dLdF, acc);
   EMD_HIP(ctx, hipMemcpyAsync(ctx->h_dsmall, acc, 4 * sizeof(double), hipMemcpyDeviceToHost,
                               ctx->stream));
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  EMD_TRY(emd_wait(ctx));
   out[0] = ctx->h_dsmall[0] + dLdE * ctx->h_dsmall[2];
   out[1] = ctx->h_dsmall[1] + dLdE * ctx->h_dsmall[3];
   return EMD_OK;

@@ -1335,7 +1335,7 @@ extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const e
              (const double *)sn->u_r, (const double *)sn->u_i, ea);
   std::vector<double> h(n);
   EMD_HIP(ctx, hipMemcpyAsync(h.data(), ea, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  EMD_TRY(emd_wait(ctx));
   double sum = 0.0, c = 0.0;
   for (int k = 0; k < n; k++) {
     const double v = h[k], tt = sum + v;
@@ -1614,7 +1614,7 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
   std::vector<double> hS(SNAP_MAXT + 64);
   EMD_HIP(ctx, hipMemcpyAsync(hS.data(), sn->sbuf, hS.size() * sizeof(double),
                               hipMemcpyDeviceToHost, ctx->stream));
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  EMD_TRY(emd_wait(ctx));
   dLdbeta[0] = dLdE * n;
   for (int k = 0; k < sn->ncoeff; k++) {
     const int j1 = sn->idxj[3 * k], j2 = sn->idxj[3 * k + 1], j = sn->idxj[3 * k + 2];

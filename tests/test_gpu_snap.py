@@ -194,15 +194,17 @@ def test_ghost_force_fold_against_numpy(monkeypatch, nc, cut, mixed):
     assert np.array_equal(results[0][1], results[1][1])
 
 
-@pytest.mark.parametrize("nc", [10, 6])
-def test_halo_exchange_one_launch_equals_scans(monkeypatch, nc):
+@pytest.mark.parametrize("nc,cut", [(10, 5.73442), (6, 5.73442), (8, 20.0)])
+def test_halo_exchange_one_launch_equals_scans(monkeypatch, nc, cut):
     """exchange_halo of a small system in one workgroup launch gives the
     ghosts of the six per-phase scans: counts, pack lists, positions,
     velocities, ids, roots and image codes, bit for bit (including the first
-    call, which starts with no capacity and grows)"""
+    call, which starts with no capacity and grows).  BCC 8^3 with a 20 A
+    halo depth has ~27 000 local + ghost atoms, more than one workgroup
+    handles: that call falls back to the scans mid-way."""
     import ctypes as C
     from examinimd_amd.pipeline import Pipeline, lattice
-    print(f"case: exchange_halo one launch vs scans, BCC {nc}^3", flush=True)
+    print(f"case: exchange_halo one launch vs scans, BCC {nc}^3, depth {cut}", flush=True)
     a = 3.1803
     x = lattice("bcc", nc, a) + np.random.default_rng(43).normal(0, 0.08, (2 * nc ** 3, 3))
     v = np.random.default_rng(44).normal(0, 1, x.shape)
@@ -213,8 +215,8 @@ def test_halo_exchange_one_launch_equals_scans(monkeypatch, nc):
         pl.load(x, (nc * a,) * 3, mass=183.84, v=v)
         L, c, s = pl.L, pl.ctx.h, C.byref(pl.s)
         pl.chk(L.emd_comm_exchange(c, s), "exchange")
-        pl.chk(L.emd_binning(c, s, 5.73442, 5.73442, 5.73442, 1, 1, 0, 1, C.byref(pl.b)), "bin")
-        pl.chk(L.emd_comm_exchange_halo(c, s, 5.73442, C.byref(pl.h)), "halo")
+        pl.chk(L.emd_binning(c, s, cut, cut, cut, 1, 1, 0, 1, C.byref(pl.b)), "bin")
+        pl.chk(L.emd_comm_exchange_halo(c, s, cut, C.byref(pl.h)), "halo")
         n, ng = pl.s.n_local, pl.s.n_ghost
         tot = n + ng
         rec = {"counts": [pl.h.num_ghost[p] for p in range(6)], "n_ghost": ng}
@@ -233,7 +235,7 @@ def test_halo_exchange_one_launch_equals_scans(monkeypatch, nc):
                                         arr.nbytes), name)
             rec[name] = arr
         # a second exchange on the grown capacities gives the same ghosts
-        pl.chk(L.emd_comm_exchange_halo(c, s, 5.73442, C.byref(pl.h)), "halo again")
+        pl.chk(L.emd_comm_exchange_halo(c, s, cut, C.byref(pl.h)), "halo again")
         assert pl.s.n_ghost == ng
         out.append(rec)
         pl.close()
