@@ -654,15 +654,21 @@ __global__ void k_yf_weights(int nslots, const unsigned short *__restrict__ key,
 // one pair per wavefront.  A half-wavefront without a neighbour computes on
 // its partner's and discards the result (no lane leaves before the wave
 // barriers).
+// LG = 16 runs four neighbours per wavefront instead: a layer of h entries
+// takes ceil(h/16) steps for four neighbours (14 steps over layers 1-8)
+// instead of ceil(h/32) for two (10 steps), at twice the LDS per wavefront.
 constexpr int FIJ_WAVES = 2;                      // wavefronts per block
-constexpr int FIJ_NB = 2 * FIJ_WAVES;             // neighbours per block
 constexpr int FIJ_HP = 32;                        // half-plane entries of layer 7
+template <int LG>
+constexpr int fij_nb() { return FIJ_WAVES * (64 / LG); }  // neighbours per block
 
-template <class T, class TF>
+template <int LG, class T, class TF>
 __global__ void __launch_bounds__(64 * FIJ_WAVES)
 k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
            const double *__restrict__ nr, int maxn, int n, const T *__restrict__ y_r,
            const T *__restrict__ y_i, TF *__restrict__ f) {
+  constexpr int FIJ_NB = fij_nb<LG>();
+  constexpr int NBW = 64 / LG;  // neighbours per wavefront
   // previous layer (j-1 <= 7) of each neighbour: U and dU (3 components) of
   // its half plane (mb <= (j-1)/2, at mb*j + ma: at most 32 entries), double
   // buffered; the middle column of an even layer reads its mirror image
@@ -672,7 +678,7 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   __shared__ T s_y[2][SNAP_NY];
   __shared__ layer_tab<T> LT;
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-  const int hs = l / 32, hl = l % 32;
+  const int hs = l / LG, hl = l % LG;  // neighbour slot of the wavefront, lane in it
   const int i = blockIdx.x;
   if (i >= n) return;
   const int cnt_i = nin[i];
@@ -684,8 +690,8 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   }
   layer_tab_fill(t, LT);
   __syncthreads();
-  const int jj0 = blockIdx.y * FIJ_NB + 2 * w;  // first half-wavefront's neighbour
-  if (jj0 >= cnt_i) return;                     // uniform over the wavefront
+  const int jj0 = blockIdx.y * FIJ_NB + NBW * w;  // the wavefront's first neighbour
+  if (jj0 >= cnt_i) return;                       // uniform over the wavefront
   const int jj = jj0 + hs;
   const bool act = jj < cnt_i;
   const size_t o = (size_t)i * maxn + (act ? jj : jj0);
@@ -720,7 +726,7 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   sfac *= (T)t.wj;
   dsfac *= (T)t.wj;
 
-  T (*buf)[8][FIJ_HP] = s_u[2 * w + hs];
+  T (*buf)[8][FIJ_HP] = s_u[NBW * w + hs];
   const T *yr = s_y[0], *yi = s_y[1];
   T fx = 0, fy = 0, fz = 0;
   // layer 0: U = 1, dU = 0; its dU_tot = dsfac * u; contraction with Y_0
@@ -737,7 +743,7 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   for (int j = 1; j <= t.twojmax; j++) {
     const int p = (j - 1) & 1, c = j & 1;
     const int ne = (j + 1) * (j / 2 + 1);
-    for (int s0 = 0; s0 < ne; s0 += 32) {
+    for (int s0 = 0; s0 < ne; s0 += LG) {
       const int e = s0 + hl;
       const int info = LT.mm[j][e];
       const int ma = info & 255, mb = (info >> 8) & 255;
@@ -805,12 +811,12 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
     }
     wave_sync();
   }
-  // half-wavefront sum of the force components
+  // sum of the force components over the neighbour's lanes
 #pragma unroll
-  for (int off = 16; off > 0; off >>= 1) {
-    fx += __shfl_xor(fx, off, 32);
-    fy += __shfl_xor(fy, off, 32);
-    fz += __shfl_xor(fz, off, 32);
+  for (int off = LG / 2; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off, LG);
+    fy += __shfl_xor(fy, off, LG);
+    fz += __shfl_xor(fz, off, LG);
   }
   if (hl == 0 && act) {
     // ad-hoc repulsion of the reference, force_snap_neigh_impl.h:705-711
@@ -1295,6 +1301,25 @@ static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   return EMD_OK;
 }
 
+// F_ij for every (atom, neighbour): 16 lanes per neighbour by default
+// (EMD_SNAP_FIJ_LANES=32 keeps two neighbours per wavefront)
+template <class T, class TF>
+static emd_status launch_fij(emd_ctx *ctx, emd_snap *sn, int n, const T *yr, const T *yi,
+                             TF *f) {
+  if (ctx->snap_fij_lanes == 32) {
+    constexpr int nb = fij_nb<32>();
+    const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
+    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<32, T, TF>), grid, 64 * FIJ_WAVES, 0, sn->t,
+               sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
+  } else {
+    constexpr int nb = fij_nb<16>();
+    const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
+    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<16, T, TF>), grid, 64 * FIJ_WAVES, 0, sn->t,
+               sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
+  }
+  return EMD_OK;
+}
+
 extern "C" emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                                      emd_snap *sn, int precision) {
   if (!ctx || !s || !nl || !sn) return EMD_ERR_ARG;
@@ -1302,17 +1327,14 @@ extern "C" emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neig
   const int n = s->n_local;
   if (n <= 0) return EMD_OK;
   EMD_TRY(snap_neigh(ctx, s, nl, sn));
-  const dim3 grid(n, (sn->cap_maxn + FIJ_NB - 1) / FIJ_NB);
   if (precision == EMD_MIXED) {
     EMD_TRY(snap_ui_y<float>(ctx, sn, n));
-    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<float, float>), grid, 64 * FIJ_WAVES, 0, sn->t, sn->nin,
-               sn->nj, sn->nr, sn->cap_maxn, n, (const float *)sn->y_r, (const float *)sn->y_i,
-               s->f32);
+    EMD_TRY((launch_fij<float, float>(ctx, sn, n, (const float *)sn->y_r,
+                                      (const float *)sn->y_i, s->f32)));
   } else {
     EMD_TRY(snap_ui_y<double>(ctx, sn, n));
-    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<double, double>), grid, 64 * FIJ_WAVES, 0, sn->t,
-               sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, (const double *)sn->y_r,
-               (const double *)sn->y_i, s->f);
+    EMD_TRY((launch_fij<double, double>(ctx, sn, n, (const double *)sn->y_r,
+                                        (const double *)sn->y_i, s->f)));
   }
   return EMD_OK;
 }
