@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6     # MI355X vector FP64
-FP32_PEAK_TFLOPS = 157.3    # MI355X vector FP32 (published, packed)
+FP32_PEAK_TFLOPS = 157.3    # MI355X vector FP32 (published)
 
 
 _T0 = time.perf_counter()
@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--loop-steps", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-forcematch", action="store_true")
+    ap.add_argument("--no-idial", action="store_true",
+                    help="skip the lj/cut/idial arithmetic-intensity sweep")
     ap.add_argument("--neigh", default="2D", choices=["CSR", "2D"],
                     help="neighbour list layout of the LJ runs: Neighbor2D (the reference's "
                          "default, src/input.cpp:132) or NeighborCSR")
@@ -259,6 +261,64 @@ def lj_force_error_disordered(emd):
     return float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
 
 
+def idial_leg(emd, n=100, dials=(1, 4, 16, 64, 256, 1024), launches=10):
+    """pair_style lj/cut/idial as an arithmetic-intensity dial: the full-list
+    force kernel on the LJ 4M configuration (FCC 100^3 displaced by 0.05
+    sigma, CSR list at rc + skin = 2.8), FP64 and mixed, at each dial.  Each
+    pair inside rc = 2.5 adds its part `dial` times; the extra time per
+    extra repeat between the two largest dials gives the rate of those
+    dependent additions, compared with the published vector peak counted
+    in additions (one add = one flop, half the FMA-based TFLOP/s figure)."""
+    import numpy as np
+    from examinimd_amd.pipeline import Pipeline, lattice
+    a = (4 / 0.8442) ** (1 / 3)
+    x = lattice("fcc", n, a) + np.random.default_rng(3).normal(0, 0.05, (4 * n ** 3, 3))
+    pl = Pipeline()
+    pl.load(x, (n * a,) * 3)
+    del x
+    L, h = pl.L, pl.ctx.h
+    pl.build(2.5)
+    n_in = float(emd.download_ints(h, pl.nl.num, pl.nl.n).astype(np.int64).sum())
+    pl.build(2.8)
+    natoms = pl.s.n_local
+    p = emd.abi.EmdLJIDialParams()
+    p.lj.ntypes = 1
+    p.lj.lj1[0][0], p.lj.lj2[0][0], p.lj.cutsq[0][0] = 48.0, 24.0, 6.25
+    res = {}
+    for prec, name in ((emd.EMD_FP64, "lj_idial_fp64"), (emd.EMD_MIXED, "lj_idial_mixed")):
+        key = "fp64" if prec == emd.EMD_FP64 else "mixed"
+        res[key] = {}
+        for d in dials:
+            p.intensity[0][0] = d
+            pl.chk(L.emd_lj_idial_force_store(h, C.byref(pl.s), C.byref(pl.nl), C.byref(p), prec),
+                   "idial warm-up")
+            L.emd_profile_select(h, name.encode())
+            L.emd_profile_enable(h, 1)
+            L.emd_profile_reset(h)
+            for _ in range(launches):
+                pl.chk(L.emd_lj_idial_force_store(h, C.byref(pl.s), C.byref(pl.nl), C.byref(p),
+                                                  prec), "idial")
+            L.emd_synchronize(h)
+            cnt, ms = _profile_table(L, h).get(name, (0, 0.0))
+            L.emd_profile_enable(h, 0)
+            res[key][str(d)] = ms / max(cnt, 1)
+    pl.close()
+    peaks = {"fp64": FP64_PEAK_TFLOPS * 1e12 / 2, "mixed": FP32_PEAK_TFLOPS * 1e12 / 2}
+    out = {"workload": f"FCC {n}^3 ({natoms} atoms) displaced 0.05 sigma, CSR list, rc 2.5",
+           "atoms": natoms, "pairs_inside_rc_per_atom": n_in / natoms,
+           "kernel_avg_ms": res, "add_peak_per_s": peaks}
+    hi, lo = dials[-1], dials[-2]
+    for key in res:
+        slope = (res[key][str(hi)] - res[key][str(lo)]) * 1e-3 / (hi - lo)
+        rate = n_in / slope if slope > 0 else None
+        out[key + "_repeat_adds_per_s"] = rate
+        out[key + "_repeat_adds_frac_of_peak"] = rate / peaks[key] if rate else None
+        out[key + "_atom_steps_per_s_force_only"] = {d: natoms / (ms * 1e-3)
+                                                     for d, ms in res[key].items()}
+    out["mixed_over_fp64_force_time"] = {d: res["fp64"][d] / res["mixed"][d] for d in res["fp64"]}
+    return out
+
+
 def cpu_baseline(n, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -464,6 +524,10 @@ def main():
     gradients = None if a.no_forcematch else forcematch_leg(emd)
     if gradients:
         progress("force-matching gradients done")
+    idial = None
+    if not a.no_idial and n == 100:
+        idial = idial_leg(emd)
+        progress("lj/cut/idial intensity sweep done")
 
     line = {
         "metric": "atom-steps/s on 1 MI355X, LJ 4M & SNAP 2k atoms: mixed vs FP64 at fixed error",
@@ -496,6 +560,7 @@ def main():
         "snap2k": snap,
         "batched": batched,
         "force_matching": gradients,
+        "lj_idial_sweep": idial,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_atom * natoms,

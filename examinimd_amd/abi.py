@@ -47,6 +47,10 @@ class EmdLJParams(C.Structure):
                 ("lj2", (C.c_double * 16) * 16), ("cutsq", (C.c_double * 16) * 16)]
 
 
+class EmdLJIDialParams(C.Structure):
+    _fields_ = [("lj", EmdLJParams), ("intensity", (C.c_double * 16) * 16)]
+
+
 class EmdSnapParams(C.Structure):
     _fields_ = [("twojmax", C.c_int), ("rfac0", C.c_double), ("rmin0", C.c_double),
                 ("rcutfac", C.c_double), ("switchflag", C.c_int), ("bzeroflag", C.c_int),
@@ -98,6 +102,10 @@ HIP_PROTOS = {
     "emd_force_zero": (C.c_int, [vp, P(EmdSystem), C.c_int]),
     "emd_lj_force": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int, C.c_int]),
     "emd_lj_force_store": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int]),
+    "emd_lj_idial_force": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJIDialParams),
+                                     C.c_int, C.c_int]),
+    "emd_lj_idial_force_store": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh),
+                                           P(EmdLJIDialParams), C.c_int]),
     "emd_lj_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int,
                                 P(C.c_double)]),
     "emd_lj_set_lanes": (C.c_int, [vp, C.c_int, C.c_int]),

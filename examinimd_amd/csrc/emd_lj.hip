@@ -14,7 +14,27 @@ constexpr int LJ_MAXT = 16;
 
 struct lj_table {
   double lj1[LJ_MAXT * LJ_MAXT], lj2[LJ_MAXT * LJ_MAXT], cutsq[LJ_MAXT * LJ_MAXT];
+  double rep[LJ_MAXT * LJ_MAXT];  // lj/cut/idial repeat counts (1 for lj/cut)
 };
+
+// lj/cut/idial's repeat loop (src/force_types/force_lj_idial_neigh_impl.h:
+// 149-155): fpair = 0, then `rep` times fpair += pair_force / rep.  The part
+// is loop-invariant (an optimising compiler hoists it out of the
+// reference's loop as well); what remains are the sequential additions
+// from 0, done here in the same order, so the sum is the reference's to the
+// bit.  rep is a double in the reference's table, compared against the int
+// counter: `r < rep` holds for ceil(rep) values of r >= 0 (rep is finite and
+// at most 1e6, checked on the host), so the loop runs on an int trip count
+// (no per-pass conversion and FP64 compare) and unrolls.
+__device__ __forceinline__ int dial_trips(double rep) { return rep > 0 ? (int)ceil(rep) : 0; }
+
+template <class T>
+__device__ __forceinline__ T dial_sum(T part, int trips) {
+  T sum = 0;
+#pragma unroll 8
+  for (int r = 0; r < trips; r++) sum += part;
+  return sum;
+}
 
 template <class T>
 __global__ void k_zero3(T *__restrict__ f, long n3) {
@@ -50,13 +70,14 @@ struct lj_acc<true> {
   typedef float T;
 };
 
-template <int K, int U, bool ONE_TYPE, bool MIXED>
+template <int K, int U, bool ONE_TYPE, bool MIXED, bool DIAL>
 __global__ void __launch_bounds__(LJ_THREADS)
 k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
           const int *__restrict__ row, const int *__restrict__ num,
           const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
-          double cutsq_0, const lj_table *__restrict__ tab, const int *__restrict__ visit,
-          int n_visit, int store, double *__restrict__ f, float *__restrict__ f32) {
+          double cutsq_0, double rep_0, const lj_table *__restrict__ tab,
+          const int *__restrict__ visit, int n_visit, int store, double *__restrict__ f,
+          float *__restrict__ f32) {
 #pragma clang fp contract(off)
   typedef typename lj_acc<MIXED>::T T;
   const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
@@ -77,21 +98,24 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
     const double dy = y_i - yj;
     const double dz = z_i - zj;
     const double rsq = dx * dx + dy * dy + dz * dz;
-    double cutsq = cutsq_0, lj1 = lj1_0, lj2 = lj2_0;
+    double cutsq = cutsq_0, lj1 = lj1_0, lj2 = lj2_0, nrep = rep_0;
     if (!ONE_TYPE) {
       const int t = type_i * LJ_MAXT + type[j];
       cutsq = tab->cutsq[t]; lj1 = tab->lj1[t]; lj2 = tab->lj2[t];
+      if (DIAL) nrep = tab->rep[t];
     }
     if (ok && rsq < cutsq) {
       if (MIXED) {
-        const float fpair = emd_lj_fpair_mixed(rsq, (float)lj1, (float)lj2);
+        float fpair = emd_lj_fpair_mixed(rsq, (float)lj1, (float)lj2);
+        if (DIAL) fpair = dial_sum(fpair / (float)nrep, dial_trips(nrep));
         fxi = __builtin_fmaf((float)dx, fpair, fxi);
         fyi = __builtin_fmaf((float)dy, fpair, fyi);
         fzi = __builtin_fmaf((float)dz, fpair, fzi);
       } else {
         const double r2inv = 1.0 / rsq;
         const double r6inv = r2inv * r2inv * r2inv;
-        const double fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
+        double fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
+        if (DIAL) fpair = dial_sum(fpair / nrep, dial_trips(nrep));
         fxi += dx * fpair;
         fyi += dy * fpair;
         fzi += dz * fpair;
@@ -405,8 +429,11 @@ k_lj_tile(const double *__restrict__ x, const int *__restrict__ bincount,
   }
 }
 
-// ForceLJNeigh TagHalfNeigh, force_lj_neigh_impl.h:208-254 (atomic j updates)
-template <class T>
+// ForceLJNeigh TagHalfNeigh, force_lj_neigh_impl.h:208-254 (atomic j
+// updates).  DIAL: ForceLJIDialNeigh TagHalfNeigh,
+// force_lj_idial_neigh_impl.h:169-219, which subtracts the reaction only from
+// local j (:208-212).
+template <class T, bool DIAL>
 __global__ void __launch_bounds__(LJ_THREADS)
 k_lj_half(const double *__restrict__ x, const int *__restrict__ type, const int *__restrict__ row,
           const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
@@ -428,13 +455,16 @@ k_lj_half(const double *__restrict__ x, const int *__restrict__ type, const int 
     if (rsq < tab->cutsq[t]) {
       const T r2inv = T(1) / T(rsq);
       const T r6inv = r2inv * r2inv * r2inv;
-      const T fpair = (r6inv * (T(tab->lj1[t]) * r6inv - T(tab->lj2[t]))) * r2inv;
+      T fpair = (r6inv * (T(tab->lj1[t]) * r6inv - T(tab->lj2[t]))) * r2inv;
+      if (DIAL) fpair = dial_sum(fpair / T(tab->rep[t]), dial_trips(tab->rep[t]));
       fxi += T(dx) * fpair;
       fyi += T(dy) * fpair;
       fzi += T(dz) * fpair;
-      atomicAdd(&f[3 * (size_t)j], -T(dx) * fpair);
-      atomicAdd(&f[3 * (size_t)j + 1], -T(dy) * fpair);
-      atomicAdd(&f[3 * (size_t)j + 2], -T(dz) * fpair);
+      if (!DIAL || j < n) {
+        atomicAdd(&f[3 * (size_t)j], -T(dx) * fpair);
+        atomicAdd(&f[3 * (size_t)j + 1], -T(dy) * fpair);
+        atomicAdd(&f[3 * (size_t)j + 2], -T(dz) * fpair);
+      }
     }
   }
   atomicAdd(&f[3 * (size_t)i], fxi);
@@ -442,7 +472,8 @@ k_lj_half(const double *__restrict__ x, const int *__restrict__ type, const int 
   atomicAdd(&f[3 * (size_t)i + 2], fzi);
 }
 
-static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p, lj_table **dtab) {
+static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p,
+                               const double (*rep)[16], lj_table **dtab) {
   static_assert(sizeof(lj_table) <= 8 * 1024, "table size");
   lj_table h;
   memset(&h, 0, sizeof(h));
@@ -451,6 +482,7 @@ static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p, lj_table **
       h.lj1[a * LJ_MAXT + b] = p->lj1[a][b];
       h.lj2[a * LJ_MAXT + b] = p->lj2[a][b];
       h.cutsq[a * LJ_MAXT + b] = p->cutsq[a][b];
+      h.rep[a * LJ_MAXT + b] = rep ? rep[a][b] : 1.0;
     }
   // the table lives in the small double scratch area's neighbour: d_red head
   EMD_TRY(emd_ensure(ctx, (void **)&ctx->d_red, &ctx->red_cap, sizeof(lj_table) + (1 << 20)));
@@ -507,26 +539,35 @@ static emd_status launch_pipe(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
   return EMD_OK;
 }
 
-template <int K, int U>
-static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
-                              const emd_lj_params *p, int precision, const lj_table *tab,
-                              int store) {
+template <int K, int U, bool DIAL>
+static emd_status launch_full_as(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                 const emd_lj_params *p, double rep_0, int precision,
+                                 const lj_table *tab, int store) {
   const int n = s->n_local;
   const int nb = emd_blocks((long)(nl->visit ? nl->n_visit : n) * K, LJ_THREADS);
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   const bool one = p->ntypes == 1;
+  const char *name_m = DIAL ? "lj_idial_mixed" : "lj_force_mixed";
+  const char *name_d = DIAL ? "lj_idial_fp64" : "lj_force_fp64";
 #define LJ_FULL_ARGS                                                                        \
   s->x, s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],   \
-      p->cutsq[0][0], tab, nl->visit, nl->n_visit, store, s->f, s->f32
+      p->cutsq[0][0], rep_0, tab, nl->visit, nl->n_visit, store, s->f, s->f32
   if (precision == EMD_MIXED) {
-    if (one) EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, true, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
-    else EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_full<K, U, false, true>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
+    if (one) EMD_LAUNCH(ctx, name_m, (k_lj_full<K, U, true, true, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
+    else EMD_LAUNCH(ctx, name_m, (k_lj_full<K, U, false, true, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
   } else {
-    if (one) EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_full<K, U, true, false>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
-    else EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_full<K, U, false, false>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
+    if (one) EMD_LAUNCH(ctx, name_d, (k_lj_full<K, U, true, false, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
+    else EMD_LAUNCH(ctx, name_d, (k_lj_full<K, U, false, false, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
   }
 #undef LJ_FULL_ARGS
   return EMD_OK;
+}
+
+template <int K, int U>
+static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                              const emd_lj_params *p, int precision, const lj_table *tab,
+                              int store) {
+  return launch_full_as<K, U, false>(ctx, s, nl, p, 1.0, precision, tab, store);
 }
 
 // LDS budget of the bin-tile kernel: three FP64 coordinates per candidate
@@ -565,6 +606,51 @@ static int tile_capacity(const emd_ctx *ctx, const emd_neigh *nl, const emd_lj_p
   return t <= LJ_TILE_MAX_CAND ? (int)t : 0;
 }
 
+// lj/cut/idial: the full-list kernel with the repeat loop, for any lanes
+// per atom (the bin-tile and pipelined forms are lj/cut only)
+template <bool DIAL>
+static emd_status launch_full_lanes(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                    const emd_lj_params *p, double rep_0, int precision,
+                                    const lj_table *tab, int store, int K) {
+  switch (K) {
+    case 1: return launch_full_as<1, 4, DIAL>(ctx, s, nl, p, rep_0, precision, tab, store);
+    case 2: return launch_full_as<2, 4, DIAL>(ctx, s, nl, p, rep_0, precision, tab, store);
+    case 4: return launch_full_as<4, 4, DIAL>(ctx, s, nl, p, rep_0, precision, tab, store);
+    case 8: return launch_full_as<8, 5, DIAL>(ctx, s, nl, p, rep_0, precision, tab, store);
+    default: return launch_full_as<16, 5, DIAL>(ctx, s, nl, p, rep_0, precision, tab, store);
+  }
+}
+
+static emd_status lj_idial_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                 const emd_lj_idial_params *p, int precision, int half,
+                                 int store) {
+  if (!ctx || !s || !nl || !p) return EMD_ERR_ARG;
+  const emd_lj_params *lj = &p->lj;
+  EMD_ARG(ctx, lj->ntypes >= 1 && lj->ntypes <= LJ_MAXT, "ntypes out of range");
+  EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
+  EMD_ARG(ctx, !(store && half), "storing forces needs a full list");
+  for (int a = 0; a < lj->ntypes; a++)
+    for (int b = 0; b < lj->ntypes; b++)
+      EMD_ARG(ctx, p->intensity[a][b] <= 1e6, "lj/cut/idial intensity above 1e6");
+  if (s->n_local <= 0) return EMD_OK;
+  lj_table *tab = nullptr;
+  if (lj->ntypes > 1 || half) EMD_TRY(upload_table(ctx, lj, p->intensity, &tab));
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  if (half) {
+    const int nb = emd_blocks(s->n_local, LJ_THREADS);
+    if (precision == EMD_MIXED)
+      EMD_LAUNCH(ctx, "lj_idial_half", (k_lj_half<float, true>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f32);
+    else
+      EMD_LAUNCH(ctx, "lj_idial_half", (k_lj_half<double, true>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f);
+    return EMD_OK;
+  }
+  const int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
+  return launch_full_lanes<true>(ctx, s, nl, lj, p->intensity[0][0], precision, tab, store,
+                                 K ? K : default_lanes(precision));
+}
+
 static emd_status lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                            const emd_lj_params *p, int precision, int half, int store) {
   if (!ctx || !s || !nl || !p) return EMD_ERR_ARG;
@@ -595,16 +681,16 @@ static emd_status lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
     }
   }
   lj_table *tab = nullptr;
-  if (p->ntypes > 1 || half) EMD_TRY(upload_table(ctx, p, &tab));
+  if (p->ntypes > 1 || half) EMD_TRY(upload_table(ctx, p, nullptr, &tab));
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   if (half) {
     const int nb = emd_blocks(s->n_local, LJ_THREADS);
     if (precision == EMD_MIXED)
-      EMD_LAUNCH(ctx, "lj_force_half", k_lj_half<float>, nb, LJ_THREADS, 0, s->x, s->type, row,
-                 nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f32);
+      EMD_LAUNCH(ctx, "lj_force_half", (k_lj_half<float, false>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f32);
     else
-      EMD_LAUNCH(ctx, "lj_force_half", k_lj_half<double>, nb, LJ_THREADS, 0, s->x, s->type, row,
-                 nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f);
+      EMD_LAUNCH(ctx, "lj_force_half", (k_lj_half<double, false>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f);
     return EMD_OK;
   }
   int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
@@ -650,6 +736,18 @@ extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh 
 extern "C" emd_status emd_lj_force_store(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                                          const emd_lj_params *p, int precision) {
   return lj_force(ctx, s, nl, p, precision, 0, 1);
+}
+
+extern "C" emd_status emd_lj_idial_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                         const emd_lj_idial_params *p, int precision,
+                                         int half) {
+  return lj_idial_force(ctx, s, nl, p, precision, half, 0);
+}
+
+extern "C" emd_status emd_lj_idial_force_store(emd_ctx *ctx, emd_system *s,
+                                               const emd_neigh *nl,
+                                               const emd_lj_idial_params *p, int precision) {
+  return lj_idial_force(ctx, s, nl, p, precision, 0, 1);
 }
 
 
@@ -714,7 +812,7 @@ extern "C" emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd
   const int n = s->n_local;
   if (n <= 0) { *pe = 0.0; return EMD_OK; }
   lj_table *tab;
-  EMD_TRY(upload_table(ctx, p, &tab));
+  EMD_TRY(upload_table(ctx, p, nullptr, &tab));
   const int nb = emd_blocks(n, 256);
   EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)nb * 2 * sizeof(double)));
   double *part = (double *)ctx->d_tmp2;

@@ -14,8 +14,12 @@ import numpy as np
 
 
 def lj_deck(n=20, steps=100, seed=87287, temperature=1.44, every=20, thermo=0, newton=False,
-            gaussian=True, density=0.8442, skin=0.3, dt=0.005):
+            gaussian=True, density=0.8442, skin=0.3, dt=0.005, intensity=None):
+    """intensity: None for pair_style lj/cut, else lj/cut/idial with that
+    repeat count"""
     dist = " dist gaussian" if gaussian else ""
+    pair = ("pair_style      lj/cut 2.5\npair_coeff      1 1 1.0 1.0 2.5" if intensity is None else
+            f"pair_style      lj/cut/idial 2.5\npair_coeff      1 1 1.0 1.0 2.5 {intensity}")
     return f"""# LJ melt, FCC {n}^3 cells ({4 * n ** 3} atoms)
 units           lj
 atom_style      atomic
@@ -26,8 +30,7 @@ create_box      1 box
 create_atoms    1 box
 mass            1 1.0
 velocity        all create {temperature} {seed} loop geom{dist}
-pair_style      lj/cut 2.5
-pair_coeff      1 1 1.0 1.0 2.5
+{pair}
 neighbor        {skin} bin
 neigh_modify    delay 0 every {every} check no
 fix             1 all nve

@@ -1,4 +1,4 @@
-// force.cpp -- Force base (src/force.cpp) and the LJ module.
+// force.cpp -- Force base (src/force.cpp) and the LJ modules (lj/cut, lj/cut/idial).
 #include "force.h"
 
 #include <cmath>
@@ -59,4 +59,43 @@ T_F_FLOAT ForceLJNeighHIP::compute_energy(System* system, Binning*, Neighbor* ne
   EMD_CALL(system->ctx, emd_lj_energy(system->ctx, system->view(), neighbor->get_neigh_list(),
                                       &params, half_neigh ? 1 : 0, &pe));
   return pe;
+}
+
+// ForceLJIDialNeigh ctor, src/force_types/force_lj_idial_neigh_impl.h:41-47
+ForceLJIDialNeighHIP::ForceLJIDialNeighHIP(char** args, System* system, bool half_neigh_)
+    : Force(args, system, half_neigh_) {
+  memset(&params, 0, sizeof(params));
+  params.lj.ntypes = system->ntypes;
+  if (system->ntypes > 16) {
+    fprintf(stderr, "ForceLJIDialNeighHIP supports at most 16 types\n");
+    exit(1);
+  }
+}
+
+// ForceLJIDialNeigh::init_coeff, force_lj_idial_neigh_impl.h:49-88: only the
+// (t1,t2) and (t2,t1) entries are set
+void ForceLJIDialNeighHIP::init_coeff(int nargs, char** args) {
+  if (nargs < 7) {
+    fprintf(stderr, "pair_coeff lj/cut/idial needs: i j eps sigma cut nrepeat\n");
+    exit(1);
+  }
+  int t1 = atoi(args[1]) - 1, t2 = atoi(args[2]) - 1;
+  double eps = atof(args[3]), sigma = atof(args[4]), cut = atof(args[5]);
+  int nrepeat = atoi(args[6]);
+  params.lj.lj1[t1][t2] = params.lj.lj1[t2][t1] = 48.0 * eps * pow(sigma, 12.0);
+  params.lj.lj2[t1][t2] = params.lj.lj2[t2][t1] = 24.0 * eps * pow(sigma, 6.0);
+  params.lj.cutsq[t1][t2] = params.lj.cutsq[t2][t1] = cut * cut;
+  params.intensity[t1][t2] = params.intensity[t2][t1] = nrepeat;
+}
+
+// ForceLJIDialNeigh::compute, force_lj_idial_neigh_impl.h:89-111
+void ForceLJIDialNeighHIP::compute(System* system, Binning*, Neighbor* neighbor) {
+  if (!half_neigh)
+    EMD_CALL(system->ctx, emd_lj_idial_force_store(system->ctx, system->view(),
+                                                   neighbor->get_neigh_list(), &params,
+                                                   system->precision));
+  else
+    EMD_CALL(system->ctx, emd_lj_idial_force(system->ctx, system->view(),
+                                             neighbor->get_neigh_list(), &params,
+                                             system->precision, 1));
 }

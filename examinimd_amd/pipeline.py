@@ -6,8 +6,8 @@ import ctypes as C
 import numpy as np
 
 import examinimd_amd as emd
-from examinimd_amd.abi import (EMD_FP64, EMD_MIXED, EMD_NEIGH_CSR, EmdBins, EmdHalo, EmdLJParams,
-                               EmdNeigh, EmdSnapParams, EmdSystem)
+from examinimd_amd.abi import (EMD_FP64, EMD_MIXED, EMD_NEIGH_CSR, EmdBins, EmdHalo,
+                               EmdLJIDialParams, EmdLJParams, EmdNeigh, EmdSnapParams, EmdSystem)
 
 P = C.POINTER
 
@@ -76,6 +76,21 @@ class Pipeline:
         L, c, s = self.L, self.ctx.h, C.byref(self.s)
         self.chk(L.emd_force_zero(c, s, precision), "zero")
         self.chk(L.emd_lj_force(c, s, C.byref(self.nl), C.byref(p), precision, 0), "lj")
+        return self.forces(precision)
+
+    def lj_idial(self, intensity, precision=EMD_FP64, eps=1.0, sigma=1.0, rc=2.5, half=0):
+        """pair_style lj/cut/idial forces of the local atoms (input order) on
+        the list of the last build()"""
+        p = EmdLJIDialParams()
+        p.lj.ntypes = 1
+        p.lj.lj1[0][0] = 48.0 * eps * sigma ** 12
+        p.lj.lj2[0][0] = 24.0 * eps * sigma ** 6
+        p.lj.cutsq[0][0] = rc * rc
+        p.intensity[0][0] = intensity
+        L, c, s = self.L, self.ctx.h, C.byref(self.s)
+        self.chk(L.emd_force_zero(c, s, precision), "zero")
+        self.chk(L.emd_lj_idial_force(c, s, C.byref(self.nl), C.byref(p), precision, half),
+                 "lj_idial")
         return self.forces(precision)
 
     def snap_setup(self, beta, twojmax=8, rcutfac=4.73442, rfac0=0.99363, rmin0=0.0,

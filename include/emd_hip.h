@@ -202,6 +202,29 @@ emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
  * Same values as emd_force_zero + emd_lj_force(half = 0). */
 emd_status emd_lj_force_store(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                               const emd_lj_params *p, int precision);
+/* pair_style lj/cut/idial (ForceLJIDialNeigh,
+ * src/force_types/force_lj_idial_neigh{.h,_impl.h}): lj/cut's parameters
+ * plus a per-type-pair repeat count.  init_coeff (force_lj_idial_neigh_impl.h:
+ * 50-88) sets entries (t1,t2) and (t2,t1) of lj1 = 48 eps sigma^12,
+ * lj2 = 24 eps sigma^6, cutsq = cut^2 and intensity = the 6th pair_coeff word. */
+typedef struct {
+  emd_lj_params lj;
+  double intensity[16][16];
+} emd_lj_idial_params;
+/* ForceLJIDialNeigh::compute, force_lj_idial_neigh_impl.h:90-111: the lj/cut
+ * pair set (rsq < cutsq); each pair's force is summed from 0 in `intensity`
+ * parts of F/intensity, in the reference's order, so the FP64 result is the
+ * reference's to the bit (one lane per atom) and equals lj/cut's for
+ * intensity 1 or 2.  The half-list form (:169-219) subtracts the reaction
+ * from local j only.  There is no energy (Force::compute_energy's 0).
+ * Intensities above 1e6 are rejected (EMD_ERR_ARG): one launch would run
+ * for seconds at millions of atoms. */
+emd_status emd_lj_idial_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                              const emd_lj_idial_params *p, int precision, int half);
+/* the same on a full list with f written instead of added (see
+ * emd_lj_force_store): equal to emd_force_zero + emd_lj_idial_force(half = 0) */
+emd_status emd_lj_idial_force_store(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                    const emd_lj_idial_params *p, int precision);
 /* ForceLJNeigh::compute_energy, force_lj_neigh_impl.h:128-156 (compensated) */
 emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
                          const emd_lj_params *p, int half, double *pe);

@@ -643,6 +643,57 @@ void or_lj_force(or_system *s, const or_csr *c, double eps, double sigma, double
   }
 }
 
+/*
+ * ForceLJIDialNeigh, src/force_types/force_lj_idial_neigh_impl.h:41-219.
+ * Same pair set as lj/cut (strict rsq < cutsq, :143/:192), but the pair
+ * force is accumulated `intensity` times from 0 in parts of
+ * ((r6inv*(lj1*r6inv - lj2))*r2inv)/intensity (:150-155, :199-204); the
+ * half-list form subtracts the reaction only from local j (:208-212), ghost
+ * forces are left alone.  intensity is a double in the reference's table
+ * (t_fparams, :46) compared against the int loop counter.
+ */
+void or_lj_idial_force(or_system *s, const or_csr *c, double eps, double sigma, double cut,
+                       double intensity, int half) {
+  const double lj1 = 48.0 * eps * pow(sigma, 12.0);
+  const double lj2 = 24.0 * eps * pow(sigma, 6.0);
+  const double cutsq = cut * cut;
+  const int N = s->N_local;
+  /* rows are independent for a full list; the half list's j updates are
+   * kept serial */
+#pragma omp parallel for schedule(static) if (!half)
+  for (int i = 0; i < N; i++) {
+    const double x_i = s->x[3 * (size_t)i], y_i = s->x[3 * (size_t)i + 1],
+                 z_i = s->x[3 * (size_t)i + 2];
+    double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+    for (int jj = c->row[i]; jj < c->row[i + 1]; jj++) {
+      int j = c->entries[jj];
+      const double dx = x_i - s->x[3 * (size_t)j];
+      const double dy = y_i - s->x[3 * (size_t)j + 1];
+      const double dz = z_i - s->x[3 * (size_t)j + 2];
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      if (rsq < cutsq) {
+        double fpair = 0;
+        for (int repeat = 0; repeat < intensity; repeat++) {
+          double r2inv = 1.0 / rsq;
+          double r6inv = r2inv * r2inv * r2inv;
+          fpair += (r6inv * (lj1 * r6inv - lj2)) * r2inv / intensity;
+        }
+        fxi += dx * fpair;
+        fyi += dy * fpair;
+        fzi += dz * fpair;
+        if (half && j < N) {
+          s->f[3 * (size_t)j] -= dx * fpair;
+          s->f[3 * (size_t)j + 1] -= dy * fpair;
+          s->f[3 * (size_t)j + 2] -= dz * fpair;
+        }
+      }
+    }
+    s->f[3 * (size_t)i] += fxi;
+    s->f[3 * (size_t)i + 1] += fyi;
+    s->f[3 * (size_t)i + 2] += fzi;
+  }
+}
+
 /* Neumaier compensated accumulation */
 static void neumaier(double *sum, double *c, double v) {
   double t = *sum + v;
@@ -821,6 +872,9 @@ static void md_force(or_md *m) {
   memset(s->f, 0, sizeof(double) * 3 * (size_t)s->N_max);  /* examinimd.cpp:232 */
   if (m->cfg.force_type == OR_FORCE_LJ)
     or_lj_force(s, &m->c, m->cfg.lj_eps, m->cfg.lj_sigma, m->cfg.lj_cut, m->cfg.half_neigh);
+  else if (m->cfg.force_type == OR_FORCE_LJ_IDIAL)
+    or_lj_idial_force(s, &m->c, m->cfg.lj_eps, m->cfg.lj_sigma, m->cfg.lj_cut,
+                      m->cfg.lj_intensity, m->cfg.half_neigh);
   else
     or_snap_force(s, &m->c, &m->snap);
 }
@@ -860,6 +914,8 @@ int or_md_init(or_md *m) {
 }
 
 double or_md_potential(or_md *m) {
+  /* lj/cut/idial keeps Force::compute_energy's default 0 (src/force.h:54) */
+  if (m->cfg.force_type == OR_FORCE_LJ_IDIAL) return 0.0;
   if (m->cfg.force_type == OR_FORCE_LJ)
     return or_lj_energy(&m->s, &m->c, m->cfg.lj_eps, m->cfg.lj_sigma, m->cfg.lj_cut,
                         m->cfg.half_neigh);

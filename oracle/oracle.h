@@ -83,7 +83,7 @@ typedef struct {
 /* ---------------------------------------------------------------- config */
 enum { OR_UNITS_REAL = 0, OR_UNITS_LJ = 1, OR_UNITS_METAL = 2 };
 enum { OR_LATTICE_SC = 0, OR_LATTICE_FCC = 1, OR_LATTICE_BCC = 2 };
-enum { OR_FORCE_LJ = 0, OR_FORCE_SNAP = 1 };
+enum { OR_FORCE_LJ = 0, OR_FORCE_SNAP = 1, OR_FORCE_LJ_IDIAL = 2 };
 enum { OR_NEIGH_CSR = 0, OR_NEIGH_2D = 1 };
 
 typedef struct {
@@ -108,6 +108,7 @@ typedef struct {
   double dt;               /* <= 0: units default */
   int thermo_rate;
   int nthreads;            /* OpenMP threads for the parallel loops (0: default) */
+  double lj_intensity;     /* lj/cut/idial repeat count (pair_coeff's 6th word) */
 } or_config;
 
 /* ---------------------------------------------------------- RNG/lattice */
@@ -152,6 +153,10 @@ void   or_lj_force(or_system *s, const or_csr *c, double eps, double sigma,
                    double cut, int half);
 double or_lj_energy(const or_system *s, const or_csr *c, double eps,
                     double sigma, double cut, int half);
+/* ForceLJIDialNeigh (pair_style lj/cut/idial): the LJ pair force summed
+ * `intensity` times in 1/intensity parts; no energy (PE = 0) */
+void   or_lj_idial_force(or_system *s, const or_csr *c, double eps, double sigma,
+                         double cut, double intensity, int half);
 
 /* -------------------------------------------------------------- NVE */
 void or_nve_initial(or_system *s);

@@ -65,7 +65,8 @@ class Config(C.Structure):
                 ("lj_sigma", C.c_double), ("lj_cut", C.c_double),
                 ("skin", C.c_double), ("exchange_rate", C.c_int),
                 ("neigh_type", C.c_int), ("half_neigh", C.c_int), ("newton", C.c_int),
-                ("dt", C.c_double), ("thermo_rate", C.c_int), ("nthreads", C.c_int)]
+                ("dt", C.c_double), ("thermo_rate", C.c_int), ("nthreads", C.c_int),
+                ("lj_intensity", C.c_double)]
 
 
 UNITS = {"real": 0, "lj": 1, "metal": 2}
@@ -112,6 +113,8 @@ def lib():
                                    C.c_int]),
             "or_lj_energy": (C.c_double, [P(System), P(CSR), C.c_double, C.c_double,
                                           C.c_double, C.c_int]),
+            "or_lj_idial_force": (None, [P(System), P(CSR), C.c_double, C.c_double,
+                                         C.c_double, C.c_double, C.c_int]),
             "or_nve_initial": (None, [P(System)]),
             "or_nve_final": (None, [P(System)]),
             "or_snap_ncoeff": (C.c_int, [C.c_int]),
@@ -181,7 +184,9 @@ def csr_arrays(c):
 def make_config(units="lj", lattice="fcc", lattice_arg=0.8442, n=(20, 20, 20), mass=1.0,
                 temperature=1.44, seed=87287, gaussian=False, force="lj", cutoff=2.5,
                 eps=1.0, sigma=1.0, skin=0.3, every=20, half=False, newton=False, dt=0.0,
-                nthreads=0):
+                nthreads=0, intensity=None):
+    """force: "lj", "snap", or "lj/cut/idial" (with intensity = pair_coeff's
+    repeat count)"""
     cfg = Config()
     cfg.units = UNITS[units]
     cfg.lattice_style = LATTICE[lattice]
@@ -192,8 +197,9 @@ def make_config(units="lj", lattice="fcc", lattice_arg=0.8442, n=(20, 20, 20), m
     cfg.temperature = temperature
     cfg.seed = seed
     cfg.vel_gaussian = 1 if gaussian else 0
-    cfg.force_type = 0 if force == "lj" else 1
-    cfg.force_cutoff = cutoff if force == "lj" else 4.73442
+    cfg.force_type = {"lj": 0, "snap": 1, "lj/cut/idial": 2}[force]
+    cfg.force_cutoff = 4.73442 if force == "snap" else cutoff
+    cfg.lj_intensity = 1.0 if intensity is None else float(intensity)
     cfg.lj_eps, cfg.lj_sigma, cfg.lj_cut = eps, sigma, cutoff
     cfg.skin = skin
     cfg.exchange_rate = every

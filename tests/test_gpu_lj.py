@@ -23,8 +23,11 @@ N_CELLS = 20      # 32 000 atoms: the CPU-runnable LJ configuration
 STEPS = 100
 
 
-def oracle_run(oracle, steps, n=N_CELLS, half=False, newton=False, gaussian=True):
-    cfg = oracle.make_config(n=(n, n, n), gaussian=gaussian, half=half, newton=newton)
+def oracle_run(oracle, steps, n=N_CELLS, half=False, newton=False, gaussian=True,
+               intensity=None):
+    cfg = oracle.make_config(n=(n, n, n), gaussian=gaussian, half=half, newton=newton,
+                             force="lj" if intensity is None else "lj/cut/idial",
+                             intensity=intensity)
     m = oracle.MD(cfg)
     m.init()
     if steps:
@@ -33,9 +36,10 @@ def oracle_run(oracle, steps, n=N_CELLS, half=False, newton=False, gaussian=True
 
 
 def gpu_app(tmp_path, steps, n=N_CELLS, neigh="CSR", precision="fp64", half=False,
-            newton=False, gaussian=True, lanes=None):
+            newton=False, gaussian=True, lanes=None, intensity=None):
     deck = decks.write(str(tmp_path / "in.lj"),
-                       decks.lj_deck(n=n, steps=steps, newton=newton, gaussian=gaussian))
+                       decks.lj_deck(n=n, steps=steps, newton=newton, gaussian=gaussian,
+                                     intensity=intensity))
     if lanes is not None:
         os.environ["EMD_LJ_LANES"] = str(lanes)
     try:
@@ -508,3 +512,65 @@ def test_restart_from_binary_dump_continues_bitwise(tmp_path, precision):
     c.close()
     for k in ("id", "x", "v", "f"):
         assert np.array_equal(got[k], want[k]), k
+
+
+# ------------------------------------------------------- pair_style lj/cut/idial
+@pytest.mark.parametrize("lanes,intensity", [(1, 3), (1, 16), (16, 3), (16, 16)])
+def test_lj_idial_32k_100_steps(tmp_path, oracle, lanes, intensity):
+    """lj/cut/idial through the driver against the oracle's restatement of
+    ForceLJIDialNeigh: one lane per atom repeats the reference's operations
+    in its order (bit-identical trajectory), 16 lanes change only each
+    atom's summation order; no energy term (PE = 0)"""
+    print(f"case: LJ idial 32k, dial {intensity}, {lanes} lanes", flush=True)
+    app = gpu_app(tmp_path, STEPS, lanes=lanes, intensity=intensity)
+    m = oracle_run(oracle, 0, intensity=intensity)
+    if lanes == 1:
+        compare(app, m, 0.0, exact=True)
+    else:
+        # the perfect lattice's forces are rounding noise (~1e-14) whose
+        # value depends on the summation order: absolute bound
+        assert np.abs(app.arrays()["f"] - m.arrays()["f"][:app.nlocal]).max() < 1e-12
+    app.run(STEPS)
+    m.run(STEPS)
+    compare(app, m, 1e-12, exact=lanes == 1)
+    assert app.thermo()[1] == 0.0 and m.thermo()[1] == 0.0
+
+
+@pytest.mark.parametrize("newton", [False, True])
+def test_lj_idial_half_list(tmp_path, oracle, newton):
+    """half list: the reaction goes to local j only (ghost rows stay zero,
+    so with newton on the folded ghost forces are zero as in the
+    reference); atomic updates make the order, not the values, vary"""
+    print(f"case: LJ idial 4000 atoms half list newton={newton}", flush=True)
+    app = gpu_app(tmp_path, 40, n=10, half=True, newton=newton, intensity=4)
+    m = oracle_run(oracle, 0, n=10, half=True, newton=newton, intensity=4)
+    assert np.array_equal(app.arrays()["x"], m.arrays()["x"][:app.nlocal])
+    assert np.abs(app.arrays()["f"] - m.arrays()["f"][:app.nlocal]).max() < 1e-12
+    app.run(40)
+    m.run(40)
+    compare(app, m, 1e-11)
+
+
+def test_lj_idial_kernel_level_mixed(oracle):
+    """displaced FCC (0.05 sigma), dial 8: FP64 forces against the oracle,
+    FP32 pair arithmetic and FP32 repeat sums against FP64 (RMS 1e-5)"""
+    from examinimd_amd.pipeline import Pipeline, lattice
+    from test_oracle import _pair_forces
+    print("case: LJ idial 32k disordered, kernel level, dial 8", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    x = lattice("fcc", 20, a) + np.random.default_rng(9).normal(0, 0.05, (32000, 3))
+    box = 20 * a
+    pl = Pipeline()
+    pl.load(x, (box,) * 3)
+    pl.build(2.8)
+    f64 = pl.lj_idial(8, emd.EMD_FP64)
+    f32 = pl.lj_idial(8, emd.EMD_MIXED)
+    lj = pl.lj(emd.EMD_FP64)
+    pl.close()
+    F = _pair_forces(oracle, x % box, box, "idial", 8)[0]
+    assert rel_err(f64, F) <= 1e-13
+    assert rel_err(f64, lj) <= 1e-13 and not np.array_equal(f64, lj)
+    df = f32 - f64
+    rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+    print("mixed idial force RMS error", rms)
+    assert rms <= 1e-5

@@ -333,6 +333,99 @@ def test_lj_parameter_gradient_closed_form(oracle):
         assert abs(fd - exact[k]) <= 1e-7 * abs(exact).max(), (k, fd, exact[k])
 
 
+# ------------------------------------------------------------ LJ idial
+def _pair_forces(oracle, x, box, kind, intensity=1.0, half=0, newton=0, cut=2.8):
+    """oracle lj/cut (kind "lj") or lj/cut/idial forces of a configuration:
+    returns (local forces in input order, ghost forces, sum over local+ghost)"""
+    L = oracle.lib()
+    s = oracle.System()
+    L.or_system_init(C.byref(s))
+    n = len(x)
+    L.or_system_grow(C.byref(s), n)
+    s.N_local = s.N = n
+    s.mass[0] = 1.0
+    for d in range(3):
+        s.domain[d] = s.hi[d] = box
+    np.ctypeslib.as_array(s.x, shape=(3 * n,))[:] = x.ravel()
+    np.ctypeslib.as_array(s.id, shape=(n,))[:] = np.arange(1, n + 1)
+    L.or_comm_exchange(C.byref(s))
+    h, b, c = _build(oracle, s, cut, half=half, newton=newton)
+    np.ctypeslib.as_array(s.f, shape=(3 * s.N_max,))[:] = 0
+    if kind == "lj":
+        L.or_lj_force(C.byref(s), C.byref(c), 1.0, 1.0, 2.5, half)
+    else:
+        L.or_lj_idial_force(C.byref(s), C.byref(c), 1.0, 1.0, 2.5, intensity, half)
+    N, G = s.N_local, s.N_ghost
+    f = oracle.arr(s.f, 3 * (N + G)).reshape(-1, 3).copy()
+    F = np.zeros((n, 3))
+    F[oracle.arr(s.id, n, np.int32) - 1] = f[:N]
+    L.or_system_free(C.byref(s))
+    L.or_halo_free(C.byref(h))
+    L.or_bins_free(C.byref(b))
+    L.or_csr_free(C.byref(c))
+    return F, f[N:], f.sum(0)
+
+
+def _melt(seed=11, n=5, a=1.5, jitter=0.08):
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) * a
+    return g + rng.normal(0, jitter, g.shape), n * a
+
+
+def test_lj_idial_reduces_to_lj_cut(oracle):
+    """dial 1: F/1 summed once; dial 2: F/2 + F/2 (halving and doubling are
+    exact in binary floating point): both equal lj/cut to the bit; dial 0
+    runs no repeat and gives zero force"""
+    x, box = _melt()
+    ref = _pair_forces(oracle, x, box, "lj")[0]
+    for k in (1, 2):
+        assert np.array_equal(_pair_forces(oracle, x, box, "idial", k)[0], ref), k
+    assert not np.any(_pair_forces(oracle, x, box, "idial", 0)[0])
+
+
+@pytest.mark.parametrize("k", [3, 5, 7, 64])
+def test_lj_idial_sum_rounding_bound(oracle, k):
+    """k parts of F/k sum to F within k roundings of |F| (a missing or doubled
+    division by k would be off by a factor k); the dial leaves the physics
+    unchanged: total force zero"""
+    x, box = _melt()
+    ref = _pair_forces(oracle, x, box, "lj")[0]
+    got, _, tot = _pair_forces(oracle, x, box, "idial", k)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max()
+    assert err <= 4 * k * np.finfo(float).eps * scale * 30, (err, scale)
+    assert err > 0 or k in (1, 2)
+    assert np.abs(got.sum(0)).max() < 1e-9 * scale
+
+
+def test_lj_idial_strict_cutoff(oracle):
+    """pair at exactly r = rc (r^2 = 6.25 exactly) is outside the strict
+    rsq < cutsq test; a pair just inside contributes"""
+    for r, inside in ((2.5, False), (2.4990234375, True)):
+        x = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]])
+        F = _pair_forces(oracle, x, 20.0, "idial", 3)[0]
+        assert (np.abs(F).max() > 0) == inside, r
+
+
+def test_lj_idial_half_list_ghost_reactions(oracle):
+    """half list: lj/cut subtracts every reaction, ghosts included, so with
+    newton on the ghost rows carry force and local+ghost totals vanish;
+    lj/cut/idial subtracts only local j (force_lj_idial_neigh_impl.h:208),
+    its ghost rows stay zero and, with dial 1, its local rows equal lj/cut's
+    (same operations in the same order for every local j)"""
+    x, box = _melt(n=6, a=1.4)
+    Fl, gl, tl = _pair_forces(oracle, x, box, "lj", half=1, newton=1)
+    Fi, gi, ti = _pair_forces(oracle, x, box, "idial", 1, half=1, newton=1)
+    assert np.abs(gl).max() > 0 and not np.any(gi)
+    assert np.abs(tl).max() < 1e-9
+    assert np.array_equal(Fi, Fl)
+    # newton off: the half list keeps each ghost pair from the local side only
+    # where needed, and the local forces of both styles agree with the full list
+    full = _pair_forces(oracle, x, box, "lj")[0]
+    Fo = _pair_forces(oracle, x, box, "idial", 1, half=1, newton=0)[0]
+    assert np.allclose(Fo, full, rtol=0, atol=1e-12 * np.abs(full).max())
+
+
 # ------------------------------------------------------------------- NVE
 def test_nve_reversibility_and_energy(oracle):
     m = oracle.MD(oracle.make_config(n=(8, 8, 8), every=10))
