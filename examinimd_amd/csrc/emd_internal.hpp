@@ -63,12 +63,12 @@ struct emd_ctx {
 
   // tuning
   int lj_lanes[2] = {0, 0};
-  int fold_one = 1;
-  int halo_one = 1;
-  int bin_one = 1;
-  int snap_yflat = 1;
-  int snap_ui_grp = 1;
-  int snap_fij_lanes = 16; // lanes per neighbour in the SNAP force kernel (16 or 32)    // SNAP U_tot with 16-lane neighbour groups (twojmax 8)     // SNAP Y from the flat term stream (0: per-entry kernel)        // small ranges: binning in one launch       // small systems: the six exchange_halo phases in one launch       // small halos: fold all six phases in one launch
+  int fold_one = 1;        // small halos: fold all six phases in one launch
+  int halo_one = 1;        // small systems: the six exchange_halo phases in one launch
+  int bin_one = 1;         // small ranges: binning in one launch
+  int snap_yflat = 1;      // SNAP Y from the flat term stream (0: per-entry kernel)
+  int snap_ui_grp = 1;     // SNAP U_tot with 16-lane neighbour groups (twojmax 8)
+  int snap_fij_lanes = 16; // lanes per neighbour in the SNAP force kernel (16 or 32)
   int lj_tile = 0;        // use the bin-tile LJ kernel when the list carries slots
   int lj_tile_lanes = 8;  // lanes per atom of the bin-tile kernel
   int lj_unroll = 5;      // entries per lane per round, 16-lane CSR kernel

@@ -53,9 +53,10 @@ class Pipeline:
             s.lo[d] = 0.0
             s.hi[d] = box[d]
 
-    def build(self, cut, half=0, newton=0, tile=False):
+    def build(self, cut, half=0, newton=0, tile=False, layout=EMD_NEIGH_CSR):
         """tile: switch the context's bin-tile LJ kernel on and build the slot
-        view (full lists) that it reads"""
+        view (full lists) that it reads; layout: EMD_NEIGH_CSR or
+        EMD_NEIGH_2D"""
         L, c, s = self.L, self.ctx.h, C.byref(self.s)
         self.nl.want_slot = 1 if tile else 0
         if tile:
@@ -64,7 +65,7 @@ class Pipeline:
         self.chk(L.emd_binning(c, s, cut, cut, cut, 1, 1, 0, 1, C.byref(self.b)), "bin sort")
         self.chk(L.emd_comm_exchange_halo(c, s, cut, C.byref(self.h)), "halo")
         self.chk(L.emd_binning(c, s, cut, cut, cut, 1, 1, 1, 0, C.byref(self.b)), "bin")
-        self.chk(L.emd_neigh_build(c, s, C.byref(self.b), cut, half, newton, EMD_NEIGH_CSR,
+        self.chk(L.emd_neigh_build(c, s, C.byref(self.b), cut, half, newton, layout,
                                    C.byref(self.nl)), "neigh")
 
     def lj(self, precision=EMD_FP64, eps=1.0, sigma=1.0, rc=2.5):

@@ -574,3 +574,33 @@ def test_lj_idial_kernel_level_mixed(oracle):
     rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
     print("mixed idial force RMS error", rms)
     assert rms <= 1e-5
+
+
+@pytest.mark.parametrize("layout", [emd.EMD_NEIGH_CSR, emd.EMD_NEIGH_2D])
+@pytest.mark.parametrize("jitter", [0.0, 1e-9])
+def test_neighbour_list_pairs_at_the_cutoff(oracle, layout, jitter):
+    """SC lattice with spacing cut/2: the second neighbours along each axis
+    sit at the cutoff to within rounding, where the inclusive FP64 test
+    (rsq <= cutsq, neighbor_csr.h:206) decides pair by pair; rows and
+    entries equal the oracle's"""
+    from examinimd_amd.pipeline import Pipeline, lattice
+    from test_oracle import oracle_csr
+    print(f"case: neighbour list at the cutoff, layout={layout} jitter={jitter}", flush=True)
+    n, a, cut = 12, 1.4, 2.8
+    x = lattice("sc", n, a)
+    if jitter:
+        x = x + np.random.default_rng(2).normal(0, jitter, x.shape)
+    box = n * a
+    orow, oent, oghost = oracle_csr(oracle, x % box, box, cut)
+    pl = Pipeline()
+    pl.load(x, (box,) * 3)
+    pl.build(cut, layout=layout)
+    row, num, ent = emd.download_neigh(pl.ctx.h, pl.nl)
+    ghosts = pl.s.n_ghost
+    pl.close()
+    assert ghosts == oghost
+    assert np.array_equal(np.asarray(row, np.int64), orow.astype(np.int64))
+    assert np.array_equal(ent, oent)
+    # 26 neighbours are inside; of the 6 axis pairs at 2a some land on each
+    # side of the FP64 test (30 per atom unjittered, 29.05 jittered)
+    assert 26 * len(x) < len(oent) < 32 * len(x)

@@ -366,6 +366,31 @@ def _pair_forces(oracle, x, box, kind, intensity=1.0, half=0, newton=0, cut=2.8)
     return F, f[N:], f.sum(0)
 
 
+def oracle_csr(oracle, x, box, cut, half=0, newton=0):
+    """the oracle's neighbour list of a configuration after its wrap, bin
+    sort and halo: (row offsets, entries, ghost count)"""
+    L = oracle.lib()
+    s = oracle.System()
+    L.or_system_init(C.byref(s))
+    n = len(x)
+    L.or_system_grow(C.byref(s), n)
+    s.N_local = s.N = n
+    s.mass[0] = 1.0
+    for d in range(3):
+        s.domain[d] = s.hi[d] = box
+    np.ctypeslib.as_array(s.x, shape=(3 * n,))[:] = x.ravel()
+    np.ctypeslib.as_array(s.id, shape=(n,))[:] = np.arange(1, n + 1)
+    L.or_comm_exchange(C.byref(s))
+    h, b, c = _build(oracle, s, cut, half=half, newton=newton)
+    row, ent = oracle.csr_arrays(c)
+    out = (row.copy(), ent.copy(), s.N_ghost)
+    L.or_system_free(C.byref(s))
+    L.or_halo_free(C.byref(h))
+    L.or_bins_free(C.byref(b))
+    L.or_csr_free(C.byref(c))
+    return out
+
+
 def _melt(seed=11, n=5, a=1.5, jitter=0.08):
     rng = np.random.default_rng(seed)
     g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) * a
