@@ -18,6 +18,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
+#include <type_traits>
+
 __global__ void k_max_int(const int *__restrict__ a, int n, int *__restrict__ out);
 
 // ------------------------------------------------------------------ binning
@@ -368,7 +370,6 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
   const int T = s_pre[27];
   const int cb = (bx * g.nby + by) * g.nbz + bz;
   const int c_off = binoffsets[cb], c_cnt = bincount[cb];
-  const unsigned long long lt = emd_lanemask_lt();
 
   for (int cs = 0; cs < T || (cs == 0 && T == 0); cs += NB_MAXC) {
     const int cn = (T - cs) < NB_MAXC ? (T - cs) : NB_MAXC;
@@ -408,51 +409,63 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
         z_i = x[3 * (size_t)i + 2];
       }
       if (i >= n_local) continue;
+      // the row is the wavefront's: its index and everything derived from it
+      // (row start, counters) are scalars
+      i = __builtin_amdgcn_readfirstlane(i);
       int run = 0;
       if (cs > 0) run = FILL ? cursor[i] : num[i];
       size_t base = 0;
       if (FILL) base = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+      int *const rowe = entries + base;
+      unsigned short *const rows = slot ? slot + base : nullptr;
       // candidate test at staged position c (the reference's predicate,
-      // neighbor_csr.h:206 full / 290 half), branch-free: positions past the
-      // staged count cn read stale LDS (the arrays hold NB_MAXC entries, a
-      // multiple of the 128 positions of one iteration) and are masked
-      auto test = [&](int c, int j, double x_j, double y_j, double z_j) -> bool {
+      // neighbor_csr.h:206 full / 290 half), branch-free.  Only the last
+      // iteration of a chunk (TAIL) can reach positions past the staged
+      // count cn; they read stale LDS (the arrays hold NB_MAXC entries, a
+      // multiple of the 128 positions of one iteration) and are masked.
+      auto test = [&](auto tail, int c, int j, double x_j, double y_j, double z_j) -> bool {
         const double dx = x_i - x_j, dy = y_i - y_j, dz = z_i - z_j;
         const double rsq = dx * dx + dy * dy + dz * dz;
+        const bool staged = !decltype(tail)::value || c < cn;
         if (HALF) {
           const bool ahead =
               (x_j > x_i) | ((x_j == x_i) & ((y_j > y_i) | ((y_j == y_i) & (z_j > z_i))));
           const bool drop = ((j == i) | (j < n_local) | (newton != 0)) & !ahead;
-          return (c < cn) & !drop & (rsq <= cutsq);
+          return staged & !drop & (rsq <= cutsq);
         }
-        return (c < cn) & (rsq <= cutsq) & (i != j);
+        return staged & (rsq <= cutsq) & (i != j);
       };
+      // kept candidates go to the row in candidate order: the lane's slot is
+      // `at` plus the kept lanes below it (mbcnt)
       auto put = [&](bool keep, unsigned long long m, int at, int c, int j) {
         if (FILL && keep) {
-          const int pos = at + __popcll(m & lt);
-          if (pos < cap) {
-            const size_t e = base + pos;
-            entries[e] = j;
-            if (slot) slot[e] = (unsigned short)(cs + c);
+          const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+              (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)at));
+          if ((int)pos < cap) {
+            rowe[pos] = j;
+            if (rows) rows[pos] = (unsigned short)(cs + c);
           }
         }
       };
       // two 64-candidate groups per iteration: both groups' LDS reads are
       // issued before either distance test; entries keep candidate order
-      for (int c0 = 0; c0 < cn; c0 += 2 * EMD_WAVE) {
+      auto iteration = [&](auto tail, int c0) {
         const int ca = c0 + lane, cb = c0 + EMD_WAVE + lane;
         const int ja = s_j[ca], jb = s_j[cb];
         const double xa = s_x[ca], ya = s_y[ca], za = s_z[ca];
         const double xb = s_x[cb], yb = s_y[cb], zb = s_z[cb];
-        const bool ka = test(ca, ja, xa, ya, za);
-        const bool kb = test(cb, jb, xb, yb, zb);
-        const unsigned long long ma = __ballot(ka);
-        const unsigned long long mb = __ballot(kb);
+        const bool ka = test(tail, ca, ja, xa, ya, za);
+        const bool kb = test(tail, cb, jb, xb, yb, zb);
+        const unsigned long long ma = __builtin_amdgcn_ballot_w64(ka);
+        const unsigned long long mb = __builtin_amdgcn_ballot_w64(kb);
         put(ka, ma, run, ca, ja);
         const int runb = run + __popcll(ma);
         put(kb, mb, runb, cb, jb);
         run = runb + __popcll(mb);
-      }
+      };
+      int c0 = 0;
+      for (; c0 + 2 * EMD_WAVE <= cn; c0 += 2 * EMD_WAVE) iteration(std::false_type{}, c0);
+      if (c0 < cn) iteration(std::true_type{}, c0);
       if (lane == 0) {
         if (FILL) {
           if (!last) cursor[i] = run;

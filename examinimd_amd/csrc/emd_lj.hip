@@ -25,14 +25,18 @@ struct lj_table {
 // bit.  rep is a double in the reference's table, compared against the int
 // counter: `r < rep` holds for ceil(rep) values of r >= 0 (rep is finite and
 // at most 1e6, checked on the host), so the loop runs on an int trip count
-// (no per-pass conversion and FP64 compare) and unrolls.
+// (no per-pass conversion and FP64 compare), eight additions per pass.
 __device__ __forceinline__ int dial_trips(double rep) { return rep > 0 ? (int)ceil(rep) : 0; }
 
 template <class T>
 __device__ __forceinline__ T dial_sum(T part, int trips) {
   T sum = 0;
-#pragma unroll 8
-  for (int r = 0; r < trips; r++) sum += part;
+  int r = 0;
+  for (; r + 8 <= trips; r += 8) {  // the same additions in the same order, 8 per pass
+#pragma unroll
+    for (int k = 0; k < 8; k++) sum += part;
+  }
+  for (; r < trips; r++) sum += part;
   return sum;
 }
 
