@@ -567,68 +567,82 @@ k_snap_y(snap_dev_tables t, const double *__restrict__ cg, const ytab_entry *__r
 //   w * U_a(ma1, mb1) * U_b(ma2, mb2),  w = coef(a,b,J) cg(ma1,ma2) cg(mb1,mb2)
 // (the double sum of SNA::compute_zi, sna_impl.hpp:221-274, weighted as in
 // compute_dbidrj's three roles).  The host lists these products (zero
-// Clebsch-Gordan weights dropped), cuts each entry's list into chunks of at
-// most YF_CHUNK terms, deals the chunks to the block's threads by longest
-// first onto the least loaded, and interleaves the threads' terms so that
-// step s of every thread is one coalesced load.  A term's code carries its
-// two U offsets, its chunk and a last-of-chunk bit; a thread stores a
-// chunk's sum in LDS when it ends, and each entry adds its chunks in order
-// (fixed order: deterministic).
+// Clebsch-Gordan weights dropped) entry by entry, pads each entry's list with
+// zero-weight terms to whole chunks of YF_CHUNK, and gives thread t the
+// chunks [t P, (t+1) P) of that sequence, interleaved so that step s of every
+// thread is one coalesced load.  Chunk boundaries are the same for every
+// lane, so the term loop carries no per-term branch: a term is two LDS byte
+// offsets (its code), a weight and six multiply-adds.  Once per chunk a
+// thread whose run of one entry's chunks ends stores the run's sum in LDS;
+// each entry adds its runs in thread order (fixed order: deterministic).
+// The two random LDS reads per term bound the kernel (not load latency: a
+// double-buffered form with 16-byte loads measured the same).
 constexpr int YF_THREADS = 256;
-constexpr int YF_CHUNK = 48;
-constexpr int YF_UNROLL = 4;
+constexpr int YF_CHUNK = 16;
 
 template <class T>
 struct alignas(2 * sizeof(T)) cpair {
   T r, i;
 };
 
+// code: byte offsets of the two U_tot entries in LDS, 13 bits each (the
+// host writes offsets for sizeof(cpair<T>)); meta[c][t]: the LDS slot of the
+// run that ends with thread t's chunk c, or YF_NO_RUN.  A thread's runs are
+// the entries its chunk range touches, so there are at most ny + threads - 1.
+constexpr unsigned short YF_NO_RUN = 0xFFFFu;
+constexpr int YF_MAXRUNS = SNAP_NY + YF_THREADS;
+
 template <class T>
 __global__ void __launch_bounds__(YF_THREADS)
-k_snap_yflat(int nu, int ny, int steps, const uint32_t *__restrict__ code,
-             const T *__restrict__ w, const int *__restrict__ cstart, int n,
-             const T *__restrict__ utot_r, const T *__restrict__ utot_i, T *__restrict__ y_r,
-             T *__restrict__ y_i) {
-  extern __shared__ unsigned char yf_smem[];
-  cpair<T> *su = reinterpret_cast<cpair<T> *>(yf_smem);
-  cpair<T> *part = su + nu;
+k_snap_yflat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
+             const T *__restrict__ w, const unsigned short *__restrict__ meta,
+             const int *__restrict__ rstart, int n, const T *__restrict__ utot_r,
+             const T *__restrict__ utot_i, T *__restrict__ y_r, T *__restrict__ y_i) {
+  // U_tot at byte 0 (a static array: its LDS address is known when the
+  // kernel is compiled, so a code's offset is the whole LDS address), then
+  // the run sums
+  __shared__ cpair<T> s_yf[SNAP_NU + YF_MAXRUNS];
+  cpair<T> *su = s_yf;
+  cpair<T> *part = s_yf + SNAP_NU;
+  const unsigned char *sb = reinterpret_cast<const unsigned char *>(s_yf);
   const int i = blockIdx.x;
   if (i >= n) return;
   for (int k = threadIdx.x; k < nu; k += YF_THREADS)
     su[k] = cpair<T>{utot_r[(size_t)i * nu + k], utot_i[(size_t)i * nu + k]};
   __syncthreads();
   T ar = 0, ai = 0;
-  // steps is a multiple of YF_UNROLL (the host pads with zero terms); the
-  // loads of YF_UNROLL steps are issued before their arithmetic
-  for (int st = 0; st < steps; st += YF_UNROLL) {
-    uint32_t c[YF_UNROLL];
-    T ww[YF_UNROLL];
+  for (int c = 0; c < nch; c++) {
+    const unsigned short m = meta[(size_t)c * YF_THREADS + threadIdx.x];
+    // the chunk's loads are issued before its arithmetic
+    uint32_t cc[YF_CHUNK];
+    T ww[YF_CHUNK];
 #pragma unroll
-    for (int u = 0; u < YF_UNROLL; u++) {
-      const size_t slot = (size_t)(st + u) * YF_THREADS + threadIdx.x;
-      c[u] = code[slot];
-      ww[u] = w[slot];
+    for (int k = 0; k < YF_CHUNK; k++) {
+      const size_t slot = ((size_t)c * YF_CHUNK + k) * YF_THREADS + threadIdx.x;
+      cc[k] = code[slot];
+      ww[k] = w[slot];
     }
 #pragma unroll
-    for (int u = 0; u < YF_UNROLL; u++) {
-      const cpair<T> u1 = su[c[u] & 511u], u2 = su[(c[u] >> 9) & 511u];
+    for (int k = 0; k < YF_CHUNK; k++) {
+      const cpair<T> u1 = *reinterpret_cast<const cpair<T> *>(sb + (cc[k] & 0x1FFFu));
+      const cpair<T> u2 = *reinterpret_cast<const cpair<T> *>(sb + (cc[k] >> 13));
       const T pr = u1.r * u2.r - u1.i * u2.i;
       const T pi = u1.r * u2.i + u1.i * u2.r;
-      ar += ww[u] * pr;
-      ai += ww[u] * pi;
-      if (c[u] & (1u << 29)) {
-        part[(c[u] >> 18) & 2047u] = cpair<T>{ar, ai};
-        ar = 0;
-        ai = 0;
-      }
+      ar += ww[k] * pr;
+      ai += ww[k] * pi;
+    }
+    if (m != YF_NO_RUN) {
+      part[m] = cpair<T>{ar, ai};
+      ar = 0;
+      ai = 0;
     }
   }
   __syncthreads();
   for (int e = threadIdx.x; e < ny; e += YF_THREADS) {
     T yr = 0, yi = 0;
-    for (int c = cstart[e]; c < cstart[e + 1]; c++) {
-      yr += part[c].r;
-      yi += part[c].i;
+    for (int r = rstart[e]; r < rstart[e + 1]; r++) {
+      yr += part[r].r;
+      yi += part[r].i;
     }
     y_r[(size_t)i * ny + e] = yr;
     y_i[(size_t)i * ny + e] = yi;
@@ -906,14 +920,15 @@ struct emd_snap {
   size_t cap_u = 0, cap_y = 0, cap_ui = 0, cap_yi = 0;
   double *eatom = nullptr;
   int cap_e = 0;
-  // flat Y term stream (k_snap_yflat): codes, key per slot, cg products,
-  // weights for the current coefficients, chunk starts per entry
-  uint32_t *yf_code = nullptr;
-  unsigned short *yf_key = nullptr;
+  // flat Y term stream (k_snap_yflat): codes (LDS byte offsets for FP32 and
+  // FP64 U_tot), key per slot, cg products, weights for the current
+  // coefficients, run slot per (chunk, thread), first run per entry
+  uint32_t *yf_code32 = nullptr, *yf_code64 = nullptr;
+  unsigned short *yf_key = nullptr, *yf_meta = nullptr;
   double *yf_cgw = nullptr, *yf_w64 = nullptr, *yf_kcoef = nullptr;
   float *yf_w32 = nullptr;
-  int *yf_cstart = nullptr;
-  int yf_steps = 0, yf_nchunks = 0;
+  int *yf_rstart = nullptr;
+  int yf_steps = 0, yf_nch = 0, yf_nruns = 0;
   std::vector<std::tuple<int, int, int>> yf_keys;  // (a, b, J), sorted
   // adjoint: every (a,b,J) triple grouped by J, with its id
   std::map<std::tuple<int, int, int>, int> tid;
@@ -997,73 +1012,72 @@ static emd_status snap_build_yflat(emd_ctx *ctx, emd_snap *sn) {
           }
         }
       }
-  // chunks in entry order
-  std::vector<int> cstart(t.ny + 1, 0), csize;
+  // chunks of YF_CHUNK terms in entry order, an entry's last chunk padded
+  // with zero-weight terms (U offsets 0, weight 0); thread th takes chunks
+  // [th P, (th+1) P)
   std::vector<std::pair<int, int>> crange;  // (entry, first term)
-  for (int e = 0; e < t.ny; e++) {
-    cstart[e] = (int)csize.size();
-    for (int f = 0; f < (int)eterms[e].size(); f += YF_CHUNK) {
-      csize.push_back(std::min(YF_CHUNK, (int)eterms[e].size() - f));
-      crange.push_back(std::make_pair(e, f));
+  for (int e = 0; e < t.ny; e++)
+    for (int f = 0; f < (int)eterms[e].size(); f += YF_CHUNK) crange.push_back(std::make_pair(e, f));
+  const int nchunks = (int)crange.size();
+  const int P = (nchunks + YF_THREADS - 1) / YF_THREADS;
+  // runs: a thread's consecutive chunks of one entry; run slots numbered
+  // entry by entry, in thread order within an entry (= term order)
+  std::vector<std::vector<std::pair<int, int>>> eruns(t.ny);  // (thread, local chunk)
+  for (int th = 0; th < YF_THREADS; th++)
+    for (int c = 0; c < P; c++) {
+      const int g = th * P + c;
+      if (g >= nchunks) break;
+      const int e = crange[g].first;
+      const bool ends = c == P - 1 || g + 1 >= nchunks || crange[g + 1].first != e;
+      if (ends) eruns[e].push_back(std::make_pair(th, c));
     }
+  std::vector<unsigned short> meta((size_t)P * YF_THREADS, YF_NO_RUN);
+  std::vector<int> rstart(t.ny + 1, 0);
+  int nruns = 0;
+  for (int e = 0; e < t.ny; e++) {
+    rstart[e] = nruns;
+    for (const auto &r : eruns[e]) meta[(size_t)r.second * YF_THREADS + r.first] = (unsigned short)nruns++;
   }
-  cstart[t.ny] = (int)csize.size();
-  const int nchunks = (int)csize.size();
-  if (nchunks > 2048 || t.nu > 512)
+  rstart[t.ny] = nruns;
+  if (nruns > YF_MAXRUNS || t.nu > SNAP_NU)
     return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: Y stream too large");
-  // longest chunk first onto the least loaded thread
-  std::vector<int> order(nchunks);
-  for (int c = 0; c < nchunks; c++) order[c] = c;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return csize[x] > csize[y]; });
-  typedef std::pair<int, int> lt;  // (load, thread)
-  std::priority_queue<lt, std::vector<lt>, std::greater<lt>> pq;
-  for (int th = 0; th < YF_THREADS; th++) pq.push(lt(0, th));
-  std::vector<std::vector<int>> tchunks(YF_THREADS);
-  for (int c : order) {
-    lt top = pq.top();
-    pq.pop();
-    tchunks[top.second].push_back(c);
-    pq.push(lt(top.first + csize[c], top.second));
-  }
-  int steps = 0;
-  for (int th = 0; th < YF_THREADS; th++) {
-    std::sort(tchunks[th].begin(), tchunks[th].end());
-    int l = 0;
-    for (int c : tchunks[th]) l += csize[c];
-    steps = std::max(steps, l);
-  }
-  steps = (steps + YF_UNROLL - 1) / YF_UNROLL * YF_UNROLL;
+  const int steps = P * YF_CHUNK;
   const size_t nslots = (size_t)steps * YF_THREADS;
-  std::vector<uint32_t> code(nslots, 0u);
+  std::vector<uint32_t> code32(nslots, 0u), code64(nslots, 0u);
   std::vector<unsigned short> key(nslots, 0);
   std::vector<double> cgw(nslots, 0.0);
-  for (int th = 0; th < YF_THREADS; th++) {
-    int st = 0;
-    for (int c : tchunks[th]) {
-      const std::vector<term> &tl = eterms[crange[c].first];
-      for (int k = 0; k < csize[c]; k++, st++) {
-        const term &tm = tl[crange[c].second + k];
-        const size_t slot = (size_t)st * YF_THREADS + th;
-        code[slot] = (uint32_t)tm.i1 | ((uint32_t)tm.i2 << 9) | ((uint32_t)c << 18) |
-                     (k == csize[c] - 1 ? (1u << 29) : 0u);
+  for (int th = 0; th < YF_THREADS; th++)
+    for (int c = 0; c < P; c++) {
+      const int g = th * P + c;
+      if (g >= nchunks) break;
+      const std::vector<term> &tl = eterms[crange[g].first];
+      for (int k = 0; k < YF_CHUNK && crange[g].second + k < (int)tl.size(); k++) {
+        const term &tm = tl[crange[g].second + k];
+        const size_t slot = ((size_t)c * YF_CHUNK + k) * YF_THREADS + th;
+        code32[slot] = (uint32_t)(tm.i1 * 8) | ((uint32_t)(tm.i2 * 8) << 13);
+        code64[slot] = (uint32_t)(tm.i1 * 16) | ((uint32_t)(tm.i2 * 16) << 13);
         key[slot] = (unsigned short)tm.key;
         cgw[slot] = tm.cgw;
       }
     }
-  }
   sn->yf_steps = steps;
-  sn->yf_nchunks = nchunks;
-  EMD_HIP(ctx, hipMalloc(&sn->yf_code, nslots * sizeof(uint32_t)));
+  sn->yf_nch = P;
+  sn->yf_nruns = nruns;
+  EMD_HIP(ctx, hipMalloc(&sn->yf_code32, nslots * sizeof(uint32_t)));
+  EMD_HIP(ctx, hipMalloc(&sn->yf_code64, nslots * sizeof(uint32_t)));
   EMD_HIP(ctx, hipMalloc(&sn->yf_key, nslots * sizeof(unsigned short)));
+  EMD_HIP(ctx, hipMalloc(&sn->yf_meta, meta.size() * sizeof(unsigned short)));
   EMD_HIP(ctx, hipMalloc(&sn->yf_cgw, nslots * sizeof(double)));
   EMD_HIP(ctx, hipMalloc(&sn->yf_w64, nslots * sizeof(double)));
   EMD_HIP(ctx, hipMalloc(&sn->yf_w32, nslots * sizeof(float)));
-  EMD_HIP(ctx, hipMalloc(&sn->yf_cstart, (t.ny + 1) * sizeof(int)));
+  EMD_HIP(ctx, hipMalloc(&sn->yf_rstart, (t.ny + 1) * sizeof(int)));
   EMD_HIP(ctx, hipMalloc(&sn->yf_kcoef, std::max<size_t>(1, sn->yf_keys.size()) * sizeof(double)));
-  EMD_HIP(ctx, hipMemcpy(sn->yf_code, code.data(), nslots * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EMD_HIP(ctx, hipMemcpy(sn->yf_code32, code32.data(), nslots * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EMD_HIP(ctx, hipMemcpy(sn->yf_code64, code64.data(), nslots * sizeof(uint32_t), hipMemcpyHostToDevice));
   EMD_HIP(ctx, hipMemcpy(sn->yf_key, key.data(), nslots * sizeof(unsigned short), hipMemcpyHostToDevice));
+  EMD_HIP(ctx, hipMemcpy(sn->yf_meta, meta.data(), meta.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
   EMD_HIP(ctx, hipMemcpy(sn->yf_cgw, cgw.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
-  EMD_HIP(ctx, hipMemcpy(sn->yf_cstart, cstart.data(), (t.ny + 1) * sizeof(int), hipMemcpyHostToDevice));
+  EMD_HIP(ctx, hipMemcpy(sn->yf_rstart, rstart.data(), (t.ny + 1) * sizeof(int), hipMemcpyHostToDevice));
   return EMD_OK;
 }
 
@@ -1111,7 +1125,7 @@ static emd_status snap_upload_coeff(emd_ctx *ctx, emd_snap *sn) {
     EMD_HIP(ctx, hipMemcpy(sn->ytab, yt.data(), yt.size() * sizeof(ytab_entry), hipMemcpyHostToDevice));
   if (!et.empty())
     EMD_HIP(ctx, hipMemcpy(sn->etab, et.data(), et.size() * sizeof(etab_entry), hipMemcpyHostToDevice));
-  if (sn->yf_code) {
+  if (sn->yf_code32) {
     std::vector<double> kc(sn->yf_keys.size(), 0.0);
     for (size_t q = 0; q < kc.size(); q++) {
       auto it = coef.find(sn->yf_keys[q]);
@@ -1226,8 +1240,8 @@ extern "C" emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn) {
   void *ptrs[] = {sn->cg, sn->ytab, sn->etab, sn->nin, sn->nj, sn->nr, sn->stab, sn->w_r,
                   sn->w_i, sn->sbuf,
                   sn->u_r, sn->u_i, sn->y_r, sn->y_i, sn->eatom,
-                  sn->yf_code, sn->yf_key, sn->yf_cgw, sn->yf_w64, sn->yf_w32, sn->yf_cstart,
-                  sn->yf_kcoef};
+                  sn->yf_code32, sn->yf_code64, sn->yf_key, sn->yf_meta, sn->yf_cgw,
+                  sn->yf_w64, sn->yf_w32, sn->yf_rstart, sn->yf_kcoef};
   for (void *q : ptrs)
     if (q) hipFree(q);
   delete sn;
@@ -1291,9 +1305,9 @@ static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   EMD_TRY(launch_ui<T>(ctx, sn, n, ur, ui));
   if (ctx->snap_yflat) {
     const T *w = std::is_same<T, float>::value ? (const T *)sn->yf_w32 : (const T *)sn->yf_w64;
-    const size_t shmem = (size_t)(sn->t.nu + sn->yf_nchunks) * 2 * sizeof(T);
-    EMD_LAUNCH(ctx, "snap_y", k_snap_yflat<T>, n, YF_THREADS, shmem, sn->t.nu, sn->t.ny,
-               sn->yf_steps, sn->yf_code, w, sn->yf_cstart, n, ur, ui, yr, yi);
+    const uint32_t *code = std::is_same<T, float>::value ? sn->yf_code32 : sn->yf_code64;
+    EMD_LAUNCH(ctx, "snap_y", k_snap_yflat<T>, n, YF_THREADS, 0, sn->t.nu, sn->t.ny,
+               sn->yf_nch, code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
   } else {
     EMD_LAUNCH(ctx, "snap_y", k_snap_y<T>, n, Y_THREADS, 0, sn->t, sn->cg, sn->ytab, n, ur, ui,
                yr, yi);
