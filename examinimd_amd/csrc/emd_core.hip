@@ -93,6 +93,10 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
   if (const char *e = getenv("EMD_BIN_ONE")) ctx->bin_one = atoi(e) != 0;
   if (const char *e = getenv("EMD_SNAP_YFLAT")) ctx->snap_yflat = atoi(e) != 0;
   if (const char *e = getenv("EMD_SNAP_UI_GRP")) ctx->snap_ui_grp = atoi(e) != 0;
+  if (const char *e = getenv("EMD_SNAP_UG_WAVES")) {
+    const int w = atoi(e);
+    ctx->snap_ug_waves = (w == 2 || w == 4) ? w : 0;
+  }
   if (const char *e = getenv("EMD_SNAP_FIJ_LANES")) ctx->snap_fij_lanes = atoi(e) == 32 ? 32 : 16;
   // pipelined LJ kernel: blocks per XCD (0, the default, keeps k_lj_full;
   // measured slower at LJ 4M: 0.83 vs 0.74 ms for 192 blocks per XCD)

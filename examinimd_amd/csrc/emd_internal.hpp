@@ -68,6 +68,7 @@ struct emd_ctx {
   int bin_one = 1;         // small ranges: binning in one launch
   int snap_yflat = 1;      // SNAP Y from the flat term stream (0: per-entry kernel)
   int snap_ui_grp = 1;     // SNAP U_tot with 16-lane neighbour groups (twojmax 8)
+  int snap_ug_waves = 0;   // wavefronts per block of that kernel (2, 4; 0: by precision)
   int snap_fij_lanes = 16; // lanes per neighbour in the SNAP force kernel (16 or 32)
   int lj_tile = 0;        // use the bin-tile LJ kernel when the list carries slots
   int lj_tile_lanes = 8;  // lanes per atom of the bin-tile kernel

@@ -354,7 +354,6 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
 // wavefronts in order: deterministic.
 constexpr int UG_LANES = 16;
 constexpr int UG_GROUPS = 64 / UG_LANES;           // per wavefront
-constexpr int UG_WAVES = 4;                        // per block
 constexpr int UG_HP = 32;                          // half-plane entries of layer 7
 
 __host__ __device__ constexpr int ug_hp(int j) { return (j + 1) * (j / 2 + 1); }
@@ -362,7 +361,7 @@ __host__ __device__ constexpr int ug_steps(int j) { return (ug_hp(j) + UG_LANES 
 __host__ __device__ constexpr int ug_slot0(int j) { return j == 0 ? 0 : ug_slot0(j - 1) + ug_steps(j - 1); }
 constexpr int UG_SLOTS = ug_slot0(SNAP_MAXJ + 1);  // 14 for twojmax 8
 
-template <class T>
+template <class T, int UG_WAVES>  // wavefronts per block
 __global__ void __launch_bounds__(64 * UG_WAVES)
 k_snap_ui_grp(snap_dev_tables t, const int *__restrict__ nin, const double *__restrict__ nr,
               int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
@@ -1290,12 +1289,22 @@ static emd_status snap_neigh(emd_ctx *ctx, const emd_system *s, const emd_neigh 
 // per wavefront
 template <class T>
 static emd_status launch_ui(emd_ctx *ctx, emd_snap *sn, int n, T *ur, T *ui) {
-  if (ctx->snap_ui_grp && sn->t.twojmax == SNAP_MAXJ)
-    EMD_LAUNCH(ctx, "snap_ui", k_snap_ui_grp<T>, n, 64 * UG_WAVES, 0, sn->t, sn->nin, sn->nr,
-               sn->cap_maxn, n, ur, ui);
-  else
+  if (ctx->snap_ui_grp && sn->t.twojmax == SNAP_MAXJ) {
+    // FP64 (196 VGPRs: two wavefronts per SIMD) runs more atoms at once with
+    // two-wavefront blocks (SNAP-2k: 47.5 vs 57.3 us); FP32 (94 VGPRs) is
+    // the same either way (28.7 vs 28.1 us) and keeps four
+    const int waves = ctx->snap_ug_waves ? ctx->snap_ug_waves
+                                         : (std::is_same<T, double>::value ? 2 : 4);
+    if (waves == 2)
+      EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui_grp<T, 2>), n, 64 * 2, 0, sn->t, sn->nin, sn->nr,
+                 sn->cap_maxn, n, ur, ui);
+    else
+      EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui_grp<T, 4>), n, 64 * 4, 0, sn->t, sn->nin, sn->nr,
+                 sn->cap_maxn, n, ur, ui);
+  } else {
     EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr,
                sn->cap_maxn, n, ur, ui);
+  }
   return EMD_OK;
 }
 
