@@ -683,11 +683,14 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   constexpr int FIJ_NB = fij_nb<LG>();
   constexpr int NBW = 64 / LG;  // neighbours per wavefront
   // previous layer (j-1 <= 7) of each neighbour: U and dU (3 components) of
-  // its half plane (mb <= (j-1)/2, at mb*j + ma: at most 32 entries), double
-  // buffered; the middle column of an even layer reads its mirror image
+  // its half plane (mb <= (j-1)/2, at mb*j + ma: at most 32 entries); the
+  // middle column of an even layer reads its mirror image
   // U(j-1-ma, j-1-mb) = (-1)^(ma+mb) conj U(ma, mb) (the same symmetry holds
-  // for dU), so no lane writes a mirror entry
-  __shared__ T s_u[FIJ_NB][2][8][FIJ_HP];
+  // for dU), so no lane writes a mirror entry.  One buffer: a layer's
+  // entries are kept in registers until every lane has read the previous
+  // layer (a layer below the top has at most 32 entries, two steps of 16
+  // lanes), then written over it.
+  __shared__ T s_u[FIJ_NB][8][FIJ_HP];
   __shared__ T s_y[2][SNAP_NY];
   __shared__ layer_tab<T> LT;
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
@@ -739,13 +742,13 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   sfac *= (T)t.wj;
   dsfac *= (T)t.wj;
 
-  T (*buf)[8][FIJ_HP] = s_u[NBW * w + hs];
+  T (*buf)[FIJ_HP] = s_u[NBW * w + hs];
   const T *yr = s_y[0], *yi = s_y[1];
   T fx = 0, fy = 0, fz = 0;
   // layer 0: U = 1, dU = 0; its dU_tot = dsfac * u; contraction with Y_0
   if (hl == 0) {
-    buf[0][0][0] = 1; buf[0][1][0] = 0;
-    for (int k = 0; k < 3; k++) { buf[0][2 + 2 * k][0] = 0; buf[0][3 + 2 * k][0] = 0; }
+    buf[0][0] = 1; buf[1][0] = 0;
+    for (int k = 0; k < 3; k++) { buf[2 + 2 * k][0] = 0; buf[3 + 2 * k][0] = 0; }
     const T w0 = (T)0.5;  // J = 0: the single entry is the middle-column diagonal
     const T c = dsfac * w0 * yr[0];
     fx += c * u3[0];
@@ -753,17 +756,21 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
     fz += c * u3[2];
   }
   wave_sync();
+  constexpr int FIJ_STEPS = (45 + LG - 1) / LG;  // steps of the top layer (45 entries)
   for (int j = 1; j <= t.twojmax; j++) {
-    const int p = (j - 1) & 1, c = j & 1;
     const int ne = (j + 1) * (j / 2 + 1);
-    for (int s0 = 0; s0 < ne; s0 += LG) {
+    T keep[2][8];  // this layer's own entries of steps 0 and 1
+#pragma unroll
+    for (int st = 0; st < FIJ_STEPS; st++) {
+      const int s0 = st * LG;
+      if (s0 >= ne) break;
       const int e = s0 + hl;
       const int info = LT.mm[j][e];
       const int ma = info & 255, mb = (info >> 8) & 255;
       const bool own = info >> 16;
       T ur = 0, ui = 0, dr[3] = {0, 0, 0}, di[3] = {0, 0, 0};
       if (own) {
-        const T *P = &buf[p][0][0];
+        const T *P = &buf[0][0];
         // P[q*FIJ_HP + idx]: q = 0 U_r, 1 U_i, 2+2k dU_r[k], 3+2k dU_i[k].
         // Layer j-1 entries (ma, mb) and (ma-1, mb); on the middle column of
         // an even layer (mb = j/2 > (j-1)/2) they are read as mirrors.
@@ -800,12 +807,10 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
           }
         }
       }
-      if (own && j < t.twojmax) {
-        // own half-plane entry only
-        T *C = &buf[c][0][0];
-        C[e] = ur; C[FIJ_HP + e] = ui;
+      if (st < 2) {
+        keep[st][0] = ur; keep[st][1] = ui;
 #pragma unroll
-        for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * FIJ_HP + e] = dr[k]; C[(3 + 2 * k) * FIJ_HP + e] = di[k]; }
+        for (int k = 0; k < 3; k++) { keep[st][2 + 2 * k] = dr[k]; keep[st][3 + 2 * k] = di[k]; }
       }
       if (own) {
         // half-plane weight of (ma, mb) in layer j
@@ -819,6 +824,18 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
           fx += dsfac * uy * u3[0] + sfac * g[0];
           fy += dsfac * uy * u3[1] + sfac * g[1];
           fz += dsfac * uy * u3[2] + sfac * g[2];
+        }
+      }
+    }
+    wave_sync();  // every lane has read layer j-1
+    if (j < t.twojmax) {
+      // own half-plane entries only (ne <= 32 below the top layer)
+#pragma unroll
+      for (int st = 0; st < 2; st++) {
+        const int e = st * LG + hl;
+        if (st * LG < ne && (LT.mm[j][e] >> 16)) {
+#pragma unroll
+          for (int q = 0; q < 8; q++) buf[q][e] = keep[st][q];
         }
       }
     }
