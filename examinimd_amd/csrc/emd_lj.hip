@@ -755,36 +755,75 @@ extern "C" emd_status emd_lj_idial_force_store(emd_ctx *ctx, emd_system *s,
 }
 
 
-// ForceLJNeigh Tag{Full,Half}NeighPE, force_lj_neigh_impl.h:256-343
-__global__ void __launch_bounds__(256)
+// ForceLJNeigh Tag{Full,Half}NeighPE, force_lj_neigh_impl.h:256-343.  K lanes
+// per atom take every K-th entry of the row, issuing the index loads of U
+// entries and then their U gathers before the arithmetic (as k_lj_full);
+// each lane sums its pair energies (the reference's per-pair expression,
+// shift included) and the block adds all lanes in double-double.  fac is 1
+// or 1/2, so fac * x is exact and the terms are fac times the fac-free
+// expressions; with one type the shift term is formed once per lane.
+constexpr int LJE_K = 16, LJE_U = 5;
+
+__device__ __forceinline__ double lj_pair_e(double r6inv, double lj1, double lj2) {
+#pragma clang fp contract(off)
+  return r6inv * (0.5 * lj1 * r6inv - lj2) / 6.0;
+}
+
+template <bool ONE_TYPE>
+__global__ void __launch_bounds__(LJ_THREADS)
 k_lj_energy(const double *__restrict__ x, const int *__restrict__ type,
             const int *__restrict__ row, const int *__restrict__ num,
             const int *__restrict__ entries, int pitch, int n, int half,
             const lj_table *__restrict__ tab, double *__restrict__ part) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = blockIdx.x * LJ_THREADS + threadIdx.x;
+  const int i = gid / LJE_K, sub = gid % LJE_K;
   double pe = 0.0;
   if (i < n) {
     const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
     const int type_i = type[i];
-    const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+    const int *rowp = entries + (row ? (size_t)row[i] : (size_t)i * (size_t)pitch);
     const int cnt = num[i];
-    for (int jj = 0; jj < cnt; jj++) {
-      const int j = entries[start + jj];
-      const double dx = x_i - x[3 * (size_t)j];
-      const double dy = y_i - x[3 * (size_t)j + 1];
-      const double dz = z_i - x[3 * (size_t)j + 2];
-      const double rsq = dx * dx + dy * dy + dz * dz;
-      const int t = type_i * LJ_MAXT + type[j];
-      const double cutsq = tab->cutsq[t], lj1 = tab->lj1[t], lj2 = tab->lj2[t];
-      if (rsq < cutsq) {
-        const double r2inv = 1.0 / rsq;
-        const double r6inv = r2inv * r2inv * r2inv;
-        const double r2invc = 1.0 / cutsq;
-        const double r6invc = r2invc * r2invc * r2invc;
-        const double fac = half ? ((j < n) ? 1.0 : 0.5) : 0.5;
-        pe += fac * r6inv * (0.5 * lj1 * r6inv - lj2) / 6.0;
-        pe -= fac * r6invc * (0.5 * lj1 * r6invc - lj2) / 6.0;
+    double shift0 = 0.0;
+    if (ONE_TYPE) {
+      const double r2invc = 1.0 / tab->cutsq[0];
+      shift0 = lj_pair_e(r2invc * r2invc * r2invc, tab->lj1[0], tab->lj2[0]);
+    }
+    for (int jj = sub; jj < cnt; jj += LJE_U * LJE_K) {
+      int j[LJE_U];
+      bool ok[LJE_U];
+#pragma unroll
+      for (int u = 0; u < LJE_U; u++) {
+        ok[u] = jj + u * LJE_K < cnt;
+        j[u] = rowp[ok[u] ? jj + u * LJE_K : cnt - 1];
+      }
+      double xj[LJE_U], yj[LJE_U], zj[LJE_U];
+#pragma unroll
+      for (int u = 0; u < LJE_U; u++) {
+        xj[u] = x[3 * (size_t)j[u]];
+        yj[u] = x[3 * (size_t)j[u] + 1];
+        zj[u] = x[3 * (size_t)j[u] + 2];
+      }
+#pragma unroll
+      for (int u = 0; u < LJE_U; u++) {
+        const double dx = x_i - xj[u];
+        const double dy = y_i - yj[u];
+        const double dz = z_i - zj[u];
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        const int t = ONE_TYPE ? 0 : type_i * LJ_MAXT + type[j[u]];
+        const double cutsq = tab->cutsq[t], lj1 = tab->lj1[t], lj2 = tab->lj2[t];
+        if (ok[u] && rsq < cutsq) {
+          const double r2inv = 1.0 / rsq;
+          const double r6inv = r2inv * r2inv * r2inv;
+          double shift = shift0;
+          if (!ONE_TYPE) {
+            const double r2invc = 1.0 / cutsq;
+            shift = lj_pair_e(r2invc * r2invc * r2invc, lj1, lj2);
+          }
+          const double fac = half ? ((j[u] < n) ? 1.0 : 0.5) : 0.5;
+          pe += fac * lj_pair_e(r6inv, lj1, lj2);
+          pe -= fac * shift;
+        }
       }
     }
   }
@@ -817,12 +856,16 @@ extern "C" emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd
   if (n <= 0) { *pe = 0.0; return EMD_OK; }
   lj_table *tab;
   EMD_TRY(upload_table(ctx, p, nullptr, &tab));
-  const int nb = emd_blocks(n, 256);
+  const int nb = emd_blocks((long)n * LJE_K, LJ_THREADS);
   EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)nb * 2 * sizeof(double)));
   double *part = (double *)ctx->d_tmp2;
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
-  EMD_LAUNCH(ctx, "lj_energy", k_lj_energy, nb, 256, 0, s->x, s->type, row, nl->num, nl->entries,
-             nl->maxneighs, n, half, tab, part);
+  if (p->ntypes == 1)
+    EMD_LAUNCH(ctx, "lj_energy", k_lj_energy<true>, nb, LJ_THREADS, 0, s->x, s->type, row,
+               nl->num, nl->entries, nl->maxneighs, n, half, tab, part);
+  else
+    EMD_LAUNCH(ctx, "lj_energy", k_lj_energy<false>, nb, LJ_THREADS, 0, s->x, s->type, row,
+               nl->num, nl->entries, nl->maxneighs, n, half, tab, part);
   return finish_dd(ctx, part, nb, pe);
 }
 
