@@ -118,6 +118,7 @@ HIP_PROTOS = {
     "emd_snap_destroy": (C.c_int, [vp, vp]),
     "emd_snap_set_coeff": (C.c_int, [vp, vp, dptr]),
     "emd_snap_force": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, C.c_int]),
+    "emd_snap_force_zeroed": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, C.c_int]),
     "emd_snap_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, P(C.c_double), dptr]),
     "emd_snap_adjoint_beta": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdHalo), vp, dptr,
                                         C.c_double, C.c_int, dptr]),

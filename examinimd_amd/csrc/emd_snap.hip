@@ -110,16 +110,29 @@ __device__ __forceinline__ void sfac_dsfac(const snap_dev_tables &t, T r, T &sfa
 
 // ================================================================ kernels
 // in-cutoff neighbours of each atom, ForceSNAP::operator() lines 612-656
-// (rij = x_j - x_i, test rsq < rcutmax^2); slot k of atom i at i*maxn + k
+// (rij = x_j - x_i, test rsq < rcutmax^2); slot k of atom i at i*maxn + k.
+// With a force array given (f64 or f32), block i also zeroes its rows
+// i, i + n, i + 2n, ... below ntot (deep_copy(f, 0), src/examinimd.cpp:232,
+// done here instead of in a launch of its own: the force kernels that add
+// into f come after this one).
 __global__ void __launch_bounds__(64)
 k_snap_neigh(const double *__restrict__ x, const int *__restrict__ row,
              const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
              double cutsq, int maxn, int *__restrict__ nin, int *__restrict__ nj,
-             double *__restrict__ nr) {
+             double *__restrict__ nr, double *__restrict__ f64, float *__restrict__ f32,
+             int ntot) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x;
   if (i >= n) return;
   const int lane = threadIdx.x;
+  if (f64 || f32) {
+    for (int r = i + n * (lane / 3); r < ntot; r += n * 21) {  // lanes 0..62: 21 rows of 3
+      if (lane < 63) {
+        if (f64) f64[3 * (size_t)r + lane % 3] = 0.0;
+        else f32[3 * (size_t)r + lane % 3] = 0.0f;
+      }
+    }
+  }
   const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
   const int *rp = entries + (row ? (size_t)row[i] : (size_t)i * (size_t)pitch);
   const int cnt = num[i];
@@ -1291,14 +1304,20 @@ static emd_status snap_scratch(emd_ctx *ctx, emd_snap *sn, int n, int maxn) {
   return EMD_OK;
 }
 
-static emd_status snap_neigh(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl, emd_snap *sn) {
+// zero_prec: EMD_FP64 or EMD_MIXED to zero that precision's force array
+// (all n_local + n_ghost rows) in the same launch, or -1
+static emd_status snap_neigh(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl, emd_snap *sn,
+                             int zero_prec = -1) {
   const int n = s->n_local;
   const int maxn = nl->maxneighs > 0 ? nl->maxneighs : 1;
   EMD_TRY(snap_scratch(ctx, sn, n, maxn));
   const double rcutmax = 2.0 * sn->radelem * sn->rcutfac;
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  double *f64 = zero_prec == EMD_FP64 ? s->f : nullptr;
+  float *f32 = zero_prec == EMD_MIXED ? s->f32 : nullptr;
   EMD_LAUNCH(ctx, "snap_neigh", k_snap_neigh, n, 64, 0, s->x, row, nl->num, nl->entries,
-             nl->maxneighs, n, rcutmax * rcutmax, sn->cap_maxn, sn->nin, sn->nj, sn->nr);
+             nl->maxneighs, n, rcutmax * rcutmax, sn->cap_maxn, sn->nin, sn->nj, sn->nr, f64,
+             f32, s->n_local + s->n_ghost);
   return EMD_OK;
 }
 
@@ -1360,13 +1379,13 @@ static emd_status launch_fij(emd_ctx *ctx, emd_snap *sn, int n, const T *yr, con
   return EMD_OK;
 }
 
-extern "C" emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
-                                     emd_snap *sn, int precision) {
+static emd_status snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, emd_snap *sn,
+                             int precision, bool zero) {
   if (!ctx || !s || !nl || !sn) return EMD_ERR_ARG;
   EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
   const int n = s->n_local;
-  if (n <= 0) return EMD_OK;
-  EMD_TRY(snap_neigh(ctx, s, nl, sn));
+  if (n <= 0) return zero ? emd_force_zero(ctx, s, precision) : EMD_OK;
+  EMD_TRY(snap_neigh(ctx, s, nl, sn, zero ? (precision == EMD_MIXED ? EMD_MIXED : EMD_FP64) : -1));
   if (precision == EMD_MIXED) {
     EMD_TRY(snap_ui_y<float>(ctx, sn, n));
     EMD_TRY((launch_fij<float, float>(ctx, sn, n, (const float *)sn->y_r,
@@ -1377,6 +1396,16 @@ extern "C" emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neig
                                         (const double *)sn->y_i, s->f)));
   }
   return EMD_OK;
+}
+
+extern "C" emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                     emd_snap *sn, int precision) {
+  return snap_force(ctx, s, nl, sn, precision, false);
+}
+
+extern "C" emd_status emd_snap_force_zeroed(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
+                                            emd_snap *sn, int precision) {
+  return snap_force(ctx, s, nl, sn, precision, true);
 }
 
 extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,

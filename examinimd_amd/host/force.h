@@ -15,8 +15,9 @@ public:
   virtual void compute(System* system, Binning* binning, Neighbor* neigh);
   // the reference returns 0 by default (src/force.h:54)
   virtual T_F_FLOAT compute_energy(System*, Binning*, Neighbor*) { return 0.0; }
-  // compute() writes every local atom's force outright and no ghost force,
-  // so the driver's zeroing of f before it can be skipped
+  // compute() leaves f complete by itself (it writes every local atom's
+  // force outright and no ghost force, or zeroes every row first), so the
+  // driver's zeroing of f before it can be skipped
   virtual bool stores_forces() const { return false; }
   virtual const char* name();
 };

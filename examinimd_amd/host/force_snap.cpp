@@ -139,8 +139,9 @@ void ForceSNAPHIP::read_files(const char* coefffilename, const char* paramfilena
 // ForceSNAP::compute, force_snap_neigh_impl.h:159-208
 void ForceSNAPHIP::compute(System* system_, Binning*, Neighbor* neighbor) {
   if (comm_newton == false) snap_abort("ForceSNAP requires 'newton on'");
-  EMD_CALL(system_->ctx, emd_snap_force(system_->ctx, system_->view(), neighbor->get_neigh_list(),
-                                        sn, system_->precision));
+  // zeroes f itself (stores_forces): no separate zeroing launch
+  EMD_CALL(system_->ctx, emd_snap_force_zeroed(system_->ctx, system_->view(),
+                                               neighbor->get_neigh_list(), sn, system_->precision));
 }
 
 T_F_FLOAT ForceSNAPHIP::compute_energy(System* system_, Binning*, Neighbor* neighbor) {

@@ -36,6 +36,7 @@ public:
   ~ForceSNAPHIP();
   void init_coeff(int nargs, char** args) override;
   void compute(System* system, Binning* binning, Neighbor* neighbor) override;
+  bool stores_forces() const override { return true; }
   // no counterpart in the reference (its ForceSNAP returns 0): linear SNAP
   // energy plus the energy of the reference's r^-14 repulsion
   T_F_FLOAT compute_energy(System* system, Binning* binning, Neighbor* neighbor) override;

@@ -272,6 +272,11 @@ emd_status emd_snap_set_coeff(emd_ctx *ctx, emd_snap *sn, const double *coeff);
  * forces land on locals and ghosts, fold with emd_comm_update_force */
 emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, emd_snap *sn,
                           int precision);
+/* deep_copy(f, 0) (src/examinimd.cpp:232) then ForceSNAP::compute: the same
+ * forces, with the zeroing of every local and ghost row (f, or f32 for
+ * EMD_MIXED) done by the first SNAP kernel instead of a launch of its own */
+emd_status emd_snap_force_zeroed(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, emd_snap *sn,
+                                 int precision);
 /* SNAP energy (no reference counterpart: the reference's ForceSNAP has no
  * compute_energy, src/force.h:54); eatom may be NULL or a device [n_local] */
 emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
