@@ -939,6 +939,12 @@ struct emd_snap {
   double *cg = nullptr;
   ytab_entry *ytab = nullptr;
   etab_entry *etab = nullptr;
+  size_t ytab_cap = 0, etab_cap = 0;  // entries allocated (kept across coefficient updates)
+  // host sources of the last coefficient upload's stream-ordered copies
+  // (kept until the next upload, which first waits for the stream)
+  std::vector<ytab_entry> h_ytab;
+  std::vector<etab_entry> h_etab;
+  std::vector<double> h_kcoef;
   int bzeroflag = 0;
   double rcutfac = 0, radelem = 0;
   // per-system scratch
@@ -1122,7 +1128,9 @@ static emd_status snap_upload_coeff(emd_ctx *ctx, emd_snap *sn) {
     coef[std::make_tuple(std::max(j, j2), std::min(j, j2), j1)] += 2.0 * b * (j + 1) / (j1 + 1.0);
     coef[std::make_tuple(std::max(j1, j), std::min(j1, j), j2)] += 2.0 * b * (j + 1) / (j2 + 1.0);
   }
-  std::vector<ytab_entry> yt;
+  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the previous upload's copies are done
+  std::vector<ytab_entry> &yt = sn->h_ytab;
+  yt.clear();
   for (int J = 0; J <= twojmax; J++) {
     sn->t.yjstart[J] = (int)yt.size();
     for (auto &kv : coef) {
@@ -1136,7 +1144,8 @@ static emd_status snap_upload_coeff(emd_ctx *ctx, emd_snap *sn) {
   }
   sn->t.yjstart[twojmax + 1] = (int)yt.size();
   sn->t.ny_terms = (int)yt.size();
-  std::vector<etab_entry> et;
+  std::vector<etab_entry> &et = sn->h_etab;
+  et.clear();
   for (int k = 0; k < nk; k++) {
     etab_entry e;
     e.j1 = sn->idxj[3 * k]; e.j2 = sn->idxj[3 * k + 1]; e.j = sn->idxj[3 * k + 2];
@@ -1144,25 +1153,38 @@ static emd_status snap_upload_coeff(emd_ctx *ctx, emd_snap *sn) {
     e.beta = sn->beta[k + 1];
     et.push_back(e);
   }
-  if (sn->ytab) EMD_HIP(ctx, hipFree(sn->ytab));
-  if (sn->etab) EMD_HIP(ctx, hipFree(sn->etab));
-  sn->ytab = nullptr;
-  sn->etab = nullptr;
-  EMD_HIP(ctx, hipMalloc(&sn->ytab, std::max<size_t>(1, yt.size()) * sizeof(ytab_entry)));
-  EMD_HIP(ctx, hipMalloc(&sn->etab, std::max<size_t>(1, et.size()) * sizeof(etab_entry)));
+  // the tables keep their allocation while they fit (coefficient updates
+  // in a force-matching loop do not reallocate); copies are stream-ordered
+  if (yt.size() > sn->ytab_cap || !sn->ytab) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (sn->ytab) EMD_HIP(ctx, hipFree(sn->ytab));
+    sn->ytab = nullptr;
+    sn->ytab_cap = std::max<size_t>(1, yt.size());
+    EMD_HIP(ctx, hipMalloc(&sn->ytab, sn->ytab_cap * sizeof(ytab_entry)));
+  }
+  if (et.size() > sn->etab_cap || !sn->etab) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (sn->etab) EMD_HIP(ctx, hipFree(sn->etab));
+    sn->etab = nullptr;
+    sn->etab_cap = std::max<size_t>(1, et.size());
+    EMD_HIP(ctx, hipMalloc(&sn->etab, sn->etab_cap * sizeof(etab_entry)));
+  }
   if (!yt.empty())
-    EMD_HIP(ctx, hipMemcpy(sn->ytab, yt.data(), yt.size() * sizeof(ytab_entry), hipMemcpyHostToDevice));
+    EMD_HIP(ctx, hipMemcpyAsync(sn->ytab, yt.data(), yt.size() * sizeof(ytab_entry),
+                                hipMemcpyHostToDevice, ctx->stream));
   if (!et.empty())
-    EMD_HIP(ctx, hipMemcpy(sn->etab, et.data(), et.size() * sizeof(etab_entry), hipMemcpyHostToDevice));
+    EMD_HIP(ctx, hipMemcpyAsync(sn->etab, et.data(), et.size() * sizeof(etab_entry),
+                                hipMemcpyHostToDevice, ctx->stream));
   if (sn->yf_code32) {
-    std::vector<double> kc(sn->yf_keys.size(), 0.0);
+    std::vector<double> &kc = sn->h_kcoef;
+    kc.assign(sn->yf_keys.size(), 0.0);
     for (size_t q = 0; q < kc.size(); q++) {
       auto it = coef.find(sn->yf_keys[q]);
       if (it != coef.end()) kc[q] = it->second;
     }
     if (!kc.empty())
-      EMD_HIP(ctx, hipMemcpy(sn->yf_kcoef, kc.data(), kc.size() * sizeof(double),
-                             hipMemcpyHostToDevice));
+      EMD_HIP(ctx, hipMemcpyAsync(sn->yf_kcoef, kc.data(), kc.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, ctx->stream));
     const int nslots = sn->yf_steps * YF_THREADS;
     if (nslots > 0)
       EMD_LAUNCH(ctx, "snap_coeff", k_yf_weights, emd_blocks(nslots, 256), 256, 0, nslots,

@@ -51,7 +51,8 @@ def lj_set(count=32, seed=23, sizes=(5, 6, 7, 8, 9, 10), disp=0.05):
 
 
 class SnapForceMatch:
-    """loss(beta) and dloss/dbeta over SNAP configurations"""
+    """loss(beta) and dloss/dbeta over SNAP configurations (one SNAP object,
+    one coefficient upload per evaluation, shared by all configurations)"""
 
     def __init__(self, configs, beta_true, ctx=None):
         self.pls = []
@@ -63,7 +64,10 @@ class SnapForceMatch:
             # static configurations: the list needs no skin; newton on for
             # the ghost force fold
             pl.build(SNAP_CUT, newton=1)
-            pl.snap_setup(list(beta_true))
+            if self.pls:
+                pl.snap_use(self.pls[0])
+            else:
+                pl.snap_setup(list(beta_true))
             F = pl.snap_force(EMD_FP64)
             E = pl.snap_energy()
             self.pls.append(pl)
@@ -74,8 +78,9 @@ class SnapForceMatch:
         beta = np.ascontiguousarray(beta, dtype=np.float64)
         loss = 0.0
         g = np.zeros(len(beta))
+        self.pls[0].snap_set_coeff(beta)
         for pl, (Ft, Et, n) in zip(self.pls, self.targets):
-            pl.snap_set_coeff(beta)
+            pl._beta = self.pls[0]._beta
             F = pl.snap_force(precision)
             E = pl.snap_energy()
             dF = F - Ft
@@ -85,7 +90,8 @@ class SnapForceMatch:
         return loss, g
 
     def close(self):
-        for pl in self.pls:
+        # the first pipeline owns the shared SNAP object: close it last
+        for pl in reversed(self.pls):
             pl.close()
         self.pls = []
 

@@ -34,6 +34,7 @@ class Pipeline:
         self.ctx.check(st, what)
 
     def load(self, x, box, mass=1.0, v=None):
+        self._ids = None
         n = len(x)
         x = np.ascontiguousarray(x, dtype=np.float64)
         v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float64)
@@ -58,6 +59,7 @@ class Pipeline:
         view (full lists) that it reads; layout: EMD_NEIGH_CSR or
         EMD_NEIGH_2D"""
         L, c, s = self.L, self.ctx.h, C.byref(self.s)
+        self._ids = None
         self.nl.want_slot = 1 if tile else 0
         if tile:
             self.chk(L.emd_lj_set_tile(c, 1, 0), "set_tile")
@@ -105,6 +107,14 @@ class Pipeline:
         h = C.c_void_p()
         self.chk(self.L.emd_snap_create(self.ctx.h, C.byref(p), C.byref(h)), "snap_create")
         self.snap = h
+        self._snap_owned = True
+
+    def snap_use(self, other):
+        """use another Pipeline's SNAP object (same context): its tables and
+        coefficients; the owner destroys it"""
+        self.snap = other.snap
+        self._beta = other._beta
+        self._snap_owned = False
 
     def snap_force(self, precision=EMD_FP64):
         L, c, s = self.L, self.ctx.h, C.byref(self.s)
@@ -125,10 +135,7 @@ class Pipeline:
 
     def _cotangent(self, gF):
         """upload dL/dF given in input order into the device atom order"""
-        n = self.s.n_local
-        ids = np.zeros(n, np.int32)
-        self.chk(self.L.emd_system_download(self.ctx.h, C.byref(self.s), n, None, None, None,
-                                            None, _i(ids), None), "ids")
+        ids = self._device_ids()
         g = np.ascontiguousarray(np.asarray(gF, dtype=np.float64)[ids - 1])
         if getattr(self, "_gdev", None) is None or self._gcap < g.nbytes:
             if getattr(self, "_gdev", None) is not None:
@@ -168,9 +175,19 @@ class Pipeline:
                                       C.byref(e)), "lj_energy")
         return e.value
 
+    def _device_ids(self):
+        """atom ids in device order (cached until the next load or build:
+        only those reorder the atoms)"""
+        if getattr(self, "_ids", None) is None:
+            n = self.s.n_local
+            ids = np.zeros(n, np.int32)
+            self.chk(self.L.emd_system_download(self.ctx.h, C.byref(self.s), n, None, None, None,
+                                                None, _i(ids), None), "ids")
+            self._ids = ids
+        return self._ids
+
     def forces(self, precision):
         n = self.s.n_local
-        ids = np.zeros(n, np.int32)
         f = np.zeros((n, 3))
         L, c = self.L, self.ctx.h
         if precision == EMD_MIXED:
@@ -181,8 +198,7 @@ class Pipeline:
         else:
             self.chk(L.emd_system_download(c, C.byref(self.s), n, None, None, _d(f), None, None,
                                            None), "f")
-        self.chk(L.emd_system_download(c, C.byref(self.s), n, None, None, None, None, _i(ids),
-                                       None), "ids")
+        ids = self._device_ids()
         out = np.zeros_like(f)
         out[ids - 1] = f
         return out
@@ -190,7 +206,8 @@ class Pipeline:
     def close(self):
         L, c = self.L, self.ctx.h
         if self.snap:
-            L.emd_snap_destroy(c, self.snap)
+            if getattr(self, "_snap_owned", True):
+                L.emd_snap_destroy(c, self.snap)
             self.snap = None
         if getattr(self, "_gdev", None) is not None:
             L.emd_free(c, self._gdev)
