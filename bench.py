@@ -201,13 +201,16 @@ def snap_files(emd, decks, tmpdir):
     return coeff, param, beta
 
 
-def snap_run(emd, decks, tmpdir, coeff, param, precision, warmup, steps, breakdown_steps=20):
-    """timed SNAP run with no events (at 2000 atoms an event pair is a
-    noticeable share of a kernel's dispatch gap), then an untimed per-kernel
+def snap_run(emd, decks, tmpdir, coeff, param, precision, warmup, steps, breakdown_steps=20,
+             repeats=5):
+    """timed SNAP runs with no events (at 2000 atoms an event pair is a
+    noticeable share of a kernel's dispatch gap): `repeats` consecutive runs
+    of `steps` steps, the median one reported (a run of 100 steps lasts
+    20-30 ms, where a single hiccup shows), then an untimed per-kernel
     breakdown whose SNAP kernel times feed the roofline"""
     deck = decks.write(os.path.join(tmpdir, f"in.snap.{precision}"),
-                       decks.snap_deck(n=10, steps=steps + breakdown_steps, coeff=coeff,
-                                       param=param))
+                       decks.snap_deck(n=10, steps=repeats * steps + breakdown_steps,
+                                       coeff=coeff, param=param))
     app = emd.App(["-il", deck, "--neigh-type", "CSR", "--precision", precision]).init()
     app.set_phase_sync(False)
     L = emd.hip_lib()
@@ -216,10 +219,13 @@ def snap_run(emd, decks, tmpdir, coeff, param, precision, warmup, steps, breakdo
         app.run(warmup)
     L.emd_synchronize(ctx)
     L.emd_profile_enable(ctx, 0)
-    t0 = time.perf_counter()
-    app.run(steps)
-    L.emd_synchronize(ctx)
-    wall = time.perf_counter() - t0
+    walls = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        app.run(steps)
+        L.emd_synchronize(ctx)
+        walls.append(time.perf_counter() - t0)
+    wall = sorted(walls)[len(walls) // 2]
     natoms = app.natoms
     e_per_atom = app.thermo()[2]
     breakdown = _breakdown(app, L, ctx, breakdown_steps)
@@ -516,7 +522,8 @@ def main():
                                       "kernel_ms_per_step": kms}
         snap["mixed_force_rms_error_eV_per_A"] = snap_force_error(emd, beta)
         snap["workload"] = (f"SNAP W BCC 10^3 (2000 atoms), twojmax 8, rc 4.73442, skin 1.0, "
-                            f"rebuild every 10, dt 0.5 fs, {a.snap_steps} steps")
+                            f"rebuild every 10, dt 0.5 fs, {a.snap_steps} steps "
+                            f"(median of 5 consecutive runs)")
 
     batched = None if a.no_batch else batch_leg(emd, a.batch_steps, a.loop_steps)
     if batched:
