@@ -433,47 +433,105 @@ k_lj_tile(const double *__restrict__ x, const int *__restrict__ bincount,
   }
 }
 
-// ForceLJNeigh TagHalfNeigh, force_lj_neigh_impl.h:208-254 (atomic j
-// updates).  DIAL: ForceLJIDialNeigh TagHalfNeigh,
-// force_lj_idial_neigh_impl.h:169-219, which subtracts the reaction only from
-// local j (:208-212).
-template <class T, bool DIAL>
+// ForceLJNeigh TagHalfNeigh, force_lj_neigh_impl.h:208-254: each pair's
+// force is added to i and subtracted from j (atomically: rows of other
+// atoms reach j too).  LJH_K lanes per atom take every LJH_K-th entry of the
+// row, issuing the index loads of LJH_U entries and then their gathers
+// before the arithmetic (a half row holds ~37 entries: one round); the
+// lanes' i sums are reduced across the group and added once.  Pair
+// arithmetic as the full-list kernel: FP64 as the reference writes it,
+// mixed with FP64 distances and emd_lj_fpair_mixed.  DIAL: ForceLJIDialNeigh
+// TagHalfNeigh, force_lj_idial_neigh_impl.h:169-219, which subtracts the
+// reaction only from local j (:208-212).  Atomic order varies from run to
+// run, so sums agree to rounding, not bitwise.
+constexpr int LJH_K = 8, LJH_U = 5;
+
+template <bool MIXED, bool DIAL>
 __global__ void __launch_bounds__(LJ_THREADS)
 k_lj_half(const double *__restrict__ x, const int *__restrict__ type, const int *__restrict__ row,
           const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
-          const lj_table *__restrict__ tab, T *__restrict__ f) {
-  const int i = blockIdx.x * LJ_THREADS + threadIdx.x;
-  if (i >= n) return;
+          const lj_table *__restrict__ tab, double *__restrict__ f, float *__restrict__ f32) {
+#pragma clang fp contract(off)
+  typedef typename lj_acc<MIXED>::T T;
+  const int gid = blockIdx.x * LJ_THREADS + threadIdx.x;
+  const int i = gid / LJH_K, sub = gid % LJH_K;
+  if (i >= n) return;  // whole groups of LJH_K lanes leave together
   const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
   const int type_i = type[i];
-  const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
+  const int *rowp = entries + (row ? (size_t)row[i] : (size_t)i * (size_t)pitch);
   const int cnt = num[i];
   T fxi = 0, fyi = 0, fzi = 0;
-  for (int jj = 0; jj < cnt; jj++) {
-    const int j = entries[start + jj];
-    const double dx = x_i - x[3 * (size_t)j];
-    const double dy = y_i - x[3 * (size_t)j + 1];
-    const double dz = z_i - x[3 * (size_t)j + 2];
-    const double rsq = dx * dx + dy * dy + dz * dz;
-    const int t = type_i * LJ_MAXT + type[j];
-    if (rsq < tab->cutsq[t]) {
-      const T r2inv = T(1) / T(rsq);
-      const T r6inv = r2inv * r2inv * r2inv;
-      T fpair = (r6inv * (T(tab->lj1[t]) * r6inv - T(tab->lj2[t]))) * r2inv;
-      if (DIAL) fpair = dial_sum(fpair / T(tab->rep[t]), dial_trips(tab->rep[t]));
-      fxi += T(dx) * fpair;
-      fyi += T(dy) * fpair;
-      fzi += T(dz) * fpair;
-      if (!DIAL || j < n) {
-        atomicAdd(&f[3 * (size_t)j], -T(dx) * fpair);
-        atomicAdd(&f[3 * (size_t)j + 1], -T(dy) * fpair);
-        atomicAdd(&f[3 * (size_t)j + 2], -T(dz) * fpair);
+  auto sub_j = [&](int j, T vx, T vy, T vz) {
+    if (MIXED) {
+      atomicAdd(&f32[3 * (size_t)j], -(float)vx);
+      atomicAdd(&f32[3 * (size_t)j + 1], -(float)vy);
+      atomicAdd(&f32[3 * (size_t)j + 2], -(float)vz);
+    } else {
+      atomicAdd(&f[3 * (size_t)j], -(double)vx);
+      atomicAdd(&f[3 * (size_t)j + 1], -(double)vy);
+      atomicAdd(&f[3 * (size_t)j + 2], -(double)vz);
+    }
+  };
+  for (int jj = sub; jj < cnt; jj += LJH_U * LJH_K) {
+    int j[LJH_U];
+    bool ok[LJH_U];
+#pragma unroll
+    for (int u = 0; u < LJH_U; u++) {
+      ok[u] = jj + u * LJH_K < cnt;
+      j[u] = rowp[ok[u] ? jj + u * LJH_K : cnt - 1];
+    }
+    double xj[LJH_U], yj[LJH_U], zj[LJH_U];
+#pragma unroll
+    for (int u = 0; u < LJH_U; u++) {
+      xj[u] = x[3 * (size_t)j[u]];
+      yj[u] = x[3 * (size_t)j[u] + 1];
+      zj[u] = x[3 * (size_t)j[u] + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < LJH_U; u++) {
+      const double dx = x_i - xj[u];
+      const double dy = y_i - yj[u];
+      const double dz = z_i - zj[u];
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      const int t = type_i * LJ_MAXT + type[j[u]];
+      if (ok[u] && rsq < tab->cutsq[t]) {
+        T vx, vy, vz;
+        if (MIXED) {
+          float fpair = emd_lj_fpair_mixed(rsq, (float)tab->lj1[t], (float)tab->lj2[t]);
+          if (DIAL) fpair = dial_sum(fpair / (float)tab->rep[t], dial_trips(tab->rep[t]));
+          vx = (float)dx * fpair;
+          vy = (float)dy * fpair;
+          vz = (float)dz * fpair;
+        } else {
+          const double r2inv = 1.0 / rsq;
+          const double r6inv = r2inv * r2inv * r2inv;
+          double fpair = (r6inv * (tab->lj1[t] * r6inv - tab->lj2[t])) * r2inv;
+          if (DIAL) fpair = dial_sum(fpair / tab->rep[t], dial_trips(tab->rep[t]));
+          vx = dx * fpair;
+          vy = dy * fpair;
+          vz = dz * fpair;
+        }
+        fxi += vx;
+        fyi += vy;
+        fzi += vz;
+        if (!DIAL || j[u] < n) sub_j(j[u], vx, vy, vz);
       }
     }
   }
-  atomicAdd(&f[3 * (size_t)i], fxi);
-  atomicAdd(&f[3 * (size_t)i + 1], fyi);
-  atomicAdd(&f[3 * (size_t)i + 2], fzi);
+  fxi = group_sum<LJH_K>(fxi);
+  fyi = group_sum<LJH_K>(fyi);
+  fzi = group_sum<LJH_K>(fzi);
+  if (sub == 0) {
+    if (MIXED) {
+      atomicAdd(&f32[3 * (size_t)i], (float)fxi);
+      atomicAdd(&f32[3 * (size_t)i + 1], (float)fyi);
+      atomicAdd(&f32[3 * (size_t)i + 2], (float)fzi);
+    } else {
+      atomicAdd(&f[3 * (size_t)i], (double)fxi);
+      atomicAdd(&f[3 * (size_t)i + 1], (double)fyi);
+      atomicAdd(&f[3 * (size_t)i + 2], (double)fzi);
+    }
+  }
 }
 
 static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p,
@@ -641,13 +699,15 @@ static emd_status lj_idial_force(emd_ctx *ctx, emd_system *s, const emd_neigh *n
   if (lj->ntypes > 1 || half) EMD_TRY(upload_table(ctx, lj, p->intensity, &tab));
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   if (half) {
-    const int nb = emd_blocks(s->n_local, LJ_THREADS);
+    const int nb = emd_blocks((long)s->n_local * LJH_K, LJ_THREADS);
     if (precision == EMD_MIXED)
-      EMD_LAUNCH(ctx, "lj_idial_half", (k_lj_half<float, true>), nb, LJ_THREADS, 0, s->x,
-                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f32);
+      EMD_LAUNCH(ctx, "lj_idial_half", (k_lj_half<true, true>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f,
+                 s->f32);
     else
-      EMD_LAUNCH(ctx, "lj_idial_half", (k_lj_half<double, true>), nb, LJ_THREADS, 0, s->x,
-                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f);
+      EMD_LAUNCH(ctx, "lj_idial_half", (k_lj_half<false, true>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f,
+                 s->f32);
     return EMD_OK;
   }
   const int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
@@ -688,13 +748,15 @@ static emd_status lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
   if (p->ntypes > 1 || half) EMD_TRY(upload_table(ctx, p, nullptr, &tab));
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   if (half) {
-    const int nb = emd_blocks(s->n_local, LJ_THREADS);
+    const int nb = emd_blocks((long)s->n_local * LJH_K, LJ_THREADS);
     if (precision == EMD_MIXED)
-      EMD_LAUNCH(ctx, "lj_force_half", (k_lj_half<float, false>), nb, LJ_THREADS, 0, s->x,
-                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f32);
+      EMD_LAUNCH(ctx, "lj_force_half", (k_lj_half<true, false>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f,
+                 s->f32);
     else
-      EMD_LAUNCH(ctx, "lj_force_half", (k_lj_half<double, false>), nb, LJ_THREADS, 0, s->x,
-                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f);
+      EMD_LAUNCH(ctx, "lj_force_half", (k_lj_half<false, false>), nb, LJ_THREADS, 0, s->x,
+                 s->type, row, nl->num, nl->entries, nl->maxneighs, s->n_local, tab, s->f,
+                 s->f32);
     return EMD_OK;
   }
   int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
