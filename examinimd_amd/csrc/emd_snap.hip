@@ -588,7 +588,9 @@ k_snap_y(snap_dev_tables t, const double *__restrict__ cg, const ytab_entry *__r
 // thread whose run of one entry's chunks ends stores the run's sum in LDS;
 // each entry adds its runs in thread order (fixed order: deterministic).
 // The two random LDS reads per term bound the kernel (not load latency: a
-// double-buffered form with 16-byte loads measured the same).
+// double-buffered form with 16-byte loads measured the same); the host
+// orders each chunk's terms so that lanes reading in the same LDS cycle hit
+// different banks.
 constexpr int YF_THREADS = 256;
 constexpr int YF_CHUNK = 16;
 
@@ -1081,18 +1083,46 @@ static emd_status snap_build_yflat(emd_ctx *ctx, emd_snap *sn) {
   std::vector<uint32_t> code32(nslots, 0u), code64(nslots, 0u);
   std::vector<unsigned short> key(nslots, 0);
   std::vector<double> cgw(nslots, 0.0);
-  for (int th = 0; th < YF_THREADS; th++)
-    for (int c = 0; c < P; c++) {
-      const int g = th * P + c;
-      if (g >= nchunks) break;
-      const std::vector<term> &tl = eterms[crange[g].first];
-      for (int k = 0; k < YF_CHUNK && crange[g].second + k < (int)tl.size(); k++) {
-        const term &tm = tl[crange[g].second + k];
-        const size_t slot = ((size_t)c * YF_CHUNK + k) * YF_THREADS + th;
-        code32[slot] = (uint32_t)(tm.i1 * 8) | ((uint32_t)(tm.i2 * 8) << 13);
-        code64[slot] = (uint32_t)(tm.i1 * 16) | ((uint32_t)(tm.i2 * 16) << 13);
-        key[slot] = (unsigned short)tm.key;
-        cgw[slot] = tm.cgw;
+  // A chunk's terms all belong to one entry, so their order within the
+  // chunk is free (it changes rounding only).  Per chunk index and 16
+  // consecutive threads, the order is chosen step by step so that the lanes
+  // that read LDS in the same cycle hit different banks: greedily, each lane
+  // takes the remaining term whose two U entries collide least with those
+  // already placed in this step (FP32 pairs: 8 B, 16 lanes per cycle, bank
+  // pair u mod 16; FP64 pairs: 16 B, 8 lanes per cycle, u mod 8).
+  const term zero_term{0, 0, 0, 0.0};
+  for (int c = 0; c < P; c++)
+    for (int th0 = 0; th0 < YF_THREADS; th0 += 16) {
+      std::vector<term> pool[16];
+      for (int l = 0; l < 16; l++) {
+        const int g = (th0 + l) * P + c;
+        for (int k = 0; k < YF_CHUNK; k++) {
+          if (g < nchunks && crange[g].second + k < (int)eterms[crange[g].first].size())
+            pool[l].push_back(eterms[crange[g].first][crange[g].second + k]);
+          else
+            pool[l].push_back(zero_term);
+        }
+      }
+      for (int k = 0; k < YF_CHUNK; k++) {
+        int f1[16] = {0}, f2[16] = {0}, d1[2][8] = {{0}}, d2[2][8] = {{0}};
+        for (int l = 0; l < 16; l++) {
+          const int h = l / 8;
+          size_t best = 0;
+          int best_cost = 1 << 30;
+          for (size_t q = 0; q < pool[l].size(); q++) {
+            const term &tm = pool[l][q];
+            const int cost = f1[tm.i1 % 16] + f2[tm.i2 % 16] + d1[h][tm.i1 % 8] + d2[h][tm.i2 % 8];
+            if (cost < best_cost) { best_cost = cost; best = q; }
+          }
+          const term tm = pool[l][best];
+          pool[l].erase(pool[l].begin() + best);
+          f1[tm.i1 % 16]++; f2[tm.i2 % 16]++; d1[h][tm.i1 % 8]++; d2[h][tm.i2 % 8]++;
+          const size_t slot = ((size_t)c * YF_CHUNK + k) * YF_THREADS + th0 + l;
+          code32[slot] = (uint32_t)(tm.i1 * 8) | ((uint32_t)(tm.i2 * 8) << 13);
+          code64[slot] = (uint32_t)(tm.i1 * 16) | ((uint32_t)(tm.i2 * 16) << 13);
+          key[slot] = (unsigned short)tm.key;
+          cgw[slot] = tm.cgw;
+        }
       }
     }
   sn->yf_steps = steps;
