@@ -81,7 +81,7 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
           const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
           double cutsq_0, double rep_0, const lj_table *__restrict__ tab,
           const int *__restrict__ visit, int n_visit, int store, double *__restrict__ f,
-          float *__restrict__ f32) {
+          float *__restrict__ f32, int prio) {
 #pragma clang fp contract(off)
   typedef typename lj_acc<MIXED>::T T;
   const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
@@ -128,12 +128,17 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
   };
   auto gather_pairs = [&](int jj, const int (&j)[U], const bool (&ok)[U]) {
     double xj[U], yj[U], zj[U];
+    // while a wavefront issues a round's loads it runs at a raised priority,
+    // so the loads of waves about to wait start ahead of other waves'
+    // arithmetic (LJ-4M mixed: 0.693 -> 0.681 ms)
+    if (prio) __builtin_amdgcn_s_setprio(2);
 #pragma unroll
     for (int u = 0; u < U; u++) {
       xj[u] = x[3 * (size_t)j[u]];
       yj[u] = x[3 * (size_t)j[u] + 1];
       zj[u] = x[3 * (size_t)j[u] + 2];
     }
+    if (prio) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int u = 0; u < U; u++) pair(ok[u], j[u], xj[u], yj[u], zj[u]);
   };
@@ -146,6 +151,7 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
     // is gathered (x_i is in cache; the slot is masked by ok)
     int j[U];
     bool ok[U];
+    if (prio > 1) __builtin_amdgcn_s_setprio(2);
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int e = jj + u * K;
@@ -162,6 +168,7 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
   for (; jj < cnt; jj += U * K) {
     int j[U];
     bool ok[U];
+    if (prio > 1) __builtin_amdgcn_s_setprio(2);
 #pragma unroll
     for (int u = 0; u < U; u++) {
       ok[u] = jj + u * K < cnt;
@@ -613,7 +620,7 @@ static emd_status launch_full_as(emd_ctx *ctx, emd_system *s, const emd_neigh *n
   const char *name_d = DIAL ? "lj_idial_fp64" : "lj_force_fp64";
 #define LJ_FULL_ARGS                                                                        \
   s->x, s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],   \
-      p->cutsq[0][0], rep_0, tab, nl->visit, nl->n_visit, store, s->f, s->f32
+      p->cutsq[0][0], rep_0, tab, nl->visit, nl->n_visit, store, s->f, s->f32, ctx->lj_prio
   if (precision == EMD_MIXED) {
     if (one) EMD_LAUNCH(ctx, name_m, (k_lj_full<K, U, true, true, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
     else EMD_LAUNCH(ctx, name_m, (k_lj_full<K, U, false, true, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
