@@ -91,7 +91,10 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
   if (const char *e = getenv("EMD_HALO_ONE")) ctx->halo_one = atoi(e) != 0;
   // binning of a small range (<= 7168 atoms, <= 4096 bins) in one launch
   if (const char *e = getenv("EMD_BIN_ONE")) ctx->bin_one = atoi(e) != 0;
-  if (const char *e = getenv("EMD_LJ_PRIO")) ctx->lj_prio = atoi(e);
+  if (const char *e = getenv("EMD_LJ_PRIO")) {
+    const int v = atoi(e);
+    ctx->lj_prio = (v >= 0 && v <= 2) ? v : -1;
+  }
   if (const char *e = getenv("EMD_SNAP_YFLAT")) ctx->snap_yflat = atoi(e) != 0;
   if (const char *e = getenv("EMD_SNAP_UI_GRP")) ctx->snap_ui_grp = atoi(e) != 0;
   if (const char *e = getenv("EMD_SNAP_UG_WAVES")) {

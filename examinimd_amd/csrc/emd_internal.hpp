@@ -73,8 +73,9 @@ struct emd_ctx {
   int lj_tile = 0;        // use the bin-tile LJ kernel when the list carries slots
   int lj_tile_lanes = 8;  // lanes per atom of the bin-tile kernel
   int lj_unroll = 5;      // entries per lane per round, 16-lane CSR kernel
-  int lj_prio = 2;        // full-list LJ kernel: raise wave priority while a round's index
-                          // loads and gathers issue (1: gathers only, 0: never)
+  int lj_prio = -1;       // full-list LJ kernel: raise wave priority while a round's index
+                          // loads and gathers issue (2), around the gathers only (1),
+                          // never (0); -1: 2 for mixed, 0 for FP64 (as measured)
   int lj_pipe_blocks = 0;    // blocks per XCD of the pipelined LJ kernel (0 = off)
   int visit_tile = 0;     // column-tile width of the force sweep order (0 = index order)
 };

@@ -130,7 +130,8 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
     double xj[U], yj[U], zj[U];
     // while a wavefront issues a round's loads it runs at a raised priority,
     // so the loads of waves about to wait start ahead of other waves'
-    // arithmetic (LJ-4M mixed: 0.693 -> 0.681 ms)
+    // arithmetic (LJ-4M mixed: 0.693 -> 0.681 ms; FP64 0.731 -> 0.737 ms,
+    // so off there by default)
     if (prio) __builtin_amdgcn_s_setprio(2);
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -616,11 +617,12 @@ static emd_status launch_full_as(emd_ctx *ctx, emd_system *s, const emd_neigh *n
   const int nb = emd_blocks((long)(nl->visit ? nl->n_visit : n) * K, LJ_THREADS);
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   const bool one = p->ntypes == 1;
+  const int prio = ctx->lj_prio >= 0 ? ctx->lj_prio : (precision == EMD_MIXED ? 2 : 0);
   const char *name_m = DIAL ? "lj_idial_mixed" : "lj_force_mixed";
   const char *name_d = DIAL ? "lj_idial_fp64" : "lj_force_fp64";
 #define LJ_FULL_ARGS                                                                        \
   s->x, s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],   \
-      p->cutsq[0][0], rep_0, tab, nl->visit, nl->n_visit, store, s->f, s->f32, ctx->lj_prio
+      p->cutsq[0][0], rep_0, tab, nl->visit, nl->n_visit, store, s->f, s->f32, prio
   if (precision == EMD_MIXED) {
     if (one) EMD_LAUNCH(ctx, name_m, (k_lj_full<K, U, true, true, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
     else EMD_LAUNCH(ctx, name_m, (k_lj_full<K, U, false, true, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
