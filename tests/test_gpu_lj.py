@@ -604,3 +604,44 @@ def test_neighbour_list_pairs_at_the_cutoff(oracle, layout, jitter):
     # 26 neighbours are inside; of the 6 axis pairs at 2a some land on each
     # side of the FP64 test (30 per atom unjittered, 29.05 jittered)
     assert 26 * len(x) < len(oent) < 32 * len(x)
+
+
+@pytest.mark.parametrize("layout", [emd.EMD_NEIGH_CSR, emd.EMD_NEIGH_2D])
+def test_ragged_rows_dense_patch_and_isolated_atoms(oracle, layout):
+    """a dense FCC patch (2048 atoms, 8^3 cells at rho 0.8442) in one corner
+    of a 40-sigma box, 64 isolated atoms on a 5-sigma grid far from it and 8
+    dimers: rows of 0, 1, surface and bulk length, most bins empty.  Lists
+    equal the oracle's; FP64 forces match it, isolated atoms get exactly
+    zero force and each dimer equal and opposite forces"""
+    from examinimd_amd.pipeline import Pipeline, lattice
+    from test_oracle import _energy_force, oracle_csr
+    print(f"case: ragged rows, layout={layout}", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    patch = lattice("fcc", 8, a)                     # spans [0, 13.44)
+    g = np.array([18.0, 23.0, 28.0, 33.0])         # >= 4.5 from the patch, also across the wrap
+    iso = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(11)
+    iso = iso + rng.uniform(-0.3, 0.3, iso.shape)
+    d = rng.normal(0, 1, (8, 3))
+    dimer = iso[:8] + 1.1 * d / np.linalg.norm(d, axis=1, keepdims=True)
+    x = np.concatenate([patch, iso, dimer])
+    box, cut = 40.0, 2.8
+    orow, oent, _ = oracle_csr(oracle, x, box, cut)
+    pl = Pipeline()
+    pl.load(x, (box,) * 3)
+    pl.build(cut, layout=layout)
+    row, num, ent = emd.download_neigh(pl.ctx.h, pl.nl)
+    f64 = pl.lj(emd.EMD_FP64)
+    f32 = pl.lj(emd.EMD_MIXED)
+    pl.close()
+    assert np.array_equal(np.asarray(row, np.int64), orow.astype(np.int64))
+    assert np.array_equal(ent, oent)
+    lens = np.diff(orow)
+    assert lens.min() == 0 and (lens == 1).sum() == 16 and lens.max() > 70
+    E, F = _energy_force(oracle, x, box, "lj")
+    assert rel_err(f64, F) <= 1e-13
+    np_, ni = len(patch), len(iso)
+    assert not f64[np_ + 8:np_ + ni].any() and not f32[np_ + 8:np_ + ni].any()
+    assert np.array_equal(f64[np_:np_ + 8], -f64[np_ + ni:])
+    df = f32 - f64
+    assert float(np.sqrt(np.mean(np.sum(df * df, axis=1)))) <= 1e-5
