@@ -304,3 +304,42 @@ def test_snap_force_zeroed_equals_zero_then_force(prec):
     pl.close()
     assert np.isfinite(got).all()
     assert rel_err(got, want) <= 1e-12 if prec == emd.EMD_FP64 else rel_err(got, want) <= 1e-5
+
+
+def test_snap_ragged_rows_patch_and_isolated_atoms(oracle):
+    """a BCC W patch (432 atoms) in one corner of a 70 A box, 64 isolated
+    atoms on a 10 A grid and 8 dimers (2.5 A): atoms with no neighbour, one
+    neighbour, surface and bulk rows.  FP64 forces and energy match the
+    oracle; isolated atoms get exactly zero force in both precisions"""
+    from examinimd_amd.pipeline import Pipeline, lattice
+    from test_oracle import _energy_force
+    print("case: SNAP ragged rows", flush=True)
+    beta = list(decks.snap_beta(seed=2024, sigma=0.05))
+    patch = lattice("bcc", 6, 3.1803)                 # spans [0, 19.1)
+    g = np.array([28.0, 38.0, 48.0, 58.0])
+    iso = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(12)
+    iso = iso + rng.uniform(-0.3, 0.3, iso.shape)
+    d = rng.normal(0, 1, (8, 3))
+    dimer = iso[:8] + 2.5 * d / np.linalg.norm(d, axis=1, keepdims=True)
+    x = np.concatenate([patch, iso, dimer])
+    box = 70.0
+    pl = Pipeline()
+    pl.load(x, (box,) * 3, mass=183.84)
+    pl.build(4.73442 + 1.0, newton=1)
+    pl.snap_setup(beta)
+    f64 = pl.snap_force(emd.EMD_FP64)
+    e64 = pl.snap_energy()
+    f32 = pl.snap_force(emd.EMD_MIXED)
+    pl.close()
+    E, F = _energy_force(oracle, x, box, "snap", snap=oracle.snap_params(beta), cut=5.73442)
+    print("force rel err", rel_err(f64, F), "energy", e64, E)
+    assert rel_err(f64, F) <= 1e-11
+    assert abs(e64 - E) <= 1e-11 * abs(E)
+    np_, ni = len(patch), len(iso)
+    assert not f64[np_ + 8:np_ + ni].any() and not f32[np_ + 8:np_ + ni].any()
+    fd = f64[np_:np_ + 8]
+    assert np.abs(fd).min() > 0
+    assert np.abs(fd + f64[np_ + ni:]).max() <= 1e-12 * np.abs(fd).max()
+    df = f32 - f64
+    assert float(np.sqrt(np.mean(np.sum(df * df, axis=1)))) <= 1e-4
