@@ -33,19 +33,25 @@ class Pipeline:
     def chk(self, st, what):
         self.ctx.check(st, what)
 
-    def load(self, x, box, mass=1.0, v=None):
+    def load(self, x, box, mass=1.0, v=None, types=None):
+        """types: 0-based type per atom (default all 0); every type gets
+        the same mass"""
         self._ids = None
         n = len(x)
         x = np.ascontiguousarray(x, dtype=np.float64)
         v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float64)
         ids = np.arange(1, n + 1, dtype=np.int32)
-        types = np.zeros(n, np.int32)
+        types = (np.zeros(n, np.int32) if types is None
+                 else np.ascontiguousarray(types, dtype=np.int32))
+        ntypes = int(types.max()) + 1 if n else 1
+        if types.shape != (n,) or (n and types.min() < 0) or ntypes > 16:
+            raise ValueError("types: one value in [0, 16) per atom")
         q = np.zeros(n)
         s = self.s
         self.chk(self.L.emd_system_upload(self.ctx.h, C.byref(s), n, _d(x), _d(v), _i(types),
                                           _i(ids), _d(q)), "upload")
-        m = np.array([mass], dtype=np.float64)
-        self.chk(self.L.emd_system_set_mass(self.ctx.h, C.byref(s), 1, _d(m)), "mass")
+        m = np.full(ntypes, mass, dtype=np.float64)
+        self.chk(self.L.emd_system_set_mass(self.ctx.h, C.byref(s), ntypes, _d(m)), "mass")
         s.n_local = n
         s.n_ghost = 0
         s.n_global = n

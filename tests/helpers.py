@@ -118,3 +118,19 @@ def oracle_csr_from(oracle, x, ids, box, cut=2.8):
                  (L.or_csr_free, c)):
         f(C.byref(o))
     return row, ent, out_ids
+
+
+def pair_forces_typed(x, box, types, lj1, lj2, cutsq):
+    """brute-force minimum-image LJ forces with per-type-pair coefficients
+    and the strict cutoff test rsq < cutsq[ti][tj]
+    (force_lj_neigh_impl.h:161-206)"""
+    d = x[:, None, :] - x[None, :, :]
+    d -= box * np.round(d / box)
+    rsq = np.sum(d * d, axis=2)
+    ti, tj = types[:, None], types[None, :]
+    m = (rsq < cutsq[ti, tj]) & ~np.eye(len(x), dtype=bool)
+    r2inv = np.where(m, 1.0 / np.where(m, rsq, 1.0), 0.0)
+    r6inv = r2inv ** 3
+    fpair = r6inv * (lj1[ti, tj] * r6inv - lj2[ti, tj]) * r2inv
+    return np.sum(d * fpair[:, :, None], axis=1)
+

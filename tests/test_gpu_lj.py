@@ -15,7 +15,7 @@ import pytest
 
 import examinimd_amd as emd
 from examinimd_amd import decks
-from helpers import rel_err
+from helpers import pair_forces_typed, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -644,4 +644,45 @@ def test_ragged_rows_dense_patch_and_isolated_atoms(oracle, layout):
     assert not f64[np_ + 8:np_ + ni].any() and not f32[np_ + 8:np_ + ni].any()
     assert np.array_equal(f64[np_:np_ + 8], -f64[np_ + ni:])
     df = f32 - f64
+    assert float(np.sqrt(np.mean(np.sum(df * df, axis=1)))) <= 1e-5
+
+
+@pytest.mark.parametrize("layout", [emd.EMD_NEIGH_CSR, emd.EMD_NEIGH_2D])
+def test_lj_two_types_against_brute_force(layout):
+    """two atom types with different eps, sigma and cutoff per type pair
+    (the per-pair table path of the LJ kernel): FP64 forces against a
+    brute-force pair sum, FP32 pair arithmetic within the mixed bound"""
+    import ctypes as C
+    from examinimd_amd.abi import EmdLJParams
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print(f"case: LJ two types, layout {layout}", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    nc = 6
+    rng = np.random.default_rng(21)
+    x = lattice("fcc", nc, a) + rng.normal(0, 0.05, (4 * nc ** 3, 3))
+    box = nc * a
+    x %= box
+    types = rng.integers(0, 2, len(x)).astype(np.int32)
+    eps = np.array([[1.0, 0.7], [0.7, 1.3]])
+    sig = np.array([[1.0, 1.1], [1.1, 0.9]])
+    cut = np.array([[2.5, 2.7], [2.7, 2.3]])
+    lj1, lj2, cutsq = 48.0 * eps * sig ** 12, 24.0 * eps * sig ** 6, cut * cut
+    F = pair_forces_typed(x, box, types, lj1, lj2, cutsq)
+    pl = Pipeline()
+    pl.load(x, (box,) * 3, types=types)
+    pl.build(2.8, layout=layout)
+    p = EmdLJParams()
+    p.ntypes = 2
+    for t1 in range(2):
+        for t2 in range(2):
+            p.lj1[t1][t2], p.lj2[t1][t2], p.cutsq[t1][t2] = lj1[t1, t2], lj2[t1, t2], cutsq[t1, t2]
+    L, c, s = pl.L, pl.ctx.h, C.byref(pl.s)
+    out = {}
+    for prec in (emd.EMD_FP64, emd.EMD_MIXED):
+        pl.chk(L.emd_force_zero(c, s, prec), "zero")
+        pl.chk(L.emd_lj_force(c, s, C.byref(pl.nl), C.byref(p), prec, 0), "lj")
+        out[prec] = pl.forces(prec)
+    pl.close()
+    assert rel_err(out[emd.EMD_FP64], F) <= 1e-12
+    df = out[emd.EMD_MIXED] - out[emd.EMD_FP64]
     assert float(np.sqrt(np.mean(np.sum(df * df, axis=1)))) <= 1e-5

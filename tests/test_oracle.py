@@ -601,3 +601,19 @@ def test_dump_layout_round_trip(oracle, tmp_path):
     out = (C.c_double * 6)()
     assert oracle.lib().or_dump_compare(C.byref(m.system), p.encode(), out) == 0
     assert max(out) == 0.0
+
+
+def test_typed_brute_force_with_one_type_equals_oracle(oracle):
+    """the per-type-pair brute-force LJ sum that checks the GPU's two-type
+    path reduces, with every pair given eps = sigma = 1 and rc 2.5, to the
+    oracle's LJ forces on a displaced FCC lattice"""
+    from examinimd_amd.pipeline import lattice
+    from helpers import pair_forces_typed, rel_err
+    a = (4 / 0.8442) ** (1 / 3)
+    x = lattice("fcc", 6, a) + np.random.default_rng(21).normal(0, 0.05, (864, 3))
+    box = 6 * a
+    x %= box
+    ones = np.ones((2, 2))
+    F = pair_forces_typed(x, box, np.zeros(864, np.int32), 48 * ones, 24 * ones, 6.25 * ones)
+    _, Fo = _energy_force(oracle, x, box, "lj")
+    assert rel_err(F, Fo) <= 1e-13
