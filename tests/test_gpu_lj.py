@@ -647,15 +647,17 @@ def test_ragged_rows_dense_patch_and_isolated_atoms(oracle, layout):
     assert float(np.sqrt(np.mean(np.sum(df * df, axis=1)))) <= 1e-5
 
 
-@pytest.mark.parametrize("layout", [emd.EMD_NEIGH_CSR, emd.EMD_NEIGH_2D])
-def test_lj_two_types_against_brute_force(layout):
+@pytest.mark.parametrize("layout,half", [(emd.EMD_NEIGH_CSR, 0), (emd.EMD_NEIGH_2D, 0),
+                                         (emd.EMD_NEIGH_CSR, 1)])
+def test_lj_two_types_against_brute_force(layout, half):
     """two atom types with different eps, sigma and cutoff per type pair
-    (the per-pair table path of the LJ kernel): FP64 forces against a
-    brute-force pair sum, FP32 pair arithmetic within the mixed bound"""
+    (the per-pair table path of the full- and half-list LJ kernels, the half
+    list with newton off): FP64 forces against a brute-force pair sum, FP32
+    pair arithmetic within the mixed bound"""
     import ctypes as C
     from examinimd_amd.abi import EmdLJParams
     from examinimd_amd.pipeline import Pipeline, lattice
-    print(f"case: LJ two types, layout {layout}", flush=True)
+    print(f"case: LJ two types, layout {layout}, half {half}", flush=True)
     a = (4 / 0.8442) ** (1 / 3)
     nc = 6
     rng = np.random.default_rng(21)
@@ -670,7 +672,7 @@ def test_lj_two_types_against_brute_force(layout):
     F = pair_forces_typed(x, box, types, lj1, lj2, cutsq)
     pl = Pipeline()
     pl.load(x, (box,) * 3, types=types)
-    pl.build(2.8, layout=layout)
+    pl.build(2.8, half=half, layout=layout)
     p = EmdLJParams()
     p.ntypes = 2
     for t1 in range(2):
@@ -680,7 +682,7 @@ def test_lj_two_types_against_brute_force(layout):
     out = {}
     for prec in (emd.EMD_FP64, emd.EMD_MIXED):
         pl.chk(L.emd_force_zero(c, s, prec), "zero")
-        pl.chk(L.emd_lj_force(c, s, C.byref(pl.nl), C.byref(p), prec, 0), "lj")
+        pl.chk(L.emd_lj_force(c, s, C.byref(pl.nl), C.byref(p), prec, half), "lj")
         out[prec] = pl.forces(prec)
     pl.close()
     assert rel_err(out[emd.EMD_FP64], F) <= 1e-12
