@@ -44,6 +44,48 @@ def test_jenkins_one_at_a_time_vectors(oracle, text, expect):
     assert _oat_final(h) == expect
 
 
+_PM_M = 2 ** 31 - 1
+_PM_INV = pow(16807, -1, _PM_M)   # inverse of the Park-Miller multiplier
+
+
+def _unwarm(seed, draws=5):
+    """Park-Miller state before `draws` steps (exact modular inverse)"""
+    for _ in range(draws):
+        seed = seed * _PM_INV % _PM_M
+    return seed
+
+
+def test_geom_seed_mask_is_27_bits(oracle):
+    """reset() (src/input.h:97-132) keeps 27 bits of the hash as the seed.
+    Over the 32,000 sites of the 20^3 FCC deck every recovered seed is below
+    2^27 and the largest reaches 2^26 (a narrower or wider mask fails one of
+    the two), and the seed equals the masked, avalanched one-at-a-time hash
+    of the 4 seed bytes followed by the 24 coordinate bytes (the hash core and
+    avalanche are pinned by the published vectors above).  Whether the bytes
+    are hashed signed or unsigned is fixed only by restatement: `char` is
+    signed on x86-64, as the reference's hash reads it."""
+    L = oracle.lib()
+    a = (4.0 / 0.8442) ** (1.0 / 3.0)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    n = 20
+    cells = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1)
+    sites = (a * (cells.reshape(-1, 1, 3)[:, :, ::-1] + basis[None])).reshape(-1, 3)
+    seed_base = 87287
+    seeds = np.empty(len(sites), np.int64)
+    xyz = (C.c_double * 3)()
+    for k, p in enumerate(sites):
+        xyz[0], xyz[1], xyz[2] = p
+        seeds[k] = _unwarm(L.or_rng_reset(seed_base, xyz))
+    assert len(seeds) == 32000
+    assert seeds.max() < 2 ** 27 and seeds.max() >= 2 ** 26
+    assert seeds.min() >= 1
+    base = np.array([seed_base], np.int32).tobytes()
+    for k in range(0, len(sites), 997):
+        h = L.or_hash_bytes(base, 4, 0)
+        h = L.or_hash_bytes(sites[k].astype(np.float64).tobytes(), 24, h)
+        assert _oat_final(h) & 0x7FFFFFF == seeds[k]
+
+
 def test_gaussian_moments(oracle):
     L = oracle.lib()
     s = C.c_int(12345)
@@ -190,11 +232,38 @@ def test_neighbour_counts_brute_force(oracle, cut):
     brute = np.zeros(s.N_local, np.int32)
     L.or_brute_neigh_count(C.byref(s), cut, brute.ctypes.data_as(C.POINTER(C.c_int)))
     assert np.array_equal(np.diff(row), brute)
+    # exact sets: row i holds every j in [0, N + G) with |x_i - x_j|^2 <= cut^2
+    n_all = s.N_local + s.N_ghost
+    x = oracle.arr(s.x, 3 * n_all).reshape(-1, 3)
+    for i in range(s.N_local):
+        d2 = ((x - x[i]) ** 2).sum(1)
+        d2[i] = np.inf
+        assert np.array_equal(np.sort(ent[row[i]:row[i + 1]]), np.flatnonzero(d2 <= cut * cut))
     # half list (newton on): every unordered local/ghost pair exactly once
     s2 = _sc_system(oracle, 6, 1.0, jitter=0.15, seed=3)
     h2, b2, c2 = _build(oracle, s2, cut, half=1, newton=1)
     row2, ent2 = oracle.csr_arrays(c2)
     assert row2[-1] * 2 == row[-1]
+
+
+@pytest.mark.parametrize("half,newton", [(0, 0), (1, 1), (1, 0)])
+def test_neigh_2d_rows_equal_csr_rows(oracle, half, newton):
+    """Neighbor2D (src/neighbor_types/neighbor_2d.h:174-331) keeps the rows of
+    NeighborCSR (neighbor_csr.h:170-435) in padded storage: same entries in
+    the same order, row lengths below maxneighs"""
+    L = oracle.lib()
+    s = _sc_system(oracle, 7, 1.0, jitter=0.2, seed=11)
+    h, b, c = _build(oracle, s, 2.1, half=half, newton=newton)
+    row, ent = oracle.csr_arrays(c)
+    l2 = oracle.List2D()
+    L.or_neigh_2d(C.byref(s), C.byref(b), 2.1, half, newton, C.byref(l2))
+    num = oracle.arr(l2.num, l2.n, np.int32)
+    pad = oracle.arr(l2.neighs, l2.n * l2.maxneighs, np.int32).reshape(l2.n, l2.maxneighs)
+    assert l2.n == s.N_local and num.max() <= l2.maxneighs
+    assert np.array_equal(num, np.diff(row))
+    for i in range(l2.n):
+        assert np.array_equal(pad[i, :num[i]], ent[row[i]:row[i + 1]])
+    L.or_list2d_free(C.byref(l2))
 
 
 # --------------------------------------------------------------------- LJ
@@ -482,9 +551,9 @@ def test_cg_against_sympy(oracle):
     g = json.load(open(os.path.join(GOLDEN, "cg_2j8.json")))
     L = oracle.lib()
     err = 0.0
-    for j1, j2, j, k1, k2, c in g["cg"][::7]:
+    for j1, j2, j, k1, k2, c in g["cg"]:
         err = max(err, abs(L.or_snap_cg(8, j1, j2, j, k1, k2) - c))
-    assert err < 1e-12
+    assert len(g["cg"]) > 4000 and err < 1e-12
 
 
 def test_snap_ncoeff(oracle):
