@@ -2,6 +2,7 @@
 // (src/input.cpp:44-792).
 #include "input.h"
 
+#include <array>
 #include <cmath>
 #include <fstream>
 #include <iostream>
@@ -102,7 +103,7 @@ void LAMMPS_RandomVelocityGeom::reset(int ibase, double* coord) {
 }
 
 // ---------------------------------------------------------------- Input
-Input::Input(System* p) : timestepflag(false), system(p) {
+Input::Input(System* p) : system(p) {
   input_file = nullptr;
   input_file_type = -1;
   units = UNITS_LJ;
@@ -131,95 +132,126 @@ Input::Input(System* p) : timestepflag(false), system(p) {
   correctnessflag = false;
   dumpbinary_path = reference_path = correctness_file = nullptr;
   restart_file = nullptr;
+  dumpneigh_rate = 0;
+  dumpneigh_path = nullptr;
   comm_newton = 0;
   precision = EMD_FP64;
   device = 0;
+  timestep_given = false;
 }
 
-// Input::read_command_line_args, src/input.cpp:151-224
+// --------------------------------------------------------- command line
+// Options of the reference driver (src/input.cpp:151-224: -il, the module
+// selectors, --dumpbinary, --correctness; --kokkos-* is passed through) and
+// this build's own (--precision, --device, --restart).
+namespace {
+struct CliOption {
+  const char* spelling;
+  const char* meaning;
+};
+const CliOption kCliOptions[] = {
+    {"-il FILE | --input-lammps FILE", "LAMMPS-style input deck"},
+    {"--force-iteration NEIGH_FULL|NEIGH_HALF", "full or half neighbour list for the pair force"},
+    {"--neigh-type 2D|CSR", "neighbour-list layout (padded rows or compressed rows)"},
+    {"--comm-type SERIAL", "periodic images on one GPU"},
+    {"--dumpbinary N DIR", "write DIR/output.<step>.<rank> every N steps"},
+    {"--correctness N DIR FILE", "every N steps compare with DIR/output.<step>.<rank>, "
+                                 "append the differences to FILE"},
+    {"--dumpneigh N DIR", "write the neighbour list, canonically ordered, to "
+                          "DIR/neigh.<step>.<rank> every N steps"},
+    {"--precision fp64|mixed", "force arithmetic: FP64, or FP32 pair math with FP64 state"},
+    {"--device N", "GPU ordinal"},
+    {"--restart FILE", "initial atoms from a --dumpbinary file instead of the lattice"},
+};
+
+void print_usage() {
+  printf("ExaMiniMD on HIP (gfx950)\nusage: ExaMiniMD -il <deck> [options]\n");
+  for (const CliOption& o : kCliOptions) printf("  %-42s %s\n", o.spelling, o.meaning);
+}
+
+[[noreturn]] void cli_fail(const char* what, const char* arg) {
+  fprintf(stderr, "ExaMiniMD: %s%s%s\n", what, arg ? ": " : "", arg ? arg : "");
+  exit(1);
+}
+}  // namespace
+
 void Input::read_command_line_args(int argc, char* argv[]) {
+  // the module headers' MODULES_OPTION_CHECK sections read argv[i+1]
 #define MODULES_OPTION_CHECK
   for (int i = 1; i < argc; i++) {
-    if ((strcmp(argv[i], "-h") == 0) || (strcmp(argv[i], "--help") == 0)) {
-      if (system->do_print) {
-        printf("ExaMiniMD 1.0 (MI355X / HIP version)\n\n");
-        printf("Options:\n");
-        printf("  -il [file] / --input-lammps [FILE]: Provide LAMMPS input file\n");
-        printf("  --force-iteration [TYPE]:   Specify which iteration style to use\n");
-        printf("                              for force calculations (NEIGH_FULL, NEIGH_HALF)\n");
-        printf("  --comm-type [TYPE]:         Specify Communication Routines implementation \n");
-        printf("                              (SERIAL)\n");
-        printf("  --dumpbinary [N] [PATH]:    Request that binary output files PATH/output* be generated every N steps\n");
-        printf("  --correctness [N] [PATH] [FILE]:   Request that correctness check against files PATH/output* be performed every N steps, correctness data written to FILE\n");
-        printf("  --neigh-type [TYPE]:        Specify Neighbor Routines implementation \n");
-        printf("                              (2D, CSR)\n");
-        printf("  --precision [fp64|mixed]:   force arithmetic (mixed: FP32 pair math and\n");
-        printf("                              accumulation, FP64 state)\n");
-        printf("  --device [N]:               GPU ordinal\n");
-        printf("  --restart [FILE]:           start from a binary dump FILE (the --dumpbinary\n");
-        printf("                              format) instead of the deck's lattice\n");
-      }
-    } else if ((strcmp(argv[i], "-il") == 0) || (strcmp(argv[i], "--input-lammps") == 0)) {
+    const char* opt = argv[i];
+    auto is = [opt](const char* a, const char* b = nullptr) {
+      return strcmp(opt, a) == 0 || (b && strcmp(opt, b) == 0);
+    };
+    auto want = [&](int k) {
+      if (i + k >= argc) cli_fail("option needs more arguments", opt);
+    };
+    if (is("-h", "--help")) {
+      if (system->do_print) print_usage();
+    } else if (is("-il", "--input-lammps")) {
+      want(1);
       input_file = argv[++i];
       input_file_type = INPUT_LAMMPS;
-    } else if ((strcmp(argv[i], "--force-iteration") == 0)) {
+    } else if (is("--force-iteration")) {
+      want(1);
 #include "modules_force.h"
       ++i;
-    } else if ((strcmp(argv[i], "--comm-type") == 0)) {
+    } else if (is("--comm-type")) {
+      want(1);
 #include "modules_comm.h"
       ++i;
-    } else if ((strcmp(argv[i], "--neigh-type") == 0)) {
+    } else if (is("--neigh-type")) {
+      want(1);
 #include "modules_neighbor.h"
       ++i;
-    } else if ((strcmp(argv[i], "--dumpbinary") == 0)) {
-      dumpbinary_rate = atoi(argv[i + 1]);
-      dumpbinary_path = argv[i + 2];
+    } else if (is("--dumpbinary")) {
+      want(2);
       dumpbinaryflag = true;
-      i += 2;
-    } else if ((strcmp(argv[i], "--correctness") == 0)) {
-      correctness_rate = atoi(argv[i + 1]);
-      reference_path = argv[i + 2];
-      correctness_file = argv[i + 3];
+      dumpbinary_rate = atoi(argv[++i]);
+      dumpbinary_path = argv[++i];
+    } else if (is("--correctness")) {
+      want(3);
       correctnessflag = true;
-      i += 3;
-    } else if ((strcmp(argv[i], "--precision") == 0)) {
+      correctness_rate = atoi(argv[++i]);
+      reference_path = argv[++i];
+      correctness_file = argv[++i];
+    } else if (is("--dumpneigh")) {
+      want(2);
+      dumpneigh_rate = atoi(argv[++i]);
+      dumpneigh_path = argv[++i];
+      if (dumpneigh_rate <= 0) cli_fail("--dumpneigh needs a positive step interval", nullptr);
+    } else if (is("--precision")) {
+      want(1);
       const char* p = argv[++i];
       if (strcmp(p, "mixed") == 0) precision = EMD_MIXED;
       else if (strcmp(p, "fp64") == 0) precision = EMD_FP64;
-      else {
-        printf("ERROR: --precision takes fp64 or mixed\n");
-        exit(1);
-      }
-    } else if ((strcmp(argv[i], "--device") == 0)) {
+      else cli_fail("--precision takes fp64 or mixed, not", p);
+    } else if (is("--device")) {
+      want(1);
       device = atoi(argv[++i]);
-    } else if ((strcmp(argv[i], "--restart") == 0)) {
+    } else if (is("--restart")) {
+      want(1);
       restart_file = argv[++i];
-    } else if ((strstr(argv[i], "--kokkos-") == NULL)) {
-      if (system->do_print) printf("ERROR: Unknown command line argument: %s\n", argv[i]);
-      exit(1);
+    } else if (strncmp(opt, "--kokkos-", 9) != 0) {
+      cli_fail("unrecognised option", opt);
     }
   }
 #undef MODULES_OPTION_CHECK
 }
 
+// ---------------------------------------------------------------- deck
 void Input::read_file(const char* filename) {
   if (filename == NULL) filename = input_file;
-  if (input_file_type == INPUT_LAMMPS) {
-    read_lammps_file(filename);
-    return;
-  }
-  if (system->do_print) printf("ERROR: Unknown input file type\n");
-  exit(1);
+  if (input_file_type != INPUT_LAMMPS || filename == NULL)
+    cli_fail("no input deck given (use -il FILE)", nullptr);
+  read_lammps_file(filename);
 }
 
 void Input::read_lammps_file(const char* filename) {
   std::ifstream file(filename);
-  if (!file.good()) {
-    printf("ERROR: cannot open input file %s\n", filename ? filename : "(null)");
-    exit(1);
-  }
-  std::string line;
-  while (std::getline(file, line)) input_data.add_line(line.c_str());
+  if (!file.good()) cli_fail("cannot read the input deck", filename);
+  std::string text;
+  while (std::getline(file, text)) input_data.add_line(text.c_str());
   input_data.finalize();
   if (system->do_print) {
     printf("\n#InputFile:\n#=========================================================\n");
@@ -229,173 +261,279 @@ void Input::read_lammps_file(const char* filename) {
   for (int l = 0; l < input_data.nlines; l++) check_lammps_command(l);
 }
 
-// Input::check_lammps_command, src/input.cpp:265-458
+// The deck subset of src/input.cpp:265-458, as a table of keyword handlers.
+// A handler returns true when it understood its line; a line that no handler
+// understood is reported and otherwise ignored, as in the reference.  Each
+// handler states the rule it keeps from the reference.
+namespace {
+using DeckWords = char**;
+struct DeckKeyword {
+  const char* keyword;
+  bool (*handle)(Input& in, DeckWords w, int line);
+};
+
+bool say(Input& in, const char* fmt, const char* word) {
+  if (in.system->do_print) printf(fmt, word);
+  return false;
+}
+
+bool deck_units(Input& in, DeckWords w, int) {
+  System& s = *in.system;
+  // metal always resets dt; real and lj only when no timestep was given
+  if (strcmp(w[1], "lj") == 0) {
+    in.units = UNITS_LJ;
+    s.boltz = 1.0;
+    s.mvv2e = 1.0;
+    if (!in.timestep_given) s.dt = 0.005;
+  } else if (strcmp(w[1], "real") == 0) {
+    in.units = UNITS_REAL;
+    s.boltz = 0.0019872067;
+    s.mvv2e = 48.88821291 * 48.88821291;
+    if (!in.timestep_given) s.dt = 1.0;
+  } else if (strcmp(w[1], "metal") == 0) {
+    in.units = UNITS_METAL;
+    s.boltz = 8.617343e-5;
+    s.mvv2e = 1.0364269e-4;
+    s.dt = 0.001;
+  } else {
+    return say(in, "deck: units '%s' not available (lj, real, metal)\n", w[1]);
+  }
+  return true;
+}
+
+bool deck_atom_style(Input& in, DeckWords w, int) {
+  if (strcmp(w[1], "atomic") == 0) return true;
+  return say(in, "deck: atom_style '%s' not available (atomic)\n", w[1]);
+}
+
+bool deck_lattice(Input& in, DeckWords w, int) {
+  bool ok = true;
+  const double arg = atof(w[2]);
+  if (strcmp(w[1], "fcc") == 0) {
+    in.lattice_style = LATTICE_FCC;
+    in.lattice_constant = std::pow((4.0 / arg), (1.0 / 3.0));  // the argument is a density
+  } else if (strcmp(w[1], "sc") == 0 || strcmp(w[1], "bcc") == 0) {
+    in.lattice_style = w[1][0] == 's' ? LATTICE_SC : LATTICE_BCC;
+    in.lattice_constant = arg;  // the argument is the cell edge
+  } else {
+    ok = say(in, "deck: lattice '%s' not available (sc, fcc, bcc)\n", w[1]);
+  }
+  if (strcmp(w[3], "origin") == 0) {
+    in.lattice_offset_x = atof(w[4]);
+    in.lattice_offset_y = atof(w[5]);
+    in.lattice_offset_z = atof(w[6]);
+  }
+  return ok;
+}
+
+bool deck_region(Input& in, DeckWords w, int) {
+  if (strcmp(w[2], "block") != 0)
+    return say(in, "deck: region style '%s' not available (block)\n", w[2]);
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = atoi(w[3 + 2 * d]);
+    hi[d] = atoi(w[4 + 2 * d]);
+  }
+  if (lo[0] || lo[1] || lo[2]) say(in, "deck: region %s must start at 0 0 0\n", w[1]);
+  in.lattice_nx = hi[0];
+  in.lattice_ny = hi[1];
+  in.lattice_nz = hi[2];
+  return true;
+}
+
+bool deck_create_box(Input& in, DeckWords w, int) {
+  in.system->ntypes = atoi(w[1]);
+  in.system->mass.assign(in.system->ntypes, 0.0);
+  return true;
+}
+
+bool deck_mass(Input& in, DeckWords w, int) {
+  const int t = atoi(w[1]) - 1;
+  if (t >= 0 && t < (int)in.system->mass.size()) in.system->mass[t] = atof(w[2]);
+  return true;
+}
+
+bool deck_pair_style(Input& in, DeckWords w, int line) {
+  struct { const char* style; int type; } styles[] = {
+      {"lj/cut", FORCE_LJ}, {"lj/cut/idial", FORCE_LJ_IDIAL}, {"snap", FORCE_SNAP}};
+  for (auto& st : styles)
+    if (strcmp(w[1], st.style) == 0) {
+      in.force_type = st.type;
+      // the reference fixes the SNAP neighbour cutoff at 4.73442 whatever
+      // the deck says (src/input.cpp:378-383)
+      in.force_cutoff = st.type == FORCE_SNAP ? 4.73442 : atof(w[2]);
+      in.force_line = line;
+      return true;
+    }
+  return say(in, "deck: pair_style '%s' not available (lj/cut, lj/cut/idial, snap)\n", w[1]);
+}
+
+bool deck_pair_coeff(Input& in, DeckWords, int line) {
+  in.force_coeff_lines.push_back(line);  // handed to Force::init_coeff after creation
+  return true;
+}
+
+bool deck_velocity(Input& in, DeckWords w, int) {
+  if (strcmp(w[1], "all") != 0) say(in, "deck: velocity applies to group 'all' only, not '%s'\n", w[1]);
+  if (strcmp(w[2], "create") != 0) say(in, "deck: velocity supports 'create' only, not '%s'\n", w[2]);
+  in.temperature_target = atof(w[3]);
+  in.temperature_seed = atoi(w[4]);
+  for (int k = 5; k + 1 < ItemizedFile::words_per_line; k++)
+    if (strcmp(w[k], "dist") == 0) in.velocity_gaussian = strcmp(w[k + 1], "gaussian") == 0;
+  return true;
+}
+
+bool deck_neighbor(Input& in, DeckWords w, int) {
+  in.neighbor_skin = atof(w[1]);
+  return true;
+}
+
+bool deck_neigh_modify(Input& in, DeckWords w, int) {
+  for (int k = 1; k + 1 < ItemizedFile::words_per_line; k++)
+    if (strcmp(w[k], "every") == 0) in.comm_exchange_rate = atoi(w[k + 1]);
+  return true;
+}
+
+bool deck_fix(Input& in, DeckWords w, int) {
+  if (strcmp(w[3], "nve") != 0) return say(in, "deck: fix style '%s' not available (nve)\n", w[3]);
+  in.integrator_type = INTEGRATOR_NVE;
+  return true;
+}
+
+bool deck_run(Input& in, DeckWords w, int) {
+  in.nsteps = atoi(w[1]);
+  return true;
+}
+
+bool deck_thermo(Input& in, DeckWords w, int) {
+  in.thermo_rate = atoi(w[1]);
+  return true;
+}
+
+bool deck_timestep(Input& in, DeckWords w, int) {
+  in.system->dt = atof(w[1]);
+  in.timestep_given = true;
+  return true;
+}
+
+bool deck_newton(Input& in, DeckWords w, int) {
+  if (strcmp(w[1], "on") == 0) in.comm_newton = 1;
+  else if (strcmp(w[1], "off") == 0) in.comm_newton = 0;
+  else say(in, "deck: newton takes on or off, not '%s'\n", w[1]);
+  return true;
+}
+
+bool deck_accept(Input&, DeckWords, int) { return true; }
+
+bool deck_variable(Input& in, DeckWords w, int) {
+  return say(in, "deck: '%s' (deck variables) is not supported\n", w[0]);
+}
+
+const DeckKeyword kDeckKeywords[] = {
+    {"units", deck_units},         {"atom_style", deck_atom_style},
+    {"lattice", deck_lattice},     {"region", deck_region},
+    {"create_box", deck_create_box}, {"create_atoms", deck_accept},
+    {"mass", deck_mass},           {"pair_style", deck_pair_style},
+    {"pair_coeff", deck_pair_coeff}, {"velocity", deck_velocity},
+    {"neighbor", deck_neighbor},   {"neigh_modify", deck_neigh_modify},
+    {"fix", deck_fix},             {"run", deck_run},
+    {"thermo", deck_thermo},       {"timestep", deck_timestep},
+    {"newton", deck_newton},       {"variable", deck_variable},
+};
+}  // namespace
+
 void Input::check_lammps_command(int line) {
-  bool known = false;
-  char** w = input_data.words[line];
-  if (w[0][0] == 0) known = true;
-  if (strstr(w[0], "#")) known = true;
-  if (strcmp(w[0], "variable") == 0) {
-    if (system->do_print) printf("LAMMPS-Command: 'variable' keyword is not supported in ExaMiniMD\n");
-  }
-  if (strcmp(w[0], "units") == 0) {
-    if (strcmp(w[1], "metal") == 0) {
-      known = true;
-      units = UNITS_METAL;
-      system->boltz = 8.617343e-5;
-      system->mvv2e = 1.0364269e-4;
-      system->dt = 0.001;
-    } else if (strcmp(w[1], "real") == 0) {
-      known = true;
-      units = UNITS_REAL;
-      system->boltz = 0.0019872067;
-      system->mvv2e = 48.88821291 * 48.88821291;
-      if (!timestepflag) system->dt = 1.0;
-    } else if (strcmp(w[1], "lj") == 0) {
-      known = true;
-      units = UNITS_LJ;
-      system->boltz = 1.0;
-      system->mvv2e = 1.0;
-      if (!timestepflag) system->dt = 0.005;
-    } else if (system->do_print) {
-      printf("LAMMPS-Command: 'units' command only supports 'real' and 'lj' in ExaMiniMD\n");
+  DeckWords w = input_data.words[line];
+  // blank lines and lines whose first word holds a '#' are comments
+  if (w[0][0] == 0 || strchr(w[0], '#')) return;
+  bool understood = false;
+  for (const DeckKeyword& k : kDeckKeywords)
+    if (strcmp(w[0], k.keyword) == 0) {
+      understood = k.handle(*this, w, line);
+      break;
     }
-  }
-  if (strcmp(w[0], "atom_style") == 0) {
-    if (strcmp(w[1], "atomic") == 0) known = true;
-    else if (system->do_print)
-      printf("LAMMPS-Command: 'atom_style' command only supports 'atomic' in ExaMiniMD\n");
-  }
-  if (strcmp(w[0], "lattice") == 0) {
-    if (strcmp(w[1], "sc") == 0) {
-      known = true;
-      lattice_style = LATTICE_SC;
-      lattice_constant = atof(w[2]);
-    } else if (strcmp(w[1], "fcc") == 0) {
-      known = true;
-      lattice_style = LATTICE_FCC;
-      lattice_constant = std::pow((4.0 / atof(w[2])), (1.0 / 3.0));
-    } else if (strcmp(w[1], "bcc") == 0) {
-      known = true;
-      lattice_style = LATTICE_BCC;
-      lattice_constant = atof(w[2]);
-    } else if (system->do_print) {
-      printf("LAMMPS-Command: 'lattice' command only supports 'sc', 'fcc' and 'bcc' in ExaMiniMD\n");
-    }
-    if (strcmp(w[3], "origin") == 0) {
-      lattice_offset_x = atof(w[4]);
-      lattice_offset_y = atof(w[5]);
-      lattice_offset_z = atof(w[6]);
-    }
-  }
-  if (strcmp(w[0], "region") == 0) {
-    if (strcmp(w[2], "block") == 0) {
-      known = true;
-      int box[6];
-      for (int k = 0; k < 6; k++) box[k] = atoi(w[3 + k]);
-      if ((box[0] != 0) || (box[2] != 0) || (box[4] != 0))
-        if (system->do_print)
-          printf("Error: LAMMPS-Command: region only allows for boxes with 0,0,0 offset\n");
-      lattice_nx = box[1];
-      lattice_ny = box[3];
-      lattice_nz = box[5];
-    } else if (system->do_print) {
-      printf("LAMMPS-Command: 'region' command only supports 'block' option in ExaMiniMD\n");
-    }
-  }
-  if (strcmp(w[0], "create_box") == 0) {
-    known = true;
-    system->ntypes = atoi(w[1]);
-    system->mass.assign(system->ntypes, 0.0);
-  }
-  if (strcmp(w[0], "create_atoms") == 0) known = true;
-  if (strcmp(w[0], "mass") == 0) {
-    known = true;
-    int type = atoi(w[1]) - 1;
-    if (type >= 0 && type < (int)system->mass.size()) system->mass[type] = atof(w[2]);
-  }
-  if (strcmp(w[0], "pair_style") == 0) {
-    if (strcmp(w[1], "lj/cut/idial") == 0) {
-      known = true;
-      force_type = FORCE_LJ_IDIAL;
-      force_cutoff = atof(w[2]);
-      force_line = line;
-    } else if (strcmp(w[1], "lj/cut") == 0) {
-      known = true;
-      force_type = FORCE_LJ;
-      force_cutoff = atof(w[2]);
-      force_line = line;
-    }
-    if (strcmp(w[1], "snap") == 0) {
-      known = true;
-      force_type = FORCE_SNAP;
-      force_cutoff = 4.73442;  // fixed in the reference (src/input.cpp:378-383)
-      force_line = line;
-    }
-    if (system->do_print && !known)
-      printf("LAMMPS-Command: 'pair_style' command only supports 'lj/cut', 'lj/cut/idial', and 'snap' style in ExaMiniMD\n");
-  }
-  if (strcmp(w[0], "pair_coeff") == 0) {
-    known = true;
-    force_coeff_lines.push_back(line);
-  }
-  if (strcmp(w[0], "velocity") == 0) {
-    known = true;
-    if (strcmp(w[1], "all") != 0 && system->do_print)
-      printf("Error: LAMMPS-Command: 'velocity' command can only be applied to 'all'\n");
-    if (strcmp(w[2], "create") != 0 && system->do_print)
-      printf("Error: LAMMPS-Command: 'velocity' command can only be used with option 'create'\n");
-    temperature_target = atof(w[3]);
-    temperature_seed = atoi(w[4]);
-    for (int k = 5; k < ItemizedFile::words_per_line - 1; k++)
-      if (strcmp(w[k], "dist") == 0) velocity_gaussian = strcmp(w[k + 1], "gaussian") == 0;
-  }
-  if (strcmp(w[0], "neighbor") == 0) {
-    known = true;
-    neighbor_skin = atof(w[1]);
-  }
-  if (strcmp(w[0], "neigh_modify") == 0) {
-    known = true;
-    for (int i = 1; i < ItemizedFile::words_per_line - 1; i++)
-      if (strcmp(w[i], "every") == 0) comm_exchange_rate = atoi(w[i + 1]);
-  }
-  if (strcmp(w[0], "fix") == 0) {
-    if (strcmp(w[3], "nve") == 0) {
-      known = true;
-      integrator_type = INTEGRATOR_NVE;
-    } else if (system->do_print) {
-      printf("LAMMPS-Command: 'fix' command only supports 'nve' style in ExaMiniMD\n");
-    }
-  }
-  if (strcmp(w[0], "run") == 0) {
-    known = true;
-    nsteps = atoi(w[1]);
-  }
-  if (strcmp(w[0], "thermo") == 0) {
-    known = true;
-    thermo_rate = atoi(w[1]);
-  }
-  if (strcmp(w[0], "timestep") == 0) {
-    known = true;
-    system->dt = atof(w[1]);
-    timestepflag = true;
-  }
-  if (strcmp(w[0], "newton") == 0) {
-    known = true;
-    if (strcmp(w[1], "on") == 0) comm_newton = 1;
-    else if (strcmp(w[1], "off") == 0) comm_newton = 0;
-    else if (system->do_print) printf("LAMMPS-Command: 'newton' must be followed by 'on' or 'off'\n");
-  }
-  if (w[0][0] == '#') known = true;
-  if (!known && system->do_print) {
-    printf("ERROR: unknown keyword\n");
+  if (!understood && system->do_print) {
+    printf("deck: line %d not understood, ignored: ", line + 1);
     input_data.print_line(line);
   }
 }
 
-// Input::create_lattice, src/input.cpp:460-792 (one process: the sub-domain
-// is the box).  Positions and velocities are made on the host, in the same
-// loop order and arithmetic as the reference, then uploaded.
+// -------------------------------------------------------------- lattice
+// Input::create_lattice, src/input.cpp:460-792, for one process (the
+// sub-domain is the box).  What pins every trajectory and is kept exactly:
+// sites are visited cell by cell with z outermost, then y, then x, then the
+// basis sites; a site's coordinate is a * (cell index + basis offset) with
+// the deck's origin folded into the basis; a site belongs to the domain when
+// lo <= x < hi per axis; ids count 1, 2, ... in visiting order.  The velocity
+// of a site is drawn from the geom generator reseeded by (seed, position);
+// the net momentum is then removed and the temperature scaled to the target.
+namespace {
+struct LatticeSites {
+  std::vector<double> x;  // [n][3]
+  std::vector<int> type, id;
+  int n() const { return (int)id.size(); }
+};
+
+LatticeSites build_sites(const Input& in, const System& s) {
+  std::vector<std::array<double, 3>> basis;
+  switch (in.lattice_style) {
+    case LATTICE_FCC:
+      basis = {{0.0, 0.0, 0.0}, {0.5, 0.5, 0.0}, {0.5, 0.0, 0.5}, {0.0, 0.5, 0.5}};
+      break;
+    case LATTICE_BCC:
+      basis = {{0.0, 0.0, 0.0}, {0.5, 0.5, 0.5}};
+      break;
+    default:
+      basis = {{0.0, 0.0, 0.0}};
+  }
+  for (auto& b : basis) {
+    b[0] += in.lattice_offset_x;
+    b[1] += in.lattice_offset_y;
+    b[2] += in.lattice_offset_z;
+  }
+  const double a = in.lattice_constant;
+  const double lo[3] = {s.sub_domain_lo_x, s.sub_domain_lo_y, s.sub_domain_lo_z};
+  const double hi[3] = {s.sub_domain_hi_x, s.sub_domain_hi_y, s.sub_domain_hi_z};
+  const double len[3] = {s.domain_x, s.domain_y, s.domain_z};
+  const int cells[3] = {in.lattice_nx, in.lattice_ny, in.lattice_nz};
+  // cell index range covering the sub-domain, one cell of slack each side
+  // (the float-to-int conversion truncates, as the reference's does)
+  T_INT first[3], last[3];
+  for (int d = 0; d < 3; d++) {
+    first[d] = (T_INT)(lo[d] / len[d] * cells[d] - 0.5);
+    last[d] = (T_INT)(hi[d] / len[d] * cells[d] + 0.5);
+  }
+  LatticeSites out;
+  for (T_INT cz = first[2]; cz <= last[2]; cz++)
+    for (T_INT cy = first[1]; cy <= last[1]; cy++)
+      for (T_INT cx = first[0]; cx <= last[0]; cx++)
+        for (const auto& b : basis) {
+          const double p[3] = {a * (1.0 * cx + b[0]), a * (1.0 * cy + b[1]), a * (1.0 * cz + b[2])};
+          bool inside = true;
+          for (int d = 0; d < 3; d++) inside = inside && p[d] >= lo[d] && p[d] < hi[d];
+          if (!inside) continue;
+          out.x.insert(out.x.end(), p, p + 3);
+          out.type.push_back(rand() % s.ntypes);
+          out.id.push_back(out.n() + 1);
+        }
+  return out;
+}
+
+// v of one site before the momentum and temperature corrections: three
+// draws (uniform - 1/2 by default, Gaussian with `dist gaussian`) divided by
+// sqrt(m)
+void site_velocity(int seed, const double* pos, double mass, bool gaussian, double* v) {
+  LAMMPS_RandomVelocityGeom rng;
+  double p[3] = {pos[0], pos[1], pos[2]};
+  rng.reset(seed, p);
+  const double root_m = sqrt(mass);
+  for (int d = 0; d < 3; d++) {
+    const double draw = gaussian ? rng.gaussian() : rng.uniform() - 0.5;
+    v[d] = draw / root_m;
+  }
+}
+}  // namespace
+
 void Input::create_lattice(Comm* comm) {
   System& s = *system;
   s.domain_x = lattice_constant * lattice_nx;
@@ -403,121 +541,48 @@ void Input::create_lattice(Comm* comm) {
   s.domain_z = lattice_constant * lattice_nz;
   comm->create_domain_decomposition();
 
-  double basis[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  int nb = 1;
-  if (lattice_style == LATTICE_FCC) {
-    nb = 4;
-    double b[4][3] = {{0.0, 0.0, 0.0}, {0.5, 0.5, 0.0}, {0.5, 0.0, 0.5}, {0.0, 0.5, 0.5}};
-    memcpy(basis, b, sizeof(b));
-  } else if (lattice_style == LATTICE_BCC) {
-    nb = 2;
-    double b[2][3] = {{0.0, 0.0, 0.0}, {0.5, 0.5, 0.5}};
-    memcpy(basis, b, sizeof(b));
-  }
-  if (lattice_style != LATTICE_SC)
-    for (int i = 0; i < nb; i++) {
-      basis[i][0] += lattice_offset_x;
-      basis[i][1] += lattice_offset_y;
-      basis[i][2] += lattice_offset_z;
-    }
-  const double a = lattice_constant;
-  T_INT ix_start = s.sub_domain_lo_x / s.domain_x * lattice_nx - 0.5;
-  T_INT iy_start = s.sub_domain_lo_y / s.domain_y * lattice_ny - 0.5;
-  T_INT iz_start = s.sub_domain_lo_z / s.domain_z * lattice_nz - 0.5;
-  T_INT ix_end = s.sub_domain_hi_x / s.domain_x * lattice_nx + 0.5;
-  T_INT iy_end = s.sub_domain_hi_y / s.domain_y * lattice_ny + 0.5;
-  T_INT iz_end = s.sub_domain_hi_z / s.domain_z * lattice_nz + 0.5;
-
-  std::vector<double> hx;
-  std::vector<int> htype, hid;
-  for (T_INT iz = iz_start; iz <= iz_end; iz++)
-    for (T_INT iy = iy_start; iy <= iy_end; iy++)
-      for (T_INT ix = ix_start; ix <= ix_end; ix++)
-        for (int k = 0; k < nb; k++) {
-          double xt, yt, zt;
-          if (lattice_style == LATTICE_SC) {
-            zt = a * (iz + lattice_offset_z);
-            yt = a * (iy + lattice_offset_y);
-            xt = a * (ix + lattice_offset_x);
-          } else {
-            xt = a * (1.0 * ix + basis[k][0]);
-            yt = a * (1.0 * iy + basis[k][1]);
-            zt = a * (1.0 * iz + basis[k][2]);
-          }
-          if ((xt >= s.sub_domain_lo_x) && (yt >= s.sub_domain_lo_y) &&
-              (zt >= s.sub_domain_lo_z) && (xt < s.sub_domain_hi_x) &&
-              (yt < s.sub_domain_hi_y) && (zt < s.sub_domain_hi_z)) {
-            hx.push_back(xt);
-            hx.push_back(yt);
-            hx.push_back(zt);
-            htype.push_back(rand() % s.ntypes);
-            hid.push_back((int)hid.size() + 1);
-          }
-        }
-  const T_INT n = (T_INT)hid.size();
+  LatticeSites sites = build_sites(*this, s);
+  const T_INT n = sites.n();
   s.N_local = n;
   s.N = n;
   s.N_ghost = 0;
-  T_INT N_local_offset = n;
-  comm->scan_int(&N_local_offset, 1);
-  for (T_INT i = 0; i < n; i++) hid[i] += N_local_offset - n;
+  T_INT id_end = n;  // ids unique over processes: shift by the exclusive scan
+  comm->scan_int(&id_end, 1);
+  for (int& id : sites.id) id += id_end - n;
   comm->reduce_int(&s.N, 1);
   if (s.do_print) printf("Atoms: %i %i\n", s.N, s.N_local);
 
-  // velocities: src/input.cpp:731-791
-  std::vector<double> hv(3 * (size_t)n), hq(n, 0.0);
-  double total_mass = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+  std::vector<double> v(3 * (size_t)n), q(n, 0.0);
+  double mom[3] = {0.0, 0.0, 0.0}, mass_sum = 0.0;
   for (T_INT i = 0; i < n; i++) {
-    LAMMPS_RandomVelocityGeom random;
-    double xx[3] = {hx[3 * i], hx[3 * i + 1], hx[3 * i + 2]};
-    random.reset(temperature_seed, xx);
-    double mass_i = s.mass[htype[i]];
-    double vx, vy, vz;
-    if (!velocity_gaussian) {
-      vx = random.uniform() - 0.5;
-      vy = random.uniform() - 0.5;
-      vz = random.uniform() - 0.5;
-    } else {
-      vx = random.gaussian();
-      vy = random.gaussian();
-      vz = random.gaussian();
-    }
-    hv[3 * i] = vx / sqrt(mass_i);
-    hv[3 * i + 1] = vy / sqrt(mass_i);
-    hv[3 * i + 2] = vz / sqrt(mass_i);
-    total_mass += mass_i;
-    px += mass_i * hv[3 * i];
-    py += mass_i * hv[3 * i + 1];
-    pz += mass_i * hv[3 * i + 2];
+    const double m = s.mass[sites.type[i]];
+    double* vi = &v[3 * (size_t)i];
+    site_velocity(temperature_seed, &sites.x[3 * (size_t)i], m, velocity_gaussian, vi);
+    mass_sum += m;
+    for (int d = 0; d < 3; d++) mom[d] += m * vi[d];
   }
-  comm->reduce_float(&px, 1);
-  comm->reduce_float(&py, 1);
-  comm->reduce_float(&pz, 1);
-  comm->reduce_float(&total_mass, 1);
-  const double svx = px / total_mass, svy = py / total_mass, svz = pz / total_mass;
-  for (T_INT i = 0; i < n; i++) {
-    hv[3 * i] -= svx;
-    hv[3 * i + 1] -= svy;
-    hv[3 * i + 2] -= svz;
-  }
-  // Temperature::compute in the serial order (the scale factor pins every
-  // trajectory, so it is summed on the host exactly as the reference does)
-  double T = 0.0;
+  comm->reduce_float(mom, 3);
+  comm->reduce_float(&mass_sum, 1);
+  const double drift[3] = {mom[0] / mass_sum, mom[1] / mass_sum, mom[2] / mass_sum};
   for (T_INT i = 0; i < n; i++)
-    T += (hv[3 * i] * hv[3 * i] + hv[3 * i + 1] * hv[3 * i + 1] + hv[3 * i + 2] * hv[3 * i + 2]) *
-         s.mass[htype[i]];
-  comm->reduce_float(&T, 1);
-  T_INT dof = 3 * s.N - 3;
-  T *= s.mvv2e / (1.0 * dof * s.boltz);
-  const double scale = sqrt(temperature_target / T);
+    for (int d = 0; d < 3; d++) v[3 * (size_t)i + d] -= drift[d];
+
+  // Temperature::compute (src/property_temperature.cpp:43-62) summed on the
+  // host in atom order: the scale factor pins every trajectory
+  double mv2 = 0.0;
   for (T_INT i = 0; i < n; i++) {
-    hv[3 * i] *= scale;
-    hv[3 * i + 1] *= scale;
-    hv[3 * i + 2] *= scale;
+    const double* vi = &v[3 * (size_t)i];
+    mv2 += (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]) * s.mass[sites.type[i]];
   }
+  comm->reduce_float(&mv2, 1);
+  const T_INT dof = 3 * s.N - 3;
+  const double T = mv2 * (s.mvv2e / (1.0 * dof * s.boltz));
+  const double scale = sqrt(temperature_target / T);
+  for (double& c : v) c *= scale;
+
   s.grow(n);
   s.upload_mass();
-  EMD_CALL(s.ctx, emd_system_upload(s.ctx, &s.d, n, hx.data(), hv.data(), htype.data(),
-                                    hid.data(), hq.data()));
+  EMD_CALL(s.ctx, emd_system_upload(s.ctx, &s.d, n, sites.x.data(), v.data(), sites.type.data(),
+                                    sites.id.data(), q.data()));
   s.N_max = s.d.n_max;
 }

@@ -1,6 +1,7 @@
 // input.h -- LAMMPS-deck subset, command line and lattice set-up
-// (src/input.{h,cpp}).  Extensions: `lattice bcc <a>`,
-// `velocity all create T seed ... dist gaussian`, `--precision fp64|mixed`.
+// (src/input.{h,cpp}: same deck keywords, CLI options and Input fields).
+// Extensions: `lattice bcc <a>`, `velocity all create T seed ... dist
+// gaussian`, `--precision fp64|mixed`, `--device`, `--restart`.
 #ifndef EMD_INPUT_H
 #define EMD_INPUT_H
 #include <string>
@@ -36,8 +37,8 @@ public:
 };
 
 class Input {
-  bool timestepflag;
 public:
+  bool timestep_given;  // a `timestep` line was read (units real/lj keep it)
   System* system;
   char* input_file;
   int input_file_type;
@@ -68,6 +69,8 @@ public:
   int thermo_rate, dumpbinary_rate, correctness_rate;
   bool dumpbinaryflag, correctnessflag;
   char *dumpbinary_path, *reference_path, *correctness_file;
+  int dumpneigh_rate;       // --dumpneigh N DIR (0: off)
+  char *dumpneigh_path;
   char *restart_file;  // --restart: initial state from a binary dump
 
   int precision;  // EMD_FP64 or EMD_MIXED
