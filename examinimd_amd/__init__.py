@@ -157,6 +157,10 @@ class App:
     def dump(self, path):
         return self.L.emd_app_dump(self.h, path.encode())
 
+    def dump_neigh(self, path):
+        """canonical neighbour-list file of the current list (--dumpneigh format)"""
+        return self.L.emd_app_dump_neigh(self.h, path.encode())
+
     @property
     def ctx(self):
         return self.L.emd_app_ctx(self.h)
@@ -181,6 +185,23 @@ class App:
             pass
 
 
+def read_neigh_dump(path):
+    """{id: sorted (id_j, image_j) pairs as an int array [count, 2]} from a
+    --dumpneigh / emd_app_dump_neigh file"""
+    import numpy as np
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    total = int(np.frombuffer(raw, np.int64, 1, 4)[0])
+    body = np.frombuffer(raw, np.int32, offset=12)
+    rows, k = {}, 0
+    for _ in range(n):
+        aid, cnt = int(body[k]), int(body[k + 1])
+        rows[aid] = body[k + 2:k + 2 + 2 * cnt].reshape(cnt, 2)
+        k += 2 + 2 * cnt
+    assert k == len(body) and sum(len(r) for r in rows.values()) == total
+    return rows
+
+
 def download_ints(ctx, ptr, n):
     import numpy as np
     out = np.zeros(max(int(n), 0), np.int32)
@@ -203,5 +224,5 @@ def download_neigh(ctx, nl):
     ent2 = download_ints(ctx, nl.entries, n * pitch).reshape(n, pitch) if n else np.zeros((0, 0))
     row = np.zeros(n + 1, np.int64)
     row[1:] = np.cumsum(num)
-    ent = np.concatenate([ent2[i, :num[i]] for i in range(n)]) if n else np.zeros(0, np.int32)
+    ent = ent2[np.arange(pitch)[None, :] < num[:, None]] if n else np.zeros(0, np.int32)
     return row, num, ent

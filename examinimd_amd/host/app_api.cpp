@@ -115,6 +115,11 @@ extern "C" int emd_app_dump(emd_app* a, const char* path) {
   return EMD_OK;
 }
 
+extern "C" int emd_app_dump_neigh(emd_app* a, const char* path) {
+  if (!a || !path) return EMD_ERR_ARG;
+  return a->md->write_neigh_file(path);
+}
+
 extern "C" emd_ctx* emd_app_ctx(emd_app* a) { return a ? a->md->system->ctx : nullptr; }
 extern "C" emd_system* emd_app_system(emd_app* a) { return a ? a->md->system->view() : nullptr; }
 extern "C" const emd_neigh* emd_app_neigh(emd_app* a) {

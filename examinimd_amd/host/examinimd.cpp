@@ -1,7 +1,9 @@
 // examinimd.cpp -- ExaMiniMD::init / run / dump_binary / check_correctness
-// (src/examinimd.cpp:47-488), same step order and output columns.
+// (src/examinimd.cpp:47-488): the same init and step order, the same thermo,
+// PERFORMANCE, dump and correctness-file formats.
 #include "examinimd.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -19,6 +21,11 @@ struct Timer {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 };
+// module selection fails before a Comm exists to report it
+[[noreturn]] void comm_less_fatal(const char* msg) {
+  fprintf(stderr, "ExaMiniMD: %s\n", msg);
+  exit(1);
+}
 }  // namespace
 
 ExaMiniMD::ExaMiniMD() {
@@ -55,8 +62,7 @@ void ExaMiniMD::init(int argc, char* argv[]) {
 #include "modules_force.h"
 #undef FORCE_MODULES_INSTANTIATION
   else {
-    printf("Invalid ForceType\n");
-    exit(1);
+    comm_less_fatal("init: no force module matches the deck's pair_style");
   }
   for (size_t line = 0; line < input->force_coeff_lines.size(); line++) {
     int l = input->force_coeff_lines[line];
@@ -69,8 +75,7 @@ void ExaMiniMD::init(int argc, char* argv[]) {
 #include "modules_neighbor.h"
 #undef NEIGHBOR_MODULES_INSTANTIATION
   else {
-    printf("Invalid NeighborType\n");
-    exit(1);
+    comm_less_fatal("init: no neighbour module matches --neigh-type");
   }
 
   if (false) {
@@ -79,8 +84,7 @@ void ExaMiniMD::init(int argc, char* argv[]) {
 #include "modules_comm.h"
 #undef COMM_MODULES_INSTANTIATION
   else {
-    printf("Invalid CommType\n");
-    exit(1);
+    comm_less_fatal("init: no comm module matches --comm-type");
   }
 
   force->comm_newton = input->comm_newton;
@@ -120,6 +124,7 @@ void ExaMiniMD::init(int argc, char* argv[]) {
   }
   if (input->dumpbinaryflag) dump_binary(step);
   if (input->correctnessflag) check_correctness(step);
+  dump_neigh(step);
 }
 
 // T, PE/N, (PE+KE)/N as the thermo lines print them
@@ -223,6 +228,7 @@ void ExaMiniMD::run(int nsteps) {
     }
     if (input->dumpbinaryflag) dump_binary(step);
     if (input->correctnessflag) check_correctness(step);
+    dump_neigh(step);
     fence();
     other_time += other_timer.seconds();
   }
@@ -279,123 +285,211 @@ void ExaMiniMD::read_restart(const char* filename) {
                                   q.data()));
 }
 
-// dump layout (src/examinimd.cpp:296-346): int n, int id[n], int type[n],
-// double q[n], double x[3n], double v[3n], double f[3n]; f is the FP64 force
-// (the mixed path's FP32 force widened)
-void ExaMiniMD::dump_binary(int step) {
-  if (step % input->dumpbinary_rate) return;
-  char filename[MAXPATHLEN];
-  snprintf(filename, sizeof(filename), "%s%s.%010d.%03d", input->dumpbinary_path, "/output", step,
-           comm->process_rank());
-  FILE* fp = fopen(filename, "wb");
-  if (fp == NULL) {
-    char str[MAXPATHLEN + 64];
-    snprintf(str, sizeof(str), "Cannot open dump file %s", filename);
-    comm->error(str);
-  }
-  T_INT n = system->N_local;
-  std::vector<int> id(n), type(n);
-  std::vector<double> q(n), x(3 * (size_t)n), v(3 * (size_t)n), f(3 * (size_t)n);
+// The dump layout of src/examinimd.cpp:296-346: int n, int id[n],
+// int type[n], double q[n], double x[3n], double v[3n], double f[3n] (x, v, f
+// atom-major), one file DIR/output.<step, 10 digits>.<rank, 3 digits> per
+// dump step.  The mixed path's FP32 force is written widened to FP64.
+namespace {
+struct AtomState {
+  std::vector<int> id, type;
+  std::vector<double> q, x, v, f;
+  explicit AtomState(size_t n) : id(n), type(n), q(n), x(3 * n), v(3 * n), f(3 * n) {}
+};
+
+void dump_file_name(char* out, size_t cap, const char* dir, int step, int rank) {
+  snprintf(out, cap, "%s/output.%010d.%03d", dir, step, rank);
+}
+}  // namespace
+
+static AtomState download_state(System* system) {
+  const T_INT n = system->N_local;
+  AtomState a((size_t)n);
   emd_ctx* ctx = system->ctx;
-  EMD_CALL(ctx, emd_system_download(ctx, &system->d, n, x.data(), v.data(),
-                                    system->precision == EMD_MIXED ? nullptr : f.data(),
-                                    type.data(), id.data(), q.data()));
-  if (system->precision == EMD_MIXED) {
+  const bool mixed = system->precision == EMD_MIXED;
+  EMD_CALL(ctx, emd_system_download(ctx, &system->d, n, a.x.data(), a.v.data(),
+                                    mixed ? nullptr : a.f.data(), a.type.data(), a.id.data(),
+                                    a.q.data()));
+  if (mixed) {
     std::vector<float> f32(3 * (size_t)n);
     EMD_CALL(ctx, emd_memcpy_d2h(ctx, f32.data(), system->d.f32, f32.size() * sizeof(float)));
-    for (size_t k = 0; k < f32.size(); k++) f[k] = f32[k];
+    for (size_t k = 0; k < f32.size(); k++) a.f[k] = f32[k];
   }
-  fwrite(&n, sizeof(T_INT), 1, fp);
-  fwrite(id.data(), sizeof(T_INT), n, fp);
-  fwrite(type.data(), sizeof(T_INT), n, fp);
-  fwrite(q.data(), sizeof(T_FLOAT), n, fp);
-  fwrite(x.data(), sizeof(T_X_FLOAT), 3 * (size_t)n, fp);
-  fwrite(v.data(), sizeof(T_V_FLOAT), 3 * (size_t)n, fp);
-  fwrite(f.data(), sizeof(T_F_FLOAT), 3 * (size_t)n, fp);
-  fclose(fp);
+  return a;
 }
 
-// ExaMiniMD::check_correctness, src/examinimd.cpp:355-482
+void ExaMiniMD::dump_binary(int step) {
+  if (step % input->dumpbinary_rate) return;
+  char path[MAXPATHLEN];
+  dump_file_name(path, sizeof(path), input->dumpbinary_path, step, comm->process_rank());
+  FILE* fp = fopen(path, "wb");
+  if (fp == NULL) {
+    char msg[MAXPATHLEN + 64];
+    snprintf(msg, sizeof(msg), "dump_binary: cannot create %s", path);
+    comm->error(msg);
+  }
+  const T_INT n = system->N_local;
+  AtomState a = download_state(system);
+  bool ok = fwrite(&n, sizeof(T_INT), 1, fp) == 1;
+  ok = ok && fwrite(a.id.data(), sizeof(T_INT), n, fp) == (size_t)n;
+  ok = ok && fwrite(a.type.data(), sizeof(T_INT), n, fp) == (size_t)n;
+  ok = ok && fwrite(a.q.data(), sizeof(T_FLOAT), n, fp) == (size_t)n;
+  for (const std::vector<double>* arr : {&a.x, &a.v, &a.f})
+    ok = ok && fwrite(arr->data(), sizeof(double), arr->size(), fp) == arr->size();
+  fclose(fp);
+  if (!ok) comm->error("dump_binary: short write");
+}
+
+// Neighbour-list dump (no counterpart file in the reference, which never
+// writes its lists): a canonical, implementation-independent image of the
+// current list, so lists of two implementations compare byte for byte.
+// File DIR/neigh.<step, 10 digits>.<rank, 3 digits>:
+//   int n (local atoms), long total (entries),
+//   then per local atom in ascending id order: int id, int count,
+//   then count pairs (int id_j, int image_j), ascending by (id_j, image_j).
+// image_j is the periodic shift of neighbour j from the local atom carrying
+// that id: bits 2d..2d+1 for axis d, 0 none, 1 -L, 2 +L (a local j has 0;
+// a ghost has the shifts its exchange_halo phases applied).
+int ExaMiniMD::write_neigh_file(const char* path) {
+  const emd_neigh* nl = neighbor ? neighbor->get_neigh_list() : nullptr;
+  CommSerialHIP* serial = dynamic_cast<CommSerialHIP*>(comm);
+  if (!nl || !serial) return EMD_ERR_STATE;
+  const emd_halo* h = serial->halo();
+  emd_ctx* ctx = system->ctx;
+  const int n = nl->n, ng = system->N_ghost;
+  std::vector<int> num(n), id(n + ng), root(ng), image(ng);
+  std::vector<int> ent;
+  std::vector<long> start(n + 1, 0);
+  EMD_CALL(ctx, emd_memcpy_d2h(ctx, num.data(), nl->num, sizeof(int) * (size_t)n));
+  for (int i = 0; i < n; i++) start[i + 1] = start[i] + num[i];
+  if (nl->layout == EMD_NEIGH_CSR) {
+    std::vector<int> row(n + 1);
+    EMD_CALL(ctx, emd_memcpy_d2h(ctx, row.data(), nl->row, sizeof(int) * (size_t)(n + 1)));
+    std::vector<int> all(row[n]);
+    EMD_CALL(ctx, emd_memcpy_d2h(ctx, all.data(), nl->entries, sizeof(int) * all.size()));
+    ent.resize(start[n]);
+    for (int i = 0; i < n; i++)
+      std::copy(all.begin() + row[i], all.begin() + row[i] + num[i], ent.begin() + start[i]);
+  } else {
+    const size_t pitch = nl->maxneighs;
+    std::vector<int> pad(pitch * n);
+    EMD_CALL(ctx, emd_memcpy_d2h(ctx, pad.data(), nl->entries, sizeof(int) * pad.size()));
+    ent.resize(start[n]);
+    for (int i = 0; i < n; i++)
+      std::copy(pad.begin() + pitch * i, pad.begin() + pitch * i + num[i], ent.begin() + start[i]);
+  }
+  EMD_CALL(ctx, emd_memcpy_d2h(ctx, id.data(), system->d.id, sizeof(int) * id.size()));
+  if (ng > 0) {
+    EMD_CALL(ctx, emd_memcpy_d2h(ctx, root.data(), h->root, sizeof(int) * (size_t)ng));
+    EMD_CALL(ctx, emd_memcpy_d2h(ctx, image.data(), h->image, sizeof(int) * (size_t)ng));
+  }
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return id[a] < id[b]; });
+  FILE* fp = fopen(path, "wb");
+  if (!fp) return EMD_ERR_ARG;
+  const long total = start[n];
+  bool ok = fwrite(&n, sizeof(int), 1, fp) == 1 && fwrite(&total, sizeof(long), 1, fp) == 1;
+  std::vector<std::pair<int, int>> keys;
+  for (int i : order) {
+    keys.clear();
+    for (long e = start[i]; e < start[i + 1]; e++) {
+      const int j = ent[e];
+      keys.emplace_back(j < n ? id[j] : id[root[j - n]], j < n ? 0 : image[j - n]);
+    }
+    std::sort(keys.begin(), keys.end());
+    const int head[2] = {id[i], num[i]};
+    ok = ok && fwrite(head, sizeof(int), 2, fp) == 2;
+    for (const auto& k : keys) {
+      const int pair[2] = {k.first, k.second};
+      ok = ok && fwrite(pair, sizeof(int), 2, fp) == 2;
+    }
+  }
+  fclose(fp);
+  return ok ? EMD_OK : EMD_ERR_ARG;
+}
+
+void ExaMiniMD::dump_neigh(int step) {
+  if (input->dumpneigh_rate <= 0 || step % input->dumpneigh_rate) return;
+  char path[MAXPATHLEN];
+  snprintf(path, sizeof(path), "%s/neigh.%010d.%03d", input->dumpneigh_path, step,
+           comm->process_rank());
+  if (write_neigh_file(path) != EMD_OK) {
+    char msg[MAXPATHLEN + 64];
+    snprintf(msg, sizeof(msg), "dumpneigh: cannot write %s", path);
+    comm->error(msg);
+  }
+}
+
+// The --correctness comparison of src/examinimd.cpp:355-482: the reference
+// dump of this step is matched to the current atoms by id; per quantity
+// (x, v, f) the root of the summed squared differences and the largest
+// component difference are appended to the correctness file as one line
+// "step dr maxdr dv maxdv df maxdf" (%g), under a header written at step 0.
+// Atoms are matched through an id -> index table (the reference searches
+// linearly when the orders differ); a reference id with no current atom is
+// reported and skipped.
 void ExaMiniMD::check_correctness(int step) {
   if (step % input->correctness_rate) return;
-  char filename[MAXPATHLEN];
-  snprintf(filename, sizeof(filename), "%s%s.%010d.%03d", input->reference_path, "/output", step,
-           comm->process_rank());
-  FILE* fpref = fopen(filename, "rb");
-  if (fpref == NULL) {
-    char str[MAXPATHLEN + 64];
-    snprintf(str, sizeof(str), "Cannot open input file %s", filename);
-    comm->error(str);
+  char path[MAXPATHLEN];
+  dump_file_name(path, sizeof(path), input->reference_path, step, comm->process_rank());
+  FILE* fp = fopen(path, "rb");
+  if (fp == NULL) {
+    char msg[MAXPATHLEN + 64];
+    snprintf(msg, sizeof(msg), "correctness: no reference dump %s", path);
+    comm->error(msg);
   }
-  T_INT n = system->N_local, ntmp = 0;
-  if (fread(&ntmp, sizeof(T_INT), 1, fpref) != 1 || ntmp != n)
-    comm->error("Mismatch in current and reference atom counts");
-  std::vector<int> idref(n), typeref(n);
-  std::vector<double> qref(n), xref(3 * (size_t)n), vref(3 * (size_t)n), fref(3 * (size_t)n);
-  size_t got = fread(idref.data(), sizeof(T_INT), n, fpref) +
-               fread(typeref.data(), sizeof(T_INT), n, fpref) +
-               fread(qref.data(), sizeof(T_FLOAT), n, fpref) +
-               fread(xref.data(), sizeof(T_X_FLOAT), 3 * (size_t)n, fpref) +
-               fread(vref.data(), sizeof(T_V_FLOAT), 3 * (size_t)n, fpref) +
-               fread(fref.data(), sizeof(T_F_FLOAT), 3 * (size_t)n, fpref);
-  fclose(fpref);
-  if (got != (size_t)n * 12) comm->error("Short reference dump");
+  const T_INT n = system->N_local;
+  T_INT nref = -1;
+  if (fread(&nref, sizeof(T_INT), 1, fp) != 1 || nref != n) {
+    fclose(fp);
+    comm->error("correctness: the reference dump holds a different number of atoms");
+  }
+  AtomState ref((size_t)n);
+  bool ok = fread(ref.id.data(), sizeof(T_INT), n, fp) == (size_t)n;
+  ok = ok && fread(ref.type.data(), sizeof(T_INT), n, fp) == (size_t)n;
+  ok = ok && fread(ref.q.data(), sizeof(T_FLOAT), n, fp) == (size_t)n;
+  for (std::vector<double>* arr : {&ref.x, &ref.v, &ref.f})
+    ok = ok && fread(arr->data(), sizeof(double), arr->size(), fp) == arr->size();
+  fclose(fp);
+  if (!ok) comm->error("correctness: the reference dump is truncated");
 
-  std::vector<int> id(n), type(n);
-  std::vector<double> q(n), x(3 * (size_t)n), v(3 * (size_t)n), f(3 * (size_t)n);
-  emd_ctx* ctx = system->ctx;
-  EMD_CALL(ctx, emd_system_download(ctx, &system->d, n, x.data(), v.data(),
-                                    system->precision == EMD_MIXED ? nullptr : f.data(),
-                                    type.data(), id.data(), q.data()));
-  if (system->precision == EMD_MIXED) {
-    std::vector<float> f32(3 * (size_t)n);
-    EMD_CALL(ctx, emd_memcpy_d2h(ctx, f32.data(), system->d.f32, f32.size() * sizeof(float)));
-    for (size_t k = 0; k < f32.size(); k++) f[k] = f32[k];
-  }
-  T_FLOAT sumdelrsq = 0.0, sumdelvsq = 0.0, sumdelfsq = 0.0;
-  T_FLOAT maxdelr = 0.0, maxdelv = 0.0, maxdelf = 0.0;
-  for (int i = 0; i < n; i++) {
-    int ii = -1;
-    if (id[i] != idref[i]) {
-      for (int j = 0; j < n; j++)
-        if (id[j] == idref[i]) { ii = j; break; }
-    } else {
-      ii = i;
-    }
-    if (ii == -1) {
-      printf("Unable to find current id matchinf reference id %d \n", idref[i]);
+  AtomState cur = download_state(system);
+  int id_max = 0;
+  for (int id : cur.id) id_max = MAX(id_max, id);
+  std::vector<int> where(id_max + 1, -1);
+  for (T_INT k = 0; k < n; k++)
+    if (cur.id[k] >= 0) where[cur.id[k]] = k;
+
+  double sq[3] = {0.0, 0.0, 0.0}, big[3] = {0.0, 0.0, 0.0};
+  const std::vector<double>* now[3] = {&cur.x, &cur.v, &cur.f};
+  const std::vector<double>* was[3] = {&ref.x, &ref.v, &ref.f};
+  for (T_INT i = 0; i < n; i++) {
+    const int id = ref.id[i];
+    const int k = (id >= 0 && id <= id_max) ? where[id] : -1;
+    if (k < 0) {
+      printf("correctness: reference atom id %d is not among the current atoms\n", id);
       continue;
     }
-    const std::vector<double>* cur[3] = {&x, &v, &f};
-    const std::vector<double>* ref[3] = {&xref, &vref, &fref};
-    T_FLOAT* sums[3] = {&sumdelrsq, &sumdelvsq, &sumdelfsq};
-    T_FLOAT* maxs[3] = {&maxdelr, &maxdelv, &maxdelf};
-    for (int k = 0; k < 3; k++) {
-      double dx = (*cur[k])[3 * ii] - (*ref[k])[3 * i];
-      double dy = (*cur[k])[3 * ii + 1] - (*ref[k])[3 * i + 1];
-      double dz = (*cur[k])[3 * ii + 2] - (*ref[k])[3 * i + 2];
-      *sums[k] += dx * dx + dy * dy + dz * dz;
-      *maxs[k] = MAX(fabs(dx), *maxs[k]);
-      *maxs[k] = MAX(fabs(dy), *maxs[k]);
-      *maxs[k] = MAX(fabs(dz), *maxs[k]);
+    for (int q = 0; q < 3; q++) {
+      double d2 = 0.0;
+      for (int c = 0; c < 3; c++) {
+        const double d = (*now[q])[3 * (size_t)k + c] - (*was[q])[3 * (size_t)i + c];
+        d2 += d * d;
+        big[q] = MAX(fabs(d), big[q]);
+      }
+      sq[q] += d2;
     }
   }
-  comm->reduce_float(&sumdelrsq, 1);
-  comm->reduce_float(&sumdelvsq, 1);
-  comm->reduce_float(&sumdelfsq, 1);
-  comm->reduce_max_float(&maxdelr, 1);
-  comm->reduce_max_float(&maxdelv, 1);
-  comm->reduce_max_float(&maxdelf, 1);
-  if (system->do_print) {
-    FILE* fpout = fopen(input->correctness_file, step == 0 ? "w" : "a");
-    if (fpout) {
-      if (step == 0)
-        fprintf(fpout, "# timestep deltarnorm maxdelr deltavnorm maxdelv deltafnorm maxdelf\n");
-      fprintf(fpout, "%d %g %g %g %g %g %g\n", step, sqrt(sumdelrsq), maxdelr, sqrt(sumdelvsq),
-              maxdelv, sqrt(sumdelfsq), maxdelf);
-      fclose(fpout);
-    }
-  }
+  comm->reduce_float(sq, 3);
+  comm->reduce_max_float(big, 3);
+  // the file is written by rank 0 whether or not the run prints
+  if (comm->process_rank() != 0) return;
+  FILE* out = fopen(input->correctness_file, step == 0 ? "w" : "a");
+  if (out == NULL) comm->error("correctness: cannot write the correctness file");
+  if (step == 0) fprintf(out, "# timestep deltarnorm maxdelr deltavnorm maxdelv deltafnorm maxdelf\n");
+  fprintf(out, "%d %g %g %g %g %g %g\n", step, sqrt(sq[0]), big[0], sqrt(sq[1]), big[1],
+          sqrt(sq[2]), big[2]);
+  fclose(out);
 }
 
 void ExaMiniMD::print_performance() {}

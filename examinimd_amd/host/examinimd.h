@@ -35,6 +35,10 @@ public:
   void run(int nsteps);
   void thermo(double out[3]);
   void dump_binary(int step);
+  // canonical neighbour-list dump (--dumpneigh N DIR); write_neigh_file
+  // writes the current list to `path` whatever the step
+  void dump_neigh(int step);
+  int write_neigh_file(const char* path);
   void read_restart(const char* filename);
   void check_correctness(int step);
   void print_performance();

@@ -42,6 +42,13 @@ int      emd_app_download(emd_app *app, double *x, double *v, double *f, int *ty
                           double *q);
 /* reference-format binary dump of the local atoms to `path` */
 int      emd_app_dump(emd_app *app, const char *path);
+/* canonical neighbour-list file of the current list (the --dumpneigh
+ * format: int n, long total, then per local atom in ascending id order
+ * int id, int count and count (id_j, image_j) int pairs sorted ascending;
+ * image_j bits 2d..2d+1 = periodic shift on axis d, 0 none, 1 -L, 2 +L).
+ * The reference writes no neighbour lists; this exists so that lists of two
+ * implementations can be compared offline. */
+int      emd_app_dump_neigh(emd_app *app, const char *path);
 /* device objects for callers that go below the driver */
 emd_ctx          *emd_app_ctx(emd_app *app);
 emd_system       *emd_app_system(emd_app *app);
