@@ -4,7 +4,7 @@ Each replica is an independent periodic box; per replica the batch must
 give what the reference's serial-comm LJ loop gives for that box alone:
   * neighbour sets: exactly the (j, periodic image) pairs within rc + skin
     found by brute force over all 27 images (bit-exact set identity);
-  * FP64: positions, velocities and forces within 1e-11 relative of the
+  * FP64: positions, velocities and forces within 1e-12 relative of the
     oracle loop over 60 steps (3 rebuilds); the rows are in a different
     order than the oracle's ghost rows, so sums agree to rounding only;
   * mixed (FP32 pair math and forces, FP64 state): force RMS error against
@@ -98,9 +98,9 @@ def test_batch_fp64_matches_oracle(oracle, lanes, block):
         assert rel_err(f0r, fo0) < 1e-13, r
         xo, vo, fo = oracle_lj_md(oracle, x0, v0, box[r], 60)
         # oracle positions are wrapped at rebuilds, batch positions too
-        assert rel_err(xr, xo) < 1e-11, r
-        assert rel_err(vr, vo) < 1e-11, r
-        assert rel_err(split(fg, n)[r], fo) < 1e-11, r
+        errs = (rel_err(xr, xo), rel_err(vr, vo), rel_err(split(fg, n)[r], fo))
+        print("replica", r, "x v f relative errors", errs)
+        assert max(errs) <= 1e-12, r
     b.close()
 
 
@@ -143,6 +143,26 @@ def test_batch_mixed_force_error_and_energy():
     assert drift64.max() < 5e-3
     b64.close()
     b32.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
+def test_batch_mixed_force_error_every_lane_count(lanes):
+    """the autotuner may pick any of these lane counts in mixed precision:
+    the FP32 force error bound (1e-5 RMS, reduced units) holds at each, on
+    boxes melted for 100 FP64 steps (the melt is where forces are largest)"""
+    print(f"case: batch mixed force error at {lanes} lanes", flush=True)
+    n, box, x, v = small_ensemble(seed=8)
+    b = make(n, box, x, v, EMD_FP64, 4, 256)
+    b.run(100)
+    xm, vm, f64 = b.download()
+    b.close()
+    b32 = make(n, box, xm, vm, EMD_MIXED, lanes, 128)
+    _, _, f32 = b32.download()
+    b32.close()
+    df = f32 - f64
+    rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+    print("batch mixed force RMS error", lanes, "lanes:", rms)
+    assert rms <= 1e-5
 
 
 def test_batch_autotune_keeps_state():
@@ -189,6 +209,7 @@ def test_replica_loop_matches_batch():
     xb, vb, _ = b.download()
     assert sec > 0
     for r in range(len(n)):
-        assert rel_err(split(xl, n)[r], split(xb, n)[r]) < 1e-11, r
-        assert rel_err(split(vl, n)[r], split(vb, n)[r]) < 1e-11, r
+        errs = (rel_err(split(xl, n)[r], split(xb, n)[r]), rel_err(split(vl, n)[r], split(vb, n)[r]))
+        print("replica", r, "loop vs batch x v relative errors", errs)
+        assert max(errs) <= 1e-12, r
     b.close()
