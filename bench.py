@@ -259,7 +259,7 @@ def lj_force_error_disordered(emd):
     x = lattice("fcc", 20, a) + np.random.default_rng(5).normal(0, 0.05, (32000, 3))
     pl = Pipeline()
     pl.load(x, (20 * a,) * 3)
-    pl.build(2.8, tile=True)
+    pl.build(2.8)
     f64 = pl.lj(emd.EMD_FP64)
     f32 = pl.lj(emd.EMD_MIXED)
     pl.close()

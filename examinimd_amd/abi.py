@@ -20,6 +20,13 @@ class EmdSystem(C.Structure):
                 ("boltz", C.c_double), ("mvv2e", C.c_double), ("dt", C.c_double)]
 
 
+class EmdVec3View(C.Structure):
+    _fields_ = [("data", dptr), ("layout", C.c_int), ("ld", C.c_long)]
+
+
+EMD_LAYOUT_RIGHT, EMD_LAYOUT_LEFT = 0, 1
+
+
 class EmdHalo(C.Structure):
     _fields_ = [("num_ghost", C.c_int * 6), ("pack", iptr * 6), ("cap", C.c_int * 6),
                 ("root", iptr), ("image", iptr), ("root_cap", C.c_int), ("n_ghost", C.c_int)]
@@ -37,9 +44,7 @@ class EmdBins(C.Structure):
 class EmdNeigh(C.Structure):
     _fields_ = [("layout", C.c_int), ("n", C.c_int), ("total", C.c_long),
                 ("maxneighs", C.c_int), ("row", iptr), ("num", iptr), ("entries", iptr),
-                ("cap_entries", C.c_long), ("cap_rows", C.c_int), ("want_slot", C.c_int),
-                ("slot", P(C.c_ushort)), ("cap_slot", C.c_long), ("bins", C.c_void_p),
-                ("visit", iptr), ("cap_visit", C.c_int), ("n_visit", C.c_int)]
+                ("cap_entries", C.c_long), ("cap_rows", C.c_int)]
 
 
 class EmdLJParams(C.Structure):
@@ -88,6 +93,13 @@ HIP_PROTOS = {
     "emd_system_download": (C.c_int, [vp, P(EmdSystem), C.c_int, dptr, dptr, dptr, iptr, iptr,
                                       dptr]),
     "emd_system_set_mass": (C.c_int, [vp, P(EmdSystem), C.c_int, dptr]),
+    "emd_system_import": (C.c_int, [vp, P(EmdSystem), C.c_int, EmdVec3View, EmdVec3View,
+                                    EmdVec3View, vp, vp, vp]),
+    "emd_system_export": (C.c_int, [vp, P(EmdSystem), C.c_int, EmdVec3View, EmdVec3View,
+                                    EmdVec3View]),
+    "emd_ctx_shared": (C.c_int, [C.c_int, P(vp)]),
+    "emd_bins_import": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double),
+                                  P(C.c_double), vp, vp, C.c_int, vp, C.c_int, P(EmdBins)]),
     "emd_comm_exchange": (C.c_int, [vp, P(EmdSystem)]),
     "emd_comm_exchange_halo": (C.c_int, [vp, P(EmdSystem), C.c_double, P(EmdHalo)]),
     "emd_comm_update_halo": (C.c_int, [vp, P(EmdSystem), P(EmdHalo)]),
@@ -109,7 +121,6 @@ HIP_PROTOS = {
     "emd_lj_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int,
                                 P(C.c_double)]),
     "emd_lj_set_lanes": (C.c_int, [vp, C.c_int, C.c_int]),
-    "emd_lj_set_tile": (C.c_int, [vp, C.c_int, C.c_int]),
     "emd_nve_initial": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_double, C.c_int]),
     "emd_nve_final": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_int]),
     "emd_nve_final_initial": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_double, C.c_int]),
@@ -152,6 +163,7 @@ APP_PROTOS = {
     "emd_app_time": (C.c_double, [vp, C.c_int]),
     "emd_app_download": (C.c_int, [vp, dptr, dptr, dptr, iptr, iptr, dptr]),
     "emd_app_dump": (C.c_int, [vp, C.c_char_p]),
+    "emd_app_dump_neigh": (C.c_int, [vp, C.c_char_p]),
     "emd_app_ctx": (vp, [vp]),
     "emd_app_system": (P(EmdSystem), [vp]),
     "emd_app_neigh": (P(EmdNeigh), [vp]),

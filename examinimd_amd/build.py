@@ -32,7 +32,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["emd_core.hip", "emd_comm.hip", "emd_bin_neigh.hip", "emd_lj.hip",
-               "emd_snap.hip", "emd_batch.hip", "emd_adjoint.hip"]
+               "emd_snap.hip", "emd_batch.hip"]
 
 
 def _run(cmd, cwd=None):
@@ -44,7 +44,7 @@ def build_hip(jobs=8):
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
-    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = HIP_SOURCES
     objs = []
     procs = []
     for s in srcs:

@@ -317,6 +317,58 @@ extern "C" emd_status emd_binning(emd_ctx *ctx, emd_system *s, double dx_in, dou
   return bin_finish(ctx, s, b, r0, n, sort);
 }
 
+// bins of another binning (the reference's View<int***> bincount and
+// binoffsets, LayoutLeft on the GPU: (ix, iy, iz) at ix + nbx (iy + nby iz))
+// into this library's x-major arrays
+__global__ void __launch_bounds__(256)
+k_bins_import(const int *__restrict__ cnt, const int *__restrict__ off, int layout, int nbx, int nby,
+              int nbz, int *__restrict__ out_cnt, int *__restrict__ out_off) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nbx * nby * nbz) return;
+  const int iz = (int)(t % nbz), iy = (int)((t / nbz) % nby), ix = (int)(t / ((long)nbz * nby));
+  const long src = layout == EMD_LAYOUT_LEFT ? ix + (long)nbx * (iy + (long)nby * iz) : t;
+  out_cnt[t] = cnt[src];
+  out_off[t] = off[src];
+}
+
+extern "C" emd_status emd_bins_import(emd_ctx *ctx, int nbinx, int nbiny, int nbinz, int nhalo,
+                                      const double min[3], const double max[3],
+                                      const int *bincount, const int *binoffsets, int layout,
+                                      const int *permute, int n_range, emd_bins *b) {
+  if (!ctx || !b || !min || !max || !bincount || !binoffsets || (n_range > 0 && !permute))
+    return EMD_ERR_ARG;
+  EMD_ARG(ctx, layout == EMD_LAYOUT_RIGHT || layout == EMD_LAYOUT_LEFT, "unknown layout");
+  EMD_ARG(ctx, nbinx > 0 && nbiny > 0 && nbinz > 0 && nhalo >= 0 && n_range >= 0, "bad grid");
+  const long nbt = (long)nbinx * nbiny * nbinz;
+  EMD_ARG(ctx, nbt < (1l << 30), "bin grid too large");
+  b->nbinx = nbinx; b->nbiny = nbiny; b->nbinz = nbinz; b->nhalo = nhalo;
+  b->minx = min[0]; b->miny = min[1]; b->minz = min[2];
+  b->maxx = max[0]; b->maxy = max[1]; b->maxz = max[2];
+  b->nbins = (int)nbt;
+  b->n_range = n_range;
+  int cap_b2 = b->cap_bins;
+  EMD_TRY(emd_ensure_int(ctx, &b->bincount, &b->cap_bins, nbt));
+  EMD_TRY(emd_ensure_int(ctx, &b->binoffsets, &cap_b2, nbt));
+  b->cap_bins = b->cap_bins < cap_b2 ? b->cap_bins : cap_b2;
+  int cap_r2 = b->cap_range;
+  EMD_TRY(emd_ensure_int(ctx, &b->permute, &b->cap_range, n_range));
+  EMD_TRY(emd_ensure_int(ctx, &b->binid, &cap_r2, n_range));
+  b->cap_range = b->cap_range < cap_r2 ? b->cap_range : cap_r2;
+  EMD_LAUNCH(ctx, "bins_import", k_bins_import, emd_blocks(nbt, 256), 256, 0, bincount,
+             binoffsets, layout, nbinx, nbiny, nbinz, b->bincount, b->binoffsets);
+  if (n_range > 0)
+    EMD_HIP(ctx, hipMemcpyAsync(b->permute, permute, (size_t)n_range * sizeof(int),
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  EMD_LAUNCH(ctx, "bins_import", k_fill_int, 1, 64, 0, ctx->d_small + 1, 1, 0);
+  const int nbm = emd_blocks(nbt, 256) < 512 ? emd_blocks(nbt, 256) : 512;
+  EMD_LAUNCH(ctx, "bins_import", k_max_int, nbm, 256, 0, b->bincount, (int)nbt, ctx->d_small + 1);
+  EMD_HIP(ctx, hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, sizeof(int),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  EMD_TRY(emd_wait(ctx));
+  b->max_count = ctx->h_small[1];
+  return EMD_OK;
+}
+
 extern "C" emd_status emd_bins_free(emd_ctx *ctx, emd_bins *b) {
   if (!ctx || !b) return EMD_ERR_ARG;
   hipStreamSynchronize(ctx->stream);
@@ -341,8 +393,7 @@ __global__ void __launch_bounds__(NB_THREADS)
 k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
         const int *__restrict__ binoffsets, const int *__restrict__ permute, neigh_geom g,
         int n_local, double cutsq, int newton, int *__restrict__ num, const int *__restrict__ row,
-        int pitch, int *__restrict__ entries, unsigned short *__restrict__ slot,
-        int *__restrict__ cursor, int *__restrict__ maxrow, int cap, int write_num) {
+        int pitch, int *__restrict__ entries, int *__restrict__ cursor, int *__restrict__ maxrow, int cap, int write_num) {
 #pragma clang fp contract(off)
   __shared__ double s_x[NB_MAXC], s_y[NB_MAXC], s_z[NB_MAXC];
   __shared__ int s_j[NB_MAXC];
@@ -417,7 +468,6 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
       size_t base = 0;
       if (FILL) base = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
       int *const rowe = entries + base;
-      unsigned short *const rows = slot ? slot + base : nullptr;
       // candidate test at staged position c (the reference's predicate,
       // neighbor_csr.h:206 full / 290 half), branch-free.  Only the last
       // iteration of a chunk (TAIL) can reach positions past the staged
@@ -437,14 +487,11 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
       };
       // kept candidates go to the row in candidate order: the lane's slot is
       // `at` plus the kept lanes below it (mbcnt)
-      auto put = [&](bool keep, unsigned long long m, int at, int c, int j) {
+      auto put = [&](bool keep, unsigned long long m, int at, int j) {
         if (FILL && keep) {
           const unsigned pos = __builtin_amdgcn_mbcnt_hi(
               (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)at));
-          if ((int)pos < cap) {
-            rowe[pos] = j;
-            if (rows) rows[pos] = (unsigned short)(cs + c);
-          }
+          if ((int)pos < cap) rowe[pos] = j;
         }
       };
       // two 64-candidate groups per iteration: both groups' LDS reads are
@@ -458,9 +505,9 @@ k_neigh(const double *__restrict__ x, const int *__restrict__ bincount,
         const bool kb = test(tail, cb, jb, xb, yb, zb);
         const unsigned long long ma = __builtin_amdgcn_ballot_w64(ka);
         const unsigned long long mb = __builtin_amdgcn_ballot_w64(kb);
-        put(ka, ma, run, ca, ja);
+        put(ka, ma, run, ja);
         const int runb = run + __popcll(ma);
-        put(kb, mb, runb, cb, jb);
+        put(kb, mb, runb, jb);
         run = runb + __popcll(mb);
       };
       int c0 = 0;
@@ -504,18 +551,15 @@ __global__ void __launch_bounds__(256) k_max_int(const int *__restrict__ a, int 
 // padded rows (pitch P) -> CSR rows, 16 lanes per row
 constexpr int COMPACT_LANES = 16;
 __global__ void __launch_bounds__(256)
-k_compact_rows(const int *__restrict__ tmp, const unsigned short *__restrict__ tslot, int P,
-               const int *__restrict__ num, const int *__restrict__ row, int n,
-               int *__restrict__ entries, unsigned short *__restrict__ slot) {
+k_compact_rows(const int *__restrict__ tmp, int P, const int *__restrict__ num,
+               const int *__restrict__ row, int n, int *__restrict__ entries) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int r = (int)(t / COMPACT_LANES), lane = (int)(t % COMPACT_LANES);
   if (r >= n) return;
   const int c = num[r];
   const size_t src = (size_t)r * P, dst = (size_t)row[r];
-  for (int k = lane; k < c; k += COMPACT_LANES) {
+  for (int k = lane; k < c; k += COMPACT_LANES)
     entries[dst + k] = __builtin_nontemporal_load(tmp + src + k);
-    if (slot) slot[dst + k] = tslot[src + k];
-  }
 }
 
 struct row_offsets_op {
@@ -528,48 +572,6 @@ struct row_offsets_op {
     if (i == n - 1) row[n] = excl + v;
   }
 };
-
-// Force sweep order: interior bin columns (bx, by; all interior bz, which
-// are consecutive bins and so one contiguous permute range) visited in T x T
-// tiles of columns, tiles and the columns inside a tile in row-major order.
-// Slot s of the enumeration is (tile, column in tile); slots outside the
-// grid are empty.  Interior bins hold every local atom and, through the
-// cell/1000 margin of the bin bounds, possibly a few ghosts.
-__device__ __forceinline__ int visit_column(const neigh_geom &g, int T, int s, int *first_bin) {
-  const int nty = (g.niy + T - 1) / T;
-  const int tile = s / (T * T), in = s % (T * T);
-  const int cx = (tile / nty) * T + in / T, cy = (tile % nty) * T + in % T;
-  if (cx >= g.nix || cy >= g.niy) return 0;
-  *first_bin = ((cx + g.nhalo) * g.nby + (cy + g.nhalo)) * g.nbz + g.nhalo;
-  return 1;
-}
-
-struct visit_len_op {
-  neigh_geom g;
-  int T;
-  const int *bincount, *binoffsets;
-  int *start;  // [nslots + 1] exclusive prefix of the column lengths
-  __device__ int value(int s) const {
-    int b0;
-    if (!visit_column(g, T, s, &b0)) return 0;
-    const int b1 = b0 + g.niz - 1;
-    return binoffsets[b1] + bincount[b1] - binoffsets[b0];
-  }
-  __device__ void emit(int s, int excl, int) const { start[s] = excl; }
-};
-
-// one block per column slot: copy the column's atoms into the order
-__global__ void __launch_bounds__(256)
-k_visit_fill(neigh_geom g, int T, const int *__restrict__ binoffsets,
-             const int *__restrict__ bincount, const int *__restrict__ permute,
-             const int *__restrict__ start, int *__restrict__ visit) {
-  int b0;
-  if (!visit_column(g, T, blockIdx.x, &b0)) return;
-  const int b1 = b0 + g.niz - 1;
-  const int o = binoffsets[b0], len = binoffsets[b1] + bincount[b1] - o;
-  const int dst = start[blockIdx.x];
-  for (int k = threadIdx.x; k < len; k += blockDim.x) visit[dst + k] = permute[o + k];
-}
 
 extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const emd_bins *b,
                                       double neigh_cut, int half, int newton, int layout,
@@ -585,7 +587,6 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
   const long nib = (long)g.nix * g.niy * g.niz;
   EMD_ARG(ctx, b->nhalo >= 1, "neighbour build needs a halo of at least one bin");
   const double cutsq = neigh_cut * neigh_cut;
-  nl->bins = b;
 
   int rcap = nl->cap_rows;
   EMD_TRY(emd_ensure_int(ctx, &nl->num, &nl->cap_rows, n + 1));
@@ -596,28 +597,20 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
   int *cursor = (int *)ctx->d_tmp2;
   if (n > 0) EMD_LAUNCH(ctx, "neigh_count", k_fill_int, emd_blocks(n, 256), 256, 0, nl->num, n, 0);
   EMD_LAUNCH(ctx, "neigh_count", k_fill_int, 1, 64, 0, ctx->d_small + 4, 1, 0);
-  const bool with_slot = nl->want_slot && !half && ctx->lj_tile;
-  if (!with_slot && nl->slot) {
-    // a list built without the tile view carries none
-    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    EMD_HIP(ctx, hipFree(nl->slot));
-    nl->slot = nullptr;
-    nl->cap_slot = 0;
-  }
 
   // k_neigh launch: fill?, CSR row offsets (or null: rows at i * pitch),
-  // pitch, entries, slots, row capacity, record full row lengths in num?
-#define NB_LAUNCH(name, F, ROW, PITCH, ENT, SLOT, CAP, WNUM)                                  \
+  // pitch, entries, row capacity, record full row lengths in num?
+#define NB_LAUNCH(name, F, ROW, PITCH, ENT, CAP, WNUM)                                        \
   do {                                                                                       \
     if (nib > 0) {                                                                           \
       if (half)                                                                              \
         EMD_LAUNCH(ctx, name, (k_neigh<true, F>), nib, NB_THREADS, 0, s->x, b->bincount,     \
                    b->binoffsets, b->permute, g, n, cutsq, newton, nl->num, ROW, PITCH, ENT, \
-                   SLOT, cursor, ctx->d_small + 4, CAP, WNUM);                               \
+                   cursor, ctx->d_small + 4, CAP, WNUM);                                     \
       else                                                                                   \
         EMD_LAUNCH(ctx, name, (k_neigh<false, F>), nib, NB_THREADS, 0, s->x, b->bincount,    \
                    b->binoffsets, b->permute, g, n, cutsq, newton, nl->num, ROW, PITCH, ENT, \
-                   SLOT, cursor, ctx->d_small + 4, CAP, WNUM);                               \
+                   cursor, ctx->d_small + 4, CAP, WNUM);                                     \
     }                                                                                        \
   } while (0)
 
@@ -636,23 +629,6 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
     P = ((nl->maxneighs + nl->maxneighs / 8 + 16) + 15) / 16 * 16;
   }
   int *pad = nullptr;
-  unsigned short *pad_slot = nullptr;
-  auto ensure_slot = [&](long need) -> emd_status {
-    if (need > nl->cap_slot || !nl->slot) {
-      if (nl->slot) {
-        EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        EMD_HIP(ctx, hipFree(nl->slot));
-      }
-      nl->slot = nullptr;
-      nl->cap_slot = 0;
-      const long want = need + need / 8 + 64;
-      if (hipMalloc(&nl->slot, (size_t)want * sizeof(unsigned short)) != hipSuccess)
-        return emd_set_error(ctx, EMD_ERR_ALLOC, "neighbour slots: hipMalloc of %ld failed",
-                             want);
-      nl->cap_slot = want;
-    }
-    return EMD_OK;
-  };
   auto ensure_entries = [&](long need) -> emd_status {
     if (need > nl->cap_entries || !nl->entries) {
       if (nl->entries) {
@@ -666,25 +642,20 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
                              want);
       nl->cap_entries = want;
     }
-    if (with_slot) EMD_TRY(ensure_slot(need));
     return EMD_OK;
   };
   if (P > 0 && n > 0) {
     if (layout == EMD_NEIGH_2D) {
       EMD_TRY(ensure_entries((long)n * P));
       pad = nl->entries;
-      pad_slot = with_slot ? nl->slot : nullptr;
     } else {
-      const size_t ib = (size_t)n * P * sizeof(int), sb = with_slot ? (size_t)n * P * 2 : 0;
-      EMD_TRY(emd_ensure(ctx, &ctx->d_tmp, &ctx->tmp_cap, ib + sb + 256));
+      EMD_TRY(emd_ensure(ctx, &ctx->d_tmp, &ctx->tmp_cap, (size_t)n * P * sizeof(int) + 256));
       pad = (int *)ctx->d_tmp;
-      pad_slot = with_slot ? (unsigned short *)((char *)ctx->d_tmp + ((ib + 255) / 256) * 256)
-                           : nullptr;
     }
-    NB_LAUNCH("neigh_fill", true, nullptr, P, pad, pad_slot, P, 1);
+    NB_LAUNCH("neigh_fill", true, nullptr, P, pad, P, 1);
   }
   if (P == 0 || n == 0)
-    NB_LAUNCH("neigh_count", false, nullptr, 0, nullptr, nullptr, 0, 0);
+    NB_LAUNCH("neigh_count", false, nullptr, 0, nullptr, 0, 0);
   // row offsets and the longest row
   if (n > 0) {
     const int nbm = emd_blocks(n, 256) < 512 ? emd_blocks(n, 256) : 512;
@@ -703,46 +674,20 @@ extern "C" emd_status emd_neigh_build(emd_ctx *ctx, const emd_system *s, const e
     if (!fits) {
       while (maxrow > nl->maxneighs) nl->maxneighs = (int)(maxrow * 1.2);
       EMD_TRY(ensure_entries((long)n * nl->maxneighs));
-      NB_LAUNCH("neigh_fill", true, nullptr, nl->maxneighs, nl->entries,
-                with_slot ? nl->slot : nullptr, nl->maxneighs, 0);
+      NB_LAUNCH("neigh_fill", true, nullptr, nl->maxneighs, nl->entries, nl->maxneighs, 0);
     }
   } else {
     EMD_TRY(ensure_entries(total));
     if (P > 0 && maxrow <= P) {
       if (n > 0)
         EMD_LAUNCH(ctx, "neigh_compact", k_compact_rows, emd_blocks((long)n * COMPACT_LANES, 256), 256,
-                   0, pad, pad_slot, P, nl->num, nl->row, n, nl->entries,
-                   with_slot ? nl->slot : nullptr);
+                   0, pad, P, nl->num, nl->row, n, nl->entries);
     } else {
-      NB_LAUNCH("neigh_fill", true, nl->row, 0, nl->entries, with_slot ? nl->slot : nullptr,
-                0x7fffffff, 0);
+      NB_LAUNCH("neigh_fill", true, nl->row, 0, nl->entries, 0x7fffffff, 0);
     }
     nl->maxneighs = maxrow;
   }
 #undef NB_LAUNCH
-  // sweep order of full lists (every local atom sits in an interior bin)
-  const int T = ctx->visit_tile;
-  nl->n_visit = 0;
-  if (!half && T > 0 && n > 0 && nib > 0) {
-    const int nslots = ((g.nix + T - 1) / T) * ((g.niy + T - 1) / T) * T * T;
-    EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)(nslots + 1) * sizeof(int)));
-    int *start = (int *)ctx->d_tmp2;
-    visit_len_op op{g, T, b->bincount, b->binoffsets, start};
-    int placed = 0;
-    EMD_TRY(emd_scan(ctx, op, nslots, "neigh_visit", &placed));
-    if (placed < n)
-      return emd_set_error(ctx, EMD_ERR_STATE,
-                           "interior bins hold %d atoms but there are %d local atoms", placed, n);
-    EMD_TRY(emd_ensure_int(ctx, &nl->visit, &nl->cap_visit, placed));
-    EMD_LAUNCH(ctx, "neigh_visit", k_visit_fill, nslots, 256, 0, g, T, b->binoffsets,
-               b->bincount, b->permute, start, nl->visit);
-    nl->n_visit = placed;
-  } else if (nl->visit) {
-    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    EMD_HIP(ctx, hipFree(nl->visit));
-    nl->visit = nullptr;
-    nl->cap_visit = 0;
-  }
   return EMD_OK;
 }
 
@@ -752,14 +697,7 @@ extern "C" emd_status emd_neigh_free(emd_ctx *ctx, emd_neigh *nl) {
   if (nl->row) hipFree(nl->row);
   if (nl->num) hipFree(nl->num);
   if (nl->entries) hipFree(nl->entries);
-  if (nl->slot) hipFree(nl->slot);
-  if (nl->visit) hipFree(nl->visit);
-  nl->visit = nullptr;
-  nl->cap_visit = 0;
   nl->row = nl->num = nl->entries = nullptr;
-  nl->slot = nullptr;
-  nl->cap_slot = 0;
-  nl->bins = nullptr;
   nl->cap_entries = 0;
   nl->cap_rows = 0;
   return EMD_OK;

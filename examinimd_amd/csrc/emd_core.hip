@@ -1,4 +1,6 @@
 // emd_core.hip -- context, device memory, kernel timing and the System image.
+#include <mutex>
+
 #include "emd_internal.hpp"
 
 // -------------------------------------------------------------- errors
@@ -58,7 +60,10 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
   bool ok = hipMalloc(&ctx->d_small, 64 * sizeof(int)) == hipSuccess &&
             hipMalloc(&ctx->d_dsmall, 64 * sizeof(double)) == hipSuccess &&
             hipHostMalloc(&ctx->h_small, 64 * sizeof(int), hipHostMallocDefault) == hipSuccess &&
-            hipHostMalloc(&ctx->h_dsmall, 64 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+            hipHostMalloc(&ctx->h_dsmall, 64 * sizeof(double), hipHostMallocDefault) == hipSuccess &&
+            hipMalloc(&ctx->d_ljtab, EMD_LJ_TABLE_BYTES) == hipSuccess &&
+            hipHostMalloc(&ctx->h_ljtab, EMD_LJ_TABLE_BYTES, hipHostMallocDefault) == hipSuccess &&
+            hipEventCreateWithFlags(&ctx->ljtab_ev, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     emd_destroy(ctx);
     return EMD_ERR_ALLOC;
@@ -72,14 +77,6 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
   for (int p = 0; p < 2; p++) {
     const int k = ctx->lj_lanes[p];
     if (!(k == 0 || k == 1 || k == 2 || k == 4 || k == 8 || k == 16)) ctx->lj_lanes[p] = 0;
-  }
-  // bin-tile LJ kernel: EMD_LJ_TILE=0 switches it off, EMD_LJ_TILE_LANES
-  // picks its lanes per atom (4, 8 or 16)
-  if (const char *e = getenv("EMD_LJ_TILE")) ctx->lj_tile = atoi(e) != 0;
-  // column-tile width of the force sweep order built with full lists
-  if (const char *e = getenv("EMD_VISIT_TILE")) {
-    const int t = atoi(e);
-    if (t >= 0 && t <= 64) ctx->visit_tile = t;
   }
   // ghost force fold of a small halo (<= 16384 ghosts) in one launch (1) or
   // one launch per phase (0)
@@ -95,28 +92,10 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
     const int v = atoi(e);
     ctx->lj_prio = (v >= 0 && v <= 2) ? v : -1;
   }
-  if (const char *e = getenv("EMD_SNAP_YFLAT")) ctx->snap_yflat = atoi(e) != 0;
   if (const char *e = getenv("EMD_SNAP_UI_GRP")) ctx->snap_ui_grp = atoi(e) != 0;
   if (const char *e = getenv("EMD_SNAP_UG_WAVES")) {
     const int w = atoi(e);
     ctx->snap_ug_waves = (w == 2 || w == 4) ? w : 0;
-  }
-  if (const char *e = getenv("EMD_SNAP_FIJ_LANES")) ctx->snap_fij_lanes = atoi(e) == 32 ? 32 : 16;
-  // pipelined LJ kernel: blocks per XCD (0, the default, keeps k_lj_full;
-  // measured slower at LJ 4M: 0.83 vs 0.74 ms for 192 blocks per XCD)
-  if (const char *e = getenv("EMD_LJ_PIPE_BLOCKS")) {
-    const int b = atoi(e);
-    if (b >= 0 && b <= 4096) ctx->lj_pipe_blocks = b;
-  }
-  // entries per lane and memory round of the LJ kernel: 16 lanes 2..6,
-  // 8 lanes 5, 8 or 10
-  if (const char *e = getenv("EMD_LJ_UNROLL")) {
-    const int u = atoi(e);
-    if ((u >= 2 && u <= 6) || u == 8 || u == 10) ctx->lj_unroll = u;
-  }
-  if (const char *e = getenv("EMD_LJ_TILE_LANES")) {
-    const int k = atoi(e);
-    if (k == 4 || k == 8 || k == 16) ctx->lj_tile_lanes = k;
   }
   *out = ctx;
   return EMD_OK;
@@ -135,7 +114,9 @@ extern "C" emd_status emd_destroy(emd_ctx *ctx) {
   if (ctx->d_scan) hipFree(ctx->d_scan);
   if (ctx->d_tmp) hipFree(ctx->d_tmp);
   if (ctx->d_tmp2) hipFree(ctx->d_tmp2);
-  if (ctx->d_red) hipFree(ctx->d_red);
+  if (ctx->d_ljtab) hipFree(ctx->d_ljtab);
+  if (ctx->h_ljtab) hipHostFree(ctx->h_ljtab);
+  if (ctx->ljtab_ev) hipEventDestroy(ctx->ljtab_ev);
   if (ctx->wait_ev) hipEventDestroy(ctx->wait_ev);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -432,6 +413,81 @@ extern "C" emd_status emd_system_download(emd_ctx *ctx, const emd_system *s, int
   if (id) EMD_HIP(ctx, hipMemcpyAsync(id, s->id, (size_t)n * 4, hipMemcpyDeviceToHost, st));
   if (q) EMD_HIP(ctx, hipMemcpyAsync(q, s->q, (size_t)n * 8, hipMemcpyDeviceToHost, st));
   EMD_HIP(ctx, hipStreamSynchronize(st));
+  return EMD_OK;
+}
+
+// ------------------------------------------------- reference View layouts
+// element (i, d) of a [n][3] double array: LayoutRight a[3 i + d],
+// LayoutLeft a[i + d ld]
+__device__ __forceinline__ size_t vec3_at(int layout, long ld, long i, int d) {
+  return layout == EMD_LAYOUT_LEFT ? (size_t)i + (size_t)d * (size_t)ld : 3 * (size_t)i + d;
+}
+
+// one thread per element, threads ordered by the destination's storage so
+// the stores are coalesced (the loads are strided when the layouts differ)
+__global__ void __launch_bounds__(256)
+k_vec3_copy(double *__restrict__ dst, int dl, long dld, const double *__restrict__ src, int sl,
+            long sld, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  long i;
+  int d;
+  if (dl == EMD_LAYOUT_LEFT) { d = (int)(t / n); i = t - (long)d * n; }
+  else { i = t / 3; d = (int)(t - 3 * i); }
+  dst[vec3_at(dl, dld, i, d)] = src[vec3_at(sl, sld, i, d)];
+}
+
+static emd_status vec3_copy(emd_ctx *ctx, double *dst, int dl, long dld, const double *src, int sl,
+                            long sld, int n) {
+  if (n <= 0) return EMD_OK;
+  EMD_ARG(ctx, (dl == EMD_LAYOUT_RIGHT || dl == EMD_LAYOUT_LEFT) &&
+                   (sl == EMD_LAYOUT_RIGHT || sl == EMD_LAYOUT_LEFT), "unknown layout");
+  EMD_ARG(ctx, (dl != EMD_LAYOUT_LEFT || dld >= n) && (sl != EMD_LAYOUT_LEFT || sld >= n),
+          "LayoutLeft leading dimension below the atom count");
+  if (dl == EMD_LAYOUT_RIGHT && sl == EMD_LAYOUT_RIGHT) {
+    EMD_HIP(ctx, hipMemcpyAsync(dst, src, (size_t)n * 24, hipMemcpyDeviceToDevice, ctx->stream));
+    return EMD_OK;
+  }
+  EMD_LAUNCH(ctx, "layout_copy", k_vec3_copy, emd_blocks(3l * n, 256), 256, 0, dst, dl, dld, src,
+             sl, sld, (long)n);
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_system_import(emd_ctx *ctx, emd_system *s, int n, emd_vec3_view x,
+                                        emd_vec3_view v, emd_vec3_view f, const int *type,
+                                        const int *id, const double *q) {
+  if (!ctx || !s || n < 0) return EMD_ERR_ARG;
+  EMD_TRY(emd_system_grow(ctx, s, n));
+  hipStream_t st = ctx->stream;
+  if (x.data) EMD_TRY(vec3_copy(ctx, s->x, EMD_LAYOUT_RIGHT, 0, x.data, x.layout, x.ld, n));
+  if (v.data) EMD_TRY(vec3_copy(ctx, s->v, EMD_LAYOUT_RIGHT, 0, v.data, v.layout, v.ld, n));
+  if (f.data) EMD_TRY(vec3_copy(ctx, s->f, EMD_LAYOUT_RIGHT, 0, f.data, f.layout, f.ld, n));
+  if (type) EMD_HIP(ctx, hipMemcpyAsync(s->type, type, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  if (id) EMD_HIP(ctx, hipMemcpyAsync(s->id, id, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  if (q) EMD_HIP(ctx, hipMemcpyAsync(s->q, q, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_system_export(emd_ctx *ctx, const emd_system *s, int n,
+                                        emd_vec3_view x, emd_vec3_view v, emd_vec3_view f) {
+  if (!ctx || !s || n < 0 || n > s->n_max) return EMD_ERR_ARG;
+  if (x.data) EMD_TRY(vec3_copy(ctx, x.data, x.layout, x.ld, s->x, EMD_LAYOUT_RIGHT, 0, n));
+  if (v.data) EMD_TRY(vec3_copy(ctx, v.data, v.layout, v.ld, s->v, EMD_LAYOUT_RIGHT, 0, n));
+  if (f.data) EMD_TRY(vec3_copy(ctx, f.data, f.layout, f.ld, s->f, EMD_LAYOUT_RIGHT, 0, n));
+  return EMD_OK;
+}
+
+// one context per device per process, created on first use
+extern "C" emd_status emd_ctx_shared(int device, emd_ctx **out) {
+  static std::mutex lock;
+  static emd_ctx *shared[64] = {nullptr};
+  if (!out || device < 0 || device >= 64) return EMD_ERR_ARG;
+  std::lock_guard<std::mutex> g(lock);
+  if (!shared[device]) {
+    const emd_status st = emd_create(device, &shared[device]);
+    if (st != EMD_OK) return st;
+  }
+  *out = shared[device];
   return EMD_OK;
 }
 

@@ -23,6 +23,9 @@ struct emd_prof_rec {
   hipEvent_t start, stop;
 };
 
+// bytes reserved per context for the LJ per-type-pair table
+constexpr size_t EMD_LJ_TABLE_BYTES = 8 * 1024;
+
 struct emd_ctx {
   int device = 0;
   int num_cu = 256;
@@ -43,8 +46,13 @@ struct emd_ctx {
   size_t tmp_cap = 0;
   void *d_tmp2 = nullptr;
   size_t tmp2_cap = 0;
-  double *d_red = nullptr;     // reduction partials
-  size_t red_cap = 0;
+  // LJ per-type-pair table: device copy, pinned staging copy of what was
+  // last uploaded (compared before each use, uploaded only on change), and
+  // the event that marks when the staging copy has been read
+  void *d_ljtab = nullptr;
+  void *h_ljtab = nullptr;
+  hipEvent_t ljtab_ev = nullptr;
+  bool ljtab_valid = false;
 
   // host waits: an event recorded on the stream and polled, so a wait for a
   // count ends as soon as the copy lands (no blocking-sync wake-up latency)
@@ -66,18 +74,11 @@ struct emd_ctx {
   int fold_one = 1;        // small halos: fold all six phases in one launch
   int halo_one = 1;        // small systems: the six exchange_halo phases in one launch
   int bin_one = 1;         // small ranges: binning in one launch
-  int snap_yflat = 1;      // SNAP Y from the flat term stream (0: per-entry kernel)
   int snap_ui_grp = 1;     // SNAP U_tot with 16-lane neighbour groups (twojmax 8)
   int snap_ug_waves = 0;   // wavefronts per block of that kernel (2, 4; 0: by precision)
-  int snap_fij_lanes = 16; // lanes per neighbour in the SNAP force kernel (16 or 32)
-  int lj_tile = 0;        // use the bin-tile LJ kernel when the list carries slots
-  int lj_tile_lanes = 8;  // lanes per atom of the bin-tile kernel
-  int lj_unroll = 5;      // entries per lane per round, 16-lane CSR kernel
   int lj_prio = -1;       // full-list LJ kernel: raise wave priority while a round's index
                           // loads and gathers issue (2), around the gathers only (1),
                           // never (0); -1: 2 for mixed, 0 for FP64 (as measured)
-  int lj_pipe_blocks = 0;    // blocks per XCD of the pipelined LJ kernel (0 = off)
-  int visit_tile = 0;     // column-tile width of the force sweep order (0 = index order)
 };
 
 emd_status emd_set_error(emd_ctx *ctx, emd_status code, const char *fmt, ...);

@@ -79,18 +79,13 @@ __global__ void __launch_bounds__(LJ_THREADS)
 k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
           const int *__restrict__ row, const int *__restrict__ num,
           const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
-          double cutsq_0, double rep_0, const lj_table *__restrict__ tab,
-          const int *__restrict__ visit, int n_visit, int store, double *__restrict__ f,
-          float *__restrict__ f32, int prio) {
+          double cutsq_0, double rep_0, const lj_table *__restrict__ tab, int store,
+          double *__restrict__ f, float *__restrict__ f32, int prio) {
 #pragma clang fp contract(off)
   typedef typename lj_acc<MIXED>::T T;
   const int tile = emd_xcd_tile(blockIdx.x, gridDim.x);
   const int gid = tile * LJ_THREADS + threadIdx.x;
-  const int a = gid / K, sub = gid % K;
-  // atoms in the list's sweep order (column tiles) when it has one; the
-  // order may name a few ghosts, which have no row
-  if (a >= (visit ? n_visit : n)) return;
-  const int i = visit ? visit[a] : a;
+  const int i = gid / K, sub = gid % K;
   if (i >= n) return;
   const double x_i = x[3 * (size_t)i], y_i = x[3 * (size_t)i + 1], z_i = x[3 * (size_t)i + 2];
   const int type_i = ONE_TYPE ? 0 : type[i];
@@ -197,250 +192,6 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
   }
 }
 
-// Software-pipelined form of k_lj_full (one type, full CSR or 2D rows).
-// Each wavefront walks groups of 64/K atoms through its XCD's contiguous
-// share of the atoms (blocks are dealt to the 8 XCDs round-robin; the share
-// only affects cache locality, every group is done exactly once).  While a
-// wavefront gathers positions for group t and computes it, the index loads
-// of group t+1 and the row offsets of group t+2 are already in flight, so
-// the latency of the HBM index stream overlaps arithmetic.  The first U*K
-// entries of a row take the pipelined round, longer rows finish in a plain
-// loop.  Per lane the entries and their order are those of k_lj_full<K,U>
-// and the reduction is the same: the FP64 result is bitwise equal.
-constexpr int PIPE_WAVES = 4;
-
-template <int K, int U, bool MIXED, bool CSR>
-__global__ void __launch_bounds__(64 * PIPE_WAVES)
-k_lj_pipe(const double *__restrict__ x, const int *__restrict__ row, const int *__restrict__ num,
-          const int *__restrict__ entries, int pitch, int n, double lj1, double lj2,
-          double cutsq, int groups_per_xcd, int waves_per_xcd, double *__restrict__ f,
-          float *__restrict__ f32) {
-#pragma clang fp contract(off)
-  typedef typename lj_acc<MIXED>::T T;
-  constexpr int G = 64 / K;  // atoms per wavefront group
-  const int lane = threadIdx.x % 64, sub = lane % K, at = lane / K;
-  const int xcd = blockIdx.x % 8;
-  const int g_lo = xcd * groups_per_xcd;
-  const int g_hi = min((n + G - 1) / G, g_lo + groups_per_xcd);
-  int g = g_lo + (blockIdx.x / 8) * PIPE_WAVES + threadIdx.x / 64;
-  if (g >= g_hi) return;  // uniform over the wavefront
-  // All loads below are unconditional, from addresses kept in range by
-  // clamping (padding atoms of the last group read atom n-1, slots past a
-  // row's end re-read its start, which lies inside the entries allocation),
-  // and invalid results are discarded by selects, so no branch separates a
-  // load from the loads issued after it.
-  auto row_info = [&](int gg, long &base, int &cnt) {
-    const int i = gg * G + at;
-    const int ic = i < n ? i : n - 1;
-    base = CSR ? (long)row[ic] : (long)ic * pitch;
-    const int c = num[ic];
-    cnt = i < n ? c : 0;
-  };
-  auto load_idx = [&](int gg, long base, int cnt, int *j, bool *ok) {
-    const int i = gg * G + at;
-    const int ic = i < n ? i : n - 1;
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int e = sub + u * K;
-      ok[u] = e < cnt;
-      const int v = __builtin_nontemporal_load(entries + base + (ok[u] ? e : 0));
-      j[u] = ok[u] ? v : ic;
-    }
-  };
-  // groups past the end are replaced by the current one for the prefetches
-  // (their results go unused), keeping every load unconditional
-  long b0, b1;
-  int c0, c1;
-  row_info(g, b0, c0);
-  int j0[U];
-  bool ok0[U];
-  load_idx(g, b0, c0, j0, ok0);
-  int g1 = g + waves_per_xcd;
-  row_info(g1 < g_hi ? g1 : g, b1, c1);
-  while (true) {
-    const int i = g * G + at;
-    const int ic = i < n ? i : n - 1;
-    // gathers of group g, its own position and current force
-    double xj[U], yj[U], zj[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      xj[u] = x[3 * (size_t)j0[u]];
-      yj[u] = x[3 * (size_t)j0[u] + 1];
-      zj[u] = x[3 * (size_t)j0[u] + 2];
-    }
-    const double x_i = x[3 * (size_t)ic], y_i = x[3 * (size_t)ic + 1], z_i = x[3 * (size_t)ic + 2];
-    T f0x, f0y, f0z;
-    if (MIXED) {
-      f0x = f32[3 * (size_t)ic];
-      f0y = f32[3 * (size_t)ic + 1];
-      f0z = f32[3 * (size_t)ic + 2];
-    } else {
-      f0x = f[3 * (size_t)ic];
-      f0y = f[3 * (size_t)ic + 1];
-      f0z = f[3 * (size_t)ic + 2];
-    }
-    // keep the prefetches below behind this group's loads in the memory
-    // queue (waits on vector loads retire in issue order)
-    __builtin_amdgcn_sched_barrier(0);
-    // next group's indices and the row offsets of the one after
-    int j1[U];
-    bool ok1[U];
-    const int g2 = g1 + waves_per_xcd;
-    long b2;
-    int c2;
-    load_idx(g1 < g_hi ? g1 : g, b1, c1, j1, ok1);
-    row_info(g2 < g_hi ? g2 : g, b2, c2);
-    T fxi = 0, fyi = 0, fzi = 0;
-    auto pair = [&](bool ok, double xj_, double yj_, double zj_) {
-      const double dx = x_i - xj_;
-      const double dy = y_i - yj_;
-      const double dz = z_i - zj_;
-      const double rsq = dx * dx + dy * dy + dz * dz;
-      if (ok && rsq < cutsq) {
-        if (MIXED) {
-          const float fpair = emd_lj_fpair_mixed(rsq, (float)lj1, (float)lj2);
-          fxi = __builtin_fmaf((float)dx, fpair, fxi);
-          fyi = __builtin_fmaf((float)dy, fpair, fyi);
-          fzi = __builtin_fmaf((float)dz, fpair, fzi);
-        } else {
-          const double r2inv = 1.0 / rsq;
-          const double r6inv = r2inv * r2inv * r2inv;
-          const double fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
-          fxi += dx * fpair;
-          fyi += dy * fpair;
-          fzi += dz * fpair;
-        }
-      }
-    };
-#pragma unroll
-    for (int u = 0; u < U; u++) pair(ok0[u], xj[u], yj[u], zj[u]);
-    // rows longer than one round
-    for (int jj = sub + U * K; jj < c0; jj += K) {
-      const int j = __builtin_nontemporal_load(entries + b0 + jj);
-      pair(true, x[3 * (size_t)j], x[3 * (size_t)j + 1], x[3 * (size_t)j + 2]);
-    }
-    if (K > 1) {
-      fxi = group_sum<K>(fxi);
-      fyi = group_sum<K>(fyi);
-      fzi = group_sum<K>(fzi);
-    }
-    if (sub == 0 && i < n) {
-      if (MIXED) {
-        f32[3 * (size_t)i] = f0x + (float)fxi;
-        f32[3 * (size_t)i + 1] = f0y + (float)fyi;
-        f32[3 * (size_t)i + 2] = f0z + (float)fzi;
-      } else {
-        f[3 * (size_t)i] = f0x + fxi;
-        f[3 * (size_t)i + 1] = f0y + fyi;
-        f[3 * (size_t)i + 2] = f0z + fzi;
-      }
-    }
-    if (g1 >= g_hi) break;
-    g = g1;
-    g1 = g2;
-    b0 = b1;
-    c0 = c1;
-    b1 = b2;
-    c1 = c2;
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      j0[u] = j1[u];
-      ok0[u] = ok1[u];
-    }
-  }
-}
-
-// Bin-tile form of the same full-list force (one type).  One block per
-// interior bin: the block copies the positions of the bin's 27-bin
-// candidate list (the order the neighbour build enumerated them in) into
-// LDS once, then every row entry's slot addresses its neighbour's position
-// there, so the only global traffic per pair is the 2-byte slot.  The lanes
-// of an atom take the same entries in the same order as k_lj_full with the
-// same K, and reduce with the same shuffles: the FP64 result is bitwise that
-// of k_lj_full<K>.
-template <int K, int U, bool MIXED>
-__global__ void __launch_bounds__(LJ_THREADS)
-k_lj_tile(const double *__restrict__ x, const int *__restrict__ bincount,
-          const int *__restrict__ binoffsets, const int *__restrict__ permute, neigh_geom g,
-          const int *__restrict__ row, const int *__restrict__ num,
-          const unsigned short *__restrict__ slot, int pitch, int n_local, int tcap, double lj1,
-          double lj2, double cutsq, double *__restrict__ f, float *__restrict__ f32) {
-#pragma clang fp contract(off)
-  typedef typename lj_acc<MIXED>::T T;
-  extern __shared__ double s_cand[];
-  double *s_x = s_cand, *s_y = s_cand + tcap, *s_z = s_cand + 2 * tcap;
-  __shared__ int s_off[27], s_pre[28];
-  const int lr = emd_xcd_tile(blockIdx.x, gridDim.x);
-  emd_stencil_setup(g, lr, bincount, binoffsets, s_off, s_pre);
-  const int ncand = s_pre[27];
-  for (int c = threadIdx.x; c < ncand; c += LJ_THREADS) {
-    const int k = emd_stencil_slot(s_pre, c);
-    const size_t j = permute[s_off[k] + (c - s_pre[k])];
-    s_x[c] = x[3 * j];
-    s_y[c] = x[3 * j + 1];
-    s_z[c] = x[3 * j + 2];
-  }
-  __syncthreads();
-  const int c0 = s_pre[13], c_cnt = s_pre[14] - s_pre[13], c_off = s_off[13];
-  const int sub = threadIdx.x % K;
-  for (int a = threadIdx.x / K; a < c_cnt; a += LJ_THREADS / K) {
-    const int i = permute[c_off + a];
-    if (i >= n_local) continue;
-    const double x_i = s_x[c0 + a], y_i = s_y[c0 + a], z_i = s_z[c0 + a];
-    const unsigned short *rowp = slot + (row ? (size_t)row[i] : (size_t)i * (size_t)pitch);
-    const int cnt = num[i];
-    T fxi = 0, fyi = 0, fzi = 0;
-    auto pair = [&](bool ok, int c) {
-      const double dx = x_i - s_x[c];
-      const double dy = y_i - s_y[c];
-      const double dz = z_i - s_z[c];
-      const double rsq = dx * dx + dy * dy + dz * dz;
-      if (ok && rsq < cutsq) {
-        if (MIXED) {
-          const float fpair = emd_lj_fpair_mixed(rsq, (float)lj1, (float)lj2);
-          fxi = __builtin_fmaf((float)dx, fpair, fxi);
-          fyi = __builtin_fmaf((float)dy, fpair, fyi);
-          fzi = __builtin_fmaf((float)dz, fpair, fzi);
-        } else {
-          const double r2inv = 1.0 / rsq;
-          const double r6inv = r2inv * r2inv * r2inv;
-          const double fpair = (r6inv * (lj1 * r6inv - lj2)) * r2inv;
-          fxi += dx * fpair;
-          fyi += dy * fpair;
-          fzi += dz * fpair;
-        }
-      }
-    };
-    for (int jj = sub; jj < cnt; jj += U * K) {
-      int c[U];
-      bool ok[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        ok[u] = jj + u * K < cnt;
-        c[u] = __builtin_nontemporal_load(rowp + (ok[u] ? jj + u * K : cnt - 1));
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) pair(ok[u], c[u]);
-    }
-    if (K > 1) {
-      fxi = group_sum<K>(fxi);
-      fyi = group_sum<K>(fyi);
-      fzi = group_sum<K>(fzi);
-    }
-    if (sub == 0) {
-      if (MIXED) {
-        f32[3 * (size_t)i] += (float)fxi;
-        f32[3 * (size_t)i + 1] += (float)fyi;
-        f32[3 * (size_t)i + 2] += (float)fzi;
-      } else {
-        f[3 * (size_t)i] += fxi;
-        f[3 * (size_t)i + 1] += fyi;
-        f[3 * (size_t)i + 2] += fzi;
-      }
-    }
-  }
-}
-
 // ForceLJNeigh TagHalfNeigh, force_lj_neigh_impl.h:208-254: each pair's
 // force is added to i and subtracted from j (atomically: rows of other
 // atoms reach j too).  LJH_K lanes per atom take every LJH_K-th entry of the
@@ -542,10 +293,7 @@ k_lj_half(const double *__restrict__ x, const int *__restrict__ type, const int 
   }
 }
 
-static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p,
-                               const double (*rep)[16], lj_table **dtab) {
-  static_assert(sizeof(lj_table) <= 8 * 1024, "table size");
-  lj_table h;
+static void fill_table(lj_table &h, const emd_lj_params *p, const double (*rep)[16]) {
   memset(&h, 0, sizeof(h));
   for (int a = 0; a < p->ntypes; a++)
     for (int b = 0; b < p->ntypes; b++) {
@@ -554,25 +302,33 @@ static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p,
       h.cutsq[a * LJ_MAXT + b] = p->cutsq[a][b];
       h.rep[a * LJ_MAXT + b] = rep ? rep[a][b] : 1.0;
     }
-  // the table lives in the small double scratch area's neighbour: d_red head
-  EMD_TRY(emd_ensure(ctx, (void **)&ctx->d_red, &ctx->red_cap, sizeof(lj_table) + (1 << 20)));
-  EMD_HIP(ctx, hipMemcpyAsync(ctx->d_red, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
-  EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *dtab = (lj_table *)ctx->d_red;
+}
+
+// The context's device table, uploaded only when the parameters differ from
+// the last upload (the pinned staging copy holds those): a force or energy
+// call with unchanged parameters neither copies nor waits.  A change waits
+// until the previous upload has read the staging copy, then queues the copy
+// on the stream (kernels queued before it still see the old table).
+static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p,
+                               const double (*rep)[16], lj_table **dtab) {
+  static_assert(sizeof(lj_table) <= EMD_LJ_TABLE_BYTES, "table size");
+  lj_table h;
+  fill_table(h, p, rep);
+  lj_table *staged = (lj_table *)ctx->h_ljtab;
+  if (!ctx->ljtab_valid || memcmp(staged, &h, sizeof(h)) != 0) {
+    if (ctx->ljtab_valid) EMD_HIP(ctx, hipEventSynchronize(ctx->ljtab_ev));
+    memcpy(staged, &h, sizeof(h));
+    EMD_HIP(ctx, hipMemcpyAsync(ctx->d_ljtab, staged, sizeof(h), hipMemcpyHostToDevice,
+                                ctx->stream));
+    EMD_HIP(ctx, hipEventRecord(ctx->ljtab_ev, ctx->stream));
+    ctx->ljtab_valid = true;
+  }
+  *dtab = (lj_table *)ctx->d_ljtab;
   return EMD_OK;
 }
 
 // lanes per atom and gathers per round (measured on MI355X, LJ 4M atoms)
 static int default_lanes(int precision) { return precision == EMD_MIXED ? 16 : 16; }
-
-extern "C" emd_status emd_lj_set_tile(emd_ctx *ctx, int on, int lanes) {
-  if (!ctx) return EMD_ERR_ARG;
-  if (!(lanes == 0 || lanes == 4 || lanes == 8 || lanes == 16))
-    return emd_set_error(ctx, EMD_ERR_ARG, "bin-tile lanes must be 0, 4, 8 or 16");
-  ctx->lj_tile = on != 0;
-  if (lanes) ctx->lj_tile_lanes = lanes;
-  return EMD_OK;
-}
 
 extern "C" emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes) {
   if (!ctx || (precision != EMD_FP64 && precision != EMD_MIXED)) return EMD_ERR_ARG;
@@ -582,39 +338,12 @@ extern "C" emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes) {
   return EMD_OK;
 }
 
-template <int K, int U>
-static emd_status launch_pipe(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
-                              const emd_lj_params *p, int precision) {
-  const int n = s->n_local;
-  constexpr int G = 64 / K;
-  const int ngroups = (n + G - 1) / G;
-  const int gpx = (ngroups + 7) / 8;
-  int bpx = ctx->lj_pipe_blocks;  // blocks per XCD
-  const int need = (gpx + PIPE_WAVES - 1) / PIPE_WAVES;
-  if (bpx > need) bpx = need;
-  if (bpx < 1) bpx = 1;
-  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
-#define LJ_PIPE_ARGS                                                                        \
-  s->x, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],            \
-      p->cutsq[0][0], gpx, bpx * PIPE_WAVES, s->f, s->f32
-  const bool csr = nl->layout == EMD_NEIGH_CSR;
-  if (precision == EMD_MIXED) {
-    if (csr) EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_pipe<K, U, true, true>), 8 * bpx, 64 * PIPE_WAVES, 0, LJ_PIPE_ARGS);
-    else EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_pipe<K, U, true, false>), 8 * bpx, 64 * PIPE_WAVES, 0, LJ_PIPE_ARGS);
-  } else {
-    if (csr) EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_pipe<K, U, false, true>), 8 * bpx, 64 * PIPE_WAVES, 0, LJ_PIPE_ARGS);
-    else EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_pipe<K, U, false, false>), 8 * bpx, 64 * PIPE_WAVES, 0, LJ_PIPE_ARGS);
-  }
-#undef LJ_PIPE_ARGS
-  return EMD_OK;
-}
-
 template <int K, int U, bool DIAL>
 static emd_status launch_full_as(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                                  const emd_lj_params *p, double rep_0, int precision,
                                  const lj_table *tab, int store) {
   const int n = s->n_local;
-  const int nb = emd_blocks((long)(nl->visit ? nl->n_visit : n) * K, LJ_THREADS);
+  const int nb = emd_blocks((long)n * K, LJ_THREADS);
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   const bool one = p->ntypes == 1;
   const int prio = ctx->lj_prio >= 0 ? ctx->lj_prio : (precision == EMD_MIXED ? 2 : 0);
@@ -622,7 +351,7 @@ static emd_status launch_full_as(emd_ctx *ctx, emd_system *s, const emd_neigh *n
   const char *name_d = DIAL ? "lj_idial_fp64" : "lj_force_fp64";
 #define LJ_FULL_ARGS                                                                        \
   s->x, s->type, row, nl->num, nl->entries, nl->maxneighs, n, p->lj1[0][0], p->lj2[0][0],   \
-      p->cutsq[0][0], rep_0, tab, nl->visit, nl->n_visit, store, s->f, s->f32, prio
+      p->cutsq[0][0], rep_0, tab, store, s->f, s->f32, prio
   if (precision == EMD_MIXED) {
     if (one) EMD_LAUNCH(ctx, name_m, (k_lj_full<K, U, true, true, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
     else EMD_LAUNCH(ctx, name_m, (k_lj_full<K, U, false, true, DIAL>), nb, LJ_THREADS, 0, LJ_FULL_ARGS);
@@ -634,51 +363,8 @@ static emd_status launch_full_as(emd_ctx *ctx, emd_system *s, const emd_neigh *n
   return EMD_OK;
 }
 
-template <int K, int U>
-static emd_status launch_full(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
-                              const emd_lj_params *p, int precision, const lj_table *tab,
-                              int store) {
-  return launch_full_as<K, U, false>(ctx, s, nl, p, 1.0, precision, tab, store);
-}
-
-// LDS budget of the bin-tile kernel: three FP64 coordinates per candidate
-// in 60 KiB (the static stencil tables take the rest of the 64 KiB)
-constexpr int LJ_TILE_MAX_CAND = (60 * 1024) / (3 * 8);
-
-template <int K, int U>
-static emd_status launch_tile(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
-                              const emd_lj_params *p, int precision, int tcap) {
-  const emd_bins *b = nl->bins;
-  const neigh_geom g = emd_neigh_geom(b);
-  const long nib = (long)g.nix * g.niy * g.niz;
-  const size_t shmem = 3 * (size_t)tcap * sizeof(double);
-  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
-#define LJ_TILE_ARGS                                                                          \
-  s->x, b->bincount, b->binoffsets, b->permute, g, row, nl->num, nl->slot, nl->maxneighs,     \
-      s->n_local, tcap, p->lj1[0][0], p->lj2[0][0], p->cutsq[0][0], s->f, s->f32
-  if (precision == EMD_MIXED)
-    EMD_LAUNCH(ctx, "lj_force_mixed", (k_lj_tile<K, U, true>), nib, LJ_THREADS, shmem, LJ_TILE_ARGS);
-  else
-    EMD_LAUNCH(ctx, "lj_force_fp64", (k_lj_tile<K, U, false>), nib, LJ_THREADS, shmem, LJ_TILE_ARGS);
-#undef LJ_TILE_ARGS
-  return EMD_OK;
-}
-
-// the bin-tile kernel applies to one-type full lists that carry slots, when
-// the largest candidate list (27 x the fullest bin) fits the LDS budget
-static int tile_capacity(const emd_ctx *ctx, const emd_neigh *nl, const emd_lj_params *p,
-                         int half) {
-  if (!ctx->lj_tile || half || p->ntypes != 1 || !nl->slot || !nl->bins) return 0;
-  const emd_bins *b = nl->bins;
-  if (b->nhalo < 1 || b->nbinx - 2 * b->nhalo < 1 || b->nbiny - 2 * b->nhalo < 1 ||
-      b->nbinz - 2 * b->nhalo < 1)
-    return 0;
-  const long t = (27l * (b->max_count > 0 ? b->max_count : 1) + 63) / 64 * 64;
-  return t <= LJ_TILE_MAX_CAND ? (int)t : 0;
-}
-
-// lj/cut/idial: the full-list kernel with the repeat loop, for any lanes
-// per atom (the bin-tile and pipelined forms are lj/cut only)
+// the full-list kernel (lj/cut, or lj/cut/idial with DIAL) at K lanes per
+// atom; U gathers per lane and round (LJ-4M sweep: 16 x 5 and 8 x 5 best)
 template <bool DIAL>
 static emd_status launch_full_lanes(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                                     const emd_lj_params *p, double rep_0, int precision,
@@ -698,6 +384,7 @@ static emd_status lj_idial_force(emd_ctx *ctx, emd_system *s, const emd_neigh *n
   if (!ctx || !s || !nl || !p) return EMD_ERR_ARG;
   const emd_lj_params *lj = &p->lj;
   EMD_ARG(ctx, lj->ntypes >= 1 && lj->ntypes <= LJ_MAXT, "ntypes out of range");
+  EMD_ARG(ctx, s->ntypes <= lj->ntypes, "the system has more atom types than the parameters");
   EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
   EMD_ARG(ctx, !(store && half), "storing forces needs a full list");
   for (int a = 0; a < lj->ntypes; a++)
@@ -728,31 +415,10 @@ static emd_status lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                            const emd_lj_params *p, int precision, int half, int store) {
   if (!ctx || !s || !nl || !p) return EMD_ERR_ARG;
   EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
+  EMD_ARG(ctx, s->ntypes <= p->ntypes, "the system has more atom types than the parameters");
   EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
   EMD_ARG(ctx, !(store && half), "storing forces needs a full list");
   if (s->n_local <= 0) return EMD_OK;
-  if (store) {
-    // kernels without a store form accumulate into zeroed local forces
-    const int kr = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
-    const int kk = kr ? kr : default_lanes(precision);
-    const bool tile = (kr == 0 || kr >= 4) && tile_capacity(ctx, nl, p, 0) > 0;
-    const bool pipe = ctx->lj_pipe_blocks > 0 && p->ntypes == 1 && kk >= 4;
-    if (tile || pipe) {
-      EMD_TRY(emd_force_zero(ctx, s, precision));
-      store = 0;
-    }
-  }
-  // an explicit request for 1 or 2 lanes per atom keeps the CSR kernel
-  // (1 lane reproduces the reference's serial sum bit for bit)
-  const int kreq = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
-  const int tcap = (kreq == 0 || kreq >= 4) ? tile_capacity(ctx, nl, p, half) : 0;
-  if (tcap) {
-    switch (kreq ? kreq : ctx->lj_tile_lanes) {
-      case 4: return launch_tile<4, 4>(ctx, s, nl, p, precision, tcap);
-      case 16: return launch_tile<16, 2>(ctx, s, nl, p, precision, tcap);
-      default: return launch_tile<8, 2>(ctx, s, nl, p, precision, tcap);
-    }
-  }
   lj_table *tab = nullptr;
   if (p->ntypes > 1 || half) EMD_TRY(upload_table(ctx, p, nullptr, &tab));
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
@@ -768,39 +434,10 @@ static emd_status lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                  s->f32);
     return EMD_OK;
   }
-  int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
-  if (K == 0) K = default_lanes(precision);
-  if (ctx->lj_pipe_blocks > 0 && p->ntypes == 1 && K >= 4) {
-    switch (K) {
-      case 4: return launch_pipe<4, 4>(ctx, s, nl, p, precision);
-      case 8: return launch_pipe<8, 5>(ctx, s, nl, p, precision);
-      default:
-        switch (ctx->lj_unroll) {
-          case 4: return launch_pipe<16, 4>(ctx, s, nl, p, precision);
-          case 6: return launch_pipe<16, 6>(ctx, s, nl, p, precision);
-          default: return launch_pipe<16, 5>(ctx, s, nl, p, precision);
-        }
-    }
-  }
-  switch (K) {
-    case 1: return launch_full<1, 4>(ctx, s, nl, p, precision, tab, store);
-    case 2: return launch_full<2, 4>(ctx, s, nl, p, precision, tab, store);
-    case 4: return launch_full<4, 4>(ctx, s, nl, p, precision, tab, store);
-    case 8:
-      switch (ctx->lj_unroll) {
-        case 8: return launch_full<8, 8>(ctx, s, nl, p, precision, tab, store);
-        case 10: return launch_full<8, 10>(ctx, s, nl, p, precision, tab, store);
-        default: return launch_full<8, 5>(ctx, s, nl, p, precision, tab, store);
-      }
-    default:
-      switch (ctx->lj_unroll) {
-        case 2: return launch_full<16, 2>(ctx, s, nl, p, precision, tab, store);
-        case 3: return launch_full<16, 3>(ctx, s, nl, p, precision, tab, store);
-        case 4: return launch_full<16, 4>(ctx, s, nl, p, precision, tab, store);
-        case 6: return launch_full<16, 6>(ctx, s, nl, p, precision, tab, store);
-        default: return launch_full<16, 5>(ctx, s, nl, p, precision, tab, store);
-      }
-  }
+  // 1 lane per atom reproduces the reference's serial sum bit for bit
+  const int K = ctx->lj_lanes[precision == EMD_MIXED ? 1 : 0];
+  return launch_full_lanes<false>(ctx, s, nl, p, 1.0, precision, tab, store,
+                                  K ? K : default_lanes(precision));
 }
 
 extern "C" emd_status emd_lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
@@ -923,6 +560,7 @@ extern "C" emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd
                                     const emd_lj_params *p, int half, double *pe) {
   if (!ctx || !s || !nl || !p || !pe) return EMD_ERR_ARG;
   EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
+  EMD_ARG(ctx, s->ntypes <= p->ntypes, "the system has more atom types than the parameters");
   const int n = s->n_local;
   if (n <= 0) { *pe = 0.0; return EMD_OK; }
   lj_table *tab;

@@ -23,9 +23,9 @@
 //                 layer in the wavefront's LDS buffer, one lane per
 //                 half-plane entry), sums sfac*w*U; the wavefronts' sums
 //                 are added in order and U_tot (285 complex) written
-//   k_snap_y      one block per atom: U_tot staged in LDS, two lanes per
-//                 half-plane entry of Y (155), each over every other (a,b)
-//                 pair of its J, Clebsch-Gordan weights from a compact table
+//   k_snap_yflat  one block per atom: U_tot staged in LDS, Y (155 half-plane
+//                 entries) from a host-built flat stream of weighted
+//                 U_a U_b products
 //   k_snap_fij    one wavefront per (atom, neighbour): U and dU recursion
 //                 (LDS layer buffers), contraction with Y_J (staged in LDS
 //                 per block), wavefront reduction, atomic scatter to f_i / f_j
@@ -535,43 +535,6 @@ __device__ __forceinline__ void y_entry(const snap_dev_tables &t, int e, int &J,
   const int k = e - t.ystart[J];
   mb = k / (J + 1);
   ma = k - mb * (J + 1);
-}
-
-// Y of one atom per block: two adjacent lanes per half-plane entry, each
-// summing every other (a, b) pair of the entry's layer; the even lane adds
-// its partner's partial to its own (fixed order)
-constexpr int Y_THREADS = 320;  // >= 2 * SNAP_NY, whole wavefronts
-
-template <class T>
-__global__ void __launch_bounds__(Y_THREADS)
-k_snap_y(snap_dev_tables t, const double *__restrict__ cg, const ytab_entry *__restrict__ ytab,
-         int n, const T *__restrict__ utot_r, const T *__restrict__ utot_i, T *__restrict__ y_r,
-         T *__restrict__ y_i) {
-  __shared__ T su_r[SNAP_NU], su_i[SNAP_NU];
-  const int i = blockIdx.x;
-  if (i >= n) return;
-  for (int k = threadIdx.x; k < t.nu; k += blockDim.x) {
-    su_r[k] = utot_r[(size_t)i * t.nu + k];
-    su_i[k] = utot_i[(size_t)i * t.nu + k];
-  }
-  __syncthreads();
-  const int e = threadIdx.x >> 1, h = threadIdx.x & 1;
-  T yr = 0, yi = 0;
-  if (e < t.ny) {
-    int J, ma, mb;
-    y_entry(t, e, J, ma, mb);
-    for (int q = t.yjstart[J] + h; q < t.yjstart[J + 1]; q += 2) {
-      const ytab_entry te = ytab[q];
-      const cplx<T> z = z_entry<T>(t, cg, su_r, su_i, te.a, te.b, J, te.cgoff, ma, mb);
-      yr += (T)te.coef * z.r;
-      yi += (T)te.coef * z.i;
-    }
-  }
-  const T or_ = dpp_any<0xB1>(yr), oi = dpp_any<0xB1>(yi);  // partner lane (xor 1)
-  if (e < t.ny && h == 0) {
-    y_r[(size_t)i * t.ny + e] = yr + or_;
-    y_i[(size_t)i * t.ny + e] = yi + oi;
-  }
 }
 
 // Y of one atom per block from a flat term stream.  Every half-plane entry
@@ -1400,34 +1363,22 @@ template <class T>
 static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   T *ur = (T *)sn->u_r, *ui = (T *)sn->u_i, *yr = (T *)sn->y_r, *yi = (T *)sn->y_i;
   EMD_TRY(launch_ui<T>(ctx, sn, n, ur, ui));
-  if (ctx->snap_yflat) {
-    const T *w = std::is_same<T, float>::value ? (const T *)sn->yf_w32 : (const T *)sn->yf_w64;
-    const uint32_t *code = std::is_same<T, float>::value ? sn->yf_code32 : sn->yf_code64;
-    EMD_LAUNCH(ctx, "snap_y", k_snap_yflat<T>, n, YF_THREADS, 0, sn->t.nu, sn->t.ny,
-               sn->yf_nch, code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
-  } else {
-    EMD_LAUNCH(ctx, "snap_y", k_snap_y<T>, n, Y_THREADS, 0, sn->t, sn->cg, sn->ytab, n, ur, ui,
-               yr, yi);
-  }
+  const T *w = std::is_same<T, float>::value ? (const T *)sn->yf_w32 : (const T *)sn->yf_w64;
+  const uint32_t *code = std::is_same<T, float>::value ? sn->yf_code32 : sn->yf_code64;
+  EMD_LAUNCH(ctx, "snap_y", k_snap_yflat<T>, n, YF_THREADS, 0, sn->t.nu, sn->t.ny, sn->yf_nch,
+             code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
   return EMD_OK;
 }
 
-// F_ij for every (atom, neighbour): 16 lanes per neighbour by default
-// (EMD_SNAP_FIJ_LANES=32 keeps two neighbours per wavefront)
+// F_ij for every (atom, neighbour): 16 lanes per neighbour, four
+// neighbours per wavefront
 template <class T, class TF>
 static emd_status launch_fij(emd_ctx *ctx, emd_snap *sn, int n, const T *yr, const T *yi,
                              TF *f) {
-  if (ctx->snap_fij_lanes == 32) {
-    constexpr int nb = fij_nb<32>();
-    const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
-    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<32, T, TF>), grid, 64 * FIJ_WAVES, 0, sn->t,
-               sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
-  } else {
-    constexpr int nb = fij_nb<16>();
-    const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
-    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<16, T, TF>), grid, 64 * FIJ_WAVES, 0, sn->t,
-               sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
-  }
+  constexpr int nb = fij_nb<16>();
+  const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
+  EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<16, T, TF>), grid, 64 * FIJ_WAVES, 0, sn->t, sn->nin,
+             sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
   return EMD_OK;
 }
 

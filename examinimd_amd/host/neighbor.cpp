@@ -20,9 +20,6 @@ void NeighborHIP::create_neigh_list(System* system, Binning* binning, bool half_
   ctx = system->ctx;
   const emd_bins* b = binning->device_bins();
   if (!b) { fprintf(stderr, "NeighborHIP needs a GPU binning module\n"); exit(1); }
-  // full lists also carry the bin-tile view that the LJ force kernel stages
-  // through LDS (a 2-byte slot per entry)
-  nl.want_slot = half_neigh_ ? 0 : 1;
   EMD_CALL(ctx, emd_neigh_build(ctx, system->view(), b, neigh_cut, half_neigh_ ? 1 : 0,
                                 comm_newton ? 1 : 0,
                                 neigh_type == NEIGH_2D ? EMD_NEIGH_2D : EMD_NEIGH_CSR, &nl));

@@ -60,15 +60,11 @@ class Pipeline:
             s.lo[d] = 0.0
             s.hi[d] = box[d]
 
-    def build(self, cut, half=0, newton=0, tile=False, layout=EMD_NEIGH_CSR):
-        """tile: switch the context's bin-tile LJ kernel on and build the slot
-        view (full lists) that it reads; layout: EMD_NEIGH_CSR or
-        EMD_NEIGH_2D"""
+    def build(self, cut, half=0, newton=0, layout=EMD_NEIGH_CSR):
+        """exchange, binning sort, halo, binning and the neighbour list;
+        layout: EMD_NEIGH_CSR or EMD_NEIGH_2D"""
         L, c, s = self.L, self.ctx.h, C.byref(self.s)
         self._ids = None
-        self.nl.want_slot = 1 if tile else 0
-        if tile:
-            self.chk(L.emd_lj_set_tile(c, 1, 0), "set_tile")
         self.chk(L.emd_comm_exchange(c, s), "exchange")
         self.chk(L.emd_binning(c, s, cut, cut, cut, 1, 1, 0, 1, C.byref(self.b)), "bin sort")
         self.chk(L.emd_comm_exchange_halo(c, s, cut, C.byref(self.h)), "halo")

@@ -99,6 +99,32 @@ emd_status emd_system_download(emd_ctx *ctx, const emd_system *s, int n, double 
                                double *v, double *f, int *type, int *id, double *q);
 emd_status emd_system_set_mass(emd_ctx *ctx, emd_system *s, int ntypes, const double *mass);
 
+/* The reference's per-atom Views in device memory (src/types.h:88-100): x is
+ * View<double*[3], LayoutRight> (a[3 i + d]); v and f are View<double*[3]>
+ * with the default layout, which is LayoutLeft on the GPU backends
+ * (a[i + d ld], ld = the View's extent(0), at least the atom count). */
+enum { EMD_LAYOUT_RIGHT = 0, EMD_LAYOUT_LEFT = 1 };
+typedef struct {
+  double *data;   /* device pointer (View::data()); NULL = not copied */
+  int layout;     /* EMD_LAYOUT_RIGHT or EMD_LAYOUT_LEFT */
+  long ld;        /* LayoutLeft: column stride (View::extent(0)); else ignored */
+} emd_vec3_view;
+/* System arrays held in the reference's View layouts -> emd_system (device to
+ * device, stream-ordered; grows s to n).  type, id, q are device int/int/
+ * double arrays of n (or NULL).  Replaces reading System's Views directly
+ * (src/system.h:57-143) for a module handed the reference's System. */
+emd_status emd_system_import(emd_ctx *ctx, emd_system *s, int n, emd_vec3_view x,
+                             emd_vec3_view v, emd_vec3_view f, const int *type, const int *id,
+                             const double *q);
+/* emd_system -> the reference's Views (x, v, f of the first n atoms; a view
+ * with data NULL is skipped); values are copied bit for bit */
+emd_status emd_system_export(emd_ctx *ctx, const emd_system *s, int n, emd_vec3_view x,
+                             emd_vec3_view v, emd_vec3_view f);
+/* one context per device and process, created on first use and kept until
+ * exit: what a module in the reference's tree, which has no context of its
+ * own, uses (emd_ctx_of in include/emd_views.h) */
+emd_status emd_ctx_shared(int device, emd_ctx **out);
+
 /* ----------------------------------------------------- comm (serial) */
 typedef struct {
   int num_ghost[6];
@@ -141,6 +167,15 @@ typedef struct {
 emd_status emd_binning(emd_ctx *ctx, emd_system *s, double dx, double dy, double dz,
                        int halo_depth, int do_local, int do_ghost, int sort, emd_bins *b);
 emd_status emd_bins_free(emd_ctx *ctx, emd_bins *b);
+/* the reference's Binning fields (src/binning.h:50-61: nbin, nhalo, min/max,
+ * bincount and binoffsets View<int***> in the default layout -- LayoutLeft
+ * on the GPU backends -- and permute_vector of n_range) -> b, copied into
+ * b's own x-major arrays (device to device), so a neighbour module handed
+ * the reference's Binning can call emd_neigh_build */
+emd_status emd_bins_import(emd_ctx *ctx, int nbinx, int nbiny, int nbinz, int nhalo,
+                           const double min[3], const double max[3], const int *bincount,
+                           const int *binoffsets, int layout, const int *permute, int n_range,
+                           emd_bins *b);
 
 /* --------------------------------------------------------- neighbours */
 typedef struct {
@@ -153,27 +188,6 @@ typedef struct {
   int *entries;     /* CSR: [total]; 2D: [n][maxneighs] row-major */
   long cap_entries;
   int cap_rows;
-  /* bin-tile view, full lists only.  Set want_slot = 1 before
-   * emd_neigh_build to get, beside every entry, its position in the
-   * candidate list of the row atom's bin (the 27 stencil bins'
-   * atoms concatenated in stencil order, each bin in permute order).  A
-   * force kernel can then stage a bin's candidates in LDS once and read
-   * neighbour positions from there.  bins points at the binning the list was
-   * built from; it must stay unchanged while the list is used. */
-  int want_slot;
-  unsigned short *slot;   /* device, same shape as entries; NULL if not built */
-  long cap_slot;
-  const emd_bins *bins;
-  /* sweep order for force kernels over the rows of a full list: the atoms
-   * of the interior bins, bin columns (all z) taken in square tiles of
-   * columns, so the neighbourhood of the atoms in flight stays cache
-   * resident.  Every local atom appears once; so may the few ghosts that
-   * the bins' cell/1000 margin places inside interior bins (index >=
-   * n_local, to be skipped).  Built when the context's tile width is > 0
-   * (EMD_VISIT_TILE; default 0, measured no faster at LJ 4M); NULL otherwise. */
-  int *visit;             /* device [n_visit] */
-  int cap_visit;
-  int n_visit;
 } emd_neigh;
 
 /* NeighborCSR::create_neigh_list (src/neighbor_types/neighbor_csr.h:370-431)
@@ -228,12 +242,11 @@ emd_status emd_lj_idial_force_store(emd_ctx *ctx, emd_system *s, const emd_neigh
 /* ForceLJNeigh::compute_energy, force_lj_neigh_impl.h:128-156 (compensated) */
 emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
                          const emd_lj_params *p, int half, double *pe);
-/* lanes per atom of the LJ force kernel (0 = built-in default); 1 or 2
- * selects the CSR-gather kernel even where the bin-tile one applies */
+/* lanes per atom of the LJ force kernel: 1, 2, 4, 8 or 16 (0 = built-in
+ * default, 16).  One lane sums an atom's pairs in the reference's serial
+ * order (FP64: bitwise the reference's force); more lanes sum interleaved
+ * parts.  Every count keeps the mixed path's force error within 1e-5 RMS. */
 emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes);
-/* bin-tile LJ kernel (one-type full lists built with want_slot): on/off and
- * its lanes per atom (4, 8, 16; 0 keeps the current value) */
-emd_status emd_lj_set_tile(emd_ctx *ctx, int on, int lanes);
 
 /* ---------------------------------------------------------------- NVE */
 /* IntegratorNVE::initial_integrate / final_integrate,
