@@ -120,6 +120,8 @@ HIP_PROTOS = {
                                            P(EmdLJIDialParams), C.c_int]),
     "emd_lj_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int,
                                 P(C.c_double)]),
+    "emd_lj_energy_noalloc": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams),
+                                        C.c_int, P(C.c_double)]),
     "emd_lj_set_lanes": (C.c_int, [vp, C.c_int, C.c_int]),
     "emd_nve_initial": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_double, C.c_int]),
     "emd_nve_final": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_int]),

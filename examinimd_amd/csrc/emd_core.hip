@@ -69,6 +69,11 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
     return EMD_ERR_ALLOC;
   }
   hipMemset(ctx->d_small, 0, 64 * sizeof(int));
+  // the allocation-free entry points' scratch: this device's copy
+  if (emd_lj_scratch_setup(ctx) != EMD_OK) {
+    emd_destroy(ctx);
+    return EMD_ERR_HIP;
+  }
   // lanes per atom of the LJ kernel: EMD_LJ_LANES (both precisions) or
   // EMD_LJ_LANES_FP64 / EMD_LJ_LANES_MIXED; 0 or unset = built-in default
   if (const char *e = getenv("EMD_LJ_LANES")) ctx->lj_lanes[0] = ctx->lj_lanes[1] = atoi(e);

@@ -53,6 +53,9 @@ struct emd_ctx {
   void *h_ljtab = nullptr;
   hipEvent_t ljtab_ev = nullptr;
   bool ljtab_valid = false;
+  // device address of this GPU's copy of the allocation-free energy path's
+  // static scratch array (resolved once at emd_create)
+  double *noalloc_scratch = nullptr;
 
   // host waits: an event recorded on the stream and polled, so a wait for a
   // count ends as soon as the copy lands (no blocking-sync wake-up latency)
@@ -82,6 +85,9 @@ struct emd_ctx {
 };
 
 emd_status emd_set_error(emd_ctx *ctx, emd_status code, const char *fmt, ...);
+// resolves the device address of emd_lj.hip's static scratch array on the
+// context's (current) device
+emd_status emd_lj_scratch_setup(emd_ctx *ctx);
 
 #define EMD_HIP(ctx, call)                                                        \
   do {                                                                            \

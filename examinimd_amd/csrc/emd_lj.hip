@@ -578,6 +578,82 @@ extern "C" emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd
   return finish_dd(ctx, part, nb, pe);
 }
 
+// ------------------------------------------- energy without allocation
+// emd_lj_energy_noalloc: the same energy as emd_lj_energy with every scratch
+// byte in one file-scope device array (`scratch`), so the call performs no
+// device or host allocation.  Layout (doubles): the LJ table, the final
+// (hi, lo) pair, then one (hi, lo) pair per energy block.  Each GPU holds its
+// own copy for the life of the process; its device address comes from
+// hipGetSymbolAddress once per device (emd_lj_scratch_setup, run by
+// emd_create with the context's device current).  Calls on one GPU share the
+// array: overlapping calls on one GPU (two streams, two threads) race.
+constexpr int NOALLOC_MAX_ATOMS = 1 << 22;  // 4,194,304 local atoms
+constexpr int NOALLOC_TAB = EMD_LJ_TABLE_BYTES / sizeof(double);
+constexpr int NOALLOC_OUT = NOALLOC_TAB;               // 2 doubles (+ padding)
+constexpr int NOALLOC_PART = NOALLOC_OUT + 32;
+constexpr int NOALLOC_MAX_BLOCKS = NOALLOC_MAX_ATOMS / (LJ_THREADS / LJE_K);
+constexpr size_t NOALLOC_DOUBLES = NOALLOC_PART + 2 * (size_t)NOALLOC_MAX_BLOCKS;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];  // 4.01 MiB per GPU
+
+emd_status emd_lj_scratch_setup(emd_ctx *ctx) {
+  void *ptr = nullptr;
+  EMD_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  ctx->noalloc_scratch = (double *)ptr;
+  return EMD_OK;
+}
+
+// second stage: one block adds the nb block pairs (hi, lo) in double-double
+// arithmetic and writes the total pair to out[0], out[1]
+__global__ void __launch_bounds__(256) k_dd_total(const double *__restrict__ part, int nb,
+                                                  double *__restrict__ out) {
+#pragma clang fp contract(off)
+  double hi = 0.0, lo = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    double s, e;
+    two_sum(hi, part[2 * (size_t)k], s, e);
+    hi = s;
+    lo += part[2 * (size_t)k + 1] + e;
+  }
+  block_dd_sum(hi, lo, out);
+}
+
+extern "C" emd_status emd_lj_energy_noalloc(emd_ctx *ctx, const emd_system *s,
+                                            const emd_neigh *nl, const emd_lj_params *p,
+                                            int half, double *pe) {
+  if (!ctx || !s || !nl || !p || !pe) return EMD_ERR_ARG;
+  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
+  EMD_ARG(ctx, s->ntypes <= p->ntypes, "the system has more atom types than the parameters");
+  const int n = s->n_local;
+  if (n > NOALLOC_MAX_ATOMS)
+    return emd_set_error(ctx, EMD_ERR_ARG,
+                         "emd_lj_energy_noalloc: %d local atoms exceed its limit of %d", n,
+                         NOALLOC_MAX_ATOMS);
+  EMD_ARG(ctx, ctx->noalloc_scratch != nullptr, "scratch address not resolved");
+  if (n <= 0) { *pe = 0.0; return EMD_OK; }
+  double *buf = ctx->noalloc_scratch;  // device address of this GPU's `scratch`
+  lj_table h;
+  fill_table(h, p, nullptr);
+  EMD_HIP(ctx, hipMemcpyAsync(buf, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+  const lj_table *tab = (const lj_table *)buf;
+  double *out = buf + NOALLOC_OUT, *part = buf + NOALLOC_PART;
+  const int nb = emd_blocks((long)n * LJE_K, LJ_THREADS);
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  if (p->ntypes == 1)
+    EMD_LAUNCH(ctx, "lj_energy", k_lj_energy<true>, nb, LJ_THREADS, 0, s->x, s->type, row,
+               nl->num, nl->entries, nl->maxneighs, n, half, tab, part);
+  else
+    EMD_LAUNCH(ctx, "lj_energy", k_lj_energy<false>, nb, LJ_THREADS, 0, s->x, s->type, row,
+               nl->num, nl->entries, nl->maxneighs, n, half, tab, part);
+  EMD_LAUNCH(ctx, "lj_energy_total", k_dd_total, 1, 256, 0, part, nb, out);
+  EMD_HIP(ctx, hipMemcpyAsync(ctx->h_dsmall, out, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  // the stack copy of the table and the pinned result are read by the
+  // stream: wait before returning
+  EMD_TRY(emd_wait(ctx));
+  *pe = ctx->h_dsmall[0] + ctx->h_dsmall[1];
+  return EMD_OK;
+}
+
 // ------------------------------------------------ adjoint d/d(eps, sigma)
 // For a cotangent g = dL/dF over the local atoms and gE = dL/dE:
 //   dL/dtheta = sum_i sum_{j in N(i)} g_i . r_ij dfpair/dtheta + gE dE/dtheta

@@ -242,6 +242,15 @@ emd_status emd_lj_idial_force_store(emd_ctx *ctx, emd_system *s, const emd_neigh
 /* ForceLJNeigh::compute_energy, force_lj_neigh_impl.h:128-156 (compensated) */
 emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
                          const emd_lj_params *p, int half, double *pe);
+/* No counterpart in the reference.  emd_lj_energy for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same energy (to rounding: the block sums are added
+ * on the device in double-double instead of on the host) with all scratch in
+ * a static 4.01 MiB device array per GPU; no hipMalloc, hipFree or host
+ * allocation.  At most 4,194,304 local atoms (EMD_ERR_ARG beyond).  Calls on
+ * one GPU share that array: calls that overlap on one GPU race. */
+emd_status emd_lj_energy_noalloc(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                 const emd_lj_params *p, int half, double *pe);
 /* lanes per atom of the LJ force kernel: 1, 2, 4, 8 or 16 (0 = built-in
  * default, 16).  One lane sums an atom's pairs in the reference's serial
  * order (FP64: bitwise the reference's force); more lanes sum interleaved
