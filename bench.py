@@ -137,13 +137,56 @@ def lj_run(emd, decks, tmpdir, n, precision, warmup, steps, chunk=100, breakdown
 
 def energy_drift(samples, dt):
     """least-squares slope of E/N against time (reduced units per tau) and
-    the largest excursion from the first sample"""
+    the largest excursion from the first sample; no slope from fewer than
+    three samples"""
     import numpy as np
     st = np.array([s for s, _ in samples], dtype=float) * dt
     e = np.array([v for _, v in samples], dtype=float)
-    slope = float(np.polyfit(st, e, 1)[0]) if len(e) > 1 else 0.0
+    slope = float(np.polyfit(st, e, 1)[0]) if len(e) >= 3 else None
+    interval = int(samples[1][0] - samples[0][0]) if len(samples) > 1 else None
     return {"slope_per_tau": slope, "max_excursion": float(np.abs(e - e[0]).max()),
-            "samples": len(e), "e0": float(e[0])}
+            "samples": len(e), "sample_interval_steps": interval, "e0": float(e[0])}
+
+
+def drift_leg(emd, decks, tmpdir, n, precision, steps=1000, every=100, neigh="2D"):
+    """the LJ run's energy drift over a fixed 1000 steps (independent of
+    --steps), E/N sampled every 100 steps (11 samples).  Every sample step is
+    a rebuild step (100 is a multiple of the rebuild interval 20).  For the
+    mixed run, at each sample the list the run rebuilt incrementally (one-pass
+    fill into the previous build's capacity, 2D layout) is compared entry for
+    entry with a CSR list built from scratch (count, scan, fill) from the
+    same positions in a second context."""
+    import numpy as np
+    from examinimd_amd.abi import EMD_NEIGH_CSR
+    from examinimd_amd.pipeline import Pipeline
+    deck = decks.write(os.path.join(tmpdir, f"in.lj.drift.{precision}"),
+                       decks.lj_deck(n=n, steps=steps))
+    app = emd.App(["-il", deck, "--neigh-type", neigh, "--precision", precision]).init()
+    app.set_phase_sync(False)
+    box = tuple(app.L.emd_app_system(app.h).contents.domain)
+    samples = [(0, app.thermo()[2])]
+    checks = []
+    for k in range(steps // every):
+        app.run(every)
+        samples.append(((k + 1) * every, app.thermo()[2]))
+        if precision == "mixed":
+            a = app.arrays()
+            row, _, ent = app.neigh()
+            pl = Pipeline()
+            pl.load(a["x"], box)
+            pl.build(2.8, layout=EMD_NEIGH_CSR)
+            frow, _, fent = emd.download_neigh(pl.ctx.h, pl.nl)
+            same_order = np.array_equal(pl._device_ids(), a["id"])
+            pl.close()
+            checks.append(bool(same_order and np.array_equal(row, frow)
+                               and np.array_equal(ent, fent)))
+    app.close()
+    ident = None
+    if checks:
+        ident = {"rebuilds_checked": len(checks), "identical": all(checks),
+                 "steps": [every * (k + 1) for k in range(len(checks))],
+                 "against": "CSR list built from scratch from the same positions"}
+    return energy_drift(samples, 0.005), ident
 
 
 def melt_force_error(emd, app):
@@ -325,11 +368,30 @@ def idial_leg(emd, n=100, dials=(1, 4, 16, 64, 256, 1024), launches=10):
     return out
 
 
+def host_description():
+    """nproc, the CPU model (lscpu) and the thread share this process may use"""
+    import subprocess
+    model = None
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return {"nproc": os.cpu_count(), "lscpu_model": model,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(n, steps, threads):
+    """the CPU oracle timed on this host's cores.  The thread count is the
+    CPU share given to this one-GPU process (OMP_NUM_THREADS, set to 16 on the
+    GPU hosts, which share their cores between GPUs), else every core"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     if threads <= 0:
-        threads = min(os.cpu_count() or 1, 16)
+        env = os.environ.get("OMP_NUM_THREADS", "")
+        threads = int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
     cfg = pyoracle.make_config(n=(n, n, n), gaussian=True, nthreads=threads)
     m = pyoracle.MD(cfg)
     m.init()
@@ -339,7 +401,7 @@ def cpu_baseline(n, steps, threads):
     natoms = m.system.N
     m.close()
     return {"value": natoms * steps / dt, "unit": "atom-steps/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host_description(),
             "sample": f"CPU oracle (C/OpenMP restatement of the reference), same LJ "
                       f"{natoms}-atom workload, {steps} steps incl. {steps // 20} "
                       f"neighbour rebuilds, {dt:.1f} s"}
@@ -353,12 +415,25 @@ def batch_leg(emd, steps, loop_steps):
     replica through the per-system C ABI"""
     import numpy as np
     from examinimd_amd import batch
-    from examinimd_amd.abi import EMD_MIXED
+    from examinimd_amd.abi import EMD_FP64, EMD_MIXED
     e = batch.ensemble(nrep=256, seed=4)
     b = batch.Batch(e["natoms"], e["box"], precision=EMD_MIXED)
     b.upload(e["x"], e["v"])
     b.setup()
     lanes, block, tune_ms = b.autotune(trial_steps=10)
+    # force error of the tuned tile: FP32 forces of the mixed batch (set-up
+    # force recomputed with that tile) against an FP64 batch at the same
+    # positions
+    b.setup()
+    x0, v0, f32 = b.download()
+    b64 = batch.Batch(e["natoms"], e["box"], precision=EMD_FP64)
+    b64.set_tile(lanes, block)
+    b64.upload(x0, v0)
+    b64.setup()
+    _, _, f64 = b64.download()
+    b64.close()
+    df = f32 - f64
+    tile_err = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
     b.run(20)                      # warm-up, captures the period graph
     ke0, pe0 = b.thermo()
     t0 = time.perf_counter()
@@ -374,7 +449,8 @@ def batch_leg(emd, steps, loop_steps):
     return {"value": value, "unit": "atom-steps/s", "ms_per_step": 1e3 * wall / steps,
             "steps": steps, "replicas": 256, "atoms": total,
             "atoms_min_max": [int(e["natoms"].min()), int(e["natoms"].max())],
-            "tile": {"lanes": lanes, "block": block, "tuned_ms_per_step": tune_ms},
+            "tile": {"lanes": lanes, "block": block, "tuned_ms_per_step": tune_ms,
+                     "mixed_force_rms_error_vs_fp64": tile_err},
             "max_energy_change_per_atom": float(np.max(np.abs((ke1 + pe1) - (ke0 + pe0))
                                                        / e["natoms"])),
             "replica_loop": {"value": loop_value, "ms_per_step": 1e3 * sec / loop_steps,
@@ -451,12 +527,16 @@ def main():
         cnt, ms = prof.get(kname, (0, 0.0))
         out[prec] = {"wall": wall, "natoms": natoms, "prof": prof, "nlist": nlist,
                      "kernel": kname, "kernel_launches": cnt, "kernel_ms": ms,
-                     "thermo_end": app.thermo_end, "breakdown": app.breakdown,
-                     "drift": energy_drift(app.energy_samples, 0.005)}
+                     "thermo_end": app.thermo_end, "breakdown": app.breakdown}
         if prec == "mixed":
             out[prec]["melt_error"] = melt_force_error(emd, app)
         app.close()
         progress(f"LJ {natoms} atoms {prec} ({a.neigh}): {natoms * a.steps / wall:.4g} atom-steps/s")
+    for prec in ("fp64", "mixed"):
+        out[prec]["drift"], ident = drift_leg(emd, decks, tmpdir, n, prec, neigh=a.neigh)
+        if ident:
+            out[prec]["neigh_identity"] = ident
+        progress(f"LJ {prec} 1000-step drift leg done")
 
     # the mixed run again with the other list layout (same workload)
     alt = None
@@ -562,7 +642,10 @@ def main():
         "thermo_end": {"fp64": fp64["thermo_end"], "mixed": mixed["thermo_end"],
                        "fields": ["T", "PE/N", "E/N"]},
         "energy_drift": {"fp64": fp64["drift"], "mixed": mixed["drift"],
-                         "unit": "E/N per tau (LJ reduced units), sampled every 100 steps"},
+                         "unit": "E/N per tau (LJ reduced units)",
+                         "run": "1000 steps from set-up, E/N every 100 steps, separate from "
+                                "the timed run"},
+        "neighbour_identity_at_rebuilds": mixed.get("neigh_identity"),
         "mixed_force_rms_error_vs_fp64_displaced_lattice": ferr_dis,
         "snap2k": snap,
         "batched": batched,
