@@ -135,6 +135,7 @@ HIP_PROTOS = {
     "emd_snap_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, P(C.c_double), dptr]),
     "emd_snap_adjoint_beta": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdHalo), vp, dptr,
                                         C.c_double, C.c_int, dptr]),
+    "emd_snap_descriptors": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, dptr]),
     "emd_lj_adjoint": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), C.c_double, C.c_double,
                                  C.c_double, dptr, C.c_double, C.c_int, dptr]),
     "emd_batch_create": (C.c_int, [vp, P(EmdBatchDesc), P(vp)]),

@@ -1626,7 +1626,8 @@ k_snap_s(snap_dev_tables t, const double *__restrict__ cg, const stab_entry *__r
 __global__ void __launch_bounds__(256)
 k_snap_bsum(snap_dev_tables t, const double *__restrict__ cg, const etab_entry *__restrict__ etab,
             int ne, int bzero, int n, const double *__restrict__ utot_r,
-            const double *__restrict__ utot_i, double *__restrict__ bsum) {
+            const double *__restrict__ utot_i, double *__restrict__ bsum,
+            double *__restrict__ brow) {
   __shared__ double su_r[SNAP_NU], su_i[SNAP_NU];
   __shared__ double sB[SNAP_MAXT];
   const int i = blockIdx.x;
@@ -1652,10 +1653,12 @@ k_snap_bsum(snap_dev_tables t, const double *__restrict__ cg, const etab_entry *
     }
   }
   __syncthreads();
+  // the atom's B_k into row i of brow, or added into bsum
   for (int k = threadIdx.x; k < ne; k += blockDim.x) {
     double b = sB[k];
     if (bzero) b -= t.wself * t.wself * t.wself * (etab[k].j + 1);
-    atomicAdd(&bsum[k], b);
+    if (brow) brow[(size_t)i * ne + k] = b;
+    else atomicAdd(&bsum[k], b);
   }
 }
 
@@ -1683,7 +1686,6 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
                                             double dLdE, int precision, double *dLdbeta) {
   if (!ctx || !s || !nl || !sn || !dLdF || !dLdbeta) return EMD_ERR_ARG;
   const int n = s->n_local;
-  const int ntrip = (int)sn->tid.size();
   for (int k = 0; k <= sn->ncoeff; k++) dLdbeta[k] = 0.0;
   if (n <= 0) return EMD_OK;
   EMD_ARG(ctx, s->n_ghost == 0 || (h && h->root && h->n_ghost == s->n_ghost),
@@ -1703,7 +1705,8 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
   if (dLdE != 0.0) {
     EMD_TRY(launch_ui<double>(ctx, sn, n, (double *)sn->u_r, (double *)sn->u_i));
     EMD_LAUNCH(ctx, "snap_adjoint_bsum", k_snap_bsum, n, 256, 0, sn->t, sn->cg, sn->etab,
-               sn->ncoeff, sn->bzeroflag, n, (const double *)sn->u_r, (const double *)sn->u_i, B);
+               sn->ncoeff, sn->bzeroflag, n, (const double *)sn->u_r, (const double *)sn->u_i, B,
+               nullptr);
   }
   std::vector<double> hS(SNAP_MAXT + 64);
   EMD_HIP(ctx, hipMemcpyAsync(hS.data(), sn->sbuf, hS.size() * sizeof(double),
@@ -1718,6 +1721,18 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
     dLdbeta[k + 1] = 2.0 * s1 + 2.0 * (j + 1) / (j1 + 1.0) * s2 + 2.0 * (j + 1) / (j2 + 1.0) * s3 +
                      dLdE * hS[SNAP_MAXT + k];
   }
-  (void)ntrip;
+  return EMD_OK;
+}
+
+extern "C" emd_status emd_snap_descriptors(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                           emd_snap *sn, double *B) {
+  if (!ctx || !s || !nl || !sn || !B) return EMD_ERR_ARG;
+  const int n = s->n_local;
+  if (n <= 0) return EMD_OK;
+  EMD_TRY(snap_neigh(ctx, s, nl, sn));
+  EMD_TRY(launch_ui<double>(ctx, sn, n, (double *)sn->u_r, (double *)sn->u_i));
+  EMD_LAUNCH(ctx, "snap_descriptors", k_snap_bsum, n, 256, 0, sn->t, sn->cg, sn->etab,
+             sn->ncoeff, sn->bzeroflag, n, (const double *)sn->u_r, (const double *)sn->u_i,
+             nullptr, B);
   return EMD_OK;
 }

@@ -13,13 +13,18 @@ The workloads are the seeded sets of the force-matching benchmark:
               n in {4..8} (128..1024 atoms), Gaussian displacements 0.05 A;
   lj_set():   32 FCC LJ boxes at density 0.8442, 4 n^3 atoms with n in
               {5..10} (500..4000 atoms), displacements 0.05 sigma.
-Each configuration keeps its own device state (positions, neighbour list)
-for the lifetime of the set; an evaluation uploads nothing but parameters.
+SnapForceMatch and LJForceMatch keep one device state per configuration
+(an evaluation is a launch sequence and a host round trip per
+configuration); SnapForceMatchMerged merges the configurations into one
+system so an evaluation is one launch sequence for the whole set.
 """
+import ctypes as C
+
 import numpy as np
 
+import examinimd_amd as emd
 from examinimd_amd import decks
-from examinimd_amd.abi import EMD_FP64
+from examinimd_amd.abi import EMD_FP64, EMD_MIXED, EMD_NEIGH_CSR, EmdHalo, EmdNeigh, EmdSystem
 from examinimd_amd.pipeline import Pipeline, lattice
 
 A_W = 3.1803
@@ -94,6 +99,212 @@ class SnapForceMatch:
         for pl in reversed(self.pls):
             pl.close()
         self.pls = []
+
+
+class MergedSystem:
+    """Configurations merged into one system on the device: the locals of
+    every configuration, then their ghosts phase by phase (phase p of every
+    configuration before phase p + 1), one CSR list and one halo whose pack
+    lists name merged indices.  Each configuration is first built on its
+    own (exchange, binning sort, halo, neighbour list, in `pls`); the merge
+    only renumbers, so every kernel sees the same neighbourhoods.  Atom ids
+    are offset per configuration to stay unique.  Nothing is rebuilt
+    afterwards: the configurations are static."""
+
+    def __init__(self, pls, mass):
+        self.L = L = emd.hip_lib()
+        self.ctx = pls[0].ctx
+        c = self.ctx.h
+        n_c = [pl.s.n_local for pl in pls]
+        g_c = [pl.s.n_ghost for pl in pls]
+        ph_c = [list(pl.h.num_ghost) for pl in pls]
+        N, G = sum(n_c), sum(g_c)
+        Gp = [sum(ph[p] for ph in ph_c) for p in range(6)]
+        off_p = np.concatenate([[N], N + np.cumsum(Gp)[:-1]]).astype(np.int64)
+        self.n_atoms, self.starts = np.array(n_c), np.concatenate([[0], np.cumsum(n_c)[:-1]])
+        x = np.zeros((N + G, 3))
+        ids = np.zeros(N + G, np.int32)
+        packs = [[] for _ in range(6)]
+        root = np.zeros(G, np.int32)
+        image = np.zeros(G, np.int32)
+        rows, ents = [np.zeros(1, np.int64)], []
+        seen = [0] * 6                      # ghosts of phase p placed so far
+        for k, pl in enumerate(pls):
+            n, g = n_c[k], g_c[k]
+            xs = self._dl(pl.s.x, 3 * (n + g)).reshape(-1, 3)
+            idk = emd.download_ints(c, pl.s.id, n + g)
+            mp = np.zeros(n + g, np.int64)
+            mp[:n] = self.starts[k] + np.arange(n)
+            gl = n
+            for p in range(6):
+                m = ph_c[k][p]
+                mp[gl:gl + m] = off_p[p] + seen[p] + np.arange(m)
+                gl += m
+            for p in range(6):
+                m = ph_c[k][p]
+                if m:
+                    packs[p].append(mp[emd.download_ints(c, pl.h.pack[p], m)])
+                seen[p] += m
+            x[mp] = xs
+            ids[mp] = idk + self.starts[k]
+            if g:
+                gi = mp[n:] - N
+                root[gi] = mp[emd.download_ints(c, pl.h.root, g)]
+                image[gi] = emd.download_ints(c, pl.h.image, g)
+            row, num, ent = emd.download_neigh(c, pl.nl)
+            rows.append(rows[-1][-1] + row[1:].astype(np.int64))
+            ents.append(mp[ent])
+        row = np.concatenate(rows).astype(np.int32)
+        ent = np.concatenate(ents).astype(np.int32)
+        self._bufs = []
+        s = self.s = EmdSystem()
+        zeros = np.zeros_like(x)
+        self.chk(L.emd_system_upload(c, C.byref(s), N + G, _dp(x), _dp(zeros),
+                                     _ip(np.zeros(N + G, np.int32)), _ip(ids),
+                                     _dp(np.zeros(N + G))), "upload")
+        m = np.array([mass])
+        self.chk(L.emd_system_set_mass(c, C.byref(s), 1, _dp(m)), "mass")
+        s.n_local, s.n_ghost, s.n_global = N, G, N
+        h = self.h = EmdHalo()
+        for p in range(6):
+            h.num_ghost[p] = Gp[p]
+            h.cap[p] = Gp[p]
+            h.pack[p] = self._put(np.concatenate(packs[p]).astype(np.int32) if packs[p]
+                                  else np.zeros(1, np.int32))
+        h.root, h.image = self._put(root), self._put(image)
+        h.root_cap = h.n_ghost = G
+        nl = self.nl = EmdNeigh()
+        nl.layout, nl.n, nl.total = EMD_NEIGH_CSR, N, int(row[-1])
+        nl.maxneighs = int(np.diff(row).max()) if N else 0
+        nl.row, nl.num = self._put(row), self._put(np.diff(row).astype(np.int32))
+        nl.entries = self._put(ent)
+        nl.cap_entries, nl.cap_rows = int(row[-1]), N + 1
+
+    def chk(self, st, what):
+        self.ctx.check(st, what)
+
+    def _dl(self, ptr, n):
+        out = np.zeros(n)
+        if n:
+            self.chk(self.L.emd_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p),
+                                           C.cast(ptr, C.c_void_p), out.nbytes), "d2h")
+        return out
+
+    def _put(self, a):
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        self.chk(self.L.emd_malloc(self.ctx.h, C.byref(p), max(a.nbytes, 4)), "malloc")
+        self.chk(self.L.emd_memcpy_h2d(self.ctx.h, p, a.ctypes.data_as(C.c_void_p), a.nbytes),
+                 "h2d")
+        self._bufs.append(p)
+        return C.cast(p, C.POINTER(C.c_int if a.dtype == np.int32 else C.c_double))
+
+    def forces(self, precision):
+        """forces of the merged locals (library order)"""
+        n = self.s.n_local
+        if precision == EMD_MIXED:
+            f32 = np.zeros((n, 3), np.float32)
+            self.chk(self.L.emd_memcpy_d2h(self.ctx.h, f32.ctypes.data_as(C.c_void_p),
+                                           C.cast(self.s.f32, C.c_void_p), f32.nbytes), "f32")
+            return f32.astype(np.float64)
+        return self._dl(self.s.f, 3 * n).reshape(n, 3)
+
+    def close(self):
+        for p in self._bufs:
+            self.L.emd_free(self.ctx.h, p)
+        self._bufs = []
+        self.L.emd_system_free(self.ctx.h, C.byref(self.s))
+
+
+def _dp(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(C.POINTER(C.c_int))
+
+
+class SnapForceMatchMerged:
+    """loss(beta) and dloss/dbeta over SNAP configurations merged into one
+    system: per evaluation one coefficient update, one force launch
+    sequence (zeroing fused), one ghost fold and one adjoint for the whole
+    set.  The configurations are static, so their energies are affine in
+    beta through per-atom descriptors computed once at set-up
+    (emd_snap_descriptors): E_c = N_c beta_0 + S_c . beta_1.. + R_c with S_c
+    the sum of the configuration's descriptor rows and R_c its energy at
+    beta = 0 (the pair repulsion); the energy part of the gradient is
+    sum_c g_E,c (N_c, S_c), and the adjoint supplies the force part.  Same
+    loss and gradient as SnapForceMatch."""
+
+    def __init__(self, configs, beta_true, ctx=None):
+        pls = []
+        for x, box in configs:
+            pl = Pipeline(ctx)
+            ctx = pl.ctx
+            pl.load(x, box, mass=W_MASS)
+            pl.build(SNAP_CUT, newton=1)
+            pls.append(pl)
+        self.m = MergedSystem(pls, W_MASS)
+        for pl in pls:
+            pl.close()
+        self.ctx = ctx
+        self.L = emd.hip_lib()
+        m, L, c = self.m, self.L, ctx.h
+        self.pl = Pipeline(ctx)          # owner of the SNAP object
+        self.pl.snap_setup(list(beta_true))
+        n = m.s.n_local
+        nc = len(beta_true) - 1
+        # descriptors and the repulsion energy per configuration
+        bdev = m._put(np.zeros(n * nc))
+        m.chk(L.emd_snap_descriptors(c, C.byref(m.s), C.byref(m.nl), self.pl.snap, bdev), "desc")
+        B = m._dl(bdev, n * nc).reshape(n, nc)
+        self.S = np.add.reduceat(B, m.starts, axis=0)
+        eatom = m._put(np.zeros(n))
+        self.pl.snap_set_coeff(np.zeros(nc + 1))
+        e = C.c_double()
+        m.chk(L.emd_snap_energy(c, C.byref(m.s), C.byref(m.nl), self.pl.snap, C.byref(e), eatom),
+              "energy")
+        self.R = np.add.reduceat(m._dl(eatom, n), m.starts)
+        self._g = m._put(np.zeros(3 * n))
+        self.pl.snap_set_coeff(beta_true)
+        self.Ft = self._forces(EMD_FP64)
+        self.Et = self.energies(beta_true)
+
+    def energies(self, beta):
+        """per-configuration energies, E_c = N_c beta_0 + S_c . beta_1.. + R_c"""
+        beta = np.asarray(beta, dtype=np.float64)
+        return self.m.n_atoms * beta[0] + self.S @ beta[1:] + self.R
+
+    def _forces(self, precision):
+        L, c, m = self.L, self.ctx.h, self.m
+        s = C.byref(m.s)
+        m.chk(L.emd_snap_force_zeroed(c, s, C.byref(m.nl), self.pl.snap, precision), "snap")
+        m.chk(L.emd_comm_update_force(c, s, C.byref(m.h), precision), "fold")
+        return m.forces(precision)
+
+    def evaluate(self, beta, precision, grad=True):
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        self.pl.snap_set_coeff(beta)
+        dF = self._forces(precision) - self.Ft
+        n = self.m.n_atoms
+        dE = self.energies(beta) - self.Et
+        loss = float((dF * dF).sum()) + 100.0 * float(np.sum((dE / n) ** 2))
+        g = np.zeros(len(beta))
+        if grad:
+            L, c, m = self.L, self.ctx.h, self.m
+            gF = np.ascontiguousarray(2.0 * dF)
+            m.chk(L.emd_memcpy_h2d(c, self._g, gF.ctypes.data_as(C.c_void_p), gF.nbytes), "h2d")
+            m.chk(L.emd_snap_adjoint_beta(c, C.byref(m.s), C.byref(m.nl), C.byref(m.h),
+                                          self.pl.snap, self._g, 0.0, precision,
+                                          g.ctypes.data_as(C.POINTER(C.c_double))), "adjoint")
+            gE = 200.0 * dE / n ** 2
+            g[0] += float(np.sum(gE * n))
+            g[1:] += gE @ self.S
+        return loss, g
+
+    def close(self):
+        self.m.close()
+        self.pl.close()
 
 
 class LJForceMatch:

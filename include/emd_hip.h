@@ -317,6 +317,17 @@ emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *n
 emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
                                  const emd_halo *h, emd_snap *sn, const double *dLdF,
                                  double dLdE, int precision, double *dLdbeta);
+/* No counterpart in the reference: its ForceSNAP forms no energy and never
+ * the bispectrum itself (src/force.h:54; bvec, force_snap_neigh.h:95-96, is
+ * declared only).  Per local atom, the ncoeff components
+ * B_k = sum' Re(conj(U_J) Z_{j1 j2 J}) whose derivative SNA::compute_dbidrj
+ * forms (src/force_types/sna_impl.hpp:329-539), bzeroflag applied, FP64, into
+ * B (DEVICE, double[n_local][ncoeff]).  The energy of atom i is
+ * beta_0 + sum_k beta_k B_k(i) plus the pair repulsion, so for static
+ * configurations the energy and its beta-gradient are products with this
+ * matrix. */
+emd_status emd_snap_descriptors(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                emd_snap *sn, double *B);
 /* LJ (one type, full list): theta = (eps, sigma); out[2] */
 emd_status emd_lj_adjoint(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl, double eps,
                           double sigma, double cut, const double *dLdF, double dLdE,

@@ -52,3 +52,27 @@ def test_lj_set_gradient():
               - s.evaluate(tm, emd.EMD_FP64, grad=False)[0]) / (2 * h)
         assert abs(fd - g64[k]) <= 1e-7 * scale, (k, fd, g64[k])
     s.close()
+
+
+def test_snap_merged_set_equals_per_configuration():
+    """the configurations merged into one system (one force, fold, energy
+    and adjoint launch sequence for the set, per-atom energy weights) give
+    the per-configuration loss and gradient"""
+    print("case: SNAP force matching, merged system vs per configuration, 7 configs", flush=True)
+    cfgs = fm.snap_set(count=7)
+    bt = fm.snap_beta_true()
+    b = fm.snap_beta_perturbed(bt)
+    ref = fm.SnapForceMatch(cfgs, bt)
+    mer = fm.SnapForceMatchMerged(cfgs, bt, ctx=ref.ctx)
+    for prec, tol in ((emd.EMD_FP64, 1e-12), (emd.EMD_MIXED, 1e-5)):
+        L0, g0 = ref.evaluate(b, prec)
+        L1, g1 = mer.evaluate(b, prec)
+        scale = np.abs(g0).max()
+        print("precision", prec, "loss", L0, L1, "grad rel diff", np.abs(g1 - g0).max() / scale)
+        assert abs(L1 - L0) <= tol * L0
+        assert np.abs(g1 - g0).max() <= tol * scale
+    # at the true parameters the loss and gradient vanish (targets are FP64)
+    L, g = mer.evaluate(bt, emd.EMD_FP64)
+    assert L <= 1e-20 and np.abs(g).max() <= 1e-8 * scale
+    mer.close()
+    ref.close()

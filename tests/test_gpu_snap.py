@@ -314,3 +314,61 @@ def test_snap_ragged_rows_patch_and_isolated_atoms(oracle):
     assert np.abs(fd + f64[np_ + ni:]).max() <= 1e-12 * np.abs(fd).max()
     df = f32 - f64
     assert float(np.sqrt(np.mean(np.sum(df * df, axis=1)))) <= 1e-4
+
+
+def test_snap_descriptors_against_oracle_bispectrum(oracle):
+    """emd_snap_descriptors: per-atom B_k on a disordered BCC W patch equal
+    the oracle's bispectrum of each atom's neighbour shell (rij = x_j - x_i
+    within rc, minimum image) to 1e-12 relative, and beta_0 + beta . B plus
+    the repulsion reproduces the per-atom energy"""
+    import ctypes as C
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print("case: SNAP descriptors, BCC W 4^3 displaced", flush=True)
+    a, nc, rc = 3.1803, 4, 4.73442
+    rng = np.random.default_rng(12)
+    x = lattice("bcc", nc, a) + rng.normal(0, 0.1, (2 * nc ** 3, 3))
+    box = nc * a
+    beta = np.array(decks.snap_beta(seed=2024, sigma=0.05))
+    pl = Pipeline()
+    pl.load(x, (box,) * 3, mass=183.84)
+    pl.build(rc + 1.0, newton=1)
+    pl.snap_setup(list(beta))
+    L, c = pl.L, pl.ctx.h
+    n, ncoef = pl.s.n_local, len(beta) - 1
+    dev, e_dev = C.c_void_p(), C.c_void_p()
+    pl.chk(L.emd_malloc(c, C.byref(dev), n * ncoef * 8), "malloc")
+    pl.chk(L.emd_malloc(c, C.byref(e_dev), n * 8), "malloc")
+    dptr = lambda p: C.cast(p, C.POINTER(C.c_double))
+    pl.chk(L.emd_snap_descriptors(c, C.byref(pl.s), C.byref(pl.nl), pl.snap, dptr(dev)), "desc")
+    B = np.zeros((n, ncoef))
+    pl.chk(L.emd_memcpy_d2h(c, B.ctypes.data_as(C.c_void_p), dev, B.nbytes), "d2h")
+    e = C.c_double()
+    pl.chk(L.emd_snap_energy(c, C.byref(pl.s), C.byref(pl.nl), pl.snap, C.byref(e), dptr(e_dev)),
+           "energy")
+    ea = np.zeros(n)
+    pl.chk(L.emd_memcpy_d2h(c, ea.ctypes.data_as(C.c_void_p), e_dev, ea.nbytes), "d2h")
+    pl.snap_set_coeff(np.zeros_like(beta))
+    pl.chk(L.emd_snap_energy(c, C.byref(pl.s), C.byref(pl.nl), pl.snap, C.byref(e), dptr(e_dev)),
+           "energy")
+    rep = np.zeros(n)
+    pl.chk(L.emd_memcpy_d2h(c, rep.ctypes.data_as(C.c_void_p), e_dev, rep.nbytes), "d2h")
+    ids = pl._device_ids()
+    L.emd_free(c, dev)
+    L.emd_free(c, e_dev)
+    pl.close()
+    assert np.abs(ea - (beta[0] + B @ beta[1:] + rep)).max() <= 1e-12 * np.abs(ea).max()
+    sp = oracle.snap_params(list(beta))
+    xw = x % box
+    worst = 0.0
+    for k in range(0, n, 5):
+        i = ids[k] - 1
+        d = xw - xw[i]
+        d -= box * np.round(d / box)
+        r2 = (d * d).sum(1)
+        sel = (r2 < rc * rc) & (np.arange(len(x)) != i)
+        rij = np.ascontiguousarray(d[sel])
+        Bo = np.zeros(ncoef)
+        oracle.lib().or_snap_bispectrum(C.byref(sp), len(rij), oracle.dptr(rij), oracle.dptr(Bo))
+        worst = max(worst, np.abs(B[k] - Bo).max() / np.abs(Bo).max())
+    print("descriptor relative error", worst)
+    assert worst <= 1e-12
