@@ -23,6 +23,10 @@ LIBDIR = os.path.join(PKG, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+# roctx ranges around the driver's module calls (host/trace.h)
+ROCTX = ["-L", ROCM + "/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath," + ROCM + "/lib"]
+
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
     # no packed FP32 instructions in any kernel
@@ -75,14 +79,14 @@ def build_host():
         return None
     cxx = os.environ.get("CXX", "g++")
     common = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-variable",
-              "-I", INCLUDE, "-I", HOST]
+              "-I", INCLUDE, "-I", HOST, "-I", ROCM + "/include"]
     lib = os.path.join(LIBDIR, "libexaminimd.so")
     _run([cxx] + common + ["-shared", "-o", lib + ".tmp"] + [os.path.join(HOST, s) for s in srcs]
-         + ["-L", LIBDIR, "-lemd_hip", "-Wl,-rpath,$ORIGIN"])
+         + ["-L", LIBDIR, "-lemd_hip", "-Wl,-rpath,$ORIGIN"] + ROCTX)
     os.replace(lib + ".tmp", lib)
     exe = os.path.join(LIBDIR, "ExaMiniMD")
     _run([cxx] + common + ["-o", exe + ".tmp", os.path.join(HOST, "main.cpp"),
-                           "-L", LIBDIR, "-lexaminimd", "-lemd_hip", "-Wl,-rpath,$ORIGIN"])
+                           "-L", LIBDIR, "-lexaminimd", "-lemd_hip", "-Wl,-rpath,$ORIGIN"] + ROCTX)
     os.replace(exe + ".tmp", exe)
     return lib
 

@@ -531,8 +531,32 @@ emd_status read_max_row(emd_ctx *ctx, emd_batch *b, int *m) {
   return EMD_OK;
 }
 
-// capture one rebuild period (rebuild + rebuild_every fused steps) starting
-// from position buffer `parity`
+emd_status ensure_rows(emd_ctx *ctx, emd_batch *b, int cap);
+
+// the row capacity a longest row of m needs, with headroom for density
+// fluctuations until the next rebuilds
+int row_capacity(int m) { return ((m + m / 4 + 16) + 15) / 16 * 16; }
+
+// A rebuild whose longest row is checked at once (one host wait): a row
+// that outgrew the capacity grows the rows and the rebuild runs again, as
+// Neighbor2D grows maxneighs and refills (src/neighbor_types/
+// neighbor_2d.h:304-330).  The rebuild only wraps positions, which is
+// idempotent, so a repeated rebuild gives the list of the same positions.
+emd_status rebuild_checked(emd_ctx *ctx, emd_batch *b) {
+  for (int attempt = 0; attempt < 8; attempt++) {
+    EMD_LAUNCH(ctx, "batch_flags_zero", k_bzero_int, 1, 64, 0, b->flags, 16);
+    EMD_TRY(rebuild(ctx, b));
+    int m = 0;
+    EMD_TRY(read_max_row(ctx, b, &m));
+    if (m <= b->cap) return EMD_OK;
+    EMD_TRY(ensure_rows(ctx, b, row_capacity(m)));
+  }
+  return emd_set_error(ctx, EMD_ERR_OVERFLOW, "batch rebuild: rows keep outgrowing capacity %d",
+                       b->cap);
+}
+
+// capture the rebuild_every fused steps of one rebuild period (the period's
+// rebuild runs before it, checked) starting from position buffer `parity`
 emd_status capture_period(emd_ctx *ctx, emd_batch *b, int parity) {
   const bool prof = ctx->profile;
   const int saved = b->cur;
@@ -540,7 +564,7 @@ emd_status capture_period(emd_ctx *ctx, emd_batch *b, int parity) {
   b->cur = parity;
   hipGraph_t g = nullptr;
   EMD_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  emd_status st = rebuild(ctx, b);
+  emd_status st = EMD_OK;
   for (int k = 0; st == EMD_OK && k < b->rebuild_every; k++) st = step(ctx, b, MODE_FUSED);
   hipError_t e = hipStreamEndCapture(ctx->stream, &g);
   ctx->profile = prof;
@@ -754,8 +778,7 @@ extern "C" emd_status emd_batch_setup(emd_ctx *ctx, emd_batch *b) {
     EMD_TRY(rebuild(ctx, b));
     int m = 0;
     EMD_TRY(read_max_row(ctx, b, &m));
-    // headroom for density fluctuations until the next rebuilds
-    const int want = ((m + m / 4 + 16) + 15) / 16 * 16;
+    const int want = row_capacity(m);
     if (want <= b->cap) break;
     // positions were wrapped in place by the rebuild; wrapping is idempotent
     EMD_TRY(ensure_rows(ctx, b, want));
@@ -782,6 +805,7 @@ extern "C" emd_status emd_batch_run(emd_ctx *ctx, emd_batch *b, int nsteps, int 
   while (k <= nsteps) {
     const long s = b->step + 1;
     if (use_graph && s % R == 0 && k + R - 1 < nsteps) {
+      EMD_TRY(rebuild_checked(ctx, b));
       const int p = b->cur;
       if (!b->graph[p]) EMD_TRY(capture_period(ctx, b, p));
       EMD_HIP(ctx, hipGraphLaunch(b->graph[p], ctx->stream));
@@ -790,7 +814,7 @@ extern "C" emd_status emd_batch_run(emd_ctx *ctx, emd_batch *b, int nsteps, int 
       k += R;
       continue;
     }
-    if (s % R == 0) EMD_TRY(rebuild(ctx, b));
+    if (s % R == 0) EMD_TRY(rebuild_checked(ctx, b));
     EMD_TRY(step(ctx, b, k == nsteps ? MODE_FINAL : MODE_FUSED));
     b->step = s;
     k++;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "property_temperature.h"
+#include "trace.h"
 
 #define MAXPATHLEN 1024
 
@@ -129,6 +130,7 @@ void ExaMiniMD::init(int argc, char* argv[]) {
 
 // T, PE/N, (PE+KE)/N as the thermo lines print them
 void ExaMiniMD::thermo(double out[3]) {
+  TraceRange r("thermo");
   Temperature temp(comm);
   PotE pote(comm);
   KinE kine(comm);
@@ -156,50 +158,74 @@ void ExaMiniMD::run(int nsteps) {
   bool deferred_final = false;
   for (int step = 1; step <= nsteps; step++) {
     other_timer.reset();
-    if (deferred_final) integrator->final_initial_integrate();
-    else integrator->initial_integrate();
+    {
+      TraceRange r("Integrator::initial_integrate");
+      if (deferred_final) integrator->final_initial_integrate();
+      else integrator->initial_integrate();
+    }
     deferred_final = false;
     fence();
     other_time += other_timer.seconds();
 
     if (step % input->comm_exchange_rate == 0 && step > 0) {
       comm_timer.reset();
-      comm->exchange();
+      {
+        TraceRange r("Comm::exchange");
+        comm->exchange();
+      }
       fence();
       comm_time += comm_timer.seconds();
 
       other_timer.reset();
-      binning->create_binning(neigh_cutoff, neigh_cutoff, neigh_cutoff, 1, true, false, true);
+      {
+        TraceRange r("Binning::create_binning (sort)");
+        binning->create_binning(neigh_cutoff, neigh_cutoff, neigh_cutoff, 1, true, false, true);
+      }
       fence();
       other_time += other_timer.seconds();
 
       comm_timer.reset();
-      comm->exchange_halo();
+      {
+        TraceRange r("Comm::exchange_halo");
+        comm->exchange_halo();
+      }
       fence();
       comm_time += comm_timer.seconds();
 
       neigh_timer.reset();
-      binning->create_binning(neigh_cutoff, neigh_cutoff, neigh_cutoff, 1, true, true, false);
-      if (neighbor) neighbor->create_neigh_list(system, binning, force->half_neigh, false);
+      {
+        TraceRange r("Neighbor::create_neigh_list");
+        binning->create_binning(neigh_cutoff, neigh_cutoff, neigh_cutoff, 1, true, true, false);
+        if (neighbor) neighbor->create_neigh_list(system, binning, force->half_neigh, false);
+      }
       fence();
       neigh_time += neigh_timer.seconds();
     } else {
       comm_timer.reset();
-      comm->update_halo();
+      {
+        TraceRange r("Comm::update_halo");
+        comm->update_halo();
+      }
       fence();
       comm_time += comm_timer.seconds();
     }
 
     force_timer.reset();
-    if (!force->stores_forces())
-      EMD_CALL(ctx, emd_force_zero(ctx, system->view(), system->precision));
-    force->compute(system, binning, neighbor);
+    {
+      TraceRange r("Force::compute");
+      if (!force->stores_forces())
+        EMD_CALL(ctx, emd_force_zero(ctx, system->view(), system->precision));
+      force->compute(system, binning, neighbor);
+    }
     fence();
     force_time += force_timer.seconds();
 
     if (input->comm_newton) {
       comm_timer.reset();
-      comm->update_force();
+      {
+        TraceRange r("Comm::update_force");
+        comm->update_force();
+      }
       fence();
       comm_time += comm_timer.seconds();
     }
@@ -210,8 +236,12 @@ void ExaMiniMD::run(int nsteps) {
         (input->dumpbinaryflag && input->dumpbinary_rate > 0 && step % input->dumpbinary_rate == 0) ||
         (input->correctnessflag && input->correctness_rate > 0 &&
          step % input->correctness_rate == 0);
-    if (fuse_integrate && step < nsteps && !observed) deferred_final = true;
-    else integrator->final_integrate();
+    if (fuse_integrate && step < nsteps && !observed) {
+      deferred_final = true;
+    } else {
+      TraceRange r("Integrator::final_integrate");
+      integrator->final_integrate();
+    }
 
     if (input->thermo_rate > 0 && step % input->thermo_rate == 0) {
       double th[3];

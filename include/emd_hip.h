@@ -372,10 +372,11 @@ emd_status emd_batch_download(emd_ctx *ctx, emd_batch *b, double *x, double *v, 
 /* wrap + neighbour build + force at the current positions (the reference's
  * set-up force before step 1) */
 emd_status emd_batch_setup(emd_ctx *ctx, emd_batch *b);
-/* advance nsteps; use_graph != 0 replays the captured rebuild period.
- * Returns EMD_ERR_OVERFLOW if a neighbour row outgrew its capacity during
- * the run (results are then invalid; call emd_batch_setup again, which
- * re-sizes the rows) */
+/* advance nsteps; use_graph != 0 replays each rebuild period's steps as a
+ * captured graph.  Every rebuild checks its longest row at once (one host
+ * wait per rebuild): rows that outgrew their capacity are grown and the
+ * rebuild repeated, as Neighbor2D does (src/neighbor_types/
+ * neighbor_2d.h:304-330), so no step runs on truncated rows */
 emd_status emd_batch_run(emd_ctx *ctx, emd_batch *b, int nsteps, int use_graph);
 /* per replica: ke[r] = 0.5 sum m v^2, pe[r] = shifted LJ energy (FP64,
  * compensated); either may be NULL */

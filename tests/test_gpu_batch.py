@@ -213,3 +213,41 @@ def test_replica_loop_matches_batch():
         print("replica", r, "loop vs batch x v relative errors", errs)
         assert max(errs) <= 1e-12, r
     b.close()
+
+
+def test_batch_row_overflow_grows_and_rebuilds():
+    """one replica implodes (velocities toward its centre, 14 sigma/tau at the
+    faces): its rows outgrow the capacity set up on the lattice, and the
+    rebuilds at steps 20 (before a graph-replayed period) and 40 (the run's
+    last step, eager) grow the rows and rebuild; the rows of step 40 are then
+    exactly the brute-force neighbour sets over all 27 images"""
+    print("case: batch rebuild with a row overflow, grow and rebuild", flush=True)
+    n, box, x, v = small_ensemble(seed=13)
+    start = np.concatenate([[0], np.cumsum(n)])
+    r = 2
+    L = box[r][0]
+    v[start[r]:start[r + 1]] = -3.5 * (x[start[r]:start[r + 1]] - L / 2)
+    b = make(n, box, x, v, EMD_FP64)
+    cap0 = b.info()[2]
+    b.run(40)                     # ends with the step-40 rebuild and its force
+    step, mrow, cap = b.info()
+    print("capacity", cap0, "->", cap, "longest row", mrow)
+    assert step == 40 and cap > cap0 and mrow <= cap
+    xw, _, _ = b.download()
+    num, ent = b.neigh()
+    b.close()
+    xr = xw[start[r]:start[r + 1]]
+    shifts = np.array([(p, q, t) for p in (-1, 0, 1) for q in (-1, 0, 1) for t in (-1, 0, 1)])
+    for i in range(len(xr)):
+        d = xr[i][None, None, :] - (xr[None, :, :] + shifts[:, None, :] * L)
+        ks, js = np.nonzero((d * d).sum(-1) <= 2.8 ** 2)
+        want = {(start[r] + j, tuple(shifts[k])) for k, j in zip(ks, js)
+                if not (j == i and not shifts[k].any())}
+        row = ent[start[r] + i, :num[start[r] + i]]
+        got = set()
+        for e in row:
+            code = int(e) >> batch.CODE_SHIFT
+            sh = tuple(0 if c == 0 else (-1 if c == 1 else 1)
+                       for c in ((code >> 2 * a) & 3 for a in range(3)))
+            got.add((int(e) & batch.JMASK, sh))
+        assert got == want and len(row) == len(want), i

@@ -222,6 +222,27 @@ def test_lj_mixed_force_error_every_lane_count(tmp_path, lanes):
     app.close()
 
 
+def test_lost_atom_reported_by_binning():
+    """an atom displaced by more than one box length (exchange wraps by one
+    length only, as CommSerial::exchange does, src/comm_types/
+    comm_serial.h:97-107) lies outside the bin grid: the binning reports
+    EMD_ERR_OVERFLOW instead of binning it (the kernel clamps the bin index
+    it flags, so nothing is written out of bounds)"""
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print("case: lost atom, FCC 5^3 with one atom 3.5 box lengths away", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    box = 5 * a
+    x = lattice("fcc", 5, a)
+    x[17, 0] += 3.5 * box       # 2.5 box lengths out after the one wrap
+    pl = Pipeline()
+    pl.load(x, (box,) * 3)
+    with pytest.raises(emd.EmdError) as err:
+        pl.build(2.8)
+    print(err.value)
+    assert "status 4" in str(err.value) and "outside the bin grid" in str(err.value)
+    pl.close()
+
+
 def _dev(ctx, ptr, n, dtype):
     import ctypes as C
     out = np.zeros(n, dtype)
