@@ -507,19 +507,30 @@ def forcematch_leg(emd, repeats=3):
                                              float(np.abs(g_per - g32).max() / np.abs(g32).max())},
                    "method": "64 configurations merged into one system"}
     cfgs = fm.lj_set()
-    lj = fm.LJForceMatch(cfgs)
     th = np.array([1.04, 0.98])
-    lj.evaluate(th, EMD_MIXED)
-    t0 = time.perf_counter()
-    for _ in range(repeats):
-        L32, g32 = lj.evaluate(th, EMD_MIXED)
-    wall = (time.perf_counter() - t0) / repeats
+
+    def timed_lj(obj):
+        obj.evaluate(th, EMD_MIXED)
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            res = obj.evaluate(th, EMD_MIXED)
+        return (time.perf_counter() - t0) / repeats, res
+
+    lj = fm.LJForceMatchMerged(cfgs)
+    wall, (L32, g32) = timed_lj(lj)
     L64, g64 = lj.evaluate(th, EMD_FP64)
+    per = fm.LJForceMatch(cfgs, ctx=lj.ctx)
+    wall_per, (_, g_per) = timed_lj(per)
+    per.close()
     natoms = sum(len(x) for x, _ in cfgs)
     lj.close()
     out["lj"] = {"configs": len(cfgs), "atoms": natoms, "ms_per_gradient": 1e3 * wall,
                  "atom_gradients_per_s": natoms / wall,
-                 "grad_rel_err_mixed_vs_fp64": float(np.abs(g32 - g64).max() / np.abs(g64).max())}
+                 "grad_rel_err_mixed_vs_fp64": float(np.abs(g32 - g64).max() / np.abs(g64).max()),
+                 "per_configuration": {"ms_per_gradient": 1e3 * wall_per,
+                                       "grad_rel_diff_vs_merged":
+                                           float(np.abs(g_per - g32).max() / np.abs(g32).max())},
+                 "method": "32 boxes merged into one system"}
     return out
 
 

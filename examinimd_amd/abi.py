@@ -120,6 +120,8 @@ HIP_PROTOS = {
                                            P(EmdLJIDialParams), C.c_int]),
     "emd_lj_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int,
                                 P(C.c_double)]),
+    "emd_lj_energy_atoms": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams), C.c_int,
+                                      dptr]),
     "emd_lj_energy_noalloc": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), P(EmdLJParams),
                                         C.c_int, P(C.c_double)]),
     "emd_lj_set_lanes": (C.c_int, [vp, C.c_int, C.c_int]),
@@ -138,6 +140,8 @@ HIP_PROTOS = {
     "emd_snap_descriptors": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, dptr]),
     "emd_lj_adjoint": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), C.c_double, C.c_double,
                                  C.c_double, dptr, C.c_double, C.c_int, dptr]),
+    "emd_lj_adjoint_atoms": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), C.c_double, C.c_double,
+                                       C.c_double, dptr, dptr, C.c_int, dptr]),
     "emd_batch_create": (C.c_int, [vp, P(EmdBatchDesc), P(vp)]),
     "emd_batch_destroy": (C.c_int, [vp, vp]),
     "emd_batch_upload": (C.c_int, [vp, vp, dptr, dptr]),

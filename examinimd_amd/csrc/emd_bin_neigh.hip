@@ -1,11 +1,11 @@
 // emd_bin_neigh.hip -- cell binning (counting sort) and neighbour lists.
 //
 // Binning: bin id per atom (Kokkos BinOp3D arithmetic: int(mul*(x-min)),
-// mul = nbin/(max-min), x-major flattening), then a stable LSD radix sort of
-// (bin id, index) pairs over the bin id's bits (rocPRIM), so the permutation
-// lists each bin's atoms in ascending index (the order the reference produces
-// on its serial backend; deterministic here); counts and offsets are the
-// bounds of each bin's run in the sorted keys.
+// mul = nbin/(max-min), x-major flattening), then a counting sort: atomic
+// counts, an exclusive scan of the counts, a scatter through per-bin cursors
+// and a per-bin ordering pass, so the permutation lists each bin's atoms in
+// ascending index (the order the reference produces on its serial backend;
+// deterministic here).
 //
 // Neighbour build: one 256-thread block per interior bin.  The block stages
 // the 27 stencil bins' candidates (index + FP64 position) in LDS in stencil
@@ -15,53 +15,129 @@
 // with an exclusive scan between them, as NeighborCSR does.
 #include "emd_internal.hpp"
 
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
-
 #include <type_traits>
 
 __global__ void k_max_int(const int *__restrict__ a, int n, int *__restrict__ out);
 
 // ------------------------------------------------------------------ binning
-// BinOp3D::bin (Kokkos_Sort.hpp) as used by binning_kksort.cpp:105-113
-__global__ void k_bin_key(const double *__restrict__ x, int r0, int n, double minx, double miny,
-                          double minz, double mulx, double muly, double mulz, int nbx, int nby,
-                          int nbz, unsigned *__restrict__ key, int *__restrict__ err) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double *xi = x + 3 * (size_t)(r0 + i);
+// BinOp3D::bin (Kokkos_Sort.hpp) as used by binning_kksort.cpp:105-113: the
+// bin of an atom, clamped into the grid (an atom outside it raises the
+// lost-atom flag instead of indexing out of bounds)
+__device__ __forceinline__ int bin_index(const double *xi, double minx, double miny,
+                                         double minz, double mulx, double muly, double mulz,
+                                         int nbx, int nby, int nbz, bool &outside) {
   int bx = (int)(mulx * (xi[0] - minx));
   int by = (int)(muly * (xi[1] - miny));
   int bz = (int)(mulz * (xi[2] - minz));
-  if (bx < 0 || bx >= nbx || by < 0 || by >= nby || bz < 0 || bz >= nbz) {
-    atomicOr(err, 1);
-    bx = bx < 0 ? 0 : (bx >= nbx ? nbx - 1 : bx);
-    by = by < 0 ? 0 : (by >= nby ? nby - 1 : by);
-    bz = bz < 0 ? 0 : (bz >= nbz ? nbz - 1 : bz);
-  }
-  key[i] = (unsigned)((bx * nby + by) * nbz + bz);
+  outside = bx < 0 || bx >= nbx || by < 0 || by >= nby || bz < 0 || bz >= nbz;
+  bx = bx < 0 ? 0 : (bx >= nbx ? nbx - 1 : bx);
+  by = by < 0 ? 0 : (by >= nby ? nby - 1 : by);
+  bz = bz < 0 ? 0 : (bz >= nbz ? nbz - 1 : bz);
+  return (bx * nby + by) * nbz + bz;
 }
 
-// first position in the sorted keys with key >= b
-__device__ inline int lower_bound_key(const unsigned *__restrict__ k, int n, unsigned b) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (k[mid] < b) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
+// Counting sort, the large-range binning path (BinSort's result: per bin
+// its atoms in ascending index, src/binning_types/binning_kksort.cpp:105-125).
+// (1) each atom's bin id, counted into the bins with atomics;
+// (2) exclusive scan of the counts (wavefront scans, emd_scan) -> offsets;
+// (3) scatter through per-bin cursors;
+// (4) each bin's slice put in ascending atom order by one wavefront: a
+//     lane's rank is the number of smaller indices in the slice (64-wide
+//     shuffle sweep, registers only; slices over 64 atoms, which a dense
+//     cluster or a wide bin can give, take an insertion sort).
+// Atoms were sorted by bin at the previous rebuild and have moved little,
+// so consecutive atoms mostly share a bin: in (1) and (3) a wavefront's runs
+// of equal bin ids make one atomic per run (the run's first lane adds the
+// run length; the lanes of a run take consecutive slots in index order).
+
+// the run of equal bin ids (consecutive lanes) that the lane belongs to:
+// its first lane and its length; lanes without an atom form runs of their own
+__device__ __forceinline__ void bin_run(int b, bool valid, int &first, int &len) {
+  const int lane = __lane_id();
+  const int prev = __shfl_up(b, 1, EMD_WAVE);
+  // lanes without an atom are a suffix of the wavefront; each is a head
+  const bool head = lane == 0 || !valid || b != prev;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = heads & (emd_lanemask_lt() | (1ull << lane));
+  first = 63 - __clzll(upto);
+  const unsigned long long after = lane == EMD_WAVE - 1 ? 0ull : heads >> (lane + 1);
+  const int end = after ? lane + __ffsll((long long)after) : EMD_WAVE;
+  len = end - first;
 }
 
-// bincount / binoffsets from the sorted keys: each bin's run [lb(b), lb(b+1))
-__global__ void k_bin_bounds(const unsigned *__restrict__ skey, int n, int nbins,
-                             int *__restrict__ count, int *__restrict__ offset) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  const int lo = lower_bound_key(skey, n, (unsigned)b);
-  const int hi = lower_bound_key(skey, n, (unsigned)b + 1u);
-  offset[b] = lo;
-  count[b] = hi - lo;
+__global__ void k_bin_count(const double *__restrict__ x, int r0, int n, double minx,
+                            double miny, double minz, double mulx, double muly, double mulz,
+                            int nbx, int nby, int nbz, int *__restrict__ binid,
+                            int *__restrict__ count, int *__restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  int b = -1;
+  if (valid) {
+    bool outside;
+    b = bin_index(x + 3 * (size_t)(r0 + i), minx, miny, minz, mulx, muly, mulz, nbx, nby, nbz,
+                  outside);
+    if (outside) atomicOr(err, 1);
+    binid[i] = b;
+  }
+  int first, len;
+  bin_run(b, valid, first, len);
+  if (valid && __lane_id() == first) atomicAdd(&count[b], len);
+}
+
+struct bin_offsets_op {
+  const int *count;
+  int *offset, *cursor;
+  __device__ int value(int b) const { return count[b]; }
+  __device__ void emit(int b, int excl, int) const {
+    offset[b] = excl;
+    cursor[b] = excl;
+  }
+};
+
+__global__ void k_bin_scatter(const int *__restrict__ binid, int n, int *__restrict__ cursor,
+                              int *__restrict__ permute) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  const int b = valid ? binid[i] : -1;
+  int first, len;
+  bin_run(b, valid, first, len);
+  int base = 0;
+  if (valid && __lane_id() == first) base = atomicAdd(&cursor[b], len);
+  base = __shfl(base, first, EMD_WAVE);
+  if (valid) permute[base + (__lane_id() - first)] = i;
+}
+
+constexpr int BIN_ORDER_THREADS = 256;
+__global__ void __launch_bounds__(BIN_ORDER_THREADS)
+k_bin_order(const int *__restrict__ count, const int *__restrict__ offset, int nbins,
+            int *__restrict__ permute) {
+  const int b = blockIdx.x * (BIN_ORDER_THREADS / EMD_WAVE) + threadIdx.x / EMD_WAVE;
+  const int lane = threadIdx.x % EMD_WAVE;
+  int c = 0;
+  if (b < nbins) {
+    c = count[b];
+    int *p = permute + offset[b];
+    if (c <= EMD_WAVE) {
+      const int v = lane < c ? p[lane] : 0x7fffffff;
+      // slices already in order (one run of the previous order) stay
+      const int next = __shfl_down(v, 1, EMD_WAVE);
+      if (__ballot(lane + 1 < c && next < v) != 0ull) {
+        int rank = 0;
+        for (int k = 0; k < c; k++) rank += __shfl(v, k, EMD_WAVE) < v;
+        if (lane < c) p[rank] = v;
+      }
+    } else if (lane == 0) {
+      for (int k = 1; k < c; k++) {
+        const int v = p[k];
+        int m = k - 1;
+        while (m >= 0 && p[m] > v) {
+          p[m + 1] = p[m];
+          m--;
+        }
+        p[m + 1] = v;
+      }
+    }
+  }
 }
 
 // Small ranges (n <= BIN_ONE_MAX_N atoms, nbins <= BIN_ONE_MAX_BINS) in one
@@ -90,15 +166,8 @@ k_bin_one(const double *__restrict__ x, int r0, int n, double minx, double miny,
   __syncthreads();
   // pass 1: histogram (the bin id is recomputed in pass 2, not stored)
   auto bin_of = [&](int i, bool &bad) {
-    const double *xi = x + 3 * (size_t)(r0 + i);
-    int bx = (int)(mulx * (xi[0] - minx));
-    int by = (int)(muly * (xi[1] - miny));
-    int bz = (int)(mulz * (xi[2] - minz));
-    bad = bx < 0 || bx >= nbx || by < 0 || by >= nby || bz < 0 || bz >= nbz;
-    bx = bx < 0 ? 0 : (bx >= nbx ? nbx - 1 : bx);
-    by = by < 0 ? 0 : (by >= nby ? nby - 1 : by);
-    bz = bz < 0 ? 0 : (bz >= nbz ? nbz - 1 : bz);
-    return (bx * nby + by) * nbz + bz;
+    return bin_index(x + 3 * (size_t)(r0 + i), minx, miny, minz, mulx, muly, mulz, nbx, nby, nbz,
+                     bad);
   };
   for (int i = tid; i < n; i += BIN_ONE_THREADS) {
     bool bad;
@@ -284,32 +353,24 @@ extern "C" emd_status emd_binning(emd_ctx *ctx, emd_system *s, double dx_in, dou
                b->binoffsets, b->permute, ctx->d_small);
     return bin_finish(ctx, s, b, r0, n, sort);
   }
-  // scratch: unsorted keys (binid), sorted keys, radix-sort workspace
-  unsigned *key = reinterpret_cast<unsigned *>(b->binid);
-  int kbits = 1;
-  while (kbits < 31 && (1l << kbits) < nbt) kbits++;
-  size_t sort_bytes = 0;
-  const rocprim::counting_iterator<int> iota(0);
-  EMD_HIP(ctx, rocprim::radix_sort_pairs(nullptr, sort_bytes, key, (unsigned *)nullptr, iota,
-                                         b->permute, n > 0 ? n : 1, 0u, (unsigned)kbits,
-                                         ctx->stream));
-  const size_t skey_bytes = ((size_t)(n > 0 ? n : 1) * sizeof(unsigned) + 255) & ~(size_t)255;
-  EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, skey_bytes + sort_bytes));
-  unsigned *skey = reinterpret_cast<unsigned *>(ctx->d_tmp2);
-  void *sort_tmp = static_cast<char *>(ctx->d_tmp2) + skey_bytes;
-
+  // counting sort: counts, offsets (with cursors in scratch), scatter,
+  // per-bin ordering; the largest count and the lost-atom flag in d_small
+  EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)nbt * sizeof(int)));
+  int *cursor = static_cast<int *>(ctx->d_tmp2);
   EMD_LAUNCH(ctx, "binning", k_fill_int, 1, 64, 0, ctx->d_small + 1, 2, 0);
-  if (n > 0) {
-    EMD_LAUNCH(ctx, "binning", k_bin_key, emd_blocks(n, 256), 256, 0, s->x, r0, n, mn[0], mn[1],
-               mn[2], mul[0], mul[1], mul[2], nb[0], nb[1], nb[2], key, ctx->d_small + 2);
-    emd_prof_rec rec;
-    emd_prof_begin(ctx, "binning_sort_keys", &rec);
-    EMD_HIP(ctx, rocprim::radix_sort_pairs(sort_tmp, sort_bytes, key, skey, iota, b->permute, n,
-                                           0u, (unsigned)kbits, ctx->stream));
-    emd_prof_end(ctx, &rec);
-  }
-  EMD_LAUNCH(ctx, "binning", k_bin_bounds, emd_blocks(nbt, 256), 256, 0, skey, n, (int)nbt,
-             b->bincount, b->binoffsets);
+  EMD_LAUNCH(ctx, "binning", k_fill_int, emd_blocks(nbt, 256), 256, 0, b->bincount, (int)nbt, 0);
+  if (n > 0)
+    EMD_LAUNCH(ctx, "binning", k_bin_count, emd_blocks(n, 256), 256, 0, s->x, r0, n, mn[0], mn[1],
+               mn[2], mul[0], mul[1], mul[2], nb[0], nb[1], nb[2], b->binid, b->bincount,
+               ctx->d_small + 2);
+  EMD_TRY(emd_scan(ctx, bin_offsets_op{b->bincount, b->binoffsets, cursor}, (int)nbt,
+                   "binning_scan", nullptr));
+  if (n > 0)
+    EMD_LAUNCH(ctx, "binning", k_bin_scatter, emd_blocks(n, 256), 256, 0, b->binid, n, cursor,
+               b->permute);
+  EMD_LAUNCH(ctx, "binning_order", k_bin_order,
+             emd_blocks(nbt, BIN_ORDER_THREADS / EMD_WAVE), BIN_ORDER_THREADS, 0, b->bincount,
+             b->binoffsets, (int)nbt, b->permute);
   {
     const int nbm = emd_blocks(nbt, 256) < 512 ? emd_blocks(nbt, 256) : 512;
     EMD_LAUNCH(ctx, "binning", k_max_int, nbm, 256, 0, b->bincount, (int)nbt, ctx->d_small + 1);

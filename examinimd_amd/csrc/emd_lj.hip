@@ -477,7 +477,9 @@ __device__ __forceinline__ double lj_pair_e(double r6inv, double lj1, double lj2
   return r6inv * (0.5 * lj1 * r6inv - lj2) / 6.0;
 }
 
-template <bool ONE_TYPE>
+// PER_ATOM: each atom's energy (its LJE_K lanes' sum) into eatom[i] instead
+// of block sums into part
+template <bool ONE_TYPE, bool PER_ATOM = false>
 __global__ void __launch_bounds__(LJ_THREADS)
 k_lj_energy(const double *__restrict__ x, const int *__restrict__ type,
             const int *__restrict__ row, const int *__restrict__ num,
@@ -535,7 +537,12 @@ k_lj_energy(const double *__restrict__ x, const int *__restrict__ type,
       }
     }
   }
-  block_dd_sum(pe, 0.0, part);
+  if (PER_ATOM) {
+    pe = group_sum<LJE_K>(pe);
+    if (i < n && sub == 0) part[i] = pe;
+  } else {
+    block_dd_sum(pe, 0.0, part);
+  }
 }
 
 static emd_status finish_dd(emd_ctx *ctx, double *part, int nb, double *out) {
@@ -654,6 +661,27 @@ extern "C" emd_status emd_lj_energy_noalloc(emd_ctx *ctx, const emd_system *s,
   return EMD_OK;
 }
 
+extern "C" emd_status emd_lj_energy_atoms(emd_ctx *ctx, const emd_system *s,
+                                          const emd_neigh *nl, const emd_lj_params *p, int half,
+                                          double *eatom) {
+  if (!ctx || !s || !nl || !p || !eatom) return EMD_ERR_ARG;
+  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
+  EMD_ARG(ctx, s->ntypes <= p->ntypes, "the system has more atom types than the parameters");
+  const int n = s->n_local;
+  if (n <= 0) return EMD_OK;
+  lj_table *tab;
+  EMD_TRY(upload_table(ctx, p, nullptr, &tab));
+  const int nb = emd_blocks((long)n * LJE_K, LJ_THREADS);
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  if (p->ntypes == 1)
+    EMD_LAUNCH(ctx, "lj_energy", (k_lj_energy<true, true>), nb, LJ_THREADS, 0, s->x, s->type,
+               row, nl->num, nl->entries, nl->maxneighs, n, half, tab, eatom);
+  else
+    EMD_LAUNCH(ctx, "lj_energy", (k_lj_energy<false, true>), nb, LJ_THREADS, 0, s->x, s->type,
+               row, nl->num, nl->entries, nl->maxneighs, n, half, tab, eatom);
+  return EMD_OK;
+}
+
 // ------------------------------------------------ adjoint d/d(eps, sigma)
 // For a cotangent g = dL/dF over the local atoms and gE = dL/dE:
 //   dL/dtheta = sum_i sum_{j in N(i)} g_i . r_ij dfpair/dtheta + gE dE/dtheta
@@ -665,7 +693,7 @@ __global__ void __launch_bounds__(256)
 k_lj_adjoint(const double *__restrict__ x, const int *__restrict__ row,
              const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
              double eps, double sigma, double cutsq, const double *__restrict__ g,
-             double *__restrict__ out) {
+             const double *__restrict__ wE, double *__restrict__ out) {
   __shared__ double s_red[4][4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double a[4] = {0.0, 0.0, 0.0, 0.0};
@@ -678,6 +706,7 @@ k_lj_adjoint(const double *__restrict__ x, const int *__restrict__ row,
     const T rc2inv = (T)(1.0 / cutsq), rc6 = rc2inv * rc2inv * rc2inv;
     const size_t start = row ? (size_t)row[i] : (size_t)i * (size_t)pitch;
     const int cnt = num[i];
+    const double we = wE ? wE[i] : 1.0;  // the energy cotangent of atom i
     for (int jj = 0; jj < cnt; jj++) {
       const int j = entries[start + jj];
       const double dx = x_i - x[3 * (size_t)j];
@@ -692,9 +721,10 @@ k_lj_adjoint(const double *__restrict__ x, const int *__restrict__ row,
         const double gdx = g0 * dx + g1 * dy + g2 * dz;
         a[0] += gdx * (double)dfde;
         a[1] += gdx * (double)dfds;
-        a[2] += (double)((T)2 * ((s12 * r6inv * r6inv - s6 * r6inv) - (s12 * rc6 * rc6 - s6 * rc6)));
-        a[3] += (double)((T)2 * e * (((T)12 * s11 * r6inv * r6inv - (T)6 * s5 * r6inv) -
-                                     ((T)12 * s11 * rc6 * rc6 - (T)6 * s5 * rc6)));
+        a[2] += we * (double)((T)2 * ((s12 * r6inv * r6inv - s6 * r6inv) -
+                                      (s12 * rc6 * rc6 - s6 * rc6)));
+        a[3] += we * (double)((T)2 * e * (((T)12 * s11 * r6inv * r6inv - (T)6 * s5 * r6inv) -
+                                          ((T)12 * s11 * rc6 * rc6 - (T)6 * s5 * rc6)));
       }
     }
   }
@@ -716,9 +746,9 @@ __global__ void k_zero4(double *p) {
   if (threadIdx.x < 4) p[threadIdx.x] = 0.0;
 }
 
-extern "C" emd_status emd_lj_adjoint(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
-                                     double eps, double sigma, double cut, const double *dLdF,
-                                     double dLdE, int precision, double *out) {
+static emd_status lj_adjoint(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl, double eps,
+                             double sigma, double cut, const double *dLdF, double dLdE,
+                             const double *dLdE_atom, int precision, double *out) {
   if (!ctx || !s || !nl || !dLdF || !out) return EMD_ERR_ARG;
   EMD_ARG(ctx, eps > 0 && sigma > 0 && cut > 0, "eps, sigma and cut must be positive");
   out[0] = out[1] = 0.0;
@@ -729,16 +759,34 @@ extern "C" emd_status emd_lj_adjoint(emd_ctx *ctx, const emd_system *s, const em
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   if (precision == EMD_MIXED)
     EMD_LAUNCH(ctx, "lj_adjoint", k_lj_adjoint<float>, emd_blocks(n, 256), 256, 0, s->x, row,
-               nl->num, nl->entries, nl->maxneighs, n, eps, sigma, cut * cut, dLdF, acc);
+               nl->num, nl->entries, nl->maxneighs, n, eps, sigma, cut * cut, dLdF, dLdE_atom,
+               acc);
   else
     EMD_LAUNCH(ctx, "lj_adjoint", k_lj_adjoint<double>, emd_blocks(n, 256), 256, 0, s->x, row,
-               nl->num, nl->entries, nl->maxneighs, n, eps, sigma, cut * cut, dLdF, acc);
+               nl->num, nl->entries, nl->maxneighs, n, eps, sigma, cut * cut, dLdF, dLdE_atom,
+               acc);
   EMD_HIP(ctx, hipMemcpyAsync(ctx->h_dsmall, acc, 4 * sizeof(double), hipMemcpyDeviceToHost,
                               ctx->stream));
   EMD_TRY(emd_wait(ctx));
-  out[0] = ctx->h_dsmall[0] + dLdE * ctx->h_dsmall[2];
-  out[1] = ctx->h_dsmall[1] + dLdE * ctx->h_dsmall[3];
+  // per-atom weights are inside the energy sums; one weight multiplies them
+  const double w = dLdE_atom ? 1.0 : dLdE;
+  out[0] = ctx->h_dsmall[0] + w * ctx->h_dsmall[2];
+  out[1] = ctx->h_dsmall[1] + w * ctx->h_dsmall[3];
   return EMD_OK;
+}
+
+extern "C" emd_status emd_lj_adjoint(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                     double eps, double sigma, double cut, const double *dLdF,
+                                     double dLdE, int precision, double *out) {
+  return lj_adjoint(ctx, s, nl, eps, sigma, cut, dLdF, dLdE, nullptr, precision, out);
+}
+
+extern "C" emd_status emd_lj_adjoint_atoms(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                           double eps, double sigma, double cut,
+                                           const double *dLdF, const double *dLdE_atom,
+                                           int precision, double *out) {
+  if (!dLdE_atom) return EMD_ERR_ARG;
+  return lj_adjoint(ctx, s, nl, eps, sigma, cut, dLdF, 0.0, dLdE_atom, precision, out);
 }
 
 // ---------------------------------------------------------------- NVE

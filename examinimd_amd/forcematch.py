@@ -307,6 +307,74 @@ class SnapForceMatchMerged:
         self.pl.close()
 
 
+class LJForceMatchMerged:
+    """loss(eps, sigma) and its gradient over LJ boxes merged into one system:
+    per evaluation one force launch (stored forces), one per-atom energy
+    launch and one adjoint launch for the whole set, the energy cotangent of
+    each box entering the adjoint as a weight on its atoms
+    (emd_lj_adjoint_atoms).  Same loss and gradient as LJForceMatch."""
+
+    def __init__(self, configs, true=(1.0, 1.0), ctx=None, rc=2.5):
+        self.rc = rc
+        pls = []
+        for x, box in configs:
+            pl = Pipeline(ctx)
+            ctx = pl.ctx
+            pl.load(x, box)
+            pl.build(rc)
+            pls.append(pl)
+        self.m = MergedSystem(pls, 1.0)
+        for pl in pls:
+            pl.close()
+        self.ctx = ctx
+        self.L = emd.hip_lib()
+        n = self.m.s.n_local
+        self._eatom = self.m._put(np.zeros(n))
+        self._g = self.m._put(np.zeros(3 * n))
+        self._w = self.m._put(np.zeros(n))
+        self.Ft, self.Et = self._force_energy(EMD_FP64, *true)
+
+    def _params(self, eps, sig):
+        from examinimd_amd.abi import EmdLJParams
+        p = EmdLJParams()
+        p.ntypes = 1
+        p.lj1[0][0] = 48.0 * eps * sig ** 12
+        p.lj2[0][0] = 24.0 * eps * sig ** 6
+        p.cutsq[0][0] = self.rc * self.rc
+        return p
+
+    def _force_energy(self, precision, eps, sig):
+        L, c, m = self.L, self.ctx.h, self.m
+        p = self._params(eps, sig)
+        m.chk(L.emd_lj_force_store(c, C.byref(m.s), C.byref(m.nl), C.byref(p), precision), "lj")
+        F = m.forces(precision)
+        m.chk(L.emd_lj_energy_atoms(c, C.byref(m.s), C.byref(m.nl), C.byref(p), 0, self._eatom),
+              "energy")
+        return F, np.add.reduceat(m._dl(self._eatom, m.s.n_local), m.starts)
+
+    def evaluate(self, theta, precision, grad=True):
+        eps, sig = (float(t) for t in theta)
+        F, E = self._force_energy(precision, eps, sig)
+        dF = F - self.Ft
+        n = self.m.n_atoms
+        dE = E - self.Et
+        loss = float((dF * dF).sum()) + 100.0 * float(np.sum((dE / n) ** 2))
+        g = np.zeros(2)
+        if grad:
+            L, c, m = self.L, self.ctx.h, self.m
+            w = np.repeat(200.0 * dE / n ** 2, n)
+            gF = np.ascontiguousarray(2.0 * dF)
+            m.chk(L.emd_memcpy_h2d(c, self._g, gF.ctypes.data_as(C.c_void_p), gF.nbytes), "h2d")
+            m.chk(L.emd_memcpy_h2d(c, self._w, w.ctypes.data_as(C.c_void_p), w.nbytes), "h2d")
+            m.chk(L.emd_lj_adjoint_atoms(c, C.byref(m.s), C.byref(m.nl), eps, sig, self.rc,
+                                         self._g, self._w, precision,
+                                         g.ctypes.data_as(C.POINTER(C.c_double))), "adjoint")
+        return loss, g
+
+    def close(self):
+        self.m.close()
+
+
 class LJForceMatch:
     """loss(eps, sigma) and its gradient over LJ boxes (cutoff 2.5)"""
 

@@ -242,6 +242,12 @@ emd_status emd_lj_idial_force_store(emd_ctx *ctx, emd_system *s, const emd_neigh
 /* ForceLJNeigh::compute_energy, force_lj_neigh_impl.h:128-156 (compensated) */
 emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
                          const emd_lj_params *p, int half, double *pe);
+/* the same energy per local atom into eatom (DEVICE, double[n_local]): the
+ * atom's share of its pairs' shifted energies (force_lj_neigh_impl.h:
+ * 256-343: one half per full-list pair; half lists give a local j's pair
+ * wholly to the row atom and half to a ghost j's) */
+emd_status emd_lj_energy_atoms(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                               const emd_lj_params *p, int half, double *eatom);
 /* No counterpart in the reference.  emd_lj_energy for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): the same energy (to rounding: the block sums are added
@@ -332,6 +338,13 @@ emd_status emd_snap_descriptors(emd_ctx *ctx, const emd_system *s, const emd_nei
 emd_status emd_lj_adjoint(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl, double eps,
                           double sigma, double cut, const double *dLdF, double dLdE,
                           int precision, double *out);
+/* the same with one energy cotangent per atom, dLdE_atom (DEVICE,
+ * double[n_local]): sum_i dLdF_i . dF_i/dtheta + sum_i dLdE_atom[i]
+ * dE_i/dtheta, E_i the atom's half of its pair energies.  Configurations
+ * merged into one system take their whole gradient in one call. */
+emd_status emd_lj_adjoint_atoms(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                double eps, double sigma, double cut, const double *dLdF,
+                                const double *dLdE_atom, int precision, double *out);
 
 /* -------------------------------------------------- batched replicas */
 /* No counterpart in the reference: many independent periodic LJ systems

@@ -76,3 +76,22 @@ def test_snap_merged_set_equals_per_configuration():
     assert L <= 1e-20 and np.abs(g).max() <= 1e-8 * scale
     mer.close()
     ref.close()
+
+
+def test_lj_merged_set_equals_per_configuration():
+    """LJ boxes merged into one system (per-atom energies, per-atom energy
+    weights in the adjoint) give the per-box loss and gradient"""
+    print("case: LJ force matching, merged system vs per box, 6 boxes", flush=True)
+    cfgs = fm.lj_set(count=6)
+    ref = fm.LJForceMatch(cfgs)
+    mer = fm.LJForceMatchMerged(cfgs, ctx=ref.ctx)
+    th = np.array([1.04, 0.98])
+    for prec, tol in ((emd.EMD_FP64, 1e-12), (emd.EMD_MIXED, 1e-6)):
+        L0, g0 = ref.evaluate(th, prec)
+        L1, g1 = mer.evaluate(th, prec)
+        scale = np.abs(g0).max()
+        print("precision", prec, "loss", L0, L1, "grad rel diff", np.abs(g1 - g0).max() / scale)
+        assert abs(L1 - L0) <= tol * L0
+        assert np.abs(g1 - g0).max() <= tol * scale
+    mer.close()
+    ref.close()

@@ -318,6 +318,50 @@ def test_lj_mixed_neighbour_identity_at_every_rebuild(tmp_path, oracle):
         assert np.array_equal(row, orow) and np.array_equal(ent, oent), r
 
 
+@pytest.mark.parametrize("cut", [2.8, 6.0])
+def test_counting_sort_binning_against_oracle(oracle, cut):
+    """the counting-sort binning (13,500 atoms, beyond the one-launch
+    path) against the oracle's BinSort restatement: counts, offsets and the
+    permutation equal, sorted positions equal; bins of 6 sigma hold ~180
+    atoms, so the ordering pass takes its insertion-sort branch"""
+    import ctypes as C
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print(f"case: counting-sort binning vs oracle, FCC 15^3, bins {cut}", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    nc = 15
+    box = nc * a
+    x = (lattice("fcc", nc, a) + np.random.default_rng(61).normal(0, 0.1, (4 * nc ** 3, 3))) % box
+    pl = Pipeline()
+    pl.load(x, (box,) * 3)
+    L, c, s = pl.L, pl.ctx.h, C.byref(pl.s)
+    pl.chk(L.emd_binning(c, s, cut, cut, cut, 1, 1, 0, 1, C.byref(pl.b)), "bin")
+    nb, nr = pl.b.nbins, pl.b.n_range
+    g = {"count": _dev(pl.ctx, pl.b.bincount, nb, np.int32),
+         "offset": _dev(pl.ctx, pl.b.binoffsets, nb, np.int32),
+         "permute": _dev(pl.ctx, pl.b.permute, nr, np.int32),
+         "x": _dev(pl.ctx, pl.s.x, 3 * nr, np.float64)}
+    maxc = pl.b.max_count
+    pl.close()
+    Lo = oracle.lib()
+    so, bo = oracle.System(), oracle.Bins()
+    Lo.or_system_init(C.byref(so))
+    Lo.or_system_grow(C.byref(so), len(x))
+    so.N_local = so.N = len(x)
+    for d in range(3):
+        so.domain[d] = so.hi[d] = box
+    np.ctypeslib.as_array(so.x, shape=(3 * len(x),))[:] = x.ravel()
+    Lo.or_binning(C.byref(so), C.byref(bo), cut, cut, cut, 1, 1, 0, 1)
+    assert bo.nbins_total == nb
+    o = {"count": oracle.arr(bo.count, nb, np.int32), "offset": oracle.arr(bo.offset, nb, np.int32),
+         "permute": oracle.arr(bo.permute, nr, np.int32), "x": oracle.arr(so.x, 3 * nr)}
+    Lo.or_bins_free(C.byref(bo))
+    Lo.or_system_free(C.byref(so))
+    print("largest bin", maxc)
+    assert maxc == o["count"].max() and (cut < 6 or maxc > 64)
+    for k in g:
+        assert np.array_equal(g[k], o[k]), k
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_fused_step_boundary_integration_bitwise(tmp_path, precision):
     """final_integrate + the next initial_integrate in one launch gives the
@@ -411,13 +455,13 @@ def test_lj_force_store_equals_zero_then_accumulate(oracle, layout, lanes):
 
 
 @pytest.mark.parametrize("nc", [6, 9])
-def test_binning_one_launch_equals_radix_sort(monkeypatch, nc):
+def test_binning_one_launch_equals_counting_sort(monkeypatch, nc):
     """binning of a small range in one workgroup launch gives the counts,
-    offsets, permutation and largest count of the radix-sort path, for the
-    sorting local binning and the local + ghost binning after the halo"""
+    offsets, permutation and largest count of the counting-sort path, for
+    the sorting local binning and the local + ghost binning after the halo"""
     import ctypes as C
     from examinimd_amd.pipeline import Pipeline, lattice
-    print(f"case: binning one launch vs radix sort, FCC {nc}^3", flush=True)
+    print(f"case: binning one launch vs counting sort, FCC {nc}^3", flush=True)
     a = (4 / 0.8442) ** (1 / 3)
     x = lattice("fcc", nc, a) + np.random.default_rng(47).normal(0, 0.1, (4 * nc ** 3, 3))
     out = []
