@@ -22,13 +22,15 @@
 //                 neighbour: Cayley-Klein recursion layer by layer (previous
 //                 layer in the wavefront's LDS buffer, one lane per
 //                 half-plane entry), sums sfac*w*U; the wavefronts' sums
-//                 are added in order and U_tot (285 complex) written
+//                 are added in order and U_tot (285 complex at twojmax 8)
+//                 written (k_snap_ui_grp: four neighbours per wavefront,
+//                 twojmax 8)
 //   k_snap_yflat  one block per atom: U_tot staged in LDS, Y (155 half-plane
-//                 entries) from a host-built flat stream of weighted
-//                 U_a U_b products
-//   k_snap_fij    one wavefront per (atom, neighbour): U and dU recursion
-//                 (LDS layer buffers), contraction with Y_J (staged in LDS
-//                 per block), wavefront reduction, atomic scatter to f_i / f_j
+//                 entries at twojmax 8) from a host-built flat stream of
+//                 weighted U_a U_b products
+//   k_snap_fij    one wavefront per (atom, neighbour) group: U and dU
+//                 recursion (LDS layer buffers), contraction with Y_J (staged
+//                 in LDS per block), reduction, atomic scatter to f_i / f_j
 //   k_snap_energy one block per atom: B_k from U_tot and Z, energy
 #include <algorithm>
 #include <cmath>
@@ -43,21 +45,52 @@
 
 namespace {
 
-constexpr int SNAP_MAXJ = 8;                 // twojmax <= 8
-constexpr int SNAP_JD = SNAP_MAXJ + 1;
-constexpr int SNAP_NU = 285;                 // sum_j (j+1)^2 for twojmax 8
-constexpr int SNAP_NY = 155;                 // half-plane entries for twojmax 8
+// Two size classes: twojmax <= 8 (the kernels tuned for SNAP-2k, whose
+// twojmax is 8) and twojmax <= 12 (the same kernels with larger LDS tables,
+// one neighbour per wavefront in the force kernel).  Per class J:
+constexpr int SNAP_MAXJ = 8;                 // the first class
+constexpr int SNAP_JBIG = 12;                // the second class: the largest twojmax
+constexpr int SNAP_RPD = SNAP_JBIG + 2;      // row length of the rootpq table
 constexpr double SNAP_PI = 3.14159265358979323846;
+
+// U entries sum_j (j+1)^2, half-plane entries sum_j (j+1)(j/2+1), (a,b,J)
+// triples and bispectrum components (diagonalstyle 3) up to twojmax J
+constexpr int snap_nu(int J) { return J < 0 ? 0 : snap_nu(J - 1) + (J + 1) * (J + 1); }
+constexpr int snap_ny(int J) { return J < 0 ? 0 : snap_ny(J - 1) + (J + 1) * (J / 2 + 1); }
+constexpr int snap_hp(int j) { return (j + 1) * (j / 2 + 1); }  // half plane of layer j
+constexpr int snap_counts(int J, bool coeff) {
+  int n = 0;
+  for (int j1 = 0; j1 <= J; j1++)
+    for (int j2 = 0; j2 <= j1; j2++)
+      for (int j = j1 - j2; j <= (J < j1 + j2 ? J : j1 + j2); j += 2)
+        if (!coeff || j >= j1) n++;
+  return n;
+}
+template <int J>
+struct snap_dim {
+  static constexpr int JD = J + 1;
+  static constexpr int NU = snap_nu(J);
+  static constexpr int NY = snap_ny(J);
+  static constexpr int HPW = (snap_hp(J) + 63) / 64 * 64;  // lane-table width of a layer
+  static constexpr int HPPREV = snap_hp(J - 1);             // half plane of layer J-1
+  static constexpr int FULLPREV = J * J;                    // full layer J-1
+};
+constexpr int SNAP_MAXT = snap_counts(SNAP_JBIG, false);  // (a,b,J) triples: 343
+constexpr int SNAP_MAXC = snap_counts(SNAP_JBIG, true);   // components: 140
+constexpr int SNAP_NU_BIG = snap_nu(SNAP_JBIG);
+static_assert(snap_nu(8) == 285 && snap_ny(8) == 155 && snap_counts(8, false) == 125 &&
+                  snap_counts(8, true) == 55,
+              "SNAP table sizes at twojmax 8");
 
 // device tables (filled by emd_snap_create)
 struct snap_dev_tables {
   int twojmax;
   int nu, ny;                      // number of U entries / half-plane entries
-  int ustart[SNAP_JD + 1];         // full layer offsets: ustart[j] + ma*(j+1) + mb
-  int ystart[SNAP_JD + 1];         // half-plane offsets: ystart[J] + mb*(J+1) + ma
-  int yjstart[SNAP_JD + 1];        // (a,b) list for each J in ytab
+  int ustart[SNAP_JBIG + 2];       // full layer offsets: ustart[j] + ma*(j+1) + mb
+  int ystart[SNAP_JBIG + 2];       // half-plane offsets: ystart[J] + mb*(J+1) + ma
+  int yjstart[SNAP_JBIG + 2];      // (a,b) list for each J in ytab
   int ny_terms;
-  double rootpq[(SNAP_JD + 1) * (SNAP_JD + 1)];
+  double rootpq[SNAP_RPD * SNAP_RPD];  // sqrt(p/q) at p * SNAP_RPD + q
   double rfac0, rmin0, rcut, wj, wself;
   int switchflag;
 };
@@ -75,8 +108,6 @@ struct etab_entry {
 struct stab_entry {
   int a, b, cgoff, tid;
 };
-
-constexpr int SNAP_MAXT = 128;               // (a,b,J) triples for twojmax 8: 125
 
 __device__ __forceinline__ int u_idx(const snap_dev_tables &t, int j, int ma, int mb) {
   return t.ustart[j] + ma * (j + 1) + mb;
@@ -179,24 +210,26 @@ __device__ __forceinline__ bool lane_entry(int l, int j, int &ma, int &mb) {
 // the rootpq table; the per-neighbour layer loop then does no integer
 // division and reads the recursion factors rootpq(j-ma, j-mb) and
 // rootpq(ma, j-mb) from LDS
-template <class T>
+template <class T, int J>
 struct layer_tab {
-  int mm[SNAP_JD][64];          // ma | mb << 8 | (entry exists) << 16
-  T rp[SNAP_JD * SNAP_JD];      // rootpq(p, q) at p * SNAP_JD + q
-  __device__ T rpa(int j, int ma, int mb) const { return rp[(j - ma) * SNAP_JD + (j - mb)]; }
-  __device__ T rpb(int j, int ma, int mb) const { return rp[ma * SNAP_JD + (j - mb)]; }
+  static constexpr int JD = snap_dim<J>::JD, W = snap_dim<J>::HPW;
+  int mm[JD][W];                // ma | mb << 8 | (entry exists) << 16
+  T rp[JD * JD];                // rootpq(p, q) at p * JD + q
+  __device__ T rpa(int j, int ma, int mb) const { return rp[(j - ma) * JD + (j - mb)]; }
+  __device__ T rpb(int j, int ma, int mb) const { return rp[ma * JD + (j - mb)]; }
 };
 
-template <class T>
-__device__ __forceinline__ void layer_tab_fill(const snap_dev_tables &t, layer_tab<T> &L) {
-  for (int k = threadIdx.x; k < SNAP_JD * 64; k += blockDim.x) {
-    const int j = k / 64, e = k % 64;
+template <class T, int J>
+__device__ __forceinline__ void layer_tab_fill(const snap_dev_tables &t, layer_tab<T, J> &L) {
+  constexpr int JD = layer_tab<T, J>::JD, W = layer_tab<T, J>::W;
+  for (int k = threadIdx.x; k < JD * W; k += blockDim.x) {
+    const int j = k / W, e = k % W;
     const int mb = e / (j + 1), ma = e - mb * (j + 1);
-    const bool own = j <= t.twojmax && e < (j + 1) * (j / 2 + 1);
+    const bool own = j <= t.twojmax && e < snap_hp(j);
     L.mm[j][e] = ma | (mb << 8) | (own ? 1 << 16 : 0);
   }
-  for (int k = threadIdx.x; k < SNAP_JD * SNAP_JD; k += blockDim.x)
-    L.rp[k] = (T)t.rootpq[(k / SNAP_JD) * (SNAP_JD + 1) + (k % SNAP_JD)];
+  for (int k = threadIdx.x; k < JD * JD; k += blockDim.x)
+    L.rp[k] = (T)t.rootpq[(k / JD) * SNAP_RPD + (k % JD)];
 }
 
 // one entry of layer j from the full layer j-1 (row-major ma*j+mb), with the
@@ -239,13 +272,13 @@ __device__ __forceinline__ cplx<T> u_step(const snap_dev_tables &t, const T *pr,
   cplx<T> u{0, 0};
   const int jp = j;  // previous layer has j rows/cols: index ma*j + mb
   if (ma < j) {
-    const T rp = (T)t.rootpq[(j - ma) * (SNAP_JD + 1) + (j - mb)];
+    const T rp = (T)t.rootpq[(j - ma) * SNAP_RPD + (j - mb)];
     const T ur = pr[ma * jp + mb], ui = pi[ma * jp + mb];
     u.r += rp * (a_r * ur + a_i * ui);
     u.i += rp * (a_r * ui - a_i * ur);
   }
   if (ma > 0) {
-    const T rp = (T)t.rootpq[ma * (SNAP_JD + 1) + (j - mb)];
+    const T rp = (T)t.rootpq[ma * SNAP_RPD + (j - mb)];
     const T ur = pr[(ma - 1) * jp + mb], ui = pi[(ma - 1) * jp + mb];
     u.r += -rp * (b_r * ur + b_i * ui);
     u.i += -rp * (b_r * ui - b_i * ur);
@@ -262,18 +295,19 @@ __device__ __forceinline__ void wave_sync() {
 
 // U_tot of every atom (SNA::compute_ui, sna_impl.hpp:152-190): one block of
 // UI_WAVES wavefronts per atom, wavefront w takes neighbours w, w+UI_WAVES, ...
-// (lanes over the half-plane entries of a layer, the previous layer in the
-// wavefront's own LDS buffer), then the partial sums of the wavefronts are
-// added in wavefront order (deterministic)
+// (lanes over the half-plane entries of a layer, 64 at a time, the previous
+// layer in the wavefront's own LDS buffer), then the partial sums of the
+// wavefronts are added in wavefront order (deterministic)
 constexpr int UI_WAVES = 4;
 
-template <class T>
+template <class T, int J>
 __global__ void __launch_bounds__(64 * UI_WAVES)
 k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restrict__ nr,
           int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
-  __shared__ T s_r[UI_WAVES][2][SNAP_JD * SNAP_JD], s_i[UI_WAVES][2][SNAP_JD * SNAP_JD];
-  __shared__ T s_pr[UI_WAVES][SNAP_NY], s_pi[UI_WAVES][SNAP_NY];
-  __shared__ layer_tab<T> LT;
+  constexpr int JD = snap_dim<J>::JD, W = snap_dim<J>::HPW;
+  __shared__ T s_r[UI_WAVES][2][JD * JD], s_i[UI_WAVES][2][JD * JD];
+  __shared__ T s_pr[UI_WAVES][snap_dim<J>::NY], s_pi[UI_WAVES][snap_dim<J>::NY];
+  __shared__ layer_tab<T, J> LT;
   const int i = blockIdx.x;
   if (i >= n) return;
   layer_tab_fill(t, LT);
@@ -281,13 +315,13 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int twojmax = t.twojmax;
   // this wavefront's sums over its neighbours, half-plane index
-  // ystart[j] + l (only lane l ever touches entry l of a layer)
+  // ystart[j] + e (only the lane of entry e ever touches it)
   T *acc_r = s_pr[w], *acc_i = s_pi[w];
   for (int k = l; k < t.ny; k += 64) acc_r[k] = acc_i[k] = 0;
   wave_sync();
   const int cnt = nin[i];
-  T (*br)[SNAP_JD * SNAP_JD] = s_r[w];
-  T (*bi)[SNAP_JD * SNAP_JD] = s_i[w];
+  T (*br)[JD * JD] = s_r[w];
+  T (*bi)[JD * JD] = s_i[w];
   for (int jj = w; jj < cnt; jj += UI_WAVES) {
     const size_t o = (size_t)i * maxn + jj;
     const T x = (T)nr[3 * o], y = (T)nr[3 * o + 1], z = (T)nr[3 * o + 2];
@@ -308,23 +342,24 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
     wave_sync();
     for (int j = 1; j <= twojmax; j++) {
       const int p = (j - 1) & 1, c = j & 1;
-      const int info = LT.mm[j][l];
-      const bool own = info >> 16;
-      const int ma = info & 255, mb = (info >> 8) & 255;
-      cplx<T> u{0, 0};
-      if (own) {
-        u = u_step_tab<T>(br[p], bi[p], j, ma, mb, LT.rpa(j, ma, mb), LT.rpb(j, ma, mb), a_r,
-                          a_i, b_r, b_i);
-        // full layer j: own entry and, off the middle column, its mirror
-        br[c][ma * (j + 1) + mb] = u.r;
-        bi[c][ma * (j + 1) + mb] = u.i;
-        if (2 * mb != j) {
-          const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
-          br[c][(j - ma) * (j + 1) + (j - mb)] = sg * u.r;
-          bi[c][(j - ma) * (j + 1) + (j - mb)] = -sg * u.i;
+      for (int e = l; e < W; e += 64) {
+        const int info = LT.mm[j][e];
+        const bool own = info >> 16;
+        const int ma = info & 255, mb = (info >> 8) & 255;
+        if (own) {
+          const cplx<T> u = u_step_tab<T>(br[p], bi[p], j, ma, mb, LT.rpa(j, ma, mb),
+                                          LT.rpb(j, ma, mb), a_r, a_i, b_r, b_i);
+          // full layer j: own entry and, off the middle column, its mirror
+          br[c][ma * (j + 1) + mb] = u.r;
+          bi[c][ma * (j + 1) + mb] = u.i;
+          if (2 * mb != j) {
+            const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
+            br[c][(j - ma) * (j + 1) + (j - mb)] = sg * u.r;
+            bi[c][(j - ma) * (j + 1) + (j - mb)] = -sg * u.i;
+          }
+          acc_r[t.ystart[j] + e] += sfac * u.r;
+          acc_i[t.ystart[j] + e] += sfac * u.i;
         }
-        acc_r[t.ystart[j] + l] += sfac * u.r;
-        acc_i[t.ystart[j] + l] += sfac * u.i;
       }
       wave_sync();
     }
@@ -334,22 +369,24 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
   // U_tot = wself * I + sum over wavefronts; own entries and mirrors
   T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
   for (int j = 0; j <= twojmax; j++) {
-    const int info = LT.mm[j][l];
-    const int ma = info & 255, mb = (info >> 8) & 255;
-    if (info >> 16) {
-      T vr = 0, vi = 0;
+    for (int e = l; e < W; e += 64) {
+      const int info = LT.mm[j][e];
+      const int ma = info & 255, mb = (info >> 8) & 255;
+      if (info >> 16) {
+        T vr = 0, vi = 0;
 #pragma unroll
-      for (int q = 0; q < UI_WAVES; q++) {
-        vr += s_pr[q][t.ystart[j] + l];
-        vi += s_pi[q][t.ystart[j] + l];
-      }
-      if (ma == mb) vr += (T)t.wself;
-      ur[t.ustart[j] + ma * (j + 1) + mb] = vr;
-      ui[t.ustart[j] + ma * (j + 1) + mb] = vi;
-      if (2 * mb != j) {
-        const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
-        ur[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = sg * vr;
-        ui[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = -sg * vi;
+        for (int q = 0; q < UI_WAVES; q++) {
+          vr += s_pr[q][t.ystart[j] + e];
+          vi += s_pi[q][t.ystart[j] + e];
+        }
+        if (ma == mb) vr += (T)t.wself;
+        ur[t.ustart[j] + ma * (j + 1) + mb] = vr;
+        ui[t.ustart[j] + ma * (j + 1) + mb] = vi;
+        if (2 * mb != j) {
+          const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
+          ur[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = sg * vr;
+          ui[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = -sg * vi;
+        }
       }
     }
   }
@@ -380,8 +417,8 @@ k_snap_ui_grp(snap_dev_tables t, const int *__restrict__ nin, const double *__re
               int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
   constexpr int TJ = SNAP_MAXJ;
   __shared__ T s_b[UG_WAVES * UG_GROUPS][2][2][UG_HP];  // [group][buffer][r/i][entry]
-  __shared__ T s_acc[UG_WAVES][2][SNAP_NY];
-  __shared__ layer_tab<T> LT;
+  __shared__ T s_acc[UG_WAVES][2][snap_dim<TJ>::NY];
+  __shared__ layer_tab<T, TJ> LT;
   const int i = blockIdx.x;
   if (i >= n) return;
   layer_tab_fill(t, LT);
@@ -562,14 +599,16 @@ struct alignas(2 * sizeof(T)) cpair {
   T r, i;
 };
 
-// code: byte offsets of the two U_tot entries in LDS, 13 bits each (the
+// code: byte offsets of the two U_tot entries in LDS, 16 bits each (the
 // host writes offsets for sizeof(cpair<T>)); meta[c][t]: the LDS slot of the
 // run that ends with thread t's chunk c, or YF_NO_RUN.  A thread's runs are
 // the entries its chunk range touches, so there are at most ny + threads - 1.
 constexpr unsigned short YF_NO_RUN = 0xFFFFu;
-constexpr int YF_MAXRUNS = SNAP_NY + YF_THREADS;
+template <int J>
+constexpr int yf_maxruns() { return snap_dim<J>::NY + YF_THREADS; }
+static_assert(snap_nu(SNAP_JBIG) * 16 < 65536, "U_tot byte offsets fit 16 bits");
 
-template <class T>
+template <class T, int J>
 __global__ void __launch_bounds__(YF_THREADS)
 k_snap_yflat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
              const T *__restrict__ w, const unsigned short *__restrict__ meta,
@@ -578,9 +617,10 @@ k_snap_yflat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
   // U_tot at byte 0 (a static array: its LDS address is known when the
   // kernel is compiled, so a code's offset is the whole LDS address), then
   // the run sums
-  __shared__ cpair<T> s_yf[SNAP_NU + YF_MAXRUNS];
+  constexpr int NU = snap_dim<J>::NU;
+  __shared__ cpair<T> s_yf[NU + yf_maxruns<J>()];
   cpair<T> *su = s_yf;
-  cpair<T> *part = s_yf + SNAP_NU;
+  cpair<T> *part = s_yf + NU;
   const unsigned char *sb = reinterpret_cast<const unsigned char *>(s_yf);
   const int i = blockIdx.x;
   if (i >= n) return;
@@ -601,8 +641,8 @@ k_snap_yflat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
     }
 #pragma unroll
     for (int k = 0; k < YF_CHUNK; k++) {
-      const cpair<T> u1 = *reinterpret_cast<const cpair<T> *>(sb + (cc[k] & 0x1FFFu));
-      const cpair<T> u2 = *reinterpret_cast<const cpair<T> *>(sb + (cc[k] >> 13));
+      const cpair<T> u1 = *reinterpret_cast<const cpair<T> *>(sb + (cc[k] & 0xFFFFu));
+      const cpair<T> u2 = *reinterpret_cast<const cpair<T> *>(sb + (cc[k] >> 16));
       const T pr = u1.r * u2.r - u1.i * u2.i;
       const T pi = u1.r * u2.i + u1.i * u2.r;
       ar += ww[k] * pr;
@@ -648,18 +688,23 @@ __global__ void k_yf_weights(int nslots, const unsigned short *__restrict__ key,
 // LG = 16 runs four neighbours per wavefront instead: a layer of h entries
 // takes ceil(h/16) steps for four neighbours (14 steps over layers 1-8)
 // instead of ceil(h/32) for two (10 steps), at twice the LDS per wavefront.
+// The size class J = 12 runs one neighbour per wavefront (LG = 64): its
+// layers below the top have up to 72 half-plane entries, two steps of 64.
 constexpr int FIJ_WAVES = 2;                      // wavefronts per block
-constexpr int FIJ_HP = 32;                        // half-plane entries of layer 7
 template <int LG>
 constexpr int fij_nb() { return FIJ_WAVES * (64 / LG); }  // neighbours per block
 
-template <int LG, class T, class TF>
+template <int LG, class T, class TF, int J>
 __global__ void __launch_bounds__(64 * FIJ_WAVES)
 k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
            const double *__restrict__ nr, int maxn, int n, const T *__restrict__ y_r,
            const T *__restrict__ y_i, TF *__restrict__ f) {
   constexpr int FIJ_NB = fij_nb<LG>();
   constexpr int NBW = 64 / LG;  // neighbours per wavefront
+  constexpr int FIJ_HP = snap_dim<J>::HPPREV;  // half-plane entries of layer J-1
+  // a layer below the top takes at most two steps (kept in registers below)
+  static_assert((FIJ_HP + LG - 1) / LG <= 2, "two steps per layer below the top");
+  static_assert(((snap_hp(J) + LG - 1) / LG) * LG <= snap_dim<J>::HPW, "lane table width");
   // previous layer (j-1 <= 7) of each neighbour: U and dU (3 components) of
   // its half plane (mb <= (j-1)/2, at mb*j + ma: at most 32 entries); the
   // middle column of an even layer reads its mirror image
@@ -669,8 +714,8 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   // layer (a layer below the top has at most 32 entries, two steps of 16
   // lanes), then written over it.
   __shared__ T s_u[FIJ_NB][8][FIJ_HP];
-  __shared__ T s_y[2][SNAP_NY];
-  __shared__ layer_tab<T> LT;
+  __shared__ T s_y[2][snap_dim<J>::NY];
+  __shared__ layer_tab<T, J> LT;
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int hs = l / LG, hl = l % LG;  // neighbour slot of the wavefront, lane in it
   const int i = blockIdx.x;
@@ -734,7 +779,7 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
     fz += c * u3[2];
   }
   wave_sync();
-  constexpr int FIJ_STEPS = (45 + LG - 1) / LG;  // steps of the top layer (45 entries)
+  constexpr int FIJ_STEPS = (snap_hp(J) + LG - 1) / LG;  // steps of the top layer
   for (int j = 1; j <= t.twojmax; j++) {
     const int ne = (j + 1) * (j / 2 + 1);
     T keep[2][8];  // this layer's own entries of steps 0 and 1
@@ -850,7 +895,7 @@ k_snap_energy(snap_dev_tables t, const double *__restrict__ cg,
               const int *__restrict__ nin, const double *__restrict__ nr, int maxn, int n,
               const double *__restrict__ utot_r, const double *__restrict__ utot_i,
               double *__restrict__ eatom) {
-  __shared__ double su_r[SNAP_NU], su_i[SNAP_NU];
+  __shared__ double su_r[SNAP_NU_BIG], su_i[SNAP_NU_BIG];
   __shared__ double s_red[4];
   const int i = blockIdx.x;
   if (i >= n) return;
@@ -1039,7 +1084,9 @@ static emd_status snap_build_yflat(emd_ctx *ctx, emd_snap *sn) {
     for (const auto &r : eruns[e]) meta[(size_t)r.second * YF_THREADS + r.first] = (unsigned short)nruns++;
   }
   rstart[t.ny] = nruns;
-  if (nruns > YF_MAXRUNS || t.nu > SNAP_NU)
+  const bool small = t.twojmax <= SNAP_MAXJ;
+  if (nruns > (small ? yf_maxruns<SNAP_MAXJ>() : yf_maxruns<SNAP_JBIG>()) ||
+      t.nu > (small ? snap_nu(SNAP_MAXJ) : snap_nu(SNAP_JBIG)))
     return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: Y stream too large");
   const int steps = P * YF_CHUNK;
   const size_t nslots = (size_t)steps * YF_THREADS;
@@ -1081,8 +1128,8 @@ static emd_status snap_build_yflat(emd_ctx *ctx, emd_snap *sn) {
           pool[l].erase(pool[l].begin() + best);
           f1[tm.i1 % 16]++; f2[tm.i2 % 16]++; d1[h][tm.i1 % 8]++; d2[h][tm.i2 % 8]++;
           const size_t slot = ((size_t)c * YF_CHUNK + k) * YF_THREADS + th0 + l;
-          code32[slot] = (uint32_t)(tm.i1 * 8) | ((uint32_t)(tm.i2 * 8) << 13);
-          code64[slot] = (uint32_t)(tm.i1 * 16) | ((uint32_t)(tm.i2 * 16) << 13);
+          code32[slot] = (uint32_t)(tm.i1 * 8) | ((uint32_t)(tm.i2 * 8) << 16);
+          code64[slot] = (uint32_t)(tm.i1 * 16) | ((uint32_t)(tm.i2 * 16) << 16);
           key[slot] = (unsigned short)tm.key;
           cgw[slot] = tm.cgw;
         }
@@ -1189,8 +1236,8 @@ static emd_status snap_upload_coeff(emd_ctx *ctx, emd_snap *sn) {
 extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, emd_snap **out) {
   if (!ctx || !p || !out) return EMD_ERR_ARG;
   *out = nullptr;
-  if (p->twojmax < 0 || p->twojmax > SNAP_MAXJ)
-    return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: twojmax must be in 0..%d", SNAP_MAXJ);
+  if (p->twojmax < 0 || p->twojmax > SNAP_JBIG)
+    return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: twojmax must be in 0..%d", SNAP_JBIG);
   if (!p->coeff) return emd_set_error(ctx, EMD_ERR_ARG, "emd_snap_create: no coefficients");
   emd_snap *sn = new emd_snap();
   snap_dev_tables &t = sn->t;
@@ -1209,7 +1256,7 @@ extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, em
   t.nu = nu;
   t.ny = ny;
   for (int a = 1; a <= twojmax; a++)
-    for (int b = 1; b <= twojmax; b++) t.rootpq[a * (SNAP_JD + 1) + b] = std::sqrt((double)a / b);
+    for (int b = 1; b <= twojmax; b++) t.rootpq[a * SNAP_RPD + b] = std::sqrt((double)a / b);
   t.rfac0 = p->rfac0;
   t.rmin0 = p->rmin0;
   t.rcut = (p->radelem + p->radelem) * p->rcutfac;
@@ -1352,9 +1399,12 @@ static emd_status launch_ui(emd_ctx *ctx, emd_snap *sn, int n, T *ur, T *ui) {
     else
       EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui_grp<T, 4>), n, 64 * 4, 0, sn->t, sn->nin, sn->nr,
                  sn->cap_maxn, n, ur, ui);
+  } else if (sn->t.twojmax <= SNAP_MAXJ) {
+    EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui<T, SNAP_MAXJ>), n, 64 * UI_WAVES, 0, sn->t, sn->nin,
+               sn->nr, sn->cap_maxn, n, ur, ui);
   } else {
-    EMD_LAUNCH(ctx, "snap_ui", k_snap_ui<T>, n, 64 * UI_WAVES, 0, sn->t, sn->nin, sn->nr,
-               sn->cap_maxn, n, ur, ui);
+    EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui<T, SNAP_JBIG>), n, 64 * UI_WAVES, 0, sn->t, sn->nin,
+               sn->nr, sn->cap_maxn, n, ur, ui);
   }
   return EMD_OK;
 }
@@ -1365,20 +1415,31 @@ static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   EMD_TRY(launch_ui<T>(ctx, sn, n, ur, ui));
   const T *w = std::is_same<T, float>::value ? (const T *)sn->yf_w32 : (const T *)sn->yf_w64;
   const uint32_t *code = std::is_same<T, float>::value ? sn->yf_code32 : sn->yf_code64;
-  EMD_LAUNCH(ctx, "snap_y", k_snap_yflat<T>, n, YF_THREADS, 0, sn->t.nu, sn->t.ny, sn->yf_nch,
-             code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
+  if (sn->t.twojmax <= SNAP_MAXJ)
+    EMD_LAUNCH(ctx, "snap_y", (k_snap_yflat<T, SNAP_MAXJ>), n, YF_THREADS, 0, sn->t.nu, sn->t.ny,
+               sn->yf_nch, code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
+  else
+    EMD_LAUNCH(ctx, "snap_y", (k_snap_yflat<T, SNAP_JBIG>), n, YF_THREADS, 0, sn->t.nu, sn->t.ny,
+               sn->yf_nch, code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
   return EMD_OK;
 }
 
 // F_ij for every (atom, neighbour): 16 lanes per neighbour, four
-// neighbours per wavefront
+// neighbours per wavefront (twojmax <= 8); one neighbour per wavefront above
 template <class T, class TF>
 static emd_status launch_fij(emd_ctx *ctx, emd_snap *sn, int n, const T *yr, const T *yi,
                              TF *f) {
-  constexpr int nb = fij_nb<16>();
-  const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
-  EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<16, T, TF>), grid, 64 * FIJ_WAVES, 0, sn->t, sn->nin,
-             sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
+  if (sn->t.twojmax <= SNAP_MAXJ) {
+    constexpr int nb = fij_nb<16>();
+    const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
+    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<16, T, TF, SNAP_MAXJ>), grid, 64 * FIJ_WAVES, 0,
+               sn->t, sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
+  } else {
+    constexpr int nb = fij_nb<64>();
+    const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
+    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<64, T, TF, SNAP_JBIG>), grid, 64 * FIJ_WAVES, 0,
+               sn->t, sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
+  }
   return EMD_OK;
 }
 
@@ -1450,8 +1511,9 @@ extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const e
 // three-role combination of Z_{abJ}; so per triple (a,b,J) only
 // S_{abJ} = sum_i sum' Re(conj(W^i_J) Z^i_{abJ}) is needed.
 
-// one U/dU layer step for the lane's entry (same recursion as k_snap_fij)
-template <class T>
+// one U/dU layer step for the lane's entry (same recursion as k_snap_fij);
+// P holds the previous layer in full, component q at P[q * FULL + ma * j + mb]
+template <class T, int FULL>
 __device__ __forceinline__ void du_step(const snap_dev_tables &t, const T *P, int j, int ma, int mb,
                                         T a_r, T a_i, T b_r, T b_i, const T *da_r, const T *da_i,
                                         const T *db_r, const T *db_i, T &ur, T &ui, T *dr, T *di) {
@@ -1459,47 +1521,52 @@ __device__ __forceinline__ void du_step(const snap_dev_tables &t, const T *P, in
 #pragma unroll
   for (int k = 0; k < 3; k++) dr[k] = di[k] = 0;
   if (ma < j) {
-    const T rp = (T)t.rootpq[(j - ma) * (SNAP_JD + 1) + (j - mb)];
+    const T rp = (T)t.rootpq[(j - ma) * SNAP_RPD + (j - mb)];
     const int s = ma * j + mb;
-    const T pr = P[s], pi = P[64 + s];
+    const T pr = P[s], pi = P[FULL + s];
     ur += rp * (a_r * pr + a_i * pi);
     ui += rp * (a_r * pi - a_i * pr);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const T dpr = P[(2 + 2 * k) * 64 + s], dpi = P[(3 + 2 * k) * 64 + s];
+      const T dpr = P[(2 + 2 * k) * FULL + s], dpi = P[(3 + 2 * k) * FULL + s];
       dr[k] += rp * (da_r[k] * pr + da_i[k] * pi + a_r * dpr + a_i * dpi);
       di[k] += rp * (da_r[k] * pi - da_i[k] * pr + a_r * dpi - a_i * dpr);
     }
   }
   if (ma > 0) {
-    const T rp = (T)t.rootpq[ma * (SNAP_JD + 1) + (j - mb)];
+    const T rp = (T)t.rootpq[ma * SNAP_RPD + (j - mb)];
     const int s = (ma - 1) * j + mb;
-    const T pr = P[s], pi = P[64 + s];
+    const T pr = P[s], pi = P[FULL + s];
     ur += -rp * (b_r * pr + b_i * pi);
     ui += -rp * (b_r * pi - b_i * pr);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const T dpr = P[(2 + 2 * k) * 64 + s], dpi = P[(3 + 2 * k) * 64 + s];
+      const T dpr = P[(2 + 2 * k) * FULL + s], dpi = P[(3 + 2 * k) * FULL + s];
       dr[k] += -rp * (db_r[k] * pr + db_i[k] * pi + b_r * dpr + b_i * dpi);
       di[k] += -rp * (db_r[k] * pi - db_i[k] * pr + b_r * dpi - b_i * dpr);
     }
   }
 }
 
-// W^i for every atom: one wavefront per atom, neighbours in list order,
-// FP64 accumulators (T is the arithmetic of the recursion)
-template <class T>
+// W^i for every atom: one wavefront per atom, neighbours in list order, the
+// half-plane entries of a layer 64 at a time (NST steps), FP64 accumulators
+// (T is the arithmetic of the recursion)
+template <class T, int J>
 __global__ void __launch_bounds__(64)
 k_snap_w(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
          const double *__restrict__ nr, int maxn, int n, const int *__restrict__ root,
          const double *__restrict__ g, double *__restrict__ w_r, double *__restrict__ w_i) {
-  __shared__ T s_u[2][8][64];
+  constexpr int JD = snap_dim<J>::JD, FULL = snap_dim<J>::FULLPREV;
+  constexpr int NST = (snap_hp(J) + 63) / 64;
+  __shared__ T s_u[2][8][FULL];
   const int i = blockIdx.x;
   if (i >= n) return;
   const int l = threadIdx.x;
-  double acc_r[SNAP_JD], acc_i[SNAP_JD];
+  double acc_r[JD][NST], acc_i[JD][NST];
 #pragma unroll
-  for (int j = 0; j < SNAP_JD; j++) acc_r[j] = acc_i[j] = 0.0;
+  for (int j = 0; j < JD; j++)
+#pragma unroll
+    for (int st = 0; st < NST; st++) acc_r[j][st] = acc_i[j][st] = 0.0;
   const double gi0 = g[3 * (size_t)i], gi1 = g[3 * (size_t)i + 1], gi2 = g[3 * (size_t)i + 2];
   const int cnt = nin[i];
   for (int jj = 0; jj < cnt; jj++) {
@@ -1541,49 +1608,66 @@ k_snap_w(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__
       s_u[0][0][0] = 1;
       s_u[0][1][0] = 0;
       for (int k = 0; k < 3; k++) { s_u[0][2 + 2 * k][0] = 0; s_u[0][3 + 2 * k][0] = 0; }
-      acc_r[0] += 0.5 * (double)(dsfac * gu);
+      acc_r[0][0] += 0.5 * (double)(dsfac * gu);
     }
     __syncthreads();
     for (int j = 1; j <= t.twojmax; j++) {
       const int p = (j - 1) & 1, c = j & 1;
-      int ma, mb;
-      const bool own = l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb);
-      T ur = 0, ui = 0, dr[3] = {0, 0, 0}, di[3] = {0, 0, 0};
-      if (own)
-        du_step<T>(t, &s_u[p][0][0], j, ma, mb, a_r, a_i, b_r, b_i, da_r, da_i, db_r, db_i, ur,
-                   ui, dr, di);
-      __syncthreads();
-      if (own && j < t.twojmax) {
-        T *C = &s_u[c][0][0];
-        const int s = ma * (j + 1) + mb;
-        C[s] = ur; C[64 + s] = ui;
+      T ur[NST], ui[NST], dr[NST][3], di[NST][3];
+      int ma[NST], mb[NST];
+      bool own[NST];
 #pragma unroll
-        for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + s] = dr[k]; C[(3 + 2 * k) * 64 + s] = di[k]; }
-        if (2 * mb != j) {
-          const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
-          const int m = (j - ma) * (j + 1) + (j - mb);
-          C[m] = sg * ur; C[64 + m] = -sg * ui;
+      for (int st = 0; st < NST; st++) {
+        const int e = st * 64 + l;
+        own[st] = e < snap_hp(j) && lane_entry(e, j, ma[st], mb[st]);
+        ur[st] = ui[st] = 0;
 #pragma unroll
-          for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * 64 + m] = sg * dr[k]; C[(3 + 2 * k) * 64 + m] = -sg * di[k]; }
-        }
+        for (int k = 0; k < 3; k++) dr[st][k] = di[st][k] = 0;
+        if (own[st])
+          du_step<T, FULL>(t, &s_u[p][0][0], j, ma[st], mb[st], a_r, a_i, b_r, b_i, da_r, da_i,
+                           db_r, db_i, ur[st], ui[st], dr[st], di[st]);
       }
-      if (own) {
-        double wgt = 1.0;
-        if (2 * mb == j) wgt = (ma < mb) ? 1.0 : ((ma == mb) ? 0.5 : 0.0);
-        const T wr = dsfac * ur * gu + sfac * (gd[0] * dr[0] + gd[1] * dr[1] + gd[2] * dr[2]);
-        const T wi = dsfac * ui * gu + sfac * (gd[0] * di[0] + gd[1] * di[1] + gd[2] * di[2]);
-        acc_r[j] += wgt * (double)wr;
-        acc_i[j] += wgt * (double)wi;
+      __syncthreads();
+#pragma unroll
+      for (int st = 0; st < NST; st++) {
+        if (own[st] && j < t.twojmax) {
+          T *C = &s_u[c][0][0];
+          const int s = ma[st] * (j + 1) + mb[st];
+          C[s] = ur[st]; C[FULL + s] = ui[st];
+#pragma unroll
+          for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * FULL + s] = dr[st][k]; C[(3 + 2 * k) * FULL + s] = di[st][k]; }
+          if (2 * mb[st] != j) {
+            const T sg = ((ma[st] + mb[st]) & 1) ? (T)-1 : (T)1;
+            const int m = (j - ma[st]) * (j + 1) + (j - mb[st]);
+            C[m] = sg * ur[st]; C[FULL + m] = -sg * ui[st];
+#pragma unroll
+            for (int k = 0; k < 3; k++) { C[(2 + 2 * k) * FULL + m] = sg * dr[st][k]; C[(3 + 2 * k) * FULL + m] = -sg * di[st][k]; }
+          }
+        }
+        if (own[st]) {
+          double wgt = 1.0;
+          if (2 * mb[st] == j) wgt = (ma[st] < mb[st]) ? 1.0 : ((ma[st] == mb[st]) ? 0.5 : 0.0);
+          const T wr = dsfac * ur[st] * gu +
+                       sfac * (gd[0] * dr[st][0] + gd[1] * dr[st][1] + gd[2] * dr[st][2]);
+          const T wi = dsfac * ui[st] * gu +
+                       sfac * (gd[0] * di[st][0] + gd[1] * di[st][1] + gd[2] * di[st][2]);
+          acc_r[j][st] += wgt * (double)wr;
+          acc_i[j][st] += wgt * (double)wi;
+        }
       }
       __syncthreads();
     }
   }
   for (int j = 0; j <= t.twojmax; j++) {
-    int ma, mb;
-    if (l < (j + 1) * (j + 1) && lane_entry(l, j, ma, mb)) {
-      const int e = t.ystart[j] + mb * (j + 1) + ma;
-      w_r[(size_t)i * t.ny + e] = acc_r[j];
-      w_i[(size_t)i * t.ny + e] = acc_i[j];
+#pragma unroll
+    for (int st = 0; st < NST; st++) {
+      int ma, mb;
+      const int e = st * 64 + l;
+      if (e < snap_hp(j) && lane_entry(e, j, ma, mb)) {
+        const int ey = t.ystart[j] + mb * (j + 1) + ma;
+        w_r[(size_t)i * t.ny + ey] = acc_r[j][st];
+        w_i[(size_t)i * t.ny + ey] = acc_i[j][st];
+      }
     }
   }
 }
@@ -1595,7 +1679,7 @@ k_snap_s(snap_dev_tables t, const double *__restrict__ cg, const stab_entry *__r
          const int *__restrict__ sjstart, int ntrip, int n, const T *__restrict__ utot_r,
          const T *__restrict__ utot_i, const double *__restrict__ w_r,
          const double *__restrict__ w_i, double *__restrict__ S) {
-  __shared__ T su_r[SNAP_NU], su_i[SNAP_NU];
+  __shared__ T su_r[SNAP_NU_BIG], su_i[SNAP_NU_BIG];
   __shared__ double sS[SNAP_MAXT];
   const int i = blockIdx.x;
   if (i >= n) return;
@@ -1605,8 +1689,7 @@ k_snap_s(snap_dev_tables t, const double *__restrict__ cg, const stab_entry *__r
   }
   for (int k = threadIdx.x; k < ntrip; k += blockDim.x) sS[k] = 0.0;
   __syncthreads();
-  const int e = threadIdx.x;
-  if (e < t.ny) {
+  for (int e = threadIdx.x; e < t.ny; e += blockDim.x) {
     int J, ma, mb;
     y_entry(t, e, J, ma, mb);
     const double wr = w_r[(size_t)i * t.ny + e], wi = w_i[(size_t)i * t.ny + e];
@@ -1628,8 +1711,8 @@ k_snap_bsum(snap_dev_tables t, const double *__restrict__ cg, const etab_entry *
             int ne, int bzero, int n, const double *__restrict__ utot_r,
             const double *__restrict__ utot_i, double *__restrict__ bsum,
             double *__restrict__ brow) {
-  __shared__ double su_r[SNAP_NU], su_i[SNAP_NU];
-  __shared__ double sB[SNAP_MAXT];
+  __shared__ double su_r[SNAP_NU_BIG], su_i[SNAP_NU_BIG];
+  __shared__ double sB[SNAP_MAXC];
   const int i = blockIdx.x;
   if (i >= n) return;
   for (int k = threadIdx.x; k < t.nu; k += blockDim.x) {
@@ -1672,11 +1755,15 @@ static emd_status snap_adjoint_impl(emd_ctx *ctx, const emd_system *s, const emd
                                     emd_snap *sn, const double *g, double *S) {
   const int n = s->n_local;
   EMD_TRY(launch_ui<T>(ctx, sn, n, (T *)sn->u_r, (T *)sn->u_i));
-  EMD_LAUNCH(ctx, "snap_adjoint_w", k_snap_w<T>, n, 64, 0, sn->t, sn->nin, sn->nj, sn->nr,
-             sn->cap_maxn, n, h ? h->root : nullptr, g, sn->w_r, sn->w_i);
+  if (sn->t.twojmax <= SNAP_MAXJ)
+    EMD_LAUNCH(ctx, "snap_adjoint_w", (k_snap_w<T, SNAP_MAXJ>), n, 64, 0, sn->t, sn->nin, sn->nj,
+               sn->nr, sn->cap_maxn, n, h ? h->root : nullptr, g, sn->w_r, sn->w_i);
+  else
+    EMD_LAUNCH(ctx, "snap_adjoint_w", (k_snap_w<T, SNAP_JBIG>), n, 64, 0, sn->t, sn->nin, sn->nj,
+               sn->nr, sn->cap_maxn, n, h ? h->root : nullptr, g, sn->w_r, sn->w_i);
   const int ntrip = (int)sn->tid.size();
   EMD_LAUNCH(ctx, "snap_adjoint_s", k_snap_s<T>, n, 192, 0, sn->t, sn->cg, sn->stab,
-             (const int *)(sn->sbuf + SNAP_MAXT + 64), ntrip, n, (const T *)sn->u_r,
+             (const int *)(sn->sbuf + SNAP_MAXT + SNAP_MAXC), ntrip, n, (const T *)sn->u_r,
              (const T *)sn->u_i, sn->w_r, sn->w_i, S);
   return EMD_OK;
 }
@@ -1693,12 +1780,13 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
   EMD_TRY(snap_neigh(ctx, s, nl, sn));
   EMD_TRY(emd_ensure(ctx, (void **)&sn->w_r, &sn->cap_w, (size_t)n * sn->t.ny * sizeof(double)));
   EMD_TRY(emd_ensure(ctx, (void **)&sn->w_i, &sn->cap_wi, (size_t)n * sn->t.ny * sizeof(double)));
-  // sbuf: S[SNAP_MAXT] | Bsum[64] | sjstart ints
-  EMD_TRY(emd_ensure(ctx, (void **)&sn->sbuf, &sn->cap_s, (SNAP_MAXT + 64 + 64) * sizeof(double)));
+  // sbuf: S[SNAP_MAXT] | Bsum[SNAP_MAXC] | sjstart ints (twojmax + 2 of them)
+  constexpr int SB_D = SNAP_MAXT + SNAP_MAXC;
+  EMD_TRY(emd_ensure(ctx, (void **)&sn->sbuf, &sn->cap_s, (SB_D + SNAP_JBIG + 2) * sizeof(double)));
   double *S = sn->sbuf, *B = sn->sbuf + SNAP_MAXT;
-  EMD_HIP(ctx, hipMemcpyAsync(sn->sbuf + SNAP_MAXT + 64, sn->sjstart.data(),
+  EMD_HIP(ctx, hipMemcpyAsync(sn->sbuf + SB_D, sn->sjstart.data(),
                               sn->sjstart.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-  EMD_LAUNCH(ctx, "snap_adjoint_s", k_zero_d, 1, 256, 0, sn->sbuf, SNAP_MAXT + 64);
+  EMD_LAUNCH(ctx, "snap_adjoint_s", k_zero_d, emd_blocks(SB_D, 256), 256, 0, sn->sbuf, SB_D);
   if (precision == EMD_MIXED) EMD_TRY(snap_adjoint_impl<float>(ctx, s, h, sn, dLdF, S));
   else EMD_TRY(snap_adjoint_impl<double>(ctx, s, h, sn, dLdF, S));
   // dE/dbeta_k = sum_i B_k^i in FP64
@@ -1708,7 +1796,7 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
                sn->ncoeff, sn->bzeroflag, n, (const double *)sn->u_r, (const double *)sn->u_i, B,
                nullptr);
   }
-  std::vector<double> hS(SNAP_MAXT + 64);
+  std::vector<double> hS(SB_D);
   EMD_HIP(ctx, hipMemcpyAsync(hS.data(), sn->sbuf, hS.size() * sizeof(double),
                               hipMemcpyDeviceToHost, ctx->stream));
   EMD_TRY(emd_wait(ctx));

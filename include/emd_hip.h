@@ -282,11 +282,11 @@ emd_status emd_reduce_mv2(emd_ctx *ctx, const emd_system *s, double *mv2);
 
 /* --------------------------------------------------------------- SNAP */
 typedef struct {
-  int twojmax;
+  int twojmax;           /* 0..12 (the reference's sna_impl.hpp:25-53 has no cap) */
   double rfac0, rmin0, rcutfac;
   int switchflag, bzeroflag;
   double radelem, wjelem;
-  int ncoeff;            /* bispectrum components (55 for twojmax 8) */
+  int ncoeff;            /* bispectrum components (55 for twojmax 8, 140 for 12) */
   const double *coeff;   /* HOST pointer: beta_0 .. beta_ncoeff */
 } emd_snap_params;
 

@@ -166,6 +166,7 @@ void or_nve_final(or_system *s);
 int    or_snap_ncoeff(int twojmax);
 int    or_snap_idxj(int twojmax, int *j1, int *j2, int *j);  /* returns count */
 double or_snap_cg(int twojmax, int j1, int j2, int j, int m1, int m2);
+void   or_snap_cg_all(int twojmax, double *out);  /* [JD]^5, JD = twojmax + 1 */
 void   or_snap_force(or_system *s, const or_csr *c, const or_snap_params *p);
 double or_snap_energy(const or_system *s, const or_csr *c, const or_snap_params *p,
                       double *eatom);

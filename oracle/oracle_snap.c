@@ -454,6 +454,20 @@ int or_snap_idxj(int twojmax, int *o1, int *o2, int *o3) {
   return n;
 }
 
+/* the whole table, out[(((j1 * JD + j2) * JD + j) * JD + m1) * JD + m2],
+ * JD = twojmax + 1 (entries the table does not define are 0) */
+void or_snap_cg_all(int twojmax, double *out) {
+  or_snap_params p;
+  memset(&p, 0, sizeof(p));
+  p.twojmax = twojmax;
+  p.rfac0 = 0.99363;
+  sna_t t;
+  sna_init(&t, &p);
+  const int JD = t.jdim;
+  memcpy(out, t.cg, sizeof(double) * JD * JD * JD * JD * JD);
+  sna_free(&t);
+}
+
 double or_snap_cg(int twojmax, int j1, int j2, int j, int m1, int m2) {
   or_snap_params p;
   memset(&p, 0, sizeof(p));

@@ -120,6 +120,7 @@ def lib():
             "or_snap_ncoeff": (C.c_int, [C.c_int]),
             "or_snap_idxj": (C.c_int, [C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
             "or_snap_cg": (C.c_double, [C.c_int] * 6),
+            "or_snap_cg_all": (None, [C.c_int, P(C.c_double)]),
             "or_snap_force": (None, [P(System), P(CSR), P(SnapParams)]),
             "or_snap_energy": (C.c_double, [P(System), P(CSR), P(SnapParams), P(C.c_double)]),
             "or_snap_bispectrum": (None, [P(SnapParams), C.c_int, P(C.c_double),

@@ -560,6 +560,79 @@ def test_snap_ncoeff(oracle):
     L = oracle.lib()
     assert L.or_snap_ncoeff(8) == 55       # W.snapcoeff: 56 = 55 + beta_0
     assert L.or_snap_ncoeff(6) == 30       # Ta06A.snapcoeff: 31
+    # brute count of (j1 >= j2, j >= j1, |j1 - j2| <= j <= min(twojmax, j1 + j2), step 2)
+    for tj in (10, 12):
+        n = sum(1 for j1 in range(tj + 1) for j2 in range(j1 + 1)
+                for j in range(j1 - j2, min(tj, j1 + j2) + 1, 2) if j >= j1)
+        assert L.or_snap_ncoeff(tj) == n
+    assert (L.or_snap_ncoeff(10), L.or_snap_ncoeff(12)) == (91, 140)
+
+
+def _cg_table(L, tj):
+    """every <j1/2 m1; j2/2 m2 | j/2 m> of 2j1, 2j2, 2j <= tj from the oracle"""
+    jd = tj + 1
+    a = np.zeros(jd ** 5)
+    L.or_snap_cg_all(tj, oracle_dptr(a))
+    a = a.reshape((jd,) * 5)
+    t = {}
+    for j1 in range(tj + 1):
+        for j2 in range(tj + 1):
+            for j in range(abs(j1 - j2), min(tj, j1 + j2) + 1, 2):
+                for k1 in range(j1 + 1):
+                    for k2 in range(j2 + 1):
+                        m2 = 2 * k1 - j1 + 2 * k2 - j2
+                        if -j <= m2 <= j:
+                            t[(j1, j2, j, k1, k2)] = float(a[j1, j2, j, k1, k2])
+    # the accessor agrees with the one-coefficient call
+    for k in list(t)[::997]:
+        assert L.or_snap_cg(tj, *k) == t[k]
+    return t
+
+
+def oracle_dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def test_cg_twojmax12_orthogonal_and_against_sympy(oracle):
+    """the Clebsch-Gordan table at 2j <= 12 (beyond the 2j <= 8 golden
+    fixture): both orthogonality relations hold for every (j1, j2) pair, the
+    Condon-Shortley sign <j1 j1; j2 (J - j1) | J J> > 0 holds, and a seeded
+    sample of 300 coefficients with a quantum number above 8 equals sympy's
+    CG (an independent library routine)"""
+    L = oracle.lib()
+    tj = 12
+    t = _cg_table(L, tj)
+    for j1 in range(tj + 1):
+        for j2 in range(tj + 1):
+            js = list(range(abs(j1 - j2), j1 + j2 + 1, 2))
+            if js[-1] > tj:
+                continue                       # a truncated J range is not complete
+            # rows (m1, m2) x columns (J, M): a real orthogonal matrix
+            rows = [(k1, k2) for k1 in range(j1 + 1) for k2 in range(j2 + 1)]
+            cols = [(j, m) for j in js for m in range(j + 1)]
+            M = np.zeros((len(rows), len(cols)))
+            ci = {c: q for q, c in enumerate(cols)}
+            for r, (k1, k2) in enumerate(rows):
+                m2 = 2 * k1 - j1 + 2 * k2 - j2
+                for j in js:
+                    if -j <= m2 <= j:
+                        M[r, ci[(j, (m2 + j) // 2)]] = t[(j1, j2, j, k1, k2)]
+            assert M.shape[0] == M.shape[1]
+            assert np.abs(M @ M.T - np.eye(len(rows))).max() < 1e-12, (j1, j2)
+            for j in js:                       # Condon-Shortley phase
+                k2 = (j - j1 + j2) // 2
+                if 0 <= k2 <= j2:
+                    assert t[(j1, j2, j, j1, k2)] > 0, (j1, j2, j)
+    sympy = pytest.importorskip("sympy")
+    from sympy.physics.quantum.cg import CG
+    keys = [k for k in t if max(k[:3]) > 8]
+    rng = np.random.default_rng(12)
+    for q in rng.choice(len(keys), 300, replace=False):
+        j1, j2, j, k1, k2 = keys[q]
+        aa2, bb2 = 2 * k1 - j1, 2 * k2 - j2
+        R = sympy.Rational
+        c = float(CG(R(j1, 2), R(aa2, 2), R(j2, 2), R(bb2, 2), R(j, 2), R(aa2 + bb2, 2)).doit())
+        assert abs(t[keys[q]] - c) < 1e-12, keys[q]
 
 
 def _shell(rng, n=14, rmin=2.3, rmax=4.5):
@@ -585,6 +658,49 @@ def test_snap_u_unitary(oracle):
     for j in range(9):
         Uj = U[j, :j + 1, :j + 1] - np.eye(j + 1)
         assert np.allclose(Uj @ Uj.conj().T, np.eye(j + 1), atol=1e-12)
+
+
+def test_snap_twojmax12_u_unitary_b_invariant_db_fd(oracle):
+    """twojmax 12: every layer of a single neighbour's U is unitary, the
+    bispectrum (140 components) is invariant under a rotation of the shell,
+    and dB/dr_j equals a central difference of B"""
+    L = oracle.lib()
+    tj, nb = 12, 140
+    sp = oracle.snap_params([0.0] * (nb + 1), twojmax=tj, switchflag=0)
+    rij = np.array([[1.2, -0.7, 2.1]])
+    jd = tj + 1
+    ur, ui = np.zeros(jd ** 3), np.zeros(jd ** 3)
+    L.or_snap_ui(C.byref(sp), 1, oracle.dptr(rij), oracle.dptr(ur), oracle.dptr(ui))
+    U = (ur + 1j * ui).reshape(jd, jd, jd)
+    for j in range(jd):
+        Uj = U[j, :j + 1, :j + 1] - np.eye(j + 1)
+        assert np.allclose(Uj @ Uj.conj().T, np.eye(j + 1), atol=1e-12), j
+    sp = oracle.snap_params([0.0] * (nb + 1), twojmax=tj)
+    rng = np.random.default_rng(6)
+    rij = _shell(rng)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] *= -1
+    B1, B2 = np.zeros(nb), np.zeros(nb)
+    L.or_snap_bispectrum(C.byref(sp), len(rij), oracle.dptr(rij), oracle.dptr(B1))
+    r2 = np.ascontiguousarray(rij @ q.T)
+    L.or_snap_bispectrum(C.byref(sp), len(rij), oracle.dptr(r2), oracle.dptr(B2))
+    assert np.abs(B1 - B2).max() < 1e-10 * max(1.0, np.abs(B1).max())
+    rij = _shell(rng, n=10)
+    dB = np.zeros(nb * 3)
+    jj = 4
+    L.or_snap_dbidrj(C.byref(sp), len(rij), oracle.dptr(rij), jj, oracle.dptr(dB))
+    dB = dB.reshape(nb, 3)
+    h = 1e-6
+    for d in range(3):
+        rp, rm = rij.copy(), rij.copy()
+        rp[jj, d] += h
+        rm[jj, d] -= h
+        Bp, Bm = np.zeros(nb), np.zeros(nb)
+        L.or_snap_bispectrum(C.byref(sp), len(rij), oracle.dptr(rp), oracle.dptr(Bp))
+        L.or_snap_bispectrum(C.byref(sp), len(rij), oracle.dptr(rm), oracle.dptr(Bm))
+        fd = (Bp - Bm) / (2 * h)
+        assert np.abs(fd - dB[:, d]).max() < 1e-6 * max(1.0, np.abs(dB).max())
 
 
 def test_snap_bispectrum_rotation_invariant(oracle):
