@@ -49,7 +49,8 @@ class EmdNeigh(C.Structure):
 
 class EmdLJParams(C.Structure):
     _fields_ = [("ntypes", C.c_int), ("lj1", (C.c_double * 16) * 16),
-                ("lj2", (C.c_double * 16) * 16), ("cutsq", (C.c_double * 16) * 16)]
+                ("lj2", (C.c_double * 16) * 16), ("cutsq", (C.c_double * 16) * 16),
+                ("table", C.POINTER(C.c_double))]
 
 
 class EmdLJIDialParams(C.Structure):

@@ -120,6 +120,7 @@ extern "C" emd_status emd_destroy(emd_ctx *ctx) {
   if (ctx->d_tmp) hipFree(ctx->d_tmp);
   if (ctx->d_tmp2) hipFree(ctx->d_tmp2);
   if (ctx->d_ljtab) hipFree(ctx->d_ljtab);
+  if (ctx->d_ljbig) hipFree(ctx->d_ljbig);
   if (ctx->h_ljtab) hipHostFree(ctx->h_ljtab);
   if (ctx->ljtab_ev) hipEventDestroy(ctx->ljtab_ev);
   if (ctx->wait_ev) hipEventDestroy(ctx->wait_ev);
@@ -371,8 +372,8 @@ extern "C" emd_status emd_system_grow(emd_ctx *ctx, emd_system *s, int n) {
   EMD_TRY(grow_one(ctx, &s->id, old, n, 1));
   EMD_TRY(grow_one(ctx, &s->q, old, n, 1));
   if (!s->mass) {
-    EMD_HIP(ctx, hipMalloc(&s->mass, 16 * sizeof(double)));
-    EMD_HIP(ctx, hipMemsetAsync(s->mass, 0, 16 * sizeof(double), ctx->stream));
+    EMD_HIP(ctx, hipMalloc(&s->mass, EMD_MAX_TYPES * sizeof(double)));
+    EMD_HIP(ctx, hipMemsetAsync(s->mass, 0, EMD_MAX_TYPES * sizeof(double), ctx->stream));
   }
   s->n_max = n;
   EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -498,8 +499,8 @@ extern "C" emd_status emd_ctx_shared(int device, emd_ctx **out) {
 
 extern "C" emd_status emd_system_set_mass(emd_ctx *ctx, emd_system *s, int ntypes,
                                           const double *mass) {
-  if (!ctx || !s || ntypes < 1 || ntypes > 16 || !mass) return EMD_ERR_ARG;
-  if (!s->mass) EMD_HIP(ctx, hipMalloc(&s->mass, 16 * sizeof(double)));
+  if (!ctx || !s || ntypes < 1 || ntypes > EMD_MAX_TYPES || !mass) return EMD_ERR_ARG;
+  if (!s->mass) EMD_HIP(ctx, hipMalloc(&s->mass, EMD_MAX_TYPES * sizeof(double)));
   EMD_HIP(ctx, hipMemcpyAsync(s->mass, mass, ntypes * sizeof(double), hipMemcpyHostToDevice,
                               ctx->stream));
   EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -50,6 +50,12 @@ struct emd_ctx {
   // last uploaded (compared before each use, uploaded only on change), and
   // the event that marks when the staging copy has been read
   void *d_ljtab = nullptr;
+  // LJ coefficients for more than 16 types (emd_lj_params.table): device
+  // copy (lj1, lj2, cutsq, rep of ntypes x ntypes each) and its host image
+  double *d_ljbig = nullptr;
+  size_t ljbig_cap = 0;
+  std::vector<double> h_ljbig;
+  bool ljbig_valid = false;
   void *h_ljtab = nullptr;
   hipEvent_t ljtab_ev = nullptr;
   bool ljtab_valid = false;

@@ -17,6 +17,16 @@ struct lj_table {
   double rep[LJ_MAXT * LJ_MAXT];  // lj/cut/idial repeat counts (1 for lj/cut)
 };
 
+// the per-type-pair coefficients as the kernels read them: four device
+// arrays, entry (ti, tj) at ti * stride + tj.  Either the context's fixed
+// 16 x 16 table or, for more types (emd_lj_params.table), its own
+// ntypes x ntypes table.
+struct lj_tv {
+  const double *lj1, *lj2, *cutsq, *rep;
+  int stride;
+};
+constexpr int LJ_MAX_TABLE_TYPES = EMD_MAX_TYPES;
+
 // lj/cut/idial's repeat loop (src/force_types/force_lj_idial_neigh_impl.h:
 // 149-155): fpair = 0, then `rep` times fpair += pair_force / rep.  The part
 // is loop-invariant (an optimising compiler hoists it out of the
@@ -79,7 +89,7 @@ __global__ void __launch_bounds__(LJ_THREADS)
 k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
           const int *__restrict__ row, const int *__restrict__ num,
           const int *__restrict__ entries, int pitch, int n, double lj1_0, double lj2_0,
-          double cutsq_0, double rep_0, const lj_table *__restrict__ tab, int store,
+          double cutsq_0, double rep_0, lj_tv tab, int store,
           double *__restrict__ f, float *__restrict__ f32, int prio) {
 #pragma clang fp contract(off)
   typedef typename lj_acc<MIXED>::T T;
@@ -99,9 +109,9 @@ k_lj_full(const double *__restrict__ x, const int *__restrict__ type,
     const double rsq = dx * dx + dy * dy + dz * dz;
     double cutsq = cutsq_0, lj1 = lj1_0, lj2 = lj2_0, nrep = rep_0;
     if (!ONE_TYPE) {
-      const int t = type_i * LJ_MAXT + type[j];
-      cutsq = tab->cutsq[t]; lj1 = tab->lj1[t]; lj2 = tab->lj2[t];
-      if (DIAL) nrep = tab->rep[t];
+      const int t = type_i * tab.stride + type[j];
+      cutsq = tab.cutsq[t]; lj1 = tab.lj1[t]; lj2 = tab.lj2[t];
+      if (DIAL) nrep = tab.rep[t];
     }
     if (ok && rsq < cutsq) {
       if (MIXED) {
@@ -209,7 +219,7 @@ template <bool MIXED, bool DIAL>
 __global__ void __launch_bounds__(LJ_THREADS)
 k_lj_half(const double *__restrict__ x, const int *__restrict__ type, const int *__restrict__ row,
           const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
-          const lj_table *__restrict__ tab, double *__restrict__ f, float *__restrict__ f32) {
+          lj_tv tab, double *__restrict__ f, float *__restrict__ f32) {
 #pragma clang fp contract(off)
   typedef typename lj_acc<MIXED>::T T;
   const int gid = blockIdx.x * LJ_THREADS + threadIdx.x;
@@ -252,20 +262,20 @@ k_lj_half(const double *__restrict__ x, const int *__restrict__ type, const int 
       const double dy = y_i - yj[u];
       const double dz = z_i - zj[u];
       const double rsq = dx * dx + dy * dy + dz * dz;
-      const int t = type_i * LJ_MAXT + type[j[u]];
-      if (ok[u] && rsq < tab->cutsq[t]) {
+      const int t = type_i * tab.stride + type[j[u]];
+      if (ok[u] && rsq < tab.cutsq[t]) {
         T vx, vy, vz;
         if (MIXED) {
-          float fpair = emd_lj_fpair_mixed(rsq, (float)tab->lj1[t], (float)tab->lj2[t]);
-          if (DIAL) fpair = dial_sum(fpair / (float)tab->rep[t], dial_trips(tab->rep[t]));
+          float fpair = emd_lj_fpair_mixed(rsq, (float)tab.lj1[t], (float)tab.lj2[t]);
+          if (DIAL) fpair = dial_sum(fpair / (float)tab.rep[t], dial_trips(tab.rep[t]));
           vx = (float)dx * fpair;
           vy = (float)dy * fpair;
           vz = (float)dz * fpair;
         } else {
           const double r2inv = 1.0 / rsq;
           const double r6inv = r2inv * r2inv * r2inv;
-          double fpair = (r6inv * (tab->lj1[t] * r6inv - tab->lj2[t])) * r2inv;
-          if (DIAL) fpair = dial_sum(fpair / tab->rep[t], dial_trips(tab->rep[t]));
+          double fpair = (r6inv * (tab.lj1[t] * r6inv - tab.lj2[t])) * r2inv;
+          if (DIAL) fpair = dial_sum(fpair / tab.rep[t], dial_trips(tab.rep[t]));
           vx = dx * fpair;
           vy = dy * fpair;
           vz = dz * fpair;
@@ -327,6 +337,59 @@ static emd_status upload_table(emd_ctx *ctx, const emd_lj_params *p,
   return EMD_OK;
 }
 
+// the view of a 16 x 16 table at device address t
+static lj_tv view_of(const lj_table *t) {
+  constexpr size_t N = (size_t)LJ_MAXT * LJ_MAXT;
+  const double *b = reinterpret_cast<const double *>(t);
+  return lj_tv{b, b + N, b + 2 * N, b + 3 * N, LJ_MAXT};
+}
+
+// the coefficients of p as a device view: the cached 16 x 16 table, or with
+// p->table (more types, the reference's View path, force_lj_neigh_impl.h:
+// 100-126 with ntypes above MAX_TYPES_STACKPARAMS) the context's own
+// ntypes x ntypes table, uploaded when its contents change (after the
+// stream has drained: kernels queued before still read the old one)
+static emd_status upload_tv(emd_ctx *ctx, const emd_lj_params *p, const double (*rep)[16],
+                            lj_tv *out) {
+  if (!p->table) {
+    lj_table *t = nullptr;
+    EMD_TRY(upload_table(ctx, p, rep, &t));
+    *out = view_of(t);
+    return EMD_OK;
+  }
+  const int nt = p->ntypes;
+  const size_t N = (size_t)nt * nt;
+  std::vector<double> h(4 * N, 1.0);  // lj1, lj2, cutsq, rep (1: lj/cut)
+  std::copy(p->table, p->table + 3 * N, h.begin());
+  if (!ctx->ljbig_valid || ctx->h_ljbig != h) {
+    EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->ljbig_cap < h.size()) {
+      if (ctx->d_ljbig) EMD_HIP(ctx, hipFree(ctx->d_ljbig));
+      ctx->d_ljbig = nullptr;
+      ctx->ljbig_cap = 0;
+      if (hipMalloc(&ctx->d_ljbig, h.size() * sizeof(double)) != hipSuccess)
+        return emd_set_error(ctx, EMD_ERR_ALLOC, "LJ type table: hipMalloc failed");
+      ctx->ljbig_cap = h.size();
+    }
+    EMD_HIP(ctx, hipMemcpy(ctx->d_ljbig, h.data(), h.size() * sizeof(double),
+                           hipMemcpyHostToDevice));
+    ctx->h_ljbig.swap(h);
+    ctx->ljbig_valid = true;
+  }
+  const double *d = ctx->d_ljbig;
+  *out = lj_tv{d, d + N, d + 2 * N, d + 3 * N, nt};
+  return EMD_OK;
+}
+
+// ntypes in range: up to 16 in the fixed arrays, more with a table
+static emd_status check_types(emd_ctx *ctx, const emd_system *s, const emd_lj_params *p) {
+  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAX_TABLE_TYPES, "ntypes out of range");
+  EMD_ARG(ctx, p->ntypes <= LJ_MAXT || p->table != nullptr,
+          "more than 16 atom types need emd_lj_params.table");
+  EMD_ARG(ctx, s->ntypes <= p->ntypes, "the system has more atom types than the parameters");
+  return EMD_OK;
+}
+
 // lanes per atom and gathers per round (measured on MI355X, LJ 4M atoms)
 static int default_lanes(int precision) { return precision == EMD_MIXED ? 16 : 16; }
 
@@ -341,11 +404,11 @@ extern "C" emd_status emd_lj_set_lanes(emd_ctx *ctx, int precision, int lanes) {
 template <int K, int U, bool DIAL>
 static emd_status launch_full_as(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                                  const emd_lj_params *p, double rep_0, int precision,
-                                 const lj_table *tab, int store) {
+                                 const lj_tv &tab, int store) {
   const int n = s->n_local;
   const int nb = emd_blocks((long)n * K, LJ_THREADS);
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
-  const bool one = p->ntypes == 1;
+  const bool one = p->ntypes == 1 && !p->table;
   const int prio = ctx->lj_prio >= 0 ? ctx->lj_prio : (precision == EMD_MIXED ? 2 : 0);
   const char *name_m = DIAL ? "lj_idial_mixed" : "lj_force_mixed";
   const char *name_d = DIAL ? "lj_idial_fp64" : "lj_force_fp64";
@@ -368,7 +431,7 @@ static emd_status launch_full_as(emd_ctx *ctx, emd_system *s, const emd_neigh *n
 template <bool DIAL>
 static emd_status launch_full_lanes(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                                     const emd_lj_params *p, double rep_0, int precision,
-                                    const lj_table *tab, int store, int K) {
+                                    const lj_tv &tab, int store, int K) {
   switch (K) {
     case 1: return launch_full_as<1, 4, DIAL>(ctx, s, nl, p, rep_0, precision, tab, store);
     case 2: return launch_full_as<2, 4, DIAL>(ctx, s, nl, p, rep_0, precision, tab, store);
@@ -383,7 +446,8 @@ static emd_status lj_idial_force(emd_ctx *ctx, emd_system *s, const emd_neigh *n
                                  int store) {
   if (!ctx || !s || !nl || !p) return EMD_ERR_ARG;
   const emd_lj_params *lj = &p->lj;
-  EMD_ARG(ctx, lj->ntypes >= 1 && lj->ntypes <= LJ_MAXT, "ntypes out of range");
+  EMD_ARG(ctx, lj->ntypes >= 1 && lj->ntypes <= LJ_MAXT && !lj->table,
+          "lj/cut/idial: 1 to 16 atom types, in the fixed arrays");
   EMD_ARG(ctx, s->ntypes <= lj->ntypes, "the system has more atom types than the parameters");
   EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
   EMD_ARG(ctx, !(store && half), "storing forces needs a full list");
@@ -391,8 +455,8 @@ static emd_status lj_idial_force(emd_ctx *ctx, emd_system *s, const emd_neigh *n
     for (int b = 0; b < lj->ntypes; b++)
       EMD_ARG(ctx, p->intensity[a][b] <= 1e6, "lj/cut/idial intensity above 1e6");
   if (s->n_local <= 0) return EMD_OK;
-  lj_table *tab = nullptr;
-  if (lj->ntypes > 1 || half) EMD_TRY(upload_table(ctx, lj, p->intensity, &tab));
+  lj_tv tab{};
+  if (lj->ntypes > 1 || half) EMD_TRY(upload_tv(ctx, lj, p->intensity, &tab));
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   if (half) {
     const int nb = emd_blocks((long)s->n_local * LJH_K, LJ_THREADS);
@@ -414,13 +478,12 @@ static emd_status lj_idial_force(emd_ctx *ctx, emd_system *s, const emd_neigh *n
 static emd_status lj_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl,
                            const emd_lj_params *p, int precision, int half, int store) {
   if (!ctx || !s || !nl || !p) return EMD_ERR_ARG;
-  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
-  EMD_ARG(ctx, s->ntypes <= p->ntypes, "the system has more atom types than the parameters");
+  EMD_TRY(check_types(ctx, s, p));
   EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
   EMD_ARG(ctx, !(store && half), "storing forces needs a full list");
   if (s->n_local <= 0) return EMD_OK;
-  lj_table *tab = nullptr;
-  if (p->ntypes > 1 || half) EMD_TRY(upload_table(ctx, p, nullptr, &tab));
+  lj_tv tab{};
+  if (p->ntypes > 1 || half || p->table) EMD_TRY(upload_tv(ctx, p, nullptr, &tab));
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   if (half) {
     const int nb = emd_blocks((long)s->n_local * LJH_K, LJ_THREADS);
@@ -484,7 +547,7 @@ __global__ void __launch_bounds__(LJ_THREADS)
 k_lj_energy(const double *__restrict__ x, const int *__restrict__ type,
             const int *__restrict__ row, const int *__restrict__ num,
             const int *__restrict__ entries, int pitch, int n, int half,
-            const lj_table *__restrict__ tab, double *__restrict__ part) {
+            lj_tv tab, double *__restrict__ part) {
 #pragma clang fp contract(off)
   const int gid = blockIdx.x * LJ_THREADS + threadIdx.x;
   const int i = gid / LJE_K, sub = gid % LJE_K;
@@ -496,8 +559,8 @@ k_lj_energy(const double *__restrict__ x, const int *__restrict__ type,
     const int cnt = num[i];
     double shift0 = 0.0;
     if (ONE_TYPE) {
-      const double r2invc = 1.0 / tab->cutsq[0];
-      shift0 = lj_pair_e(r2invc * r2invc * r2invc, tab->lj1[0], tab->lj2[0]);
+      const double r2invc = 1.0 / tab.cutsq[0];
+      shift0 = lj_pair_e(r2invc * r2invc * r2invc, tab.lj1[0], tab.lj2[0]);
     }
     for (int jj = sub; jj < cnt; jj += LJE_U * LJE_K) {
       int j[LJE_U];
@@ -520,8 +583,8 @@ k_lj_energy(const double *__restrict__ x, const int *__restrict__ type,
         const double dy = y_i - yj[u];
         const double dz = z_i - zj[u];
         const double rsq = dx * dx + dy * dy + dz * dz;
-        const int t = ONE_TYPE ? 0 : type_i * LJ_MAXT + type[j[u]];
-        const double cutsq = tab->cutsq[t], lj1 = tab->lj1[t], lj2 = tab->lj2[t];
+        const int t = ONE_TYPE ? 0 : type_i * tab.stride + type[j[u]];
+        const double cutsq = tab.cutsq[t], lj1 = tab.lj1[t], lj2 = tab.lj2[t];
         if (ok[u] && rsq < cutsq) {
           const double r2inv = 1.0 / rsq;
           const double r6inv = r2inv * r2inv * r2inv;
@@ -566,12 +629,11 @@ static emd_status finish_dd(emd_ctx *ctx, double *part, int nb, double *out) {
 extern "C" emd_status emd_lj_energy(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
                                     const emd_lj_params *p, int half, double *pe) {
   if (!ctx || !s || !nl || !p || !pe) return EMD_ERR_ARG;
-  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
-  EMD_ARG(ctx, s->ntypes <= p->ntypes, "the system has more atom types than the parameters");
+  EMD_TRY(check_types(ctx, s, p));
   const int n = s->n_local;
   if (n <= 0) { *pe = 0.0; return EMD_OK; }
-  lj_table *tab;
-  EMD_TRY(upload_table(ctx, p, nullptr, &tab));
+  lj_tv tab;
+  EMD_TRY(upload_tv(ctx, p, nullptr, &tab));
   const int nb = emd_blocks((long)n * LJE_K, LJ_THREADS);
   EMD_TRY(emd_ensure(ctx, &ctx->d_tmp2, &ctx->tmp2_cap, (size_t)nb * 2 * sizeof(double)));
   double *part = (double *)ctx->d_tmp2;
@@ -628,7 +690,8 @@ extern "C" emd_status emd_lj_energy_noalloc(emd_ctx *ctx, const emd_system *s,
                                             const emd_neigh *nl, const emd_lj_params *p,
                                             int half, double *pe) {
   if (!ctx || !s || !nl || !p || !pe) return EMD_ERR_ARG;
-  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
+  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT && !p->table,
+          "emd_lj_energy_noalloc: 1 to 16 atom types, in the fixed arrays");
   EMD_ARG(ctx, s->ntypes <= p->ntypes, "the system has more atom types than the parameters");
   const int n = s->n_local;
   if (n > NOALLOC_MAX_ATOMS)
@@ -641,7 +704,7 @@ extern "C" emd_status emd_lj_energy_noalloc(emd_ctx *ctx, const emd_system *s,
   lj_table h;
   fill_table(h, p, nullptr);
   EMD_HIP(ctx, hipMemcpyAsync(buf, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
-  const lj_table *tab = (const lj_table *)buf;
+  const lj_tv tab = view_of((const lj_table *)buf);
   double *out = buf + NOALLOC_OUT, *part = buf + NOALLOC_PART;
   const int nb = emd_blocks((long)n * LJE_K, LJ_THREADS);
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
@@ -665,12 +728,11 @@ extern "C" emd_status emd_lj_energy_atoms(emd_ctx *ctx, const emd_system *s,
                                           const emd_neigh *nl, const emd_lj_params *p, int half,
                                           double *eatom) {
   if (!ctx || !s || !nl || !p || !eatom) return EMD_ERR_ARG;
-  EMD_ARG(ctx, p->ntypes >= 1 && p->ntypes <= LJ_MAXT, "ntypes out of range");
-  EMD_ARG(ctx, s->ntypes <= p->ntypes, "the system has more atom types than the parameters");
+  EMD_TRY(check_types(ctx, s, p));
   const int n = s->n_local;
   if (n <= 0) return EMD_OK;
-  lj_table *tab;
-  EMD_TRY(upload_table(ctx, p, nullptr, &tab));
+  lj_tv tab;
+  EMD_TRY(upload_tv(ctx, p, nullptr, &tab));
   const int nb = emd_blocks((long)n * LJE_K, LJ_THREADS);
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   if (p->ntypes == 1)

@@ -15,7 +15,15 @@ ForceLJNeighHIP::ForceLJNeighHIP(char** args, System* system, bool half_neigh_)
   use_stackparams = (ntypes <= MAX_TYPES_STACKPARAMS);
   memset(&params, 0, sizeof(params));
   params.ntypes = ntypes;
-  if (ntypes > 16) { fprintf(stderr, "ForceLJNeighHIP supports at most 16 types\n"); exit(1); }
+  if (ntypes > EMD_MAX_TYPES) {
+    fprintf(stderr, "ForceLJNeighHIP: %d atom types, at most %d\n", ntypes, EMD_MAX_TYPES);
+    exit(1);
+  }
+  // beyond the fixed 16 x 16 arrays the coefficients go in a table of their own
+  if (ntypes > 16) {
+    table.assign(3 * (size_t)ntypes * ntypes, 0.0);
+    params.table = table.data();
+  }
 }
 
 // ForceLJNeigh::init_coeff, force_lj_neigh_impl.h:60-98.  With at most
@@ -34,10 +42,19 @@ void ForceLJNeighHIP::init_coeff(int nargs, char** args) {
         params.lj2[i][j] = lj2;
         params.cutsq[i][j] = cut * cut;
       }
-  } else {
+  } else if (t1 < 0 || t2 < 0 || t1 >= ntypes || t2 >= ntypes) {
+    fprintf(stderr, "pair_coeff: atom type out of range\n");
+    exit(1);
+  } else if (!params.table) {
     params.lj1[t1][t2] = params.lj1[t2][t1] = lj1;
     params.lj2[t1][t2] = params.lj2[t2][t1] = lj2;
     params.cutsq[t1][t2] = params.cutsq[t2][t1] = cut * cut;
+  } else {
+    const size_t N = (size_t)ntypes * ntypes;
+    const size_t a = (size_t)t1 * ntypes + t2, b = (size_t)t2 * ntypes + t1;
+    table[a] = table[b] = lj1;
+    table[N + a] = table[N + b] = lj2;
+    table[2 * N + a] = table[2 * N + b] = cut * cut;
   }
 }
 

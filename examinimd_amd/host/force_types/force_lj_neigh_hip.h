@@ -23,6 +23,7 @@ class ForceLJNeighHIP : public Force {
   int ntypes;
   bool use_stackparams;
   emd_lj_params params;
+  std::vector<double> table;  // more than 16 types: lj1, lj2, cutsq [ntypes][ntypes]
 
 public:
   ForceLJNeighHIP(char** args, System* system, bool half_neigh_);

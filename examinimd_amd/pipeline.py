@@ -44,8 +44,8 @@ class Pipeline:
         types = (np.zeros(n, np.int32) if types is None
                  else np.ascontiguousarray(types, dtype=np.int32))
         ntypes = int(types.max()) + 1 if n else 1
-        if types.shape != (n,) or (n and types.min() < 0) or ntypes > 16:
-            raise ValueError("types: one value in [0, 16) per atom")
+        if types.shape != (n,) or (n and types.min() < 0) or ntypes > 1024:
+            raise ValueError("types: one value in [0, 1024) per atom")
         q = np.zeros(n)
         s = self.s
         self.chk(self.L.emd_system_upload(self.ctx.h, C.byref(s), n, _d(x), _d(v), _i(types),

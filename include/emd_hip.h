@@ -75,6 +75,9 @@ emd_status emd_profile_get(emd_ctx *ctx, const char *name, double *ms, long *lau
 size_t     emd_profile_report(emd_ctx *ctx, char *buf, size_t len);
 
 /* ------------------------------------------------------------- system */
+/* atom types a system may hold (the device mass table's size) */
+#define EMD_MAX_TYPES 1024
+
 /* Device image of System (src/system.h:57-143). */
 typedef struct {
   double *x, *v, *f;
@@ -97,6 +100,7 @@ emd_status emd_system_upload(emd_ctx *ctx, emd_system *s, int n, const double *x
                              const double *q);
 emd_status emd_system_download(emd_ctx *ctx, const emd_system *s, int n, double *x,
                                double *v, double *f, int *type, int *id, double *q);
+/* per-type masses (host array of ntypes, 1 <= ntypes <= EMD_MAX_TYPES) */
 emd_status emd_system_set_mass(emd_ctx *ctx, emd_system *s, int ntypes, const double *mass);
 
 /* The reference's per-atom Views in device memory (src/types.h:88-100): x is
@@ -202,6 +206,14 @@ emd_status emd_neigh_free(emd_ctx *ctx, emd_neigh *nl);
 typedef struct {
   int ntypes;
   double lj1[16][16], lj2[16][16], cutsq[16][16];
+  /* NULL: the coefficients are the fixed arrays (ntypes <= 16; the
+   * reference's stack parameters, force_lj_neigh_impl.h:43-58).  Otherwise
+   * a HOST array of 3 * ntypes * ntypes doubles, lj1 then lj2 then cutsq,
+   * each row-major [ti][tj], for any ntypes up to EMD_MAX_TYPES (the
+   * reference's View path for more types than its stack limit,
+   * force_lj_neigh_impl.h:100-126); the fixed arrays are then ignored.
+   * lj/cut/idial and emd_lj_energy_noalloc take the fixed arrays only. */
+  const double *table;
 } emd_lj_params;
 
 /* deep_copy(system->f, 0.0), src/examinimd.cpp:232 (f, or f32 for mixed) */

@@ -709,3 +709,115 @@ def test_lj_two_types_against_brute_force(layout, half):
     assert rel_err(out[emd.EMD_FP64], F) <= 1e-12
     df = out[emd.EMD_MIXED] - out[emd.EMD_FP64]
     assert float(np.sqrt(np.mean(np.sum(df * df, axis=1)))) <= 1e-5
+
+
+def _many_type_coeffs(rng, nt):
+    eps = rng.uniform(0.6, 1.4, (nt, nt))
+    sig = rng.uniform(0.85, 1.1, (nt, nt))
+    cut = rng.uniform(2.0, 2.7, (nt, nt))
+    eps, sig, cut = (np.triu(m) + np.triu(m, 1).T for m in (eps, sig, cut))
+    return 48.0 * eps * sig ** 12, 24.0 * eps * sig ** 6, cut * cut
+
+
+def _pair_energy_typed(x, box, types, lj1, lj2, cutsq):
+    """the reference's shifted pair energy per ordered pair of a full list,
+    0.5 (r6 (0.5 lj1 r6 - lj2) / 6 - the same at rc) (force_lj_neigh_impl.h:
+    287-293), brute force over minimum images"""
+    d = x[:, None, :] - x[None, :, :]
+    d -= box * np.round(d / box)
+    rsq = np.sum(d * d, axis=2)
+    ti, tj = types[:, None], types[None, :]
+    c, a, b = cutsq[ti, tj], lj1[ti, tj], lj2[ti, tj]
+    m = (rsq < c) & ~np.eye(len(x), dtype=bool)
+    r6 = np.where(m, 1.0 / np.where(m, rsq, 1.0), 0.0) ** 3
+    rc6 = (1.0 / c) ** 3
+    e = 0.5 * (r6 * (0.5 * a * r6 - b) / 6.0 - rc6 * (0.5 * a * rc6 - b) / 6.0)
+    return float(np.sum(np.where(m, e, 0.0)))
+
+
+@pytest.mark.parametrize("layout,half", [(emd.EMD_NEIGH_CSR, 0), (emd.EMD_NEIGH_2D, 0),
+                                         (emd.EMD_NEIGH_CSR, 1)])
+def test_lj_twenty_types_table_against_brute_force(layout, half):
+    """20 atom types, more than the fixed 16 x 16 arrays hold: the
+    coefficients come through emd_lj_params.table (the reference's View path
+    for many types, force_lj_neigh_impl.h:100-126).  FP64 forces and (full
+    lists) the energy against brute-force pair sums, FP32 pair arithmetic
+    within the mixed bound; without the table 20 types are refused"""
+    from examinimd_amd.abi import EmdLJParams
+    from examinimd_amd.pipeline import Pipeline, lattice
+    print(f"case: LJ 20 types (table), layout {layout}, half {half}", flush=True)
+    a = (4 / 0.8442) ** (1 / 3)
+    nc, nt = 6, 20
+    rng = np.random.default_rng(44)
+    x = lattice("fcc", nc, a) + rng.normal(0, 0.05, (4 * nc ** 3, 3))
+    box = nc * a
+    x %= box
+    types = rng.integers(0, nt, len(x)).astype(np.int32)
+    lj1, lj2, cutsq = _many_type_coeffs(rng, nt)
+    F = pair_forces_typed(x, box, types, lj1, lj2, cutsq)
+    pl = Pipeline()
+    pl.load(x, (box,) * 3, types=types)
+    pl.build(2.8, half=half, layout=layout)
+    table = np.ascontiguousarray(np.concatenate([lj1.ravel(), lj2.ravel(), cutsq.ravel()]))
+    p = EmdLJParams()
+    p.ntypes = nt
+    L, c, s = pl.L, pl.ctx.h, C.byref(pl.s)
+    assert L.emd_lj_force(c, s, C.byref(pl.nl), C.byref(p), emd.EMD_FP64, half) == 1
+    p.table = table.ctypes.data_as(C.POINTER(C.c_double))
+    out = {}
+    for prec in (emd.EMD_FP64, emd.EMD_MIXED):
+        pl.chk(L.emd_force_zero(c, s, prec), "zero")
+        pl.chk(L.emd_lj_force(c, s, C.byref(pl.nl), C.byref(p), prec, half), "lj")
+        out[prec] = pl.forces(prec)
+    if not half:
+        e = C.c_double()
+        pl.chk(L.emd_lj_energy(c, s, C.byref(pl.nl), C.byref(p), 0, C.byref(e)), "energy")
+        E = _pair_energy_typed(x, box, types, lj1, lj2, cutsq)
+        print("energy", e.value, E)
+        assert abs(e.value - E) <= 1e-12 * abs(E)
+    pl.close()
+    print("force rel err", rel_err(out[emd.EMD_FP64], F))
+    assert rel_err(out[emd.EMD_FP64], F) <= 1e-12
+    df = out[emd.EMD_MIXED] - out[emd.EMD_FP64]
+    rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+    frms = float(np.sqrt(np.mean(np.sum(F * F, axis=1))))
+    print("mixed force RMS error", rms, "force RMS", frms)
+    assert rms <= 1e-6 * frms
+
+
+def test_lj_twenty_types_through_the_driver(tmp_path):
+    """a deck with create_box 20 and one pair_coeff line per type pair: the
+    driver's LJ module builds the coefficient table, and its set-up forces
+    equal a brute-force pair sum over its own positions and types"""
+    print("case: LJ 20 types, driver deck, set-up forces", flush=True)
+    nt, n = 20, 5
+    rng = np.random.default_rng(7)
+    eps = rng.uniform(0.6, 1.4, (nt, nt))
+    sig = rng.uniform(0.85, 1.1, (nt, nt))
+    cut = rng.uniform(2.0, 2.5, (nt, nt))
+    lines = [f"mass            {t + 1} 1.0" for t in range(nt)]
+    for i in range(nt):
+        for j in range(i, nt):
+            lines.append(f"pair_coeff      {i + 1} {j + 1} {eps[i, j]:.17g} {sig[i, j]:.17g} "
+                         f"{cut[i, j]:.17g}")
+    deck = "\n".join(["units           lj", "atom_style      atomic", "newton          off",
+                      "lattice         fcc 0.8442", f"region          box block 0 {n} 0 {n} 0 {n}",
+                      f"create_box      {nt} box", "create_atoms    1 box"] + lines[:nt] +
+                     ["velocity        all create 1.0 87287 loop geom",
+                      "pair_style      lj/cut 2.5"] + lines[nt:] +
+                     ["neighbor        0.3 bin", "neigh_modify    delay 0 every 20 check no",
+                      "fix             1 all nve", "timestep        0.005", "thermo          0",
+                      "run             0"]) + "\n"
+    path = decks.write(str(tmp_path / "in.lj20"), deck)
+    app = emd.App(["-il", path, "--neigh-type", "CSR", "--precision", "fp64"]).init()
+    g = app.arrays()
+    app.close()
+    up = lambda m: np.triu(m) + np.triu(m, 1).T
+    eps, sig, cut = up(eps), up(sig), up(cut)
+    box = n * (4 / 0.8442) ** (1 / 3)
+    t = g["type"]
+    assert t.min() >= 0 and t.max() < nt and len(np.unique(t)) > 16
+    F = pair_forces_typed(g["x"], box, t, 48.0 * eps * sig ** 12, 24.0 * eps * sig ** 6,
+                          cut * cut)
+    print("force rel err", rel_err(g["f"], F))
+    assert rel_err(g["f"], F) <= 1e-12
