@@ -126,6 +126,11 @@ def test_lj_half_list(tmp_path, oracle, newton):
     app.run(40)
     m.run(40)
     compare(app, m, 1e-12)
+    # PE/N with the reference's half-list accounting (a pair with a ghost j
+    # counts half, force_lj_neigh_impl.h:329-339, with newton on as well)
+    pe, pe_o = app.thermo()[1], m.thermo()[1]
+    print("PE/N", pe, pe_o)
+    assert abs(pe - pe_o) <= 1e-12 * abs(pe_o)
 
 
 def test_lj_uniform_velocity_reference_deck(tmp_path, oracle):
