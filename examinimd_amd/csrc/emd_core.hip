@@ -97,11 +97,6 @@ extern "C" emd_status emd_create(int device, emd_ctx **out) {
     const int v = atoi(e);
     ctx->lj_prio = (v >= 0 && v <= 2) ? v : -1;
   }
-  if (const char *e = getenv("EMD_SNAP_UI_GRP")) ctx->snap_ui_grp = atoi(e) != 0;
-  if (const char *e = getenv("EMD_SNAP_UG_WAVES")) {
-    const int w = atoi(e);
-    ctx->snap_ug_waves = (w == 2 || w == 4) ? w : 0;
-  }
   *out = ctx;
   return EMD_OK;
 }

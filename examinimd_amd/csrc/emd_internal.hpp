@@ -83,8 +83,6 @@ struct emd_ctx {
   int fold_one = 1;        // small halos: fold all six phases in one launch
   int halo_one = 1;        // small systems: the six exchange_halo phases in one launch
   int bin_one = 1;         // small ranges: binning in one launch
-  int snap_ui_grp = 1;     // SNAP U_tot with 16-lane neighbour groups (twojmax 8)
-  int snap_ug_waves = 0;   // wavefronts per block of that kernel (2, 4; 0: by precision)
   int lj_prio = -1;       // full-list LJ kernel: raise wave priority while a round's index
                           // loads and gathers issue (2), around the gathers only (1),
                           // never (0); -1: 2 for mixed, 0 for FP64 (as measured)

@@ -17,20 +17,22 @@
 //
 // Kernels (T = double, or float for the mixed path):
 //   k_snap_neigh  one wavefront per atom: compacts the in-cutoff neighbours
-//                 (FP64 distance test, list order kept) with ballots
-//   k_snap_ui     four wavefronts per atom, each over every fourth
-//                 neighbour: Cayley-Klein recursion layer by layer (previous
-//                 layer in the wavefront's LDS buffer, one lane per
-//                 half-plane entry), sums sfac*w*U; the wavefronts' sums
-//                 are added in order and U_tot (285 complex at twojmax 8)
-//                 written (k_snap_ui_grp: four neighbours per wavefront,
-//                 twojmax 8)
+//                 (FP64 distance test, list order kept) with ballots and
+//                 writes each neighbour's Cayley-Klein parameters, switching
+//                 function and derivative factors (snap_par) once
+//   k_snap_ui_rg  (twojmax <= 8) one wavefront per atom, seven neighbours at
+//                 a time, nine lanes per neighbour: the layer recursion in
+//                 registers (lane ma holds column entries (ma, mb)), sums
+//                 sfac*U per lane, then over the groups; U_tot (285 complex
+//                 at twojmax 8) written with its mirror entries
 //   k_snap_yflat  one block per atom: U_tot staged in LDS, Y (155 half-plane
 //                 entries at twojmax 8) from a host-built flat stream of
 //                 weighted U_a U_b products
-//   k_snap_fij    one wavefront per (atom, neighbour) group: U and dU
-//                 recursion (LDS layer buffers), contraction with Y_J (staged
-//                 in LDS per block), reduction, atomic scatter to f_i / f_j
+//   k_snap_fij_rg (twojmax <= 8) U and dU recursion in registers, seven
+//                 neighbours per wavefront, contraction with Y_J, sums over
+//                 the nine lanes, atomic scatter to f_i / f_j
+//   k_snap_ui, k_snap_fij  the same for the size class twojmax <= 12: one
+//                 neighbour per wavefront, layers in LDS
 //   k_snap_energy one block per atom: B_k from U_tot and Z, energy
 #include <algorithm>
 #include <cmath>
@@ -137,6 +139,62 @@ __device__ __forceinline__ void sfac_dsfac(const snap_dev_tables &t, T r, T &sfa
   }
 }
 
+// Per in-cutoff neighbour, computed once by the neighbour compaction: the
+// Cayley-Klein parameters a = (z0 - i z)/r0, b = (y - i x)/r0 of rij, the
+// switching function and its derivative (times wj), the unit vector and the
+// factors of da/dr and db/dr, which are rank one in the unit vector u:
+//   da_r = c1 u, da_i = c2 u - r0inv e_z, db_r = c3 u + r0inv e_y,
+//   db_i = c4 u - r0inv e_x
+// (SNA::compute_uarray / compute_duarray, sna_impl.hpp:640-700, 728-790)
+template <class T>
+struct alignas(16) snap_par {
+  T a_r, a_i, b_r, b_i;
+  T sfac, dsfac;
+  T u[3];
+  T c1, c2, c3, c4;
+  T r0inv;
+  T pad[2];
+};
+
+template <class T>
+__device__ __forceinline__ void snap_par_of(const snap_dev_tables &t, double dx, double dy,
+                                            double dz, snap_par<T> &p) {
+  const T x = (T)dx, y = (T)dy, z = (T)dz;
+  const T rsq = x * x + y * y + z * z;
+  const T r = sqrt(rsq);
+  const T rscale0 = (T)t.rfac0 * (T)SNAP_PI / ((T)t.rcut - (T)t.rmin0);
+  const T theta0 = (r - (T)t.rmin0) * rscale0;
+  const T cs = cos(theta0), sn = sin(theta0);
+  const T z0 = r * cs / sn;
+  const T dz0dr = z0 / r - (r * rscale0) * (rsq + z0 * z0) / rsq;
+  const T rinv = (T)1 / r;
+  const T r0inv = (T)1 / sqrt(r * r + z0 * z0);
+  const T dr0invdr = -(r0inv * r0inv * r0inv) * (r + z0 * dz0dr);
+  p.a_r = z0 * r0inv;
+  p.a_i = -z * r0inv;
+  p.b_r = y * r0inv;
+  p.b_i = -x * r0inv;
+  T sfac, dsfac;
+  sfac_dsfac(t, r, sfac, dsfac);
+  p.sfac = sfac * (T)t.wj;
+  p.dsfac = dsfac * (T)t.wj;
+  p.u[0] = x * rinv;
+  p.u[1] = y * rinv;
+  p.u[2] = z * rinv;
+  p.c1 = dz0dr * r0inv + z0 * dr0invdr;
+  p.c2 = -z * dr0invdr;
+  p.c3 = y * dr0invdr;
+  p.c4 = -x * dr0invdr;
+  p.r0inv = r0inv;
+  p.pad[0] = p.pad[1] = 0;
+}
+
+// the reference's ad-hoc pair repulsion as F/r, force_snap_neigh_impl.h:705-711
+__device__ __forceinline__ double snap_fdivr(double dx, double dy, double dz) {
+  const double rsq = dx * dx + dy * dy + dz * dz;
+  return -1.5e6 / pow(rsq, 7.0);
+}
+
 }  // namespace
 
 // ================================================================ kernels
@@ -145,14 +203,18 @@ __device__ __forceinline__ void sfac_dsfac(const snap_dev_tables &t, T r, T &sfa
 // With a force array given (f64 or f32), block i also zeroes its rows
 // i, i + n, i + 2n, ... below ntot (deep_copy(f, 0), src/examinimd.cpp:232,
 // done here instead of in a launch of its own: the force kernels that add
-// into f come after this one).
+// into f come after this one).  With p32 or p64 given, the slot's neighbour
+// parameters (snap_par, FP32 or FP64) and the repulsion F/r (fdv) are
+// written as well.
 __global__ void __launch_bounds__(64)
-k_snap_neigh(const double *__restrict__ x, const int *__restrict__ row,
+k_snap_neigh(snap_dev_tables t, const double *__restrict__ x, const int *__restrict__ row,
              const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
              double cutsq, int maxn, int *__restrict__ nin, int *__restrict__ nj,
              double *__restrict__ nr, double *__restrict__ f64, float *__restrict__ f32,
-             int ntot) {
-#pragma clang fp contract(off)
+             int ntot, snap_par<float> *__restrict__ p32, snap_par<double> *__restrict__ p64,
+             double *__restrict__ fdv) {
+#pragma clang fp contract(off)  // the distance test as the reference (the parameters are
+                                // computed in functions of their own)
   const int i = blockIdx.x;
   if (i >= n) return;
   const int lane = threadIdx.x;
@@ -191,11 +253,30 @@ k_snap_neigh(const double *__restrict__ x, const int *__restrict__ row,
         nr[3 * o] = dx;
         nr[3 * o + 1] = dy;
         nr[3 * o + 2] = dz;
+        if (p64) snap_par_of<double>(t, dx, dy, dz, p64[o]);
+        if (p32) snap_par_of<float>(t, dx, dy, dz, p32[o]);
+        if (fdv) fdv[o] = snap_fdivr(dx, dy, dz);
       }
     }
     run += __popcll(m);
   }
   if (lane == 0) nin[i] = run < maxn ? run : maxn;
+}
+
+// the neighbour parameters of one precision (and the repulsion F/r) from the
+// compacted neighbours, for a U kernel of another precision than the one the
+// compaction wrote
+template <class T>
+__global__ void __launch_bounds__(64)
+k_snap_par(snap_dev_tables t, const int *__restrict__ nin, const double *__restrict__ nr,
+           int maxn, int n, snap_par<T> *__restrict__ par, double *__restrict__ fdv) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  for (int k = threadIdx.x; k < nin[i]; k += 64) {
+    const size_t o = (size_t)i * maxn + k;
+    snap_par_of<T>(t, nr[3 * o], nr[3 * o + 1], nr[3 * o + 2], par[o]);
+    fdv[o] = snap_fdivr(nr[3 * o], nr[3 * o + 1], nr[3 * o + 2]);
+  }
 }
 
 // layer entry owned by lane l in layer j: (ma, mb) with mb <= j/2
@@ -233,7 +314,10 @@ __device__ __forceinline__ void layer_tab_fill(const snap_dev_tables &t, layer_t
 }
 
 // one entry of layer j from the full layer j-1 (row-major ma*j+mb), with the
-// recursion factors from the layer table (see u_step)
+// recursion factors from the layer table:
+// U(j,ma,mb) = [ma<j] rp(j-ma,j-mb) conj(a) U(j-1,ma,mb)
+//            - [ma>0] rp(ma,j-mb)  conj(b) U(j-1,ma-1,mb)
+// (SNA::compute_uarray, sna_impl.hpp:663-695)
 template <class T>
 __device__ __forceinline__ cplx<T> u_step_tab(const T *pr, const T *pi, int j, int ma, int mb,
                                               T rpa, T rpb, T a_r, T a_i, T b_r, T b_i) {
@@ -262,30 +346,6 @@ __device__ __forceinline__ void ck_params(T x, T y, T z, T z0, T r, T &a_r, T &a
   b_i = -r0inv * x;
 }
 
-// one entry of layer j from layer j-1 (full layer in LDS, row-major ma*(j)+mb):
-// U(j,ma,mb) = [ma<j] rp(j-ma,j-mb) conj(a) U(j-1,ma,mb)
-//            - [ma>0] rp(ma,j-mb)  conj(b) U(j-1,ma-1,mb)
-// (SNA::compute_uarray, sna_impl.hpp:663-695)
-template <class T>
-__device__ __forceinline__ cplx<T> u_step(const snap_dev_tables &t, const T *pr, const T *pi,
-                                          int j, int ma, int mb, T a_r, T a_i, T b_r, T b_i) {
-  cplx<T> u{0, 0};
-  const int jp = j;  // previous layer has j rows/cols: index ma*j + mb
-  if (ma < j) {
-    const T rp = (T)t.rootpq[(j - ma) * SNAP_RPD + (j - mb)];
-    const T ur = pr[ma * jp + mb], ui = pi[ma * jp + mb];
-    u.r += rp * (a_r * ur + a_i * ui);
-    u.i += rp * (a_r * ui - a_i * ur);
-  }
-  if (ma > 0) {
-    const T rp = (T)t.rootpq[ma * SNAP_RPD + (j - mb)];
-    const T ur = pr[(ma - 1) * jp + mb], ui = pi[(ma - 1) * jp + mb];
-    u.r += -rp * (b_r * ur + b_i * ui);
-    u.i += -rp * (b_r * ui - b_i * ur);
-  }
-  return u;
-}
-
 // ordering of LDS traffic among the lanes of one wavefront (no block barrier)
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -293,7 +353,8 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// U_tot of every atom (SNA::compute_ui, sna_impl.hpp:152-190): one block of
+// U_tot of every atom for the size class twojmax <= 12 (SNA::compute_ui,
+// sna_impl.hpp:152-190): one block of
 // UI_WAVES wavefronts per atom, wavefront w takes neighbours w, w+UI_WAVES, ...
 // (lanes over the half-plane entries of a layer, 64 at a time, the previous
 // layer in the wavefront's own LDS buffer), then the partial sums of the
@@ -387,150 +448,6 @@ k_snap_ui(snap_dev_tables t, const int *__restrict__ nin, const double *__restri
           ur[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = sg * vr;
           ui[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = -sg * vi;
         }
-      }
-    }
-  }
-}
-
-// U_tot with 16-lane groups (twojmax 8): each wavefront runs four
-// neighbours side by side, one per 16-lane group, so a layer of h half-plane
-// entries takes ceil(h/16) steps for four neighbours instead of one step per
-// neighbour (13 steps per four neighbours instead of 32).  A group keeps
-// only the half plane of the previous layer in LDS and reads the middle
-// column of even layers as mirrors, U(J-ma, J-mb) = (-1)^(ma+mb) conj
-// U(ma, mb); each lane accumulates its own entries (fixed slots, fully
-// unrolled layers) over its group's neighbours in registers.  The sums are
-// then added over the four groups of a wavefront (xor tree) and over the
-// wavefronts in order: deterministic.
-constexpr int UG_LANES = 16;
-constexpr int UG_GROUPS = 64 / UG_LANES;           // per wavefront
-constexpr int UG_HP = 32;                          // half-plane entries of layer 7
-
-__host__ __device__ constexpr int ug_hp(int j) { return (j + 1) * (j / 2 + 1); }
-__host__ __device__ constexpr int ug_steps(int j) { return (ug_hp(j) + UG_LANES - 1) / UG_LANES; }
-__host__ __device__ constexpr int ug_slot0(int j) { return j == 0 ? 0 : ug_slot0(j - 1) + ug_steps(j - 1); }
-constexpr int UG_SLOTS = ug_slot0(SNAP_MAXJ + 1);  // 14 for twojmax 8
-
-template <class T, int UG_WAVES>  // wavefronts per block
-__global__ void __launch_bounds__(64 * UG_WAVES)
-k_snap_ui_grp(snap_dev_tables t, const int *__restrict__ nin, const double *__restrict__ nr,
-              int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
-  constexpr int TJ = SNAP_MAXJ;
-  __shared__ T s_b[UG_WAVES * UG_GROUPS][2][2][UG_HP];  // [group][buffer][r/i][entry]
-  __shared__ T s_acc[UG_WAVES][2][snap_dim<TJ>::NY];
-  __shared__ layer_tab<T, TJ> LT;
-  const int i = blockIdx.x;
-  if (i >= n) return;
-  layer_tab_fill(t, LT);
-  __syncthreads();
-  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-  const int grp = l / UG_LANES, lg = l % UG_LANES;
-  T (*buf)[2][UG_HP] = s_b[w * UG_GROUPS + grp];
-  T acc_r[UG_SLOTS], acc_i[UG_SLOTS];
-#pragma unroll
-  for (int q = 0; q < UG_SLOTS; q++) acc_r[q] = acc_i[q] = 0;
-  const int cnt = nin[i];
-  for (int base = w * UG_GROUPS; base < cnt; base += UG_WAVES * UG_GROUPS) {
-    const int jj = base + grp;
-    const bool act = jj < cnt;
-    const size_t o = (size_t)i * maxn + (act ? jj : base);
-    const T x = (T)nr[3 * o], y = (T)nr[3 * o + 1], z = (T)nr[3 * o + 2];
-    const T r = sqrt(x * x + y * y + z * z);
-    const T theta0 = (r - (T)t.rmin0) * (T)t.rfac0 * (T)SNAP_PI / ((T)t.rcut - (T)t.rmin0);
-    const T z0 = r / tan(theta0);
-    T a_r, a_i, b_r, b_i;
-    ck_params(x, y, z, z0, r, a_r, a_i, b_r, b_i);
-    T sfac, dsfac;
-    sfac_dsfac(t, r, sfac, dsfac);
-    sfac = act ? sfac * (T)t.wj : (T)0;  // an idle group adds nothing
-    // layer 0: U = 1
-    if (lg == 0) {
-      buf[0][0][0] = 1;
-      buf[0][1][0] = 0;
-      acc_r[0] += sfac;
-    }
-    wave_sync();
-#pragma unroll
-    for (int j = 1; j <= TJ; j++) {
-      const int p = (j - 1) & 1, c = j & 1;
-#pragma unroll
-      for (int st = 0; st < ug_steps(j); st++) {
-        const int e = st * UG_LANES + lg;
-        const int info = LT.mm[j][e];
-        const bool own = e < ug_hp(j);
-        const int ma = info & 255, mb = (info >> 8) & 255;
-        T ur = 0, ui = 0;
-        if (own) {
-          const bool mir = 2 * mb == j;
-          const T sgn = ((ma + mb) & 1) ? (T)-1 : (T)1;
-          const T s_r = mir ? sgn : (T)1, s_i = mir ? -sgn : (T)1;
-          if (ma < j) {
-            const T rp = LT.rpa(j, ma, mb);
-            const int sidx = mir ? (j - 1 - mb) * j + (j - 1 - ma) : mb * j + ma;
-            const T pr = s_r * buf[p][0][sidx], pi = s_i * buf[p][1][sidx];
-            ur += rp * (a_r * pr + a_i * pi);
-            ui += rp * (a_r * pi - a_i * pr);
-          }
-          if (ma > 0) {
-            const T rp = LT.rpb(j, ma, mb);
-            const int sidx = mir ? (j - 1 - mb) * j + (j - ma) : mb * j + ma - 1;
-            const T tr = mir ? -s_r : (T)1, ti = mir ? -s_i : (T)1;
-            const T pr = tr * buf[p][0][sidx], pi = ti * buf[p][1][sidx];
-            ur += -rp * (b_r * pr + b_i * pi);
-            ui += -rp * (b_r * pi - b_i * pr);
-          }
-          if (j < TJ) {
-            buf[c][0][e] = ur;
-            buf[c][1][e] = ui;
-          }
-        }
-        acc_r[ug_slot0(j) + st] += sfac * ur;
-        acc_i[ug_slot0(j) + st] += sfac * ui;
-      }
-      wave_sync();
-    }
-  }
-  // sum over the four groups of the wavefront, then over the wavefronts
-#pragma unroll
-  for (int q = 0; q < UG_SLOTS; q++) {
-    acc_r[q] += __shfl_xor(acc_r[q], UG_LANES, 64);
-    acc_i[q] += __shfl_xor(acc_i[q], UG_LANES, 64);
-    acc_r[q] += __shfl_xor(acc_r[q], 2 * UG_LANES, 64);
-    acc_i[q] += __shfl_xor(acc_i[q], 2 * UG_LANES, 64);
-  }
-  if (grp == 0) {
-#pragma unroll
-    for (int j = 0; j <= TJ; j++)
-#pragma unroll
-      for (int st = 0; st < ug_steps(j); st++) {
-        const int e = st * UG_LANES + lg;
-        if (e < ug_hp(j)) {
-          s_acc[w][0][t.ystart[j] + e] = acc_r[ug_slot0(j) + st];
-          s_acc[w][1][t.ystart[j] + e] = acc_i[ug_slot0(j) + st];
-        }
-      }
-  }
-  __syncthreads();
-  if (w != 0) return;
-  // U_tot = wself * I + the wavefront sums in order; own entries and mirrors
-  T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
-  for (int j = 0; j <= TJ; j++) {
-    const int info = LT.mm[j][l];
-    const int ma = info & 255, mb = (info >> 8) & 255;
-    if (info >> 16) {
-      T vr = 0, vi = 0;
-#pragma unroll
-      for (int q = 0; q < UG_WAVES; q++) {
-        vr += s_acc[q][0][t.ystart[j] + l];
-        vi += s_acc[q][1][t.ystart[j] + l];
-      }
-      if (ma == mb) vr += (T)t.wself;
-      ur[t.ustart[j] + ma * (j + 1) + mb] = vr;
-      ui[t.ustart[j] + ma * (j + 1) + mb] = vi;
-      if (2 * mb != j) {
-        const T sg = ((ma + mb) & 1) ? (T)-1 : (T)1;
-        ur[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = sg * vr;
-        ui[t.ustart[j] + (j - ma) * (j + 1) + (j - mb)] = -sg * vi;
       }
     }
   }
@@ -677,30 +594,22 @@ __global__ void k_yf_weights(int nslots, const unsigned short *__restrict__ key,
   w32[s] = (float)v;
 }
 
-// F_ij for two (atom, neighbour) pairs per wavefront, 32 lanes each
+// F_ij of the size class J = 12, one (atom, neighbour) pair per wavefront
 // (SNA::compute_duidrj + compute_duarray, sna_impl.hpp:290-321 / 728-893,
-// then the Y contraction): the lanes of a half-wavefront take the half-plane
-// entries of a layer 32 at a time (one step up to layer 7, two for the 45
-// entries of layer 8), so about twice as many lanes do useful work as with
-// one pair per wavefront.  A half-wavefront without a neighbour computes on
-// its partner's and discards the result (no lane leaves before the wave
-// barriers).
-// LG = 16 runs four neighbours per wavefront instead: a layer of h entries
-// takes ceil(h/16) steps for four neighbours (14 steps over layers 1-8)
-// instead of ceil(h/32) for two (10 steps), at twice the LDS per wavefront.
-// The size class J = 12 runs one neighbour per wavefront (LG = 64): its
-// layers below the top have up to 72 half-plane entries, two steps of 64.
-constexpr int FIJ_WAVES = 2;                      // wavefronts per block
-template <int LG>
-constexpr int fij_nb() { return FIJ_WAVES * (64 / LG); }  // neighbours per block
+// then the Y contraction): the lanes take the half-plane entries of a layer
+// 64 at a time (layers below the top have up to 72 entries, two steps), the
+// previous layer of U and dU in LDS.  (For twojmax <= 8 the register-resident
+// k_snap_fij_rg replaces it.)
+constexpr int FIJ_WAVES = 2;  // wavefronts per block
 
-template <int LG, class T, class TF, int J>
+template <class T, class TF, int J>
 __global__ void __launch_bounds__(64 * FIJ_WAVES)
 k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
            const double *__restrict__ nr, int maxn, int n, const T *__restrict__ y_r,
            const T *__restrict__ y_i, TF *__restrict__ f) {
-  constexpr int FIJ_NB = fij_nb<LG>();
-  constexpr int NBW = 64 / LG;  // neighbours per wavefront
+  constexpr int LG = 64;                      // lanes per neighbour
+  constexpr int NBW = 64 / LG;                // neighbours per wavefront
+  constexpr int FIJ_NB = FIJ_WAVES * NBW;     // neighbours per block
   constexpr int FIJ_HP = snap_dim<J>::HPPREV;  // half-plane entries of layer J-1
   // a layer below the top takes at most two steps (kept in registers below)
   static_assert((FIJ_HP + LG - 1) / LG <= 2, "two steps per layer below the top");
@@ -888,6 +797,307 @@ k_snap_fij(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict
   }
 }
 
+// ------------------------------------------- register-resident recursion
+// For twojmax <= 8 a neighbour's layer fits nine lanes: lane ma (0..8) of
+// the neighbour's group holds the entries (ma, mb) of every half-plane
+// column mb (0..4) of the current layer in registers.  The recursion
+//   U_j(ma,mb) = rpa conj(a) U_{j-1}(ma,mb) - rpb conj(b) U_{j-1}(ma-1,mb)
+// reads U_{j-1}(ma,mb) from the lane's own register and U_{j-1}(ma-1,mb)
+// from the lane below (a DPP shift of the whole wavefront by one lane; at
+// ma = 0 rpb is 0 and the value from the group below is not used).  The
+// middle column mb = j/2 of an even layer needs U_{j-1}(ma, j/2), outside
+// the previous half plane: it is the mirror (-1)^(ma+j/2) conj
+// U_{j-1}(j-1-ma, j/2-1), fetched from lane j-1-ma of the group by one
+// permute per value.  Seven neighbours per wavefront (lane 63 idle); no LDS
+// traffic for the layers and no barriers between them.  The recursion
+// factors and the half-plane-weighted Y of each (column, ma) come from one
+// LDS record, zero where the entry does not exist (ma > j), so lanes past
+// the layer compute exact zeros.
+constexpr int RG_W = SNAP_MAXJ + 1;  // lanes per neighbour (ma = 0..8)
+constexpr int RG_NB = 64 / RG_W;     // neighbours per wavefront: 7
+constexpr int RG_COLS = SNAP_MAXJ / 2 + 1;
+// first column slot of layer j: sum over layers j' < j of (j'/2 + 1) columns
+__host__ __device__ constexpr int rg_slot0(int j) { return j <= 0 ? 0 : j + (j - 1) * (j - 1) / 4; }
+constexpr int RG_SLOTS = rg_slot0(SNAP_MAXJ + 1);  // (layer, column) pairs of layers 0..8: 25
+static_assert(rg_slot0(1) == 1 && rg_slot0(2) == 2 && rg_slot0(3) == 4 && rg_slot0(8) == 20 &&
+                  RG_SLOTS == 25,
+              "column slots");
+
+template <class T>
+struct alignas(4 * sizeof(T)) rg_rec {
+  T rpa, rpb, yr, yi;  // recursion factors and weighted Y of (j, ma, mb)
+};
+
+// value of lane l - 1 (0 in lane 0): DPP wavefront shift right by one
+// (bound_ctrl: the lane without a source reads 0)
+__device__ __forceinline__ float lane_below(float v) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double lane_below(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x138, 0xf, 0xf, true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x138, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// the records of one atom: column slot s = rg_slot0(j) + mb, lane ma
+template <class T>
+__device__ __forceinline__ void rg_fill(const snap_dev_tables &t, const T *__restrict__ y_r,
+                                        const T *__restrict__ y_i, int i, rg_rec<T> *rec) {
+  for (int k = threadIdx.x; k < RG_SLOTS * RG_W; k += blockDim.x) {
+    const int s = k / RG_W, ma = k - s * RG_W;
+    int j = 0;
+    while (j < SNAP_MAXJ && rg_slot0(j + 1) <= s) j++;
+    const int mb = s - rg_slot0(j);
+    T rpa = 0, rpb = 0, yr = 0, yi = 0;
+    if (j <= t.twojmax && ma <= j) {
+      if (ma < j) rpa = (T)t.rootpq[(j - ma) * SNAP_RPD + (j - mb)];
+      if (ma > 0) rpb = (T)t.rootpq[ma * SNAP_RPD + (j - mb)];
+      T wgt = 1;  // the reference's half plane: middle column ma < mb, diagonal 1/2
+      if (2 * mb == j) wgt = (ma < mb) ? (T)1 : ((ma == mb) ? (T)0.5 : (T)0);
+      if (y_r) {
+        const int e = t.ystart[j] + mb * (j + 1) + ma;
+        yr = wgt * y_r[(size_t)i * t.ny + e];
+        yi = wgt * y_i[(size_t)i * t.ny + e];
+      }
+    }
+    rec[k] = rg_rec<T>{rpa, rpb, yr, yi};
+  }
+}
+
+// U_tot with the register-resident recursion (twojmax <= 8; SNA::compute_ui,
+// sna_impl.hpp:152-190): a block of UR_WAVES wavefronts per atom, seven
+// neighbours per wavefront at a time, lane ma of a neighbour's group holding
+// the entries (ma, mb) of every half-plane column (as in k_snap_fij_rg).
+// Each lane accumulates sfac * U of its entries over its group's neighbours
+// in registers; the sums are added over the seven groups of a wavefront by a
+// fixed tree of lane shifts (groups 4-6 onto 0-2, then 2-3 onto 0-1, then 1
+// onto 0) and over the wavefronts in order: deterministic.
+template <class T, int UR_WAVES>
+__global__ void __launch_bounds__(64 * UR_WAVES)
+k_snap_ui_rg(snap_dev_tables t, const int *__restrict__ nin, const snap_par<T> *__restrict__ par,
+             int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
+  __shared__ rg_rec<T> s_rec[RG_SLOTS * RG_W];
+  __shared__ T s_acc[UR_WAVES][2][RG_SLOTS * RG_W];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  rg_fill<T>(t, nullptr, nullptr, i, s_rec);
+  __syncthreads();
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int g = l / RG_W, ma = l - g * RG_W;  // g = 7 only for the idle lane 63
+  T acc[RG_SLOTS][2];
+#pragma unroll
+  for (int s = 0; s < RG_SLOTS; s++) acc[s][0] = acc[s][1] = 0;
+  const int cnt = nin[i];
+  for (int base = w * RG_NB; base < cnt; base += UR_WAVES * RG_NB) {
+    const int jj = base + g;
+    const bool act = g < RG_NB && jj < cnt;
+    const size_t o = (size_t)i * maxn + (act ? jj : base);
+    const T a_r = par[o].a_r, a_i = par[o].a_i, b_r = par[o].b_r, b_i = par[o].b_i;
+    const T sfac = act ? par[o].sfac : (T)0;  // an idle group adds nothing
+    T c[RG_COLS][2];
+#pragma unroll
+    for (int mb = 0; mb < RG_COLS; mb++) c[mb][0] = c[mb][1] = 0;
+    c[0][0] = ma == 0 ? (T)1 : (T)0;  // layer 0: U = 1
+    acc[0][0] += sfac * c[0][0];
+#pragma unroll
+    for (int j = 1; j <= SNAP_MAXJ; j++) {
+      if (j > t.twojmax) break;
+      T mir[2], mirb[2];  // even j: U_{j-1}(ma, j/2) and U_{j-1}(ma-1, j/2)
+      if ((j & 1) == 0) {
+        const int src = g * RG_W + (j - 1 - ma > 0 ? j - 1 - ma : 0);
+        const T sg = ((ma + j / 2) & 1) ? (T)-1 : (T)1;
+        mir[0] = sg * __shfl(c[j / 2 - 1][0], src, 64);
+        mir[1] = -sg * __shfl(c[j / 2 - 1][1], src, 64);
+        mirb[0] = lane_below(mir[0]);
+        mirb[1] = lane_below(mir[1]);
+      }
+#pragma unroll
+      for (int mb = 0; mb <= j / 2; mb++) {
+        const int slot = rg_slot0(j) + mb;
+        const rg_rec<T> q = s_rec[slot * RG_W + ma];
+        const bool mid = (j & 1) == 0 && mb == j / 2;
+        const T p0 = mid ? mir[0] : c[mb][0], p1 = mid ? mir[1] : c[mb][1];
+        const T q0 = mid ? mirb[0] : lane_below(c[mb][0]);
+        const T q1 = mid ? mirb[1] : lane_below(c[mb][1]);
+        const T ur = q.rpa * (a_r * p0 + a_i * p1) - q.rpb * (b_r * q0 + b_i * q1);
+        const T ui = q.rpa * (a_r * p1 - a_i * p0) - q.rpb * (b_r * q1 - b_i * q0);
+        c[mb][0] = ur;
+        c[mb][1] = ui;
+        acc[slot][0] += sfac * ur;
+        acc[slot][1] += sfac * ui;
+      }
+    }
+  }
+  // over the seven groups of the wavefront, into lanes 0..8
+#pragma unroll
+  for (int s = 0; s < RG_SLOTS; s++) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      T v = acc[s][q];
+      T u = __shfl_down(v, 4 * RG_W, 64);
+      if (l < 3 * RG_W) v += u;
+      u = __shfl_down(v, 2 * RG_W, 64);
+      if (l < 2 * RG_W) v += u;
+      u = __shfl_down(v, RG_W, 64);
+      if (l < RG_W) {
+        v += u;
+        s_acc[w][q][s * RG_W + l] = v;
+      }
+    }
+  }
+  __syncthreads();
+  // U_tot = wself * I + the wavefront sums in order; own entries and mirrors
+  T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
+  for (int k = threadIdx.x; k < RG_SLOTS * RG_W; k += blockDim.x) {
+    const int s = k / RG_W, m = k - s * RG_W;
+    int j = 0;
+    while (j < SNAP_MAXJ && rg_slot0(j + 1) <= s) j++;
+    const int mb = s - rg_slot0(j);
+    if (j > t.twojmax || m > j) continue;
+    T vr = 0, vi = 0;
+#pragma unroll
+    for (int q = 0; q < UR_WAVES; q++) {
+      vr += s_acc[q][0][k];
+      vi += s_acc[q][1][k];
+    }
+    if (m == mb) vr += (T)t.wself;
+    ur[t.ustart[j] + m * (j + 1) + mb] = vr;
+    ui[t.ustart[j] + m * (j + 1) + mb] = vi;
+    if (2 * mb != j) {
+      const T sg = ((m + mb) & 1) ? (T)-1 : (T)1;
+      ur[t.ustart[j] + (j - m) * (j + 1) + (j - mb)] = sg * vr;
+      ui[t.ustart[j] + (j - m) * (j + 1) + (j - mb)] = -sg * vi;
+    }
+  }
+}
+
+// F_ij with the register-resident recursion: blocks of RG_WAVES wavefronts
+// take 7 * RG_WAVES neighbours of one atom (grid.y over the neighbour slots).
+constexpr int RG_WAVES = 4;
+
+template <class T, class TF>
+__global__ void __launch_bounds__(64 * RG_WAVES) __attribute__((amdgpu_waves_per_eu(3)))
+k_snap_fij_rg(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
+              const double *__restrict__ nr, const snap_par<T> *__restrict__ par,
+              const double *__restrict__ fdv, int maxn, int n, const T *__restrict__ y_r,
+              const T *__restrict__ y_i, TF *__restrict__ f) {
+  __shared__ rg_rec<T> s_rec[RG_SLOTS * RG_W];
+  __shared__ T s_red[RG_WAVES][3][64];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const int cnt_i = nin[i];
+  const int row0 = blockIdx.y * (RG_WAVES * RG_NB);
+  if (row0 >= cnt_i) return;  // no neighbour for this block
+  rg_fill<T>(t, y_r, y_i, i, s_rec);
+  __syncthreads();
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int g = l / RG_W, ma = l - g * RG_W;  // g = 7 only for the idle lane 63
+  const int jj0 = row0 + w * RG_NB;
+  if (jj0 >= cnt_i) return;  // uniform over the wavefront; no block barrier follows
+  const int jj = jj0 + g;
+  const bool act = g < RG_NB && jj < cnt_i;
+  const size_t o = (size_t)i * maxn + (act ? jj : jj0);
+  const snap_par<T> pp = par[o];
+  const T a_r = pp.a_r, a_i = pp.a_i, b_r = pp.b_r, b_i = pp.b_i;
+  const T u3[3] = {pp.u[0], pp.u[1], pp.u[2]};
+  T da_r[3], da_i[3], db_r[3], db_i[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    da_r[k] = pp.c1 * u3[k];
+    da_i[k] = pp.c2 * u3[k];
+    db_r[k] = pp.c3 * u3[k];
+    db_i[k] = pp.c4 * u3[k];
+  }
+  da_i[2] += -pp.r0inv;
+  db_i[0] += -pp.r0inv;
+  db_r[1] += pp.r0inv;
+  const T sfac = pp.sfac, dsfac = pp.dsfac;
+
+  // c[mb][q]: q = 0 U_r, 1 U_i, 2 + 2k dU_r/dr_k, 3 + 2k dU_i/dr_k of (ma, mb)
+  T c[RG_COLS][8];
+#pragma unroll
+  for (int mb = 0; mb < RG_COLS; mb++)
+#pragma unroll
+    for (int q = 0; q < 8; q++) c[mb][q] = 0;
+  c[0][0] = ma == 0 ? (T)1 : (T)0;  // layer 0: U = 1, dU = 0
+  // sums over the lane's entries of Re(conj(U) Y) and Re(conj(dU_k) Y)
+  T auy = 0, ag[3] = {0, 0, 0};
+  {
+    const rg_rec<T> q = s_rec[ma];
+    auy += c[0][0] * q.yr;
+  }
+#pragma unroll
+  for (int j = 1; j <= SNAP_MAXJ; j++) {
+    if (j > t.twojmax) break;
+    T mir[8], mirb[8];  // even j: U_{j-1}(ma, j/2) and U_{j-1}(ma-1, j/2)
+    if ((j & 1) == 0) {
+      const int src = g * RG_W + (j - 1 - ma > 0 ? j - 1 - ma : 0);
+      const T sg = ((ma + j / 2) & 1) ? (T)-1 : (T)1;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const T v = __shfl(c[j / 2 - 1][q], src, 64);
+        mir[q] = (q & 1) ? -sg * v : sg * v;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) mirb[q] = lane_below(mir[q]);
+    }
+#pragma unroll
+    for (int mb = 0; mb <= j / 2; mb++) {
+      const rg_rec<T> q = s_rec[(rg_slot0(j) + mb) * RG_W + ma];
+      const bool mid = (j & 1) == 0 && mb == j / 2;
+      T P[8], Q[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        P[k] = mid ? mir[k] : c[mb][k];
+        Q[k] = mid ? mirb[k] : lane_below(c[mb][k]);
+      }
+      const T ur = q.rpa * (a_r * P[0] + a_i * P[1]) - q.rpb * (b_r * Q[0] + b_i * Q[1]);
+      const T ui = q.rpa * (a_r * P[1] - a_i * P[0]) - q.rpb * (b_r * Q[1] - b_i * Q[0]);
+      c[mb][0] = ur;
+      c[mb][1] = ui;
+      auy += ur * q.yr + ui * q.yi;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const T dr = q.rpa * (da_r[k] * P[0] + da_i[k] * P[1] + a_r * P[2 + 2 * k] + a_i * P[3 + 2 * k]) -
+                     q.rpb * (db_r[k] * Q[0] + db_i[k] * Q[1] + b_r * Q[2 + 2 * k] + b_i * Q[3 + 2 * k]);
+        const T di = q.rpa * (da_r[k] * P[1] - da_i[k] * P[0] + a_r * P[3 + 2 * k] - a_i * P[2 + 2 * k]) -
+                     q.rpb * (db_r[k] * Q[1] - db_i[k] * Q[0] + b_r * Q[3 + 2 * k] - b_i * Q[2 + 2 * k]);
+        c[mb][2 + 2 * k] = dr;
+        c[mb][3 + 2 * k] = di;
+        ag[k] += dr * q.yr + di * q.yi;
+      }
+    }
+  }
+  // the lane's share of F_ij, summed over the nine lanes of the group in order
+  s_red[w][0][l] = dsfac * auy * u3[0] + sfac * ag[0];
+  s_red[w][1][l] = dsfac * auy * u3[1] + sfac * ag[1];
+  s_red[w][2][l] = dsfac * auy * u3[2] + sfac * ag[2];
+  wave_sync();
+  if (ma == 0 && act) {
+    T fx = 0, fy = 0, fz = 0;
+#pragma unroll
+    for (int m = 0; m < RG_W; m++) {
+      fx += s_red[w][0][l + m];
+      fy += s_red[w][1][l + m];
+      fz += s_red[w][2][l + m];
+    }
+    // ad-hoc repulsion of the reference, force_snap_neigh_impl.h:705-711
+    const double fdivr = fdv[o];
+    const TF Fx = (TF)fx + (TF)(nr[3 * o] * fdivr);
+    const TF Fy = (TF)fy + (TF)(nr[3 * o + 1] * fdivr);
+    const TF Fz = (TF)fz + (TF)(nr[3 * o + 2] * fdivr);
+    const int j = nj[o];
+    atomicAdd(&f[3 * (size_t)i], Fx);
+    atomicAdd(&f[3 * (size_t)i + 1], Fy);
+    atomicAdd(&f[3 * (size_t)i + 2], Fz);
+    atomicAdd(&f[3 * (size_t)j], -Fx);
+    atomicAdd(&f[3 * (size_t)j + 1], -Fy);
+    atomicAdd(&f[3 * (size_t)j + 2], -Fz);
+  }
+}
+
 // per-atom energy: beta_0 + sum_k beta_k B_k + pair repulsion (FP64)
 __global__ void __launch_bounds__(256)
 k_snap_energy(snap_dev_tables t, const double *__restrict__ cg,
@@ -961,6 +1171,12 @@ struct emd_snap {
   int cap_n = 0, cap_maxn = 0;
   int *nin = nullptr, *nj = nullptr;
   double *nr = nullptr;
+  // neighbour parameters (snap_par<float> or <double>, per slot) and the
+  // repulsion F/r, with the precision the parameters were last written in
+  void *par = nullptr;
+  double *fdv = nullptr;
+  size_t cap_par = 0, cap_fdv = 0;
+  int par_prec = -1;
   void *u_r = nullptr, *u_i = nullptr, *y_r = nullptr, *y_i = nullptr;
   size_t cap_u = 0, cap_y = 0, cap_ui = 0, cap_yi = 0;
   double *eatom = nullptr;
@@ -1328,7 +1544,8 @@ extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, em
 extern "C" emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn) {
   if (!sn) return EMD_OK;
   if (ctx) hipStreamSynchronize(ctx->stream);
-  void *ptrs[] = {sn->cg, sn->ytab, sn->etab, sn->nin, sn->nj, sn->nr, sn->stab, sn->w_r,
+  void *ptrs[] = {sn->cg, sn->ytab, sn->etab, sn->nin, sn->nj, sn->nr, sn->par, sn->fdv,
+                  sn->stab, sn->w_r,
                   sn->w_i, sn->sbuf,
                   sn->u_r, sn->u_i, sn->y_r, sn->y_i, sn->eatom,
                   sn->yf_code32, sn->yf_code64, sn->yf_key, sn->yf_meta, sn->yf_cgw,
@@ -1359,6 +1576,9 @@ static emd_status snap_scratch(emd_ctx *ctx, emd_snap *sn, int n, int maxn) {
     sn->cap_n = cn;
     sn->cap_maxn = cm;
   }
+  const size_t slots = (size_t)n * sn->cap_maxn;
+  EMD_TRY(emd_ensure(ctx, &sn->par, &sn->cap_par, slots * sizeof(snap_par<double>)));
+  EMD_TRY(emd_ensure(ctx, (void **)&sn->fdv, &sn->cap_fdv, slots * sizeof(double)));
   EMD_TRY(emd_ensure(ctx, &sn->u_r, &sn->cap_u, (size_t)n * sn->t.nu * sizeof(double)));
   EMD_TRY(emd_ensure(ctx, &sn->u_i, &sn->cap_ui, (size_t)n * sn->t.nu * sizeof(double)));
   EMD_TRY(emd_ensure(ctx, &sn->y_r, &sn->cap_y, (size_t)n * sn->t.ny * sizeof(double)));
@@ -1367,9 +1587,11 @@ static emd_status snap_scratch(emd_ctx *ctx, emd_snap *sn, int n, int maxn) {
 }
 
 // zero_prec: EMD_FP64 or EMD_MIXED to zero that precision's force array
-// (all n_local + n_ghost rows) in the same launch, or -1
+// (all n_local + n_ghost rows) in the same launch, or -1.  par_prec:
+// EMD_FP64 or EMD_MIXED to write the neighbour parameters in that precision
+// (for the U and force kernels that follow), or -1
 static emd_status snap_neigh(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl, emd_snap *sn,
-                             int zero_prec = -1) {
+                             int zero_prec, int par_prec) {
   const int n = s->n_local;
   const int maxn = nl->maxneighs > 0 ? nl->maxneighs : 1;
   EMD_TRY(snap_scratch(ctx, sn, n, maxn));
@@ -1377,31 +1599,37 @@ static emd_status snap_neigh(emd_ctx *ctx, const emd_system *s, const emd_neigh 
   const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
   double *f64 = zero_prec == EMD_FP64 ? s->f : nullptr;
   float *f32 = zero_prec == EMD_MIXED ? s->f32 : nullptr;
-  EMD_LAUNCH(ctx, "snap_neigh", k_snap_neigh, n, 64, 0, s->x, row, nl->num, nl->entries,
+  snap_par<float> *p32 = par_prec == EMD_MIXED ? (snap_par<float> *)sn->par : nullptr;
+  snap_par<double> *p64 = par_prec == EMD_FP64 ? (snap_par<double> *)sn->par : nullptr;
+  EMD_LAUNCH(ctx, "snap_neigh", k_snap_neigh, n, 64, 0, sn->t, s->x, row, nl->num, nl->entries,
              nl->maxneighs, n, rcutmax * rcutmax, sn->cap_maxn, sn->nin, sn->nj, sn->nr, f64,
-             f32, s->n_local + s->n_ghost);
+             f32, s->n_local + s->n_ghost, p32, p64, (p32 || p64) ? sn->fdv : nullptr);
+  sn->par_prec = (p32 || p64) ? par_prec : -1;
   return EMD_OK;
 }
 
-// U_tot of every atom: the grouped kernel for twojmax 8, else one neighbour
-// per wavefront
+// the neighbour parameters in T's precision (written by the compaction, or
+// here when it wrote the other precision)
+template <class T>
+static emd_status snap_par_ready(emd_ctx *ctx, emd_snap *sn, int n) {
+  const int prec = std::is_same<T, float>::value ? EMD_MIXED : EMD_FP64;
+  if (sn->par_prec == prec) return EMD_OK;
+  EMD_LAUNCH(ctx, "snap_neigh", k_snap_par<T>, n, 64, 0, sn->t, sn->nin, sn->nr, sn->cap_maxn, n,
+             (snap_par<T> *)sn->par, sn->fdv);
+  sn->par_prec = prec;
+  return EMD_OK;
+}
+
+// U_tot of every atom: the register-resident kernel for twojmax <= 8 (one
+// wavefront per atom: SNAP-2k 18.6 us mixed / 23.5 us FP64, against 18.8 /
+// 32.8 with two and 26.8 / 47.9 with four wavefronts per atom), else one
+// neighbour per wavefront
 template <class T>
 static emd_status launch_ui(emd_ctx *ctx, emd_snap *sn, int n, T *ur, T *ui) {
-  if (ctx->snap_ui_grp && sn->t.twojmax == SNAP_MAXJ) {
-    // FP64 (196 VGPRs: two wavefronts per SIMD) runs more atoms at once with
-    // two-wavefront blocks (SNAP-2k: 47.5 vs 57.3 us); FP32 (94 VGPRs) is
-    // the same either way (28.7 vs 28.1 us) and keeps four
-    const int waves = ctx->snap_ug_waves ? ctx->snap_ug_waves
-                                         : (std::is_same<T, double>::value ? 2 : 4);
-    if (waves == 2)
-      EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui_grp<T, 2>), n, 64 * 2, 0, sn->t, sn->nin, sn->nr,
-                 sn->cap_maxn, n, ur, ui);
-    else
-      EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui_grp<T, 4>), n, 64 * 4, 0, sn->t, sn->nin, sn->nr,
-                 sn->cap_maxn, n, ur, ui);
-  } else if (sn->t.twojmax <= SNAP_MAXJ) {
-    EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui<T, SNAP_MAXJ>), n, 64 * UI_WAVES, 0, sn->t, sn->nin,
-               sn->nr, sn->cap_maxn, n, ur, ui);
+  if (sn->t.twojmax <= SNAP_MAXJ) {
+    EMD_TRY(snap_par_ready<T>(ctx, sn, n));
+    EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui_rg<T, 1>), n, 64, 0, sn->t, sn->nin,
+               (const snap_par<T> *)sn->par, sn->cap_maxn, n, ur, ui);
   } else {
     EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui<T, SNAP_JBIG>), n, 64 * UI_WAVES, 0, sn->t, sn->nin,
                sn->nr, sn->cap_maxn, n, ur, ui);
@@ -1424,21 +1652,22 @@ static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   return EMD_OK;
 }
 
-// F_ij for every (atom, neighbour): 16 lanes per neighbour, four
-// neighbours per wavefront (twojmax <= 8); one neighbour per wavefront above
+// F_ij for every (atom, neighbour): the register-resident kernel (seven
+// neighbours per wavefront) for twojmax <= 8, one neighbour per wavefront
+// above
 template <class T, class TF>
 static emd_status launch_fij(emd_ctx *ctx, emd_snap *sn, int n, const T *yr, const T *yi,
                              TF *f) {
   if (sn->t.twojmax <= SNAP_MAXJ) {
-    constexpr int nb = fij_nb<16>();
-    const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
-    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<16, T, TF, SNAP_MAXJ>), grid, 64 * FIJ_WAVES, 0,
-               sn->t, sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
+    EMD_TRY(snap_par_ready<T>(ctx, sn, n));
+    const dim3 grid(n, (sn->cap_maxn + RG_WAVES * RG_NB - 1) / (RG_WAVES * RG_NB));
+    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij_rg<T, TF>), grid, 64 * RG_WAVES, 0, sn->t, sn->nin,
+               sn->nj, sn->nr, (const snap_par<T> *)sn->par, sn->fdv, sn->cap_maxn, n, yr, yi,
+               f);
   } else {
-    constexpr int nb = fij_nb<64>();
-    const dim3 grid(n, (sn->cap_maxn + nb - 1) / nb);
-    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<64, T, TF, SNAP_JBIG>), grid, 64 * FIJ_WAVES, 0,
-               sn->t, sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
+    const dim3 grid(n, (sn->cap_maxn + FIJ_WAVES - 1) / FIJ_WAVES);
+    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<T, TF, SNAP_JBIG>), grid, 64 * FIJ_WAVES, 0, sn->t,
+               sn->nin, sn->nj, sn->nr, sn->cap_maxn, n, yr, yi, f);
   }
   return EMD_OK;
 }
@@ -1449,7 +1678,8 @@ static emd_status snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, e
   EMD_ARG(ctx, nl->n == s->n_local, "neighbour list does not match the system");
   const int n = s->n_local;
   if (n <= 0) return zero ? emd_force_zero(ctx, s, precision) : EMD_OK;
-  EMD_TRY(snap_neigh(ctx, s, nl, sn, zero ? (precision == EMD_MIXED ? EMD_MIXED : EMD_FP64) : -1));
+  const int prec = precision == EMD_MIXED ? EMD_MIXED : EMD_FP64;
+  EMD_TRY(snap_neigh(ctx, s, nl, sn, zero ? prec : -1, prec));
   if (precision == EMD_MIXED) {
     EMD_TRY(snap_ui_y<float>(ctx, sn, n));
     EMD_TRY((launch_fij<float, float>(ctx, sn, n, (const float *)sn->y_r,
@@ -1478,7 +1708,7 @@ extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const e
   const int n = s->n_local;
   *energy = 0.0;
   if (n <= 0) return EMD_OK;
-  EMD_TRY(snap_neigh(ctx, s, nl, sn));
+  EMD_TRY(snap_neigh(ctx, s, nl, sn, -1, EMD_FP64));
   EMD_TRY(launch_ui<double>(ctx, sn, n, (double *)sn->u_r, (double *)sn->u_i));
   double *ea = eatom;
   if (!ea) {
@@ -1777,7 +2007,7 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
   if (n <= 0) return EMD_OK;
   EMD_ARG(ctx, s->n_ghost == 0 || (h && h->root && h->n_ghost == s->n_ghost),
           "ghost atoms need the halo (owner of each ghost)");
-  EMD_TRY(snap_neigh(ctx, s, nl, sn));
+  EMD_TRY(snap_neigh(ctx, s, nl, sn, -1, precision == EMD_MIXED ? EMD_MIXED : EMD_FP64));
   EMD_TRY(emd_ensure(ctx, (void **)&sn->w_r, &sn->cap_w, (size_t)n * sn->t.ny * sizeof(double)));
   EMD_TRY(emd_ensure(ctx, (void **)&sn->w_i, &sn->cap_wi, (size_t)n * sn->t.ny * sizeof(double)));
   // sbuf: S[SNAP_MAXT] | Bsum[SNAP_MAXC] | sjstart ints (twojmax + 2 of them)
@@ -1817,7 +2047,7 @@ extern "C" emd_status emd_snap_descriptors(emd_ctx *ctx, const emd_system *s, co
   if (!ctx || !s || !nl || !sn || !B) return EMD_ERR_ARG;
   const int n = s->n_local;
   if (n <= 0) return EMD_OK;
-  EMD_TRY(snap_neigh(ctx, s, nl, sn));
+  EMD_TRY(snap_neigh(ctx, s, nl, sn, -1, EMD_FP64));
   EMD_TRY(launch_ui<double>(ctx, sn, n, (double *)sn->u_r, (double *)sn->u_i));
   EMD_LAUNCH(ctx, "snap_descriptors", k_snap_bsum, n, 256, 0, sn->t, sn->cg, sn->etab,
              sn->ncoeff, sn->bzeroflag, n, (const double *)sn->u_r, (const double *)sn->u_i,
