@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <queue>
 #include <set>
@@ -491,7 +492,7 @@ __device__ __forceinline__ void y_entry(const snap_dev_tables &t, int e, int &J,
   ma = k - mb * (J + 1);
 }
 
-// Y of one atom per block from a flat term stream.  Every half-plane entry
+// Y of AT atoms per block from a flat term stream.  Every half-plane entry
 // e = (J, ma, mb) is the sum over its (a, b) keys and (ma1, mb1) of
 //   w * U_a(ma1, mb1) * U_b(ma2, mb2),  w = coef(a,b,J) cg(ma1,ma2) cg(mb1,mb2)
 // (the double sum of SNA::compute_zi, sna_impl.hpp:221-274, weighted as in
@@ -504,16 +505,50 @@ __device__ __forceinline__ void y_entry(const snap_dev_tables &t, int e, int &J,
 // offsets (its code), a weight and six multiply-adds.  Once per chunk a
 // thread whose run of one entry's chunks ends stores the run's sum in LDS;
 // each entry adds its runs in thread order (fixed order: deterministic).
-// The two random LDS reads per term bound the kernel (not load latency: a
-// double-buffered form with 16-byte loads measured the same); the host
-// orders each chunk's terms so that lanes reading in the same LDS cycle hit
-// different banks.
+// With AT atoms per block (U_tot of each in LDS) every code and weight load
+// serves AT atoms: with the LDS reads free of bank conflicts (below) the
+// stream of codes and weights from L2 is what bounds the kernel.
+//
+// LDS banks: the LDS serves 256 bytes per cycle, so a wavefront's
+// ds_read_b64 is served 32 lanes per cycle, and two lanes of one cycle that
+// read different 8-byte words at the same bank residue (word index mod 32)
+// are served in turn.  (Measured: 5.15 LDS-active cycles per wavefront read
+// of an earlier FP32 term order, for which this model gives 5.19.)  A
+// ds_read_b128 of 16-byte FP64 pairs measured 8 cycles: its 8-byte halves
+// are served like two b64 reads, and a 16-byte stride leaves only 16
+// residues.  So FP32 U_tot is stored as (re, im) pairs and FP64 U_tot as an
+// array of real parts and one of imaginary parts, each read with b64 loads
+// at word residue u mod 32.  For twojmax <= 8 U_tot is staged four times,
+// copy c at word offset c * stride + shift(c), and the host gives every
+// read of every term the copy whose residue no other address of its
+// 32-lane group takes (a matching per group; with the shifts 0, 5, 11, 19
+// about 4 % of the groups keep one conflict).
 constexpr int YF_THREADS = 256;
 constexpr int YF_CHUNK = 16;
 
 template <class T>
 struct alignas(2 * sizeof(T)) cpair {
   T r, i;
+};
+
+// U_tot copies in the Y kernel's LDS, in 8-byte words: an FP32 pair is one
+// word, an FP64 entry a word in the real array and one in the imaginary
+// array (IMB bytes further)
+template <class T, int J>
+struct yf_layout {
+  static constexpr bool SOA = sizeof(T) == 8;
+  static constexpr int NCOPY = J <= SNAP_MAXJ ? 4 : 1;
+  static constexpr int MOD = 32;    // bank residues of an 8-byte word
+  static constexpr int GROUP = 32;  // lanes served per cycle
+  static constexpr int STRIDE = (snap_dim<J>::NU + MOD - 1) / MOD * MOD;
+  __host__ __device__ static constexpr int shift(int c) {
+    return NCOPY == 1 ? 0 : (c == 0 ? 0 : c == 1 ? 5 : c == 2 ? 11 : 19);
+  }
+  __host__ __device__ static constexpr int base(int c) { return c * STRIDE + shift(c); }
+  static constexpr int NUC = base(NCOPY - 1) + snap_dim<J>::NU;  // words the copies span
+  static constexpr int NUC_PAD = (NUC + MOD - 1) / MOD * MOD;
+  static constexpr int IMB = SOA ? NUC_PAD * 8 : 0;            // imaginary array (FP64)
+  static constexpr int UBYTES = SOA ? 2 * NUC_PAD * 8 : NUC_PAD * 8;
 };
 
 // code: byte offsets of the two U_tot entries in LDS, 16 bits each (the
@@ -523,28 +558,47 @@ struct alignas(2 * sizeof(T)) cpair {
 constexpr unsigned short YF_NO_RUN = 0xFFFFu;
 template <int J>
 constexpr int yf_maxruns() { return snap_dim<J>::NY + YF_THREADS; }
-static_assert(snap_nu(SNAP_JBIG) * 16 < 65536, "U_tot byte offsets fit 16 bits");
+static_assert(yf_layout<double, SNAP_JBIG>::NUC * 8 < 65536 &&
+                  yf_layout<float, SNAP_JBIG>::NUC * 8 < 65536,
+              "U_tot byte offsets fit 16 bits");
 
-template <class T, int J>
+template <class T, int J, int AT>
 __global__ void __launch_bounds__(YF_THREADS)
 k_snap_yflat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
              const T *__restrict__ w, const unsigned short *__restrict__ meta,
              const int *__restrict__ rstart, int n, const T *__restrict__ utot_r,
              const T *__restrict__ utot_i, T *__restrict__ y_r, T *__restrict__ y_i) {
-  // U_tot at byte 0 (a static array: its LDS address is known when the
-  // kernel is compiled, so a code's offset is the whole LDS address), then
-  // the run sums
-  constexpr int NU = snap_dim<J>::NU;
-  __shared__ cpair<T> s_yf[NU + yf_maxruns<J>()];
-  cpair<T> *su = s_yf;
-  cpair<T> *part = s_yf + NU;
-  const unsigned char *sb = reinterpret_cast<const unsigned char *>(s_yf);
-  const int i = blockIdx.x;
-  if (i >= n) return;
-  for (int k = threadIdx.x; k < nu; k += YF_THREADS)
-    su[k] = cpair<T>{utot_r[(size_t)i * nu + k], utot_i[(size_t)i * nu + k]};
+  // the copies of U_tot from byte 0 (a static array: its LDS address is
+  // known when the kernel is compiled, so a code's offset is the whole LDS
+  // address), then the run sums
+  using L = yf_layout<T, J>;
+  static_assert(L::UBYTES % sizeof(cpair<T>) == 0, "run sums aligned");
+  constexpr int UP = L::UBYTES / sizeof(cpair<T>);  // pairs per atom's U_tot copies
+  __shared__ cpair<T> s_yf[AT * (UP + yf_maxruns<J>())];
+  unsigned char *sb = reinterpret_cast<unsigned char *>(s_yf);
+  const int i0 = blockIdx.x * AT;
+  if (i0 >= n) return;
+  for (int a = 0; a < AT; a++) {
+    const int i = i0 + a < n ? i0 + a : i0;
+    unsigned char *ub = sb + a * L::UBYTES;
+    for (int k = threadIdx.x; k < nu; k += YF_THREADS) {
+      const T vr = utot_r[(size_t)i * nu + k], vi = utot_i[(size_t)i * nu + k];
+#pragma unroll
+      for (int c = 0; c < L::NCOPY; c++) {
+        if (L::SOA) {
+          reinterpret_cast<T *>(ub)[L::base(c) + k] = vr;
+          reinterpret_cast<T *>(ub + L::IMB)[L::base(c) + k] = vi;
+        } else {
+          reinterpret_cast<cpair<T> *>(ub)[L::base(c) + k] = cpair<T>{vr, vi};
+        }
+      }
+    }
+  }
   __syncthreads();
-  T ar = 0, ai = 0;
+  cpair<T> *part = s_yf + AT * UP;
+  T ar[AT], ai[AT];
+#pragma unroll
+  for (int a = 0; a < AT; a++) ar[a] = ai[a] = 0;
   for (int c = 0; c < nch; c++) {
     const unsigned short m = meta[(size_t)c * YF_THREADS + threadIdx.x];
     // the chunk's loads are issued before its arithmetic
@@ -558,28 +612,47 @@ k_snap_yflat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
     }
 #pragma unroll
     for (int k = 0; k < YF_CHUNK; k++) {
-      const cpair<T> u1 = *reinterpret_cast<const cpair<T> *>(sb + (cc[k] & 0xFFFFu));
-      const cpair<T> u2 = *reinterpret_cast<const cpair<T> *>(sb + (cc[k] >> 16));
-      const T pr = u1.r * u2.r - u1.i * u2.i;
-      const T pi = u1.r * u2.i + u1.i * u2.r;
-      ar += ww[k] * pr;
-      ai += ww[k] * pi;
+#pragma unroll
+      for (int a = 0; a < AT; a++) {
+        const unsigned char *ub = sb + a * L::UBYTES;
+        cpair<T> u1, u2;
+        if (L::SOA) {
+          u1.r = *reinterpret_cast<const T *>(ub + (cc[k] & 0xFFFFu));
+          u1.i = *reinterpret_cast<const T *>(ub + L::IMB + (cc[k] & 0xFFFFu));
+          u2.r = *reinterpret_cast<const T *>(ub + (cc[k] >> 16));
+          u2.i = *reinterpret_cast<const T *>(ub + L::IMB + (cc[k] >> 16));
+        } else {
+          u1 = *reinterpret_cast<const cpair<T> *>(ub + (cc[k] & 0xFFFFu));
+          u2 = *reinterpret_cast<const cpair<T> *>(ub + (cc[k] >> 16));
+        }
+        const T pr = u1.r * u2.r - u1.i * u2.i;
+        const T pi = u1.r * u2.i + u1.i * u2.r;
+        ar[a] += ww[k] * pr;
+        ai[a] += ww[k] * pi;
+      }
     }
     if (m != YF_NO_RUN) {
-      part[m] = cpair<T>{ar, ai};
-      ar = 0;
-      ai = 0;
+#pragma unroll
+      for (int a = 0; a < AT; a++) {
+        part[a * yf_maxruns<J>() + m] = cpair<T>{ar[a], ai[a]};
+        ar[a] = 0;
+        ai[a] = 0;
+      }
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ny; e += YF_THREADS) {
-    T yr = 0, yi = 0;
-    for (int r = rstart[e]; r < rstart[e + 1]; r++) {
-      yr += part[r].r;
-      yi += part[r].i;
+  for (int a = 0; a < AT; a++) {
+    const int i = i0 + a;
+    if (i >= n) break;
+    for (int e = threadIdx.x; e < ny; e += YF_THREADS) {
+      T yr = 0, yi = 0;
+      for (int r = rstart[e]; r < rstart[e + 1]; r++) {
+        yr += part[a * yf_maxruns<J>() + r].r;
+        yi += part[a * yf_maxruns<J>() + r].i;
+      }
+      y_r[(size_t)i * ny + e] = yr;
+      y_i[(size_t)i * ny + e] = yi;
     }
-    y_r[(size_t)i * ny + e] = yr;
-    y_i[(size_t)i * ny + e] = yi;
   }
 }
 
@@ -1231,6 +1304,69 @@ static double host_cg(int j1, int j2, int j, int k1, int k2) {
 }
 
 // the beta-independent part of the flat Y stream (see k_snap_yflat)
+// LDS byte offsets of the two U_tot reads of every slot of the Y stream
+// (code = offset1 | offset2 << 16, 8-byte words): for each step, each
+// wavefront, each of the two reads and each lane group served in one LDS
+// cycle, the group's distinct U entries are matched to distinct bank
+// residues (u + shift(c)) mod MOD, each entry choosing one copy c
+// (augmenting paths); an entry left unmatched takes the least loaded
+// residue among its copies.  Lanes reading the same entry read the same copy.
+template <class L>
+static void yf_assign_copies(const std::vector<int> &u1, const std::vector<int> &u2, int steps,
+                             std::vector<uint32_t> &code) {
+  std::vector<int> cp1(u1.size(), 0), cp2(u2.size(), 0);
+  for (int st = 0; st < steps; st++)
+    for (int w0 = 0; w0 < YF_THREADS; w0 += 64)
+      for (int op = 0; op < 2; op++)
+        for (int g0 = w0; g0 < w0 + 64; g0 += L::GROUP) {
+          const std::vector<int> &u = op ? u2 : u1;
+          std::vector<int> &cp = op ? cp2 : cp1;
+          const size_t s0 = (size_t)st * YF_THREADS + g0;
+          std::vector<int> addr;  // distinct entries of the group
+          for (int l = 0; l < L::GROUP; l++)
+            if (std::find(addr.begin(), addr.end(), u[s0 + l]) == addr.end()) addr.push_back(u[s0 + l]);
+          const int na = (int)addr.size();
+          std::vector<int> owner(L::MOD, -1), copy_of(na, -1);
+          std::function<bool(int, std::vector<char> &)> augment = [&](int a,
+                                                                      std::vector<char> &seen) {
+            for (int c = 0; c < L::NCOPY; c++) {
+              const int b = (addr[a] + L::shift(c)) % L::MOD;
+              if (seen[b]) continue;
+              seen[b] = 1;
+              if (owner[b] < 0 || augment(owner[b], seen)) {
+                owner[b] = a;
+                copy_of[a] = c;
+                return true;
+              }
+            }
+            return false;
+          };
+          std::vector<int> load(L::MOD, 0);
+          for (int a = 0; a < na; a++) {
+            std::vector<char> seen(L::MOD, 0);
+            augment(a, seen);
+          }
+          for (int a = 0; a < na; a++)
+            if (copy_of[a] >= 0) load[(addr[a] + L::shift(copy_of[a])) % L::MOD]++;
+          for (int a = 0; a < na; a++)
+            if (copy_of[a] < 0) {
+              int best = 0;
+              for (int c = 1; c < L::NCOPY; c++)
+                if (load[(addr[a] + L::shift(c)) % L::MOD] < load[(addr[a] + L::shift(best)) % L::MOD])
+                  best = c;
+              copy_of[a] = best;
+              load[(addr[a] + L::shift(best)) % L::MOD]++;
+            }
+          for (int l = 0; l < L::GROUP; l++) {
+            const int a = (int)(std::find(addr.begin(), addr.end(), u[s0 + l]) - addr.begin());
+            cp[s0 + l] = copy_of[a];
+          }
+        }
+  for (size_t k = 0; k < u1.size(); k++)
+    code[k] = (uint32_t)((L::base(cp1[k]) + u1[k]) * 8) |
+              ((uint32_t)((L::base(cp2[k]) + u2[k]) * 8) << 16);
+}
+
 static emd_status snap_build_yflat(emd_ctx *ctx, emd_snap *sn) {
   const snap_dev_tables &t = sn->t;
   const int nk = (int)sn->idxj.size() / 3;
@@ -1307,6 +1443,7 @@ static emd_status snap_build_yflat(emd_ctx *ctx, emd_snap *sn) {
   const int steps = P * YF_CHUNK;
   const size_t nslots = (size_t)steps * YF_THREADS;
   std::vector<uint32_t> code32(nslots, 0u), code64(nslots, 0u);
+  std::vector<int> u1(nslots, 0), u2(nslots, 0);
   std::vector<unsigned short> key(nslots, 0);
   std::vector<double> cgw(nslots, 0.0);
   // A chunk's terms all belong to one entry, so their order within the
@@ -1344,13 +1481,20 @@ static emd_status snap_build_yflat(emd_ctx *ctx, emd_snap *sn) {
           pool[l].erase(pool[l].begin() + best);
           f1[tm.i1 % 16]++; f2[tm.i2 % 16]++; d1[h][tm.i1 % 8]++; d2[h][tm.i2 % 8]++;
           const size_t slot = ((size_t)c * YF_CHUNK + k) * YF_THREADS + th0 + l;
-          code32[slot] = (uint32_t)(tm.i1 * 8) | ((uint32_t)(tm.i2 * 8) << 16);
-          code64[slot] = (uint32_t)(tm.i1 * 16) | ((uint32_t)(tm.i2 * 16) << 16);
+          u1[slot] = tm.i1;
+          u2[slot] = tm.i2;
           key[slot] = (unsigned short)tm.key;
           cgw[slot] = tm.cgw;
         }
       }
     }
+  if (small) {
+    yf_assign_copies<yf_layout<float, SNAP_MAXJ>>(u1, u2, steps, code32);
+    code64 = code32;  // the same word offsets (FP64: into the real array)
+  } else {
+    yf_assign_copies<yf_layout<float, SNAP_JBIG>>(u1, u2, steps, code32);
+    code64 = code32;
+  }
   sn->yf_steps = steps;
   sn->yf_nch = P;
   sn->yf_nruns = nruns;
@@ -1643,11 +1787,14 @@ static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   EMD_TRY(launch_ui<T>(ctx, sn, n, ur, ui));
   const T *w = std::is_same<T, float>::value ? (const T *)sn->yf_w32 : (const T *)sn->yf_w64;
   const uint32_t *code = std::is_same<T, float>::value ? sn->yf_code32 : sn->yf_code64;
+  // twojmax <= 8: two atoms per block share each term's code and weight
+  // load (SNAP-2k: 26.8 / 47.2 us mixed / FP64, against 31.5 / 60.9 with one
+  // atom and 28.2 / 55.9 with three)
   if (sn->t.twojmax <= SNAP_MAXJ)
-    EMD_LAUNCH(ctx, "snap_y", (k_snap_yflat<T, SNAP_MAXJ>), n, YF_THREADS, 0, sn->t.nu, sn->t.ny,
-               sn->yf_nch, code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
+    EMD_LAUNCH(ctx, "snap_y", (k_snap_yflat<T, SNAP_MAXJ, 2>), (n + 1) / 2, YF_THREADS, 0, sn->t.nu,
+               sn->t.ny, sn->yf_nch, code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
   else
-    EMD_LAUNCH(ctx, "snap_y", (k_snap_yflat<T, SNAP_JBIG>), n, YF_THREADS, 0, sn->t.nu, sn->t.ny,
+    EMD_LAUNCH(ctx, "snap_y", (k_snap_yflat<T, SNAP_JBIG, 1>), n, YF_THREADS, 0, sn->t.nu, sn->t.ny,
                sn->yf_nch, code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
   return EMD_OK;
 }
