@@ -915,28 +915,43 @@ __device__ __forceinline__ double lane_below(double v) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// the records of one atom: column slot s = rg_slot0(j) + mb, lane ma
+// Per (column slot s, lane ma), k = s * RG_W + ma, built on the host once:
+// the recursion factors (zero where the entry does not exist), and
+//   ey   the half-plane index of Y (or -1), wgt its half-plane weight
+//   uo   the U_tot index of (ma, mb) (or -1), um that of the mirror
+//        (j-ma, j-mb) off the middle column (or -1), neg its sign (-1)^(ma+mb)
+//        and diag for ma == mb (wself)
 template <class T>
-__device__ __forceinline__ void rg_fill(const snap_dev_tables &t, const T *__restrict__ y_r,
-                                        const T *__restrict__ y_i, int i, rg_rec<T> *rec) {
-  for (int k = threadIdx.x; k < RG_SLOTS * RG_W; k += blockDim.x) {
-    const int s = k / RG_W, ma = k - s * RG_W;
-    int j = 0;
-    while (j < SNAP_MAXJ && rg_slot0(j + 1) <= s) j++;
-    const int mb = s - rg_slot0(j);
-    T rpa = 0, rpb = 0, yr = 0, yi = 0;
-    if (j <= t.twojmax && ma <= j) {
-      if (ma < j) rpa = (T)t.rootpq[(j - ma) * SNAP_RPD + (j - mb)];
-      if (ma > 0) rpb = (T)t.rootpq[ma * SNAP_RPD + (j - mb)];
-      T wgt = 1;  // the reference's half plane: middle column ma < mb, diagonal 1/2
-      if (2 * mb == j) wgt = (ma < mb) ? (T)1 : ((ma == mb) ? (T)0.5 : (T)0);
+struct rg_rp {
+  T rpa, rpb;
+};
+struct rg_map {
+  int ey, uo, um;
+  unsigned char wq, neg, diag, pad;  // wgt = wq / 2
+};
+
+// the records of one atom: column slot s = rg_slot0(j) + mb, lane ma
+template <class T, int NT>
+__device__ __forceinline__ void rg_fill(const rg_rp<T> *__restrict__ rp,
+                                        const rg_map *__restrict__ map, int ny,
+                                        const T *__restrict__ y_r, const T *__restrict__ y_i,
+                                        int i, rg_rec<T> *rec) {
+#pragma unroll
+  for (int q = 0; q < (RG_SLOTS * RG_W + NT - 1) / NT; q++) {
+    const int k = q * NT + (int)threadIdx.x;
+    if (k < RG_SLOTS * RG_W) {
+      const rg_rp<T> r = rp[k];
+      T yr = 0, yi = 0;
       if (y_r) {
-        const int e = t.ystart[j] + mb * (j + 1) + ma;
-        yr = wgt * y_r[(size_t)i * t.ny + e];
-        yi = wgt * y_i[(size_t)i * t.ny + e];
+        const rg_map m = map[k];
+        if (m.ey >= 0) {
+          const T wgt = (T)0.5 * (T)m.wq;
+          yr = wgt * y_r[(size_t)i * ny + m.ey];
+          yi = wgt * y_i[(size_t)i * ny + m.ey];
+        }
       }
+      rec[k] = rg_rec<T>{r.rpa, r.rpb, yr, yi};
     }
-    rec[k] = rg_rec<T>{rpa, rpb, yr, yi};
   }
 }
 
@@ -950,13 +965,14 @@ __device__ __forceinline__ void rg_fill(const snap_dev_tables &t, const T *__res
 // onto 0) and over the wavefronts in order: deterministic.
 template <class T, int UR_WAVES>
 __global__ void __launch_bounds__(64 * UR_WAVES)
-k_snap_ui_rg(snap_dev_tables t, const int *__restrict__ nin, const snap_par<T> *__restrict__ par,
-             int maxn, int n, T *__restrict__ utot_r, T *__restrict__ utot_i) {
+k_snap_ui_rg(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *__restrict__ rgm,
+             const int *__restrict__ nin, const snap_par<T> *__restrict__ par, int maxn, int n,
+             T *__restrict__ utot_r, T *__restrict__ utot_i) {
   __shared__ rg_rec<T> s_rec[RG_SLOTS * RG_W];
   __shared__ T s_acc[UR_WAVES][2][RG_SLOTS * RG_W];
   const int i = blockIdx.x;
   if (i >= n) return;
-  rg_fill<T>(t, nullptr, nullptr, i, s_rec);
+  rg_fill<T, 64 * UR_WAVES>(rgp, rgm, 0, nullptr, nullptr, i, s_rec);
   __syncthreads();
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int g = l / RG_W, ma = l - g * RG_W;  // g = 7 only for the idle lane 63
@@ -1025,24 +1041,21 @@ k_snap_ui_rg(snap_dev_tables t, const int *__restrict__ nin, const snap_par<T> *
   // U_tot = wself * I + the wavefront sums in order; own entries and mirrors
   T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
   for (int k = threadIdx.x; k < RG_SLOTS * RG_W; k += blockDim.x) {
-    const int s = k / RG_W, m = k - s * RG_W;
-    int j = 0;
-    while (j < SNAP_MAXJ && rg_slot0(j + 1) <= s) j++;
-    const int mb = s - rg_slot0(j);
-    if (j > t.twojmax || m > j) continue;
+    const rg_map m = rgm[k];
+    if (m.uo < 0) continue;
     T vr = 0, vi = 0;
 #pragma unroll
     for (int q = 0; q < UR_WAVES; q++) {
       vr += s_acc[q][0][k];
       vi += s_acc[q][1][k];
     }
-    if (m == mb) vr += (T)t.wself;
-    ur[t.ustart[j] + m * (j + 1) + mb] = vr;
-    ui[t.ustart[j] + m * (j + 1) + mb] = vi;
-    if (2 * mb != j) {
-      const T sg = ((m + mb) & 1) ? (T)-1 : (T)1;
-      ur[t.ustart[j] + (j - m) * (j + 1) + (j - mb)] = sg * vr;
-      ui[t.ustart[j] + (j - m) * (j + 1) + (j - mb)] = -sg * vi;
+    if (m.diag) vr += (T)t.wself;
+    ur[m.uo] = vr;
+    ui[m.uo] = vi;
+    if (m.um >= 0) {
+      const T sg = m.neg ? (T)-1 : (T)1;
+      ur[m.um] = sg * vr;
+      ui[m.um] = -sg * vi;
     }
   }
 }
@@ -1081,7 +1094,8 @@ __device__ __forceinline__ double lane_above(double v) {
 // permutation.  About half the arithmetic of carrying dU/dr forward.
 template <class T, class TF>
 __global__ void __launch_bounds__(64 * RG_WAVES) __attribute__((amdgpu_waves_per_eu(3)))
-k_snap_fij_rev(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__ nj,
+k_snap_fij_rev(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *__restrict__ rgm,
+               const int *__restrict__ nin, const int *__restrict__ nj,
                const double *__restrict__ nr, const snap_par<T> *__restrict__ par,
                const double *__restrict__ fdv, int maxn, int n, const T *__restrict__ y_r,
                const T *__restrict__ y_i, TF *__restrict__ f) {
@@ -1093,7 +1107,7 @@ k_snap_fij_rev(snap_dev_tables t, const int *__restrict__ nin, const int *__rest
   const int cnt_i = nin[i];
   const int row0 = blockIdx.y * (RG_WAVES * RG_NB);
   if (row0 >= cnt_i) return;  // no neighbour for this block
-  rg_fill<T>(t, y_r, y_i, i, s_rec);
+  rg_fill<T, 64 * RG_WAVES>(rgp, rgm, t.ny, y_r, y_i, i, s_rec);
   __syncthreads();
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int g = l / RG_W, ma = l - g * RG_W;  // g = 7 only for the idle lane 63
@@ -1327,6 +1341,11 @@ struct emd_snap {
   // repulsion F/r, with the precision the parameters were last written in
   void *par = nullptr;
   double *fdv = nullptr;
+  // register-layout tables (twojmax <= 8): recursion factors in both
+  // precisions and the index map, per (column slot, lane)
+  rg_rp<float> *rg_rp32 = nullptr;
+  rg_rp<double> *rg_rp64 = nullptr;
+  rg_map *rg_map_d = nullptr;
   size_t cap_par = 0, cap_fdv = 0;
   int par_prec = -1;
   void *u_r = nullptr, *u_i = nullptr, *y_r = nullptr, *y_i = nullptr;
@@ -1595,6 +1614,51 @@ static emd_status snap_build_yflat(emd_ctx *ctx, emd_snap *sn) {
   return EMD_OK;
 }
 
+// the register-layout tables for twojmax <= 8 (see rg_map)
+static emd_status snap_build_rg(emd_ctx *ctx, emd_snap *sn) {
+  const snap_dev_tables &t = sn->t;
+  std::vector<rg_rp<float>> r32(RG_SLOTS * RG_W);
+  std::vector<rg_rp<double>> r64(RG_SLOTS * RG_W);
+  std::vector<rg_map> mp(RG_SLOTS * RG_W);
+  for (int sl = 0; sl < RG_SLOTS; sl++)
+    for (int ma = 0; ma < RG_W; ma++) {
+      const int k = sl * RG_W + ma;
+      int j = 0;
+      while (j < SNAP_MAXJ && rg_slot0(j + 1) <= sl) j++;
+      const int mb = sl - rg_slot0(j);
+      double rpa = 0, rpb = 0;
+      rg_map m{-1, -1, -1, 0, 0, 0, 0};
+      if (j <= t.twojmax && ma <= j) {
+        if (ma < j) rpa = t.rootpq[(j - ma) * SNAP_RPD + (j - mb)];
+        if (ma > 0) rpb = t.rootpq[ma * SNAP_RPD + (j - mb)];
+        // the reference's half plane: middle column ma < mb, diagonal 1/2
+        m.wq = 2 * mb == j ? (ma < mb ? 2 : (ma == mb ? 1 : 0)) : 2;
+        m.ey = t.ystart[j] + mb * (j + 1) + ma;
+        m.uo = t.ustart[j] + ma * (j + 1) + mb;
+        m.um = 2 * mb != j ? t.ustart[j] + (j - ma) * (j + 1) + (j - mb) : -1;
+        m.neg = (ma + mb) & 1;
+        m.diag = ma == mb;
+      }
+      r32[k] = rg_rp<float>{(float)rpa, (float)rpb};
+      r64[k] = rg_rp<double>{rpa, rpb};
+      mp[k] = m;
+    }
+  EMD_HIP(ctx, hipMalloc(&sn->rg_rp32, r32.size() * sizeof(r32[0])));
+  EMD_HIP(ctx, hipMalloc(&sn->rg_rp64, r64.size() * sizeof(r64[0])));
+  EMD_HIP(ctx, hipMalloc(&sn->rg_map_d, mp.size() * sizeof(mp[0])));
+  EMD_HIP(ctx, hipMemcpy(sn->rg_rp32, r32.data(), r32.size() * sizeof(r32[0]), hipMemcpyHostToDevice));
+  EMD_HIP(ctx, hipMemcpy(sn->rg_rp64, r64.data(), r64.size() * sizeof(r64[0]), hipMemcpyHostToDevice));
+  EMD_HIP(ctx, hipMemcpy(sn->rg_map_d, mp.data(), mp.size() * sizeof(mp[0]), hipMemcpyHostToDevice));
+  return EMD_OK;
+}
+
+template <class T>
+static const rg_rp<T> *rg_rp_of(const emd_snap *sn);
+template <>
+const rg_rp<float> *rg_rp_of<float>(const emd_snap *sn) { return sn->rg_rp32; }
+template <>
+const rg_rp<double> *rg_rp_of<double>(const emd_snap *sn) { return sn->rg_rp64; }
+
 static emd_status snap_upload_coeff(emd_ctx *ctx, emd_snap *sn) {
   const int twojmax = sn->t.twojmax;
   // Y weights: the three roles of each component (compute_dbidrj)
@@ -1755,6 +1819,7 @@ extern "C" emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, em
   EMD_HIP(ctx, hipMemcpy(sn->cg, sn->cg_h.data(), sn->cg_h.size() * sizeof(double),
                          hipMemcpyHostToDevice));
   emd_status st = snap_build_yflat(ctx, sn);
+  if (st == EMD_OK && twojmax <= SNAP_MAXJ) st = snap_build_rg(ctx, sn);
   if (st == EMD_OK) st = snap_upload_coeff(ctx, sn);
   if (st != EMD_OK) {
     emd_snap_destroy(ctx, sn);
@@ -1768,6 +1833,7 @@ extern "C" emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn) {
   if (!sn) return EMD_OK;
   if (ctx) hipStreamSynchronize(ctx->stream);
   void *ptrs[] = {sn->cg, sn->ytab, sn->etab, sn->nin, sn->nj, sn->nr, sn->par, sn->fdv,
+                  sn->rg_rp32, sn->rg_rp64, sn->rg_map_d,
                   sn->stab, sn->w_r,
                   sn->w_i, sn->sbuf,
                   sn->u_r, sn->u_i, sn->y_r, sn->y_i, sn->eatom,
@@ -1851,8 +1917,8 @@ template <class T>
 static emd_status launch_ui(emd_ctx *ctx, emd_snap *sn, int n, T *ur, T *ui) {
   if (sn->t.twojmax <= SNAP_MAXJ) {
     EMD_TRY(snap_par_ready<T>(ctx, sn, n));
-    EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui_rg<T, 1>), n, 64, 0, sn->t, sn->nin,
-               (const snap_par<T> *)sn->par, sn->cap_maxn, n, ur, ui);
+    EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui_rg<T, 1>), n, 64, 0, sn->t, rg_rp_of<T>(sn), sn->rg_map_d,
+               sn->nin, (const snap_par<T> *)sn->par, sn->cap_maxn, n, ur, ui);
   } else {
     EMD_LAUNCH(ctx, "snap_ui", (k_snap_ui<T, SNAP_JBIG>), n, 64 * UI_WAVES, 0, sn->t, sn->nin,
                sn->nr, sn->cap_maxn, n, ur, ui);
@@ -1888,8 +1954,9 @@ static emd_status launch_fij(emd_ctx *ctx, emd_snap *sn, int n, const T *yr, con
   if (sn->t.twojmax <= SNAP_MAXJ) {
     EMD_TRY(snap_par_ready<T>(ctx, sn, n));
     const dim3 grid(n, (sn->cap_maxn + RG_WAVES * RG_NB - 1) / (RG_WAVES * RG_NB));
-    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij_rev<T, TF>), grid, 64 * RG_WAVES, 0, sn->t, sn->nin,
-               sn->nj, sn->nr, (const snap_par<T> *)sn->par, sn->fdv, sn->cap_maxn, n, yr, yi, f);
+    EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij_rev<T, TF>), grid, 64 * RG_WAVES, 0, sn->t,
+               rg_rp_of<T>(sn), sn->rg_map_d, sn->nin, sn->nj, sn->nr,
+               (const snap_par<T> *)sn->par, sn->fdv, sn->cap_maxn, n, yr, yi, f);
   } else {
     const dim3 grid(n, (sn->cap_maxn + FIJ_WAVES - 1) / FIJ_WAVES);
     EMD_LAUNCH(ctx, "snap_fij", (k_snap_fij<T, TF, SNAP_JBIG>), grid, 64 * FIJ_WAVES, 0, sn->t,
