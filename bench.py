@@ -610,13 +610,16 @@ def main():
             progress(f"SNAP {sn} atoms {prec}: {sn * a.snap_steps / sw:.4g} atom-steps/s")
             snap[prec] = {"value": sn * a.snap_steps / sw, "ms_per_step": 1e3 * sw / a.snap_steps,
                           "snap_kernels_ms_per_step": {k: v for k, v in sb.items()
-                                                       if k in ("snap_ui", "snap_y", "snap_fij")},
+                                                       if k in ("snap_neigh", "snap_ui", "snap_y",
+                                                                "snap_fij")},
                           "kernels_ms_per_step_untimed_breakdown": sb,
                           "etot_per_atom_end": se}
         snap["mixed_over_fp64"] = snap["mixed"]["value"] / snap["fp64"]["value"]
         # VALU roofline of the SNAP kernels: loop-bound FLOPs of the Y form
-        # (examinimd_amd/flopcount.py; 26 neighbours inside the cutoff of BCC
-        # W at a = 3.1803) over the summed SNAP kernel time per step
+        # with the reverse-mode force (examinimd_amd/flopcount.py; 26
+        # neighbours inside the cutoff of BCC W at a = 3.1803) over the summed
+        # time per step of the four SNAP kernels (compaction with the
+        # neighbour parameters, U, Y, force)
         from examinimd_amd.flopcount import snap_flops
         fl = snap_flops(8, 26)["total"]
         for prec, peak in (("fp64", FP64_PEAK_TFLOPS), ("mixed", FP32_PEAK_TFLOPS)):

@@ -563,40 +563,17 @@ static_assert(yf_layout<double, SNAP_JBIG>::NUC * 8 < 65536 &&
                   yf_layout<float, SNAP_JBIG>::NUC * 8 < 65536,
               "U_tot byte offsets fit 16 bits");
 
+// The term stream of AT atoms whose U_tot copies are in LDS at sb (a copy
+// set of yf_layout<T,J>::UBYTES per atom), run sums in part: Y of atoms
+// i0 .. i0 + AT - 1 (those below n)
 template <class T, int J, int AT>
-__global__ void __launch_bounds__(YF_THREADS)
-k_snap_yflat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
-             const T *__restrict__ w, const unsigned short *__restrict__ meta,
-             const int *__restrict__ rstart, int n, const T *__restrict__ utot_r,
-             const T *__restrict__ utot_i, T *__restrict__ y_r, T *__restrict__ y_i) {
-  // the copies of U_tot from byte 0 (a static array: its LDS address is
-  // known when the kernel is compiled, so a code's offset is the whole LDS
-  // address), then the run sums
+__device__ __forceinline__ void yf_stream(const unsigned char *sb, cpair<T> *part, int i0, int n,
+                                          int ny, int nch, const uint32_t *__restrict__ code,
+                                          const T *__restrict__ w,
+                                          const unsigned short *__restrict__ meta,
+                                          const int *__restrict__ rstart, T *__restrict__ y_r,
+                                          T *__restrict__ y_i) {
   using L = yf_layout<T, J>;
-  static_assert(L::UBYTES % sizeof(cpair<T>) == 0, "run sums aligned");
-  constexpr int UP = L::UBYTES / sizeof(cpair<T>);  // pairs per atom's U_tot copies
-  __shared__ cpair<T> s_yf[AT * (UP + yf_maxruns<J>())];
-  unsigned char *sb = reinterpret_cast<unsigned char *>(s_yf);
-  const int i0 = blockIdx.x * AT;
-  if (i0 >= n) return;
-  for (int a = 0; a < AT; a++) {
-    const int i = i0 + a < n ? i0 + a : i0;
-    unsigned char *ub = sb + a * L::UBYTES;
-    for (int k = threadIdx.x; k < nu; k += YF_THREADS) {
-      const T vr = utot_r[(size_t)i * nu + k], vi = utot_i[(size_t)i * nu + k];
-#pragma unroll
-      for (int c = 0; c < L::NCOPY; c++) {
-        if (L::SOA) {
-          reinterpret_cast<T *>(ub)[L::base(c) + k] = vr;
-          reinterpret_cast<T *>(ub + L::IMB)[L::base(c) + k] = vi;
-        } else {
-          reinterpret_cast<cpair<T> *>(ub)[L::base(c) + k] = cpair<T>{vr, vi};
-        }
-      }
-    }
-  }
-  __syncthreads();
-  cpair<T> *part = s_yf + AT * UP;
   T ar[AT], ai[AT];
 #pragma unroll
   for (int a = 0; a < AT; a++) ar[a] = ai[a] = 0;
@@ -655,6 +632,42 @@ k_snap_yflat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
       y_i[(size_t)i * ny + e] = yi;
     }
   }
+}
+
+template <class T, int J, int AT>
+__global__ void __launch_bounds__(YF_THREADS)
+k_snap_yflat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
+             const T *__restrict__ w, const unsigned short *__restrict__ meta,
+             const int *__restrict__ rstart, int n, const T *__restrict__ utot_r,
+             const T *__restrict__ utot_i, T *__restrict__ y_r, T *__restrict__ y_i) {
+  // the copies of U_tot from byte 0 (a static array: its LDS address is
+  // known when the kernel is compiled, so a code's offset is the whole LDS
+  // address), then the run sums
+  using L = yf_layout<T, J>;
+  static_assert(L::UBYTES % sizeof(cpair<T>) == 0, "run sums aligned");
+  constexpr int UP = L::UBYTES / sizeof(cpair<T>);  // pairs per atom's U_tot copies
+  __shared__ cpair<T> s_yf[AT * (UP + yf_maxruns<J>())];
+  unsigned char *sb = reinterpret_cast<unsigned char *>(s_yf);
+  const int i0 = blockIdx.x * AT;
+  if (i0 >= n) return;
+  for (int a = 0; a < AT; a++) {
+    const int i = i0 + a < n ? i0 + a : i0;
+    unsigned char *ub = sb + a * L::UBYTES;
+    for (int k = threadIdx.x; k < nu; k += YF_THREADS) {
+      const T vr = utot_r[(size_t)i * nu + k], vi = utot_i[(size_t)i * nu + k];
+#pragma unroll
+      for (int c = 0; c < L::NCOPY; c++) {
+        if (L::SOA) {
+          reinterpret_cast<T *>(ub)[L::base(c) + k] = vr;
+          reinterpret_cast<T *>(ub + L::IMB)[L::base(c) + k] = vi;
+        } else {
+          reinterpret_cast<cpair<T> *>(ub)[L::base(c) + k] = cpair<T>{vr, vi};
+        }
+      }
+    }
+  }
+  __syncthreads();
+  yf_stream<T, J, AT>(sb, s_yf + AT * UP, i0, n, ny, nch, code, w, meta, rstart, y_r, y_i);
 }
 
 // term weights for the current coefficients: w = coef(key) * cg product
@@ -955,32 +968,16 @@ __device__ __forceinline__ void rg_fill(const rg_rp<T> *__restrict__ rp,
   }
 }
 
-// U_tot with the register-resident recursion (twojmax <= 8; SNA::compute_ui,
-// sna_impl.hpp:152-190): a block of UR_WAVES wavefronts per atom, seven
-// neighbours per wavefront at a time, lane ma of a neighbour's group holding
-// the entries (ma, mb) of every half-plane column (the layout above).
-// Each lane accumulates sfac * U of its entries over its group's neighbours
-// in registers; the sums are added over the seven groups of a wavefront by a
-// fixed tree of lane shifts (groups 4-6 onto 0-2, then 2-3 onto 0-1, then 1
-// onto 0) and over the wavefronts in order: deterministic.
-template <class T, int UR_WAVES>
-__global__ void __launch_bounds__(64 * UR_WAVES)
-k_snap_ui_rg(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *__restrict__ rgm,
-             const int *__restrict__ nin, const snap_par<T> *__restrict__ par, int maxn, int n,
-             T *__restrict__ utot_r, T *__restrict__ utot_i) {
-  __shared__ rg_rec<T> s_rec[RG_SLOTS * RG_W];
-  __shared__ T s_acc[UR_WAVES][2][RG_SLOTS * RG_W];
-  const int i = blockIdx.x;
-  if (i >= n) return;
-  rg_fill<T, 64 * UR_WAVES>(rgp, rgm, 0, nullptr, nullptr, i, s_rec);
-  __syncthreads();
-  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-  const int g = l / RG_W, ma = l - g * RG_W;  // g = 7 only for the idle lane 63
-  T acc[RG_SLOTS][2];
-#pragma unroll
-  for (int s = 0; s < RG_SLOTS; s++) acc[s][0] = acc[s][1] = 0;
-  const int cnt = nin[i];
-  for (int base = w * RG_NB; base < cnt; base += UR_WAVES * RG_NB) {
+// Sums over the neighbours base0, base0 + stride, ... (seven at a time) of
+// atom i of sfac * U for the lane's entries (ma, mb) of every column of every
+// layer (SNA::compute_ui, sna_impl.hpp:152-190, with the register layout
+// above), added to acc[column slot]
+template <class T>
+__device__ __forceinline__ void rg_u_sums(const snap_dev_tables &t, const rg_rec<T> *s_rec,
+                                          const snap_par<T> *__restrict__ par, int i, int maxn,
+                                          int cnt, int base0, int stride, int g, int ma,
+                                          T (&acc)[RG_SLOTS][2]) {
+  for (int base = base0; base < cnt; base += stride) {
     const int jj = base + g;
     const bool act = g < RG_NB && jj < cnt;
     const size_t o = (size_t)i * maxn + (act ? jj : base);
@@ -1020,7 +1017,13 @@ k_snap_ui_rg(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *
       }
     }
   }
-  // over the seven groups of the wavefront, into lanes 0..8
+}
+
+// acc summed over the seven groups of the wavefront by a fixed tree of lane
+// shifts (groups 4-6 onto 0-2, then 2-3 onto 0-1, then 1 onto 0); lanes
+// 0..8 store the sums at out[column slot * RG_W + lane]
+template <class T>
+__device__ __forceinline__ void rg_group_sum(T (&acc)[RG_SLOTS][2], int l, T *out_r, T *out_i) {
 #pragma unroll
   for (int s = 0; s < RG_SLOTS; s++) {
 #pragma unroll
@@ -1033,10 +1036,34 @@ k_snap_ui_rg(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *
       u = __shfl_down(v, RG_W, 64);
       if (l < RG_W) {
         v += u;
-        s_acc[w][q][s * RG_W + l] = v;
+        (q ? out_i : out_r)[s * RG_W + l] = v;
       }
     }
   }
+}
+
+// U_tot with the register-resident recursion (twojmax <= 8): a block of
+// UR_WAVES wavefronts per atom, wavefront w over the neighbours 7w, 7w +
+// 7 UR_WAVES, ...; the lane sums added over the groups and then over the
+// wavefronts in order: deterministic.
+template <class T, int UR_WAVES>
+__global__ void __launch_bounds__(64 * UR_WAVES)
+k_snap_ui_rg(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *__restrict__ rgm,
+             const int *__restrict__ nin, const snap_par<T> *__restrict__ par, int maxn, int n,
+             T *__restrict__ utot_r, T *__restrict__ utot_i) {
+  __shared__ rg_rec<T> s_rec[RG_SLOTS * RG_W];
+  __shared__ T s_acc[UR_WAVES][2][RG_SLOTS * RG_W];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  rg_fill<T, 64 * UR_WAVES>(rgp, rgm, 0, nullptr, nullptr, i, s_rec);
+  __syncthreads();
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int g = l / RG_W, ma = l - g * RG_W;  // g = 7 only for the idle lane 63
+  T acc[RG_SLOTS][2];
+#pragma unroll
+  for (int s = 0; s < RG_SLOTS; s++) acc[s][0] = acc[s][1] = 0;
+  rg_u_sums<T>(t, s_rec, par, i, maxn, nin[i], w * RG_NB, UR_WAVES * RG_NB, g, ma, acc);
+  rg_group_sum<T>(acc, l, s_acc[w][0], s_acc[w][1]);
   __syncthreads();
   // U_tot = wself * I + the wavefront sums in order; own entries and mirrors
   T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
@@ -1929,9 +1956,9 @@ static emd_status launch_ui(emd_ctx *ctx, emd_snap *sn, int n, T *ur, T *ui) {
 template <class T>
 static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
   T *ur = (T *)sn->u_r, *ui = (T *)sn->u_i, *yr = (T *)sn->y_r, *yi = (T *)sn->y_i;
-  EMD_TRY(launch_ui<T>(ctx, sn, n, ur, ui));
   const T *w = std::is_same<T, float>::value ? (const T *)sn->yf_w32 : (const T *)sn->yf_w64;
   const uint32_t *code = std::is_same<T, float>::value ? sn->yf_code32 : sn->yf_code64;
+  EMD_TRY(launch_ui<T>(ctx, sn, n, ur, ui));
   // twojmax <= 8: two atoms per block share each term's code and weight
   // load (SNAP-2k: 26.8 / 47.2 us mixed / FP64, against 31.5 / 60.9 with one
   // atom and 28.2 / 55.9 with three)

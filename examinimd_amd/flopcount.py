@@ -2,19 +2,29 @@
 atom and step, for the algorithmic side of the SNAP roofline.
 
 Counts the floating-point adds and multiplies (an FMA counts 2; sqrt,
-division and transcendentals 1 each) executed by the three kernels of
-emd_snap.hip for an atom with n_in neighbours inside the cutoff:
+division and transcendentals 1 each) executed by the four kernels of
+emd_snap.hip (twojmax <= 8) for an atom with n_in neighbours inside the
+cutoff:
 
-  U   k_snap_ui : per neighbour the Cayley-Klein set-up and, for every
-                  half-plane entry of every layer j, the two recursion terms
-                  (SNA::compute_uarray, sna_impl.hpp:663-695) and the sfac
-                  accumulation (add_uarraytot, sna_impl.hpp:578-604);
-  Y   k_snap_y  : per atom, for every half-plane entry (J, ma, mb) and every
-                  (a, b) pair with a non-zero role weight, the Clebsch-Gordan
-                  double sum of compute_zi (sna_impl.hpp:221-274) and the
-                  weighted accumulation into Y;
-  F   k_snap_fij: per neighbour the U and dU/dr recursion (compute_duarray,
-                  sna_impl.hpp:728-893) and the contraction with Y.
+  P   k_snap_neigh : per in-cutoff neighbour the Cayley-Klein parameters,
+                     their derivative factors, the switching function and
+                     the r^-14 repulsion, once;
+  U   k_snap_ui_rg : per neighbour, for every half-plane entry of every layer
+                     j, the two recursion terms (SNA::compute_uarray,
+                     sna_impl.hpp:663-695) and the sfac accumulation
+                     (add_uarraytot, sna_impl.hpp:578-604);
+  Y   k_snap_yflat : per atom, for every half-plane entry (J, ma, mb) and
+                     every (a, b) pair with a non-zero role weight, the
+                     Clebsch-Gordan double sum of compute_zi
+                     (sna_impl.hpp:221-274) and the weighted accumulation;
+  F   k_snap_fij_rev: per neighbour the force as the gradient of
+                     sum Re(conj(U) sfac w Y): the U recursion forward, the
+                     adjoint recursion backward with the two sums
+                     Sa = sum rpa conj(Ubar) P, Sb = sum rpb conj(Ubar) Q, and
+                     F_k = dsfac u_k sum Re(conj(U) w Y) + Re(conj(da_k) Sa)
+                     - Re(conj(db_k) Sb) (carrying dU/dr forward instead,
+                     compute_duarray, sna_impl.hpp:728-893, costs about
+                     twice as much).
 
 The reference's Z/dB form (compute_zi over all 125 triples, then dB/dr for
 all 55 components per neighbour) does about twice this work; roofline
@@ -59,9 +69,14 @@ def half_plane(j):
     return [(ma, mb) for mb in range(j // 2 + 1) for ma in range(j + 1)]
 
 
+def p_flops_per_neighbour(twojmax):
+    # r, theta0, sin/cos, z0 and dz0/dr, 1/r0 and its derivative, a, b,
+    # c1..c4, the unit vector, sfac and dsfac (times wj), r^-14 repulsion
+    return 70
+
+
 def u_flops_per_neighbour(twojmax):
-    setup = 30  # r, theta0, z0, Cayley-Klein a and b, switching function
-    f = setup
+    f = 0
     for j in range(1, twojmax + 1):
         for ma, mb in half_plane(j):
             if ma < j:
@@ -91,26 +106,30 @@ def y_flops_per_atom(twojmax):
 
 
 def f_flops_per_neighbour(twojmax):
-    setup = 60  # r, theta0 and its derivative, a, b and their gradients, sfac, dsfac
-    f = setup
+    f = 15  # da/dr and db/dr from c1..c4 and the unit vector
     for j in range(1, twojmax + 1):
         for ma, mb in half_plane(j):
-            if ma < j:
-                f += 10 + 54   # U term and three dU/dr components
-            if ma > 0:
-                f += 10 + 54
-            if not (2 * mb == j and ma > mb):
-                f += 29        # weighted contraction with Y, three components
-    f += 18 + 10  # wavefront reduction, r^-14 repulsion
+            weighted = not (2 * mb == j and ma > mb)
+            # forward: the recursion terms, Re(conj(U) w Y)
+            f += (10 if ma < j else 0) + (10 if ma > 0 else 0) + (4 if weighted else 0)
+            # backward: Sa / Sb terms, and (above layer 1) the adjoint sent to
+            # layer j-1; the entry's own term sfac w Y
+            f += (10 if ma < j else 0) + (10 if ma > 0 else 0)
+            if j >= 2:
+                f += (10 if ma < j else 0) + (10 if ma > 0 else 0)
+            if weighted and j < twojmax:
+                f += 4
+    f += 3 * 10 + 3 * 8 + 6  # F_k from the sums, the lane reduction, repulsion
     return f
 
 
 def snap_flops(twojmax=8, n_in=26):
     """per-atom-step FLOPs of the Y form, by kernel"""
+    p = n_in * p_flops_per_neighbour(twojmax)
     u = n_in * u_flops_per_neighbour(twojmax)
     y = y_flops_per_atom(twojmax)
     fij = n_in * f_flops_per_neighbour(twojmax)
-    return {"U": u, "Y": y, "F": fij, "total": u + y + fij,
+    return {"P": p, "U": u, "Y": y, "F": fij, "total": p + u + y + fij,
             "twojmax": twojmax, "n_in": n_in, "ncoeff": len(components(twojmax))}
 
 

@@ -23,5 +23,17 @@ def test_counts_scale_with_neighbours():
     a = flopcount.snap_flops(8, 26)
     b = flopcount.snap_flops(8, 52)
     assert b["Y"] == a["Y"]                      # per atom, not per neighbour
-    assert b["U"] == 2 * a["U"] and b["F"] == 2 * a["F"]
-    assert 0.9e6 < a["total"] < 1.2e6
+    assert b["U"] == 2 * a["U"] and b["F"] == 2 * a["F"] and b["P"] == 2 * a["P"]
+    assert 0.7e6 < a["total"] < 0.9e6
+
+
+def test_reverse_mode_force_costs_less_than_forward():
+    # carrying dU/dr forward costs per term 10 (U) + 54 (three dU components)
+    # plus a 29-FLOP contraction per weighted entry; the reverse-mode count
+    # must stay well below that
+    fwd = 0
+    for j in range(1, 9):
+        for ma, mb in flopcount.half_plane(j):
+            fwd += (64 if ma < j else 0) + (64 if ma > 0 else 0)
+            fwd += 29 if not (2 * mb == j and ma > mb) else 0
+    assert flopcount.f_flops_per_neighbour(8) < 0.55 * fwd
