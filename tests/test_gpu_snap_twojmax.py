@@ -31,7 +31,7 @@ def ncoeff(oracle, tj):
     return oracle.lib().or_snap_ncoeff(tj)
 
 
-@pytest.mark.parametrize("twojmax", [6, 10, 12])
+@pytest.mark.parametrize("twojmax", [2, 4, 6, 10, 12])
 def test_snap_twojmax_force_energy_against_oracle(oracle, twojmax):
     from examinimd_amd.pipeline import Pipeline
     from test_oracle import _energy_force
@@ -155,3 +155,42 @@ def test_snap_twojmax10_md_against_oracle(tmp_path, oracle):
     m.run(20)
     compare(app, m, 1e-12)
     app.close()
+
+
+@pytest.mark.parametrize("twojmax", [8, 6])
+def test_snap_dense_neighbourhoods_against_oracle(oracle, twojmax):
+    """compressed BCC W (a = 2.6 A, 0.05 A disorder): about 58 neighbours
+    inside the cutoff, so the force kernel runs three block rows of 28
+    neighbours per atom and the U kernel nine rounds of seven; FP64 forces
+    and energy against the oracle, the mixed path's force RMS error"""
+    from examinimd_amd.pipeline import Pipeline, lattice
+    from test_oracle import _energy_force
+    print(f"case: SNAP dense BCC W a = 2.6, twojmax {twojmax}", flush=True)
+    a, nc = 2.6, 5
+    beta = list(decks.snap_beta(seed=11 + twojmax, sigma=0.05, ncoeff=ncoeff(oracle, twojmax)))
+    x = lattice("bcc", nc, a) + np.random.default_rng(7).normal(0, 0.05, (2 * nc ** 3, 3))
+    box = nc * a
+    pl = Pipeline()
+    pl.load(x, (box,) * 3, mass=183.84)
+    pl.build(RC + 1.0, newton=1)
+    pl.snap_setup(beta, twojmax=twojmax)
+    f64 = pl.snap_force(emd.EMD_FP64)
+    e64 = pl.snap_energy()
+    f32 = pl.snap_force(emd.EMD_MIXED)
+    pl.close()
+    xw = x % box
+    d = xw[:, None, :] - xw[None, :, :]
+    d -= box * np.round(d / box)
+    nin = ((d ** 2).sum(-1) < RC * RC).sum(1) - 1
+    print("in-cutoff neighbours min / max", nin.min(), nin.max())
+    assert nin.min() > 28
+    E, F = _energy_force(oracle, xw, box, "snap",
+                         snap=oracle.snap_params(beta, twojmax=twojmax), cut=RC + 1.0)
+    print("force rel err", rel_err(f64, F), "energy", e64, E)
+    assert rel_err(f64, F) <= 1e-12
+    assert abs(e64 - E) <= 1e-12 * abs(E)
+    df = f32 - f64
+    rms = float(np.sqrt(np.mean(np.sum(df * df, axis=1))))
+    frms = float(np.sqrt(np.mean(np.sum(f64 * f64, axis=1))))
+    print("mixed force RMS error", rms, "force RMS", frms)
+    assert rms <= 1e-4 * frms
