@@ -20,6 +20,8 @@
 //                 (FP64 distance test, list order kept) with ballots and
 //                 writes each neighbour's Cayley-Klein parameters, switching
 //                 function and derivative factors (snap_par) once
+//   k_snap_neigh_ui (twojmax <= 8, force path) the compaction and U_tot of
+//                 k_snap_ui_rg in the same wavefront: one launch
 //   k_snap_ui_rg  (twojmax <= 8) one wavefront per atom, seven neighbours at
 //                 a time, nine lanes per neighbour: the layer recursion in
 //                 registers (lane ma holds column entries (ma, mb)), sums
@@ -208,18 +210,19 @@ __device__ __forceinline__ double snap_fdivr(double dx, double dy, double dz) {
 // into f come after this one).  With p32 or p64 given, the slot's neighbour
 // parameters (snap_par, FP32 or FP64) and the repulsion F/r (fdv) are
 // written as well.
-__global__ void __launch_bounds__(64)
-k_snap_neigh(snap_dev_tables t, const double *__restrict__ x, const int *__restrict__ row,
-             const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
-             double cutsq, int maxn, int *__restrict__ nin, int *__restrict__ nj,
-             double *__restrict__ nr, double *__restrict__ f64, float *__restrict__ f32,
-             int ntot, snap_par<float> *__restrict__ p32, snap_par<double> *__restrict__ p64,
-             double *__restrict__ fdv) {
+// (the body of k_snap_neigh for atom i and one wavefront; returns the count)
+__device__ __forceinline__ int snap_compact(const snap_dev_tables &t, int i, const double *__restrict__ x,
+                                            const int *__restrict__ row, const int *__restrict__ num,
+                                            const int *__restrict__ entries, int pitch, int n,
+                                            double cutsq, int maxn, int *__restrict__ nin,
+                                            int *__restrict__ nj, double *__restrict__ nr,
+                                            double *__restrict__ f64, float *__restrict__ f32,
+                                            int ntot, snap_par<float> *__restrict__ p32,
+                                            snap_par<double> *__restrict__ p64,
+                                            double *__restrict__ fdv) {
 #pragma clang fp contract(off)  // the distance test as the reference (the parameters are
                                 // computed in functions of their own)
-  const int i = blockIdx.x;
-  if (i >= n) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x % 64;
   if (f64 || f32) {
     for (int r = i + n * (lane / 3); r < ntot; r += n * 21) {  // lanes 0..62: 21 rows of 3
       if (lane < 63) {
@@ -263,6 +266,20 @@ k_snap_neigh(snap_dev_tables t, const double *__restrict__ x, const int *__restr
     run += __popcll(m);
   }
   if (lane == 0) nin[i] = run < maxn ? run : maxn;
+  return run < maxn ? run : maxn;
+}
+
+__global__ void __launch_bounds__(64)
+k_snap_neigh(snap_dev_tables t, const double *__restrict__ x, const int *__restrict__ row,
+             const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
+             double cutsq, int maxn, int *__restrict__ nin, int *__restrict__ nj,
+             double *__restrict__ nr, double *__restrict__ f64, float *__restrict__ f32,
+             int ntot, snap_par<float> *__restrict__ p32, snap_par<double> *__restrict__ p64,
+             double *__restrict__ fdv) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  snap_compact(t, i, x, row, num, entries, pitch, n, cutsq, maxn, nin, nj, nr, f64, f32, ntot, p32,
+               p64, fdv);
 }
 
 // the neighbour parameters of one precision (and the repulsion F/r) from the
@@ -1076,6 +1093,56 @@ k_snap_ui_rg(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *
       vr += s_acc[q][0][k];
       vi += s_acc[q][1][k];
     }
+    if (m.diag) vr += (T)t.wself;
+    ur[m.uo] = vr;
+    ui[m.uo] = vi;
+    if (m.um >= 0) {
+      const T sg = m.neg ? (T)-1 : (T)1;
+      ur[m.um] = sg * vr;
+      ui[m.um] = -sg * vi;
+    }
+  }
+}
+
+// The compaction of k_snap_neigh (with the neighbour parameters in T's
+// precision) followed by U_tot in the same wavefront (twojmax <= 8, one
+// wavefront per atom as k_snap_ui_rg<T, 1>): one launch less per force
+// evaluation, and the parameters just written are read back from the cache.
+template <class T>
+__global__ void __launch_bounds__(64)
+k_snap_neigh_ui(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *__restrict__ rgm,
+                const double *__restrict__ x, const int *__restrict__ row,
+                const int *__restrict__ num, const int *__restrict__ entries, int pitch, int n,
+                double cutsq, int maxn, int *__restrict__ nin, int *__restrict__ nj,
+                double *__restrict__ nr, double *__restrict__ f64, float *__restrict__ f32,
+                int ntot, snap_par<T> *__restrict__ par, double *__restrict__ fdv,
+                T *__restrict__ utot_r, T *__restrict__ utot_i) {
+  __shared__ rg_rec<T> s_rec[RG_SLOTS * RG_W];
+  __shared__ T s_acc[2][RG_SLOTS * RG_W];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  rg_fill<T, 64>(rgp, rgm, 0, nullptr, nullptr, i, s_rec);
+  snap_par<float> *p32 = nullptr;
+  snap_par<double> *p64 = nullptr;
+  if (sizeof(T) == 4) p32 = reinterpret_cast<snap_par<float> *>(par);
+  else p64 = reinterpret_cast<snap_par<double> *>(par);
+  const int cnt = snap_compact(t, i, x, row, num, entries, pitch, n, cutsq, maxn, nin, nj, nr, f64,
+                               f32, ntot, p32, p64, fdv);
+  __syncthreads();  // the wavefront's parameter writes, and the records, are visible
+  const int l = threadIdx.x;
+  const int g = l / RG_W, ma = l - g * RG_W;
+  T acc[RG_SLOTS][2];
+#pragma unroll
+  for (int q = 0; q < RG_SLOTS; q++) acc[q][0] = acc[q][1] = 0;
+  rg_u_sums<T>(t, s_rec, par, i, maxn, cnt, 0, RG_NB, g, ma, acc);
+  rg_group_sum<T>(acc, l, s_acc[0], s_acc[1]);
+  __syncthreads();
+  T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
+  for (int k = l; k < RG_SLOTS * RG_W; k += 64) {
+    const rg_map m = rgm[k];
+    if (m.uo < 0) continue;
+    T vr = s_acc[0][k];
+    const T vi = s_acc[1][k];
     if (m.diag) vr += (T)t.wself;
     ur[m.uo] = vr;
     ui[m.uo] = vi;
@@ -1936,6 +2003,31 @@ static emd_status snap_par_ready(emd_ctx *ctx, emd_snap *sn, int n) {
   return EMD_OK;
 }
 
+// twojmax <= 8: the compaction with the neighbour parameters in precision
+// prec and U_tot of every atom, in one launch (k_snap_neigh_ui)
+static emd_status snap_neigh_ui(emd_ctx *ctx, const emd_system *s, const emd_neigh *nl,
+                                emd_snap *sn, int zero_prec, int prec) {
+  const int n = s->n_local;
+  const int maxn = nl->maxneighs > 0 ? nl->maxneighs : 1;
+  EMD_TRY(snap_scratch(ctx, sn, n, maxn));
+  const double rcutmax = 2.0 * sn->radelem * sn->rcutfac;
+  const int *row = nl->layout == EMD_NEIGH_CSR ? nl->row : nullptr;
+  double *f64 = zero_prec == EMD_FP64 ? s->f : nullptr;
+  float *f32 = zero_prec == EMD_MIXED ? s->f32 : nullptr;
+  if (prec == EMD_MIXED)
+    EMD_LAUNCH(ctx, "snap_ui", k_snap_neigh_ui<float>, n, 64, 0, sn->t, sn->rg_rp32, sn->rg_map_d,
+               s->x, row, nl->num, nl->entries, nl->maxneighs, n, rcutmax * rcutmax, sn->cap_maxn,
+               sn->nin, sn->nj, sn->nr, f64, f32, s->n_local + s->n_ghost,
+               (snap_par<float> *)sn->par, sn->fdv, (float *)sn->u_r, (float *)sn->u_i);
+  else
+    EMD_LAUNCH(ctx, "snap_ui", k_snap_neigh_ui<double>, n, 64, 0, sn->t, sn->rg_rp64, sn->rg_map_d,
+               s->x, row, nl->num, nl->entries, nl->maxneighs, n, rcutmax * rcutmax, sn->cap_maxn,
+               sn->nin, sn->nj, sn->nr, f64, f32, s->n_local + s->n_ghost,
+               (snap_par<double> *)sn->par, sn->fdv, (double *)sn->u_r, (double *)sn->u_i);
+  sn->par_prec = prec;
+  return EMD_OK;
+}
+
 // U_tot of every atom: the register-resident kernel for twojmax <= 8 (one
 // wavefront per atom: SNAP-2k 18.6 us mixed / 23.5 us FP64, against 18.8 /
 // 32.8 with two and 26.8 / 47.9 with four wavefronts per atom), else one
@@ -1954,11 +2046,10 @@ static emd_status launch_ui(emd_ctx *ctx, emd_snap *sn, int n, T *ur, T *ui) {
 }
 
 template <class T>
-static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
+static emd_status snap_y(emd_ctx *ctx, emd_snap *sn, int n) {
   T *ur = (T *)sn->u_r, *ui = (T *)sn->u_i, *yr = (T *)sn->y_r, *yi = (T *)sn->y_i;
   const T *w = std::is_same<T, float>::value ? (const T *)sn->yf_w32 : (const T *)sn->yf_w64;
   const uint32_t *code = std::is_same<T, float>::value ? sn->yf_code32 : sn->yf_code64;
-  EMD_TRY(launch_ui<T>(ctx, sn, n, ur, ui));
   // twojmax <= 8: two atoms per block share each term's code and weight
   // load (SNAP-2k: 26.8 / 47.2 us mixed / FP64, against 31.5 / 60.9 with one
   // atom and 28.2 / 55.9 with three)
@@ -1969,6 +2060,12 @@ static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
     EMD_LAUNCH(ctx, "snap_y", (k_snap_yflat<T, SNAP_JBIG, 1>), n, YF_THREADS, 0, sn->t.nu, sn->t.ny,
                sn->yf_nch, code, w, sn->yf_meta, sn->yf_rstart, n, ur, ui, yr, yi);
   return EMD_OK;
+}
+
+template <class T>
+static emd_status snap_ui_y(emd_ctx *ctx, emd_snap *sn, int n) {
+  EMD_TRY(launch_ui<T>(ctx, sn, n, (T *)sn->u_r, (T *)sn->u_i));
+  return snap_y<T>(ctx, sn, n);
 }
 
 // F_ij for every (atom, neighbour): the register-resident reverse-mode
@@ -1999,13 +2096,20 @@ static emd_status snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, e
   const int n = s->n_local;
   if (n <= 0) return zero ? emd_force_zero(ctx, s, precision) : EMD_OK;
   const int prec = precision == EMD_MIXED ? EMD_MIXED : EMD_FP64;
-  EMD_TRY(snap_neigh(ctx, s, nl, sn, zero ? prec : -1, prec));
+  if (sn->t.twojmax <= SNAP_MAXJ) {
+    // compaction, neighbour parameters and U_tot in one launch, then Y
+    EMD_TRY(snap_neigh_ui(ctx, s, nl, sn, zero ? prec : -1, prec));
+    if (precision == EMD_MIXED) EMD_TRY(snap_y<float>(ctx, sn, n));
+    else EMD_TRY(snap_y<double>(ctx, sn, n));
+  } else {
+    EMD_TRY(snap_neigh(ctx, s, nl, sn, zero ? prec : -1, prec));
+    if (precision == EMD_MIXED) EMD_TRY(snap_ui_y<float>(ctx, sn, n));
+    else EMD_TRY(snap_ui_y<double>(ctx, sn, n));
+  }
   if (precision == EMD_MIXED) {
-    EMD_TRY(snap_ui_y<float>(ctx, sn, n));
     EMD_TRY((launch_fij<float, float>(ctx, sn, n, (const float *)sn->y_r,
                                       (const float *)sn->y_i, s->f32)));
   } else {
-    EMD_TRY(snap_ui_y<double>(ctx, sn, n));
     EMD_TRY((launch_fij<double, double>(ctx, sn, n, (const double *)sn->y_r,
                                         (const double *)sn->y_i, s->f)));
   }
