@@ -194,3 +194,30 @@ def test_snap_dense_neighbourhoods_against_oracle(oracle, twojmax):
     frms = float(np.sqrt(np.mean(np.sum(f64 * f64, axis=1))))
     print("mixed force RMS error", rms, "force RMS", frms)
     assert rms <= 1e-4 * frms
+
+
+@pytest.mark.parametrize("natoms", [1, 127])
+def test_snap_odd_atom_counts_against_oracle(oracle, natoms):
+    """an odd number of atoms (the Y kernel takes two atoms per block, so the
+    last block has one) and a single atom with no neighbours: FP64 forces
+    and energy against the oracle in both precisions' code paths"""
+    from examinimd_amd.pipeline import Pipeline
+    from test_oracle import _energy_force
+    print(f"case: SNAP {natoms} atoms (odd count)", flush=True)
+    beta = list(decks.snap_beta(seed=9, sigma=0.05))
+    x, box = shaken_bcc(4, 21)
+    x = x[:natoms]
+    pl = Pipeline()
+    pl.load(x, (box,) * 3, mass=183.84)
+    pl.build(RC + 1.0, newton=1)
+    pl.snap_setup(beta)
+    f64 = pl.snap_force(emd.EMD_FP64)
+    e64 = pl.snap_energy()
+    f32 = pl.snap_force(emd.EMD_MIXED)
+    pl.close()
+    E, F = _energy_force(oracle, x % box, box, "snap", snap=oracle.snap_params(beta), cut=RC + 1.0)
+    scale = max(np.abs(F).max(), 1.0)
+    print("force max abs err", np.abs(f64 - F).max(), "energy", e64, E)
+    assert np.abs(f64 - F).max() <= 1e-12 * scale
+    assert abs(e64 - E) <= 1e-12 * max(abs(E), 1.0)
+    assert np.abs(f32 - f64).max() <= 1e-4 * scale
