@@ -541,7 +541,10 @@ __device__ __forceinline__ void y_entry(const snap_dev_tables &t, int e, int &J,
 // read of every term the copy whose residue no other address of its
 // 32-lane group takes (a matching per group; with the shifts 0, 5, 11, 19
 // about 4 % of the groups keep one conflict).
-constexpr int YF_THREADS = 256;
+// threads per Y block (the stream is split this many ways): SNAP-2k 25.3 /
+// 43.7 us (mixed / FP64) against 27.0 / 47.3 with 256 and 28.7 / 47.2 with
+// 1024 (more padding, fewer blocks per CU)
+constexpr int YF_THREADS = 512;
 constexpr int YF_CHUNK = 16;
 
 template <class T>
