@@ -1039,17 +1039,17 @@ __device__ __forceinline__ void rg_u_sums(const snap_dev_tables &t, const rg_rec
   }
 }
 
-// acc summed over the seven groups of the wavefront by a fixed tree of lane
-// shifts (groups 4-6 onto 0-2, then 2-3 onto 0-1, then 1 onto 0); lanes
-// 0..8 store the sums at out[column slot * RG_W + lane]
-template <class T>
-__device__ __forceinline__ void rg_group_sum(T (&acc)[RG_SLOTS][2], int l, T *out_r, T *out_i) {
+// acc summed over the seven groups of the wavefront (in TA) by a fixed tree
+// of lane shifts (groups 4-6 onto 0-2, then 2-3 onto 0-1, then 1 onto 0);
+// lanes 0..8 store the sums at out[column slot * RG_W + lane]
+template <class T, class TA>
+__device__ __forceinline__ void rg_group_sum(T (&acc)[RG_SLOTS][2], int l, TA *out_r, TA *out_i) {
 #pragma unroll
   for (int s = 0; s < RG_SLOTS; s++) {
 #pragma unroll
     for (int q = 0; q < 2; q++) {
-      T v = acc[s][q];
-      T u = __shfl_down(v, 4 * RG_W, 64);
+      TA v = (TA)acc[s][q];
+      TA u = __shfl_down(v, 4 * RG_W, 64);
       if (l < 3 * RG_W) v += u;
       u = __shfl_down(v, 2 * RG_W, 64);
       if (l < 2 * RG_W) v += u;
@@ -1083,7 +1083,7 @@ k_snap_ui_rg(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *
 #pragma unroll
   for (int s = 0; s < RG_SLOTS; s++) acc[s][0] = acc[s][1] = 0;
   rg_u_sums<T>(t, s_rec, par, i, maxn, nin[i], w * RG_NB, UR_WAVES * RG_NB, g, ma, acc);
-  rg_group_sum<T>(acc, l, s_acc[w][0], s_acc[w][1]);
+  rg_group_sum<T, T>(acc, l, s_acc[w][0], s_acc[w][1]);
   __syncthreads();
   // U_tot = wself * I + the wavefront sums in order; own entries and mirrors
   T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
@@ -1138,7 +1138,7 @@ k_snap_neigh_ui(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_ma
 #pragma unroll
   for (int q = 0; q < RG_SLOTS; q++) acc[q][0] = acc[q][1] = 0;
   rg_u_sums<T>(t, s_rec, par, i, maxn, cnt, 0, RG_NB, g, ma, acc);
-  rg_group_sum<T>(acc, l, s_acc[0], s_acc[1]);
+  rg_group_sum<T, T>(acc, l, s_acc[0], s_acc[1]);
   __syncthreads();
   T *ur = utot_r + (size_t)i * t.nu, *ui = utot_i + (size_t)i * t.nu;
   for (int k = l; k < RG_SLOTS * RG_W; k += 64) {
@@ -2329,6 +2329,111 @@ k_snap_w(snap_dev_tables t, const int *__restrict__ nin, const int *__restrict__
   }
 }
 
+// W^i for every atom with the register-resident recursion (twojmax <= 8):
+//   W^i_J(e) = sum_{j in N(i)} w(e) [dsfac (g . u) U_J(e) + sfac dU_J(e)/dr . g],
+// g = g_i - g_j (a ghost j carries its owner's cotangent).  Only the
+// derivative along g is needed, so each lane carries U and dU_g = g . dU/dr
+// (4 values per entry): its recursion is that of U with da_g = g . da/dr and
+// db_g = g . db/dr, both rank one in u plus constants:
+//   da_g = c1 (g.u) + i (c2 (g.u) - r0inv g_z),  db_g = c3 (g.u) + r0inv g_y
+//                                                   + i (c4 (g.u) - r0inv g_x).
+// One wavefront per atom, seven neighbours at a time; each lane sums its
+// group's neighbours in T, the groups are added in FP64 (fixed tree):
+// deterministic.
+template <class T>
+__global__ void __launch_bounds__(64)
+k_snap_w_rg(snap_dev_tables t, const rg_rp<T> *__restrict__ rgp, const rg_map *__restrict__ rgm,
+            const int *__restrict__ nin, const int *__restrict__ nj,
+            const snap_par<T> *__restrict__ par, int maxn, int n, const int *__restrict__ root,
+            const double *__restrict__ g, double *__restrict__ w_r, double *__restrict__ w_i) {
+  __shared__ rg_rec<T> s_rec[RG_SLOTS * RG_W];
+  __shared__ double s_acc[2][RG_SLOTS * RG_W];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  rg_fill<T, 64>(rgp, rgm, 0, nullptr, nullptr, i, s_rec);
+  __syncthreads();
+  for (int k = threadIdx.x; k < RG_SLOTS * RG_W; k += 64)
+    s_rec[k].yr = (T)0.5 * (T)rgm[k].wq;  // the half-plane weight of the entry
+  __syncthreads();
+  const int l = threadIdx.x;
+  const int gr = l / RG_W, ma = l - gr * RG_W;  // gr = 7 only for the idle lane 63
+  T acc[RG_SLOTS][2];
+#pragma unroll
+  for (int s = 0; s < RG_SLOTS; s++) acc[s][0] = acc[s][1] = 0;
+  const double gi0 = g[3 * (size_t)i], gi1 = g[3 * (size_t)i + 1], gi2 = g[3 * (size_t)i + 2];
+  const int cnt = nin[i];
+  for (int base = 0; base < cnt; base += RG_NB) {
+    const int jj = base + gr;
+    const bool act = gr < RG_NB && jj < cnt;
+    const size_t o = (size_t)i * maxn + (act ? jj : base);
+    const snap_par<T> pp = par[o];
+    const int jn = nj[o];
+    const int own_j = jn < n ? jn : root[jn - n];
+    const T g0 = act ? (T)(gi0 - g[3 * (size_t)own_j]) : (T)0;
+    const T g1 = act ? (T)(gi1 - g[3 * (size_t)own_j + 1]) : (T)0;
+    const T g2 = act ? (T)(gi2 - g[3 * (size_t)own_j + 2]) : (T)0;
+    const T gu = g0 * pp.u[0] + g1 * pp.u[1] + g2 * pp.u[2];
+    const T a_r = pp.a_r, a_i = pp.a_i, b_r = pp.b_r, b_i = pp.b_i;
+    const T da_r = pp.c1 * gu, da_i = pp.c2 * gu - pp.r0inv * g2;
+    const T db_r = pp.c3 * gu + pp.r0inv * g1, db_i = pp.c4 * gu - pp.r0inv * g0;
+    const T sfac = pp.sfac, dsg = pp.dsfac * gu;  // dsg = 0 for an idle group
+    // c[mb]: U_r, U_i, dU_g r, dU_g i of (ma, mb)
+    T c[RG_COLS][4];
+#pragma unroll
+    for (int mb = 0; mb < RG_COLS; mb++) c[mb][0] = c[mb][1] = c[mb][2] = c[mb][3] = 0;
+    c[0][0] = ma == 0 ? (T)1 : (T)0;  // layer 0: U = 1, dU = 0
+    acc[0][0] += (T)0.5 * dsg * c[0][0];
+#pragma unroll
+    for (int j = 1; j <= SNAP_MAXJ; j++) {
+      if (j > t.twojmax) break;
+      T mir[4], mirb[4];  // even j: (U, dU_g) of (ma, j/2) and (ma-1, j/2) in layer j-1
+      if ((j & 1) == 0) {
+        const int src = gr * RG_W + (j - 1 - ma > 0 ? j - 1 - ma : 0);
+        const T sg = ((ma + j / 2) & 1) ? (T)-1 : (T)1;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const T v = __shfl(c[j / 2 - 1][q], src, 64);
+          mir[q] = (q & 1) ? -sg * v : sg * v;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) mirb[q] = lane_below(mir[q]);
+      }
+#pragma unroll
+      for (int mb = 0; mb <= j / 2; mb++) {
+        const int slot = rg_slot0(j) + mb;
+        const rg_rec<T> q = s_rec[slot * RG_W + ma];
+        const bool mid = (j & 1) == 0 && mb == j / 2;
+        T P[4], Q[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          P[k] = mid ? mir[k] : c[mb][k];
+          Q[k] = mid ? mirb[k] : lane_below(c[mb][k]);
+        }
+        const T ur = q.rpa * (a_r * P[0] + a_i * P[1]) - q.rpb * (b_r * Q[0] + b_i * Q[1]);
+        const T ui = q.rpa * (a_r * P[1] - a_i * P[0]) - q.rpb * (b_r * Q[1] - b_i * Q[0]);
+        const T dr = q.rpa * (da_r * P[0] + da_i * P[1] + a_r * P[2] + a_i * P[3]) -
+                     q.rpb * (db_r * Q[0] + db_i * Q[1] + b_r * Q[2] + b_i * Q[3]);
+        const T di = q.rpa * (da_r * P[1] - da_i * P[0] + a_r * P[3] - a_i * P[2]) -
+                     q.rpb * (db_r * Q[1] - db_i * Q[0] + b_r * Q[3] - b_i * Q[2]);
+        c[mb][0] = ur;
+        c[mb][1] = ui;
+        c[mb][2] = dr;
+        c[mb][3] = di;
+        acc[slot][0] += q.yr * (dsg * ur + sfac * dr);
+        acc[slot][1] += q.yr * (dsg * ui + sfac * di);
+      }
+    }
+  }
+  rg_group_sum<T, double>(acc, l, s_acc[0], s_acc[1]);
+  __syncthreads();
+  for (int k = l; k < RG_SLOTS * RG_W; k += 64) {
+    const rg_map m = rgm[k];
+    if (m.ey < 0) continue;
+    w_r[(size_t)i * t.ny + m.ey] = s_acc[0][k];
+    w_i[(size_t)i * t.ny + m.ey] = s_acc[1][k];
+  }
+}
+
 // S_{abJ} += sum' Re(conj(W^i_J) Z^i_{abJ}) over atoms (FP64 accumulation)
 template <class T>
 __global__ void __launch_bounds__(192)
@@ -2412,9 +2517,12 @@ static emd_status snap_adjoint_impl(emd_ctx *ctx, const emd_system *s, const emd
                                     emd_snap *sn, const double *g, double *S) {
   const int n = s->n_local;
   EMD_TRY(launch_ui<T>(ctx, sn, n, (T *)sn->u_r, (T *)sn->u_i));
-  if (sn->t.twojmax <= SNAP_MAXJ)
-    EMD_LAUNCH(ctx, "snap_adjoint_w", (k_snap_w<T, SNAP_MAXJ>), n, 64, 0, sn->t, sn->nin, sn->nj,
-               sn->nr, sn->cap_maxn, n, h ? h->root : nullptr, g, sn->w_r, sn->w_i);
+  if (sn->t.twojmax <= SNAP_MAXJ) {
+    EMD_TRY(snap_par_ready<T>(ctx, sn, n));
+    EMD_LAUNCH(ctx, "snap_adjoint_w", k_snap_w_rg<T>, n, 64, 0, sn->t, rg_rp_of<T>(sn),
+               sn->rg_map_d, sn->nin, sn->nj, (const snap_par<T> *)sn->par, sn->cap_maxn, n,
+               h ? h->root : nullptr, g, sn->w_r, sn->w_i);
+  }
   else
     EMD_LAUNCH(ctx, "snap_adjoint_w", (k_snap_w<T, SNAP_JBIG>), n, 64, 0, sn->t, sn->nin, sn->nj,
                sn->nr, sn->cap_maxn, n, h ? h->root : nullptr, g, sn->w_r, sn->w_i);
