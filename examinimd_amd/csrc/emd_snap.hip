@@ -2620,7 +2620,8 @@ template <class T>
 static emd_status snap_adjoint_impl(emd_ctx *ctx, const emd_system *s, const emd_halo *h,
                                     emd_snap *sn, const double *g, double *S) {
   const int n = s->n_local;
-  EMD_TRY(launch_ui<T>(ctx, sn, n, (T *)sn->u_r, (T *)sn->u_i));
+  // twojmax <= 8: U_tot came with the compaction (k_snap_neigh_ui)
+  if (sn->t.twojmax > SNAP_MAXJ) EMD_TRY(launch_ui<T>(ctx, sn, n, (T *)sn->u_r, (T *)sn->u_i));
   if (sn->t.twojmax <= SNAP_MAXJ) {
     EMD_TRY(snap_par_ready<T>(ctx, sn, n));
     EMD_LAUNCH(ctx, "snap_adjoint_w", k_snap_w_rg<T>, n, 64, 0, sn->t, rg_rp_of<T>(sn),
@@ -2654,7 +2655,9 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
   if (n <= 0) return EMD_OK;
   EMD_ARG(ctx, s->n_ghost == 0 || (h && h->root && h->n_ghost == s->n_ghost),
           "ghost atoms need the halo (owner of each ghost)");
-  EMD_TRY(snap_neigh(ctx, s, nl, sn, -1, precision == EMD_MIXED ? EMD_MIXED : EMD_FP64));
+  const int aprec = precision == EMD_MIXED ? EMD_MIXED : EMD_FP64;
+  if (sn->t.twojmax <= SNAP_MAXJ) EMD_TRY(snap_neigh_ui(ctx, s, nl, sn, -1, aprec));
+  else EMD_TRY(snap_neigh(ctx, s, nl, sn, -1, aprec));
   EMD_TRY(emd_ensure(ctx, (void **)&sn->w_r, &sn->cap_w, (size_t)n * sn->t.ny * sizeof(double)));
   EMD_TRY(emd_ensure(ctx, (void **)&sn->w_i, &sn->cap_wi, (size_t)n * sn->t.ny * sizeof(double)));
   // sbuf: S[SNAP_MAXT] | Bsum[SNAP_MAXC] | sjstart ints (twojmax + 2 of them)
