@@ -129,6 +129,8 @@ HIP_PROTOS = {
     "emd_nve_initial": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_double, C.c_int]),
     "emd_nve_final": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_int]),
     "emd_nve_final_initial": (C.c_int, [vp, P(EmdSystem), C.c_double, C.c_double, C.c_int]),
+    "emd_nve_step_boundary": (C.c_int, [vp, P(EmdSystem), P(EmdHalo), C.c_int, C.c_int,
+                                        C.c_double, C.c_double, C.c_int]),
     "emd_reduce_mv2": (C.c_int, [vp, P(EmdSystem), P(C.c_double)]),
     "emd_snap_create": (C.c_int, [vp, P(EmdSnapParams), P(vp)]),
     "emd_snap_destroy": (C.c_int, [vp, vp]),

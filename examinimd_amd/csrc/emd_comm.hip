@@ -260,16 +260,7 @@ __global__ void k_halo_update(double *__restrict__ x, const int *__restrict__ ro
                               double ly, double lz) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_ghost) return;
-  const int r = root[g];
-  const int img = image[g];
-  const double L[3] = {lx, ly, lz};
-  const size_t dst = 3 * (size_t)(n_local + g), src = 3 * (size_t)r;
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const int c = (img >> (2 * d)) & 3;
-    const double xs = x[src + d];
-    x[dst + d] = (c == 0) ? xs : xs + (c == 1 ? -L[d] : L[d]);
-  }
+  emd_halo_update_at(g, x, root, image, n_local, lx, ly, lz);
 }
 
 extern "C" emd_status emd_comm_update_halo(emd_ctx *ctx, emd_system *s, const emd_halo *h) {
@@ -294,54 +285,33 @@ __global__ void k_halo_fold(T *__restrict__ f, const int *__restrict__ pack, int
   f[3 * (size_t)i + 2] += fz;
 }
 
-// all six phases in one launch of one workgroup, phases in reverse order with
-// a workgroup barrier between them (a phase reads ghost forces that later
-// phases folded into it); per atom the same additions in the same order as
-// the per-phase launches.  Within a phase the destinations pack[p][k] are
-// distinct atoms below the phase's ghost range, so no two lanes write one
-// address and no lane reads what another writes.
-struct fold_phases {
-  const int *pack[6];
-  int num[6];
-  int off[6];
-};
-constexpr int FOLD_ONE_THREADS = 1024;
-constexpr int FOLD_ONE_MAX = 16384;  // ghosts; above this the per-phase launches are faster
-
+// all six phases in one launch of one workgroup (emd_fold_phases_one): per
+// atom the same additions in the same order as the per-phase launches
 template <class T>
 __global__ void __launch_bounds__(FOLD_ONE_THREADS)
 k_halo_fold_one(T *__restrict__ f, fold_phases ph) {
-  for (int p = 5; p >= 0; p--) {
-    const int n = ph.num[p], off = ph.off[p];
-    const int *__restrict__ pack = ph.pack[p];
-    for (int ii = threadIdx.x; ii < n; ii += FOLD_ONE_THREADS) {
-      const int i = pack[ii];
-      const size_t g = 3 * (size_t)(off + ii);
-      const T fx = f[g], fy = f[g + 1], fz = f[g + 2];
-      f[3 * (size_t)i] += fx;
-      f[3 * (size_t)i + 1] += fy;
-      f[3 * (size_t)i + 2] += fz;
-    }
-    __syncthreads();
+  emd_fold_phases_one<T, FOLD_ONE_THREADS>(f, ph);
+}
+
+long emd_fold_phases_of(const emd_system *s, const emd_halo *h, fold_phases &ph) {
+  long total = 0;
+  for (int p = 0; p < 6; p++) {
+    ph.pack[p] = h->pack[p];
+    ph.num[p] = h->num_ghost[p] > 0 ? h->num_ghost[p] : 0;
+    ph.off[p] = p == 0 ? s->n_local : ph.off[p - 1] + h->num_ghost[p - 1];
+    total += ph.num[p];
   }
+  return total;
 }
 
 extern "C" emd_status emd_comm_update_force(emd_ctx *ctx, emd_system *s, const emd_halo *h,
                                             int precision) {
   if (!ctx || !s || !h) return EMD_ERR_ARG;
-  int off[6];
-  off[0] = s->n_local;
-  for (int p = 1; p < 6; p++) off[p] = off[p - 1] + h->num_ghost[p - 1];
-  long total = 0;
-  for (int p = 0; p < 6; p++) total += h->num_ghost[p] > 0 ? h->num_ghost[p] : 0;
+  fold_phases ph;
+  const long total = emd_fold_phases_of(s, h, ph);
   if (total == 0) return EMD_OK;
+  const int *off = ph.off;
   if (total <= FOLD_ONE_MAX && ctx->fold_one) {
-    fold_phases ph;
-    for (int p = 0; p < 6; p++) {
-      ph.pack[p] = h->pack[p];
-      ph.num[p] = h->num_ghost[p] > 0 ? h->num_ghost[p] : 0;
-      ph.off[p] = off[p];
-    }
     if (precision == EMD_MIXED)
       EMD_LAUNCH(ctx, "comm_update_force", k_halo_fold_one<float>, 1, FOLD_ONE_THREADS, 0,
                  s->f32, ph);

@@ -384,6 +384,85 @@ __device__ __forceinline__ T group_sum(T v) {
 // ------------------------------------------------------------ misc helpers
 inline int emd_blocks(long n, int threads) { return (int)((n + threads - 1) / threads); }
 
+// ------------------------------------------- step-boundary pieces (device)
+// CommSerial::update_force phases of a halo: pack lists and the first ghost
+// index of each phase (comm_serial.h:200-213)
+struct fold_phases {
+  const int *pack[6];
+  int num[6];
+  int off[6];
+};
+
+// all six phases folded by one workgroup of NT threads, phases in reverse
+// order with a workgroup barrier between them (a phase reads ghost forces
+// that later phases folded into it).  Within a phase the destinations
+// pack[p][k] are distinct atoms below the phase's ghost range, so no two
+// lanes write one address and no lane reads what another writes.
+template <class T, int NT>
+__device__ __forceinline__ void emd_fold_phases_one(T *__restrict__ f, const fold_phases &ph) {
+  for (int p = 5; p >= 0; p--) {
+    const int n = ph.num[p], off = ph.off[p];
+    const int *__restrict__ pack = ph.pack[p];
+    for (int ii = threadIdx.x; ii < n; ii += NT) {
+      const int i = pack[ii];
+      const size_t g = 3 * (size_t)(off + ii);
+      const T fx = f[g], fy = f[g + 1], fz = f[g + 2];
+      f[3 * (size_t)i] += fx;
+      f[3 * (size_t)i + 1] += fy;
+      f[3 * (size_t)i + 2] += fz;
+    }
+    __syncthreads();
+  }
+}
+
+// IntegratorNVE FinalIntegrateFunctor of one step then InitialIntegrateFunctor
+// of the next for atom i (src/integrator_nve.cpp:104-112 then 65-74): two
+// velocity half-kicks and the drift, in the reference's operation order
+template <bool MIXED>
+__device__ __forceinline__ void emd_nve_final_initial_at(
+    int i, double *__restrict__ x, double *__restrict__ v, const double *__restrict__ f,
+    const float *__restrict__ f32, const int *__restrict__ type, const double *__restrict__ mass,
+    double dtf, double dtv) {
+#pragma clang fp contract(off)
+  const double dtfm = dtf / mass[type[i]];
+  const size_t o = 3 * (size_t)i;
+  const double f0 = MIXED ? (double)f32[o] : f[o];
+  const double f1 = MIXED ? (double)f32[o + 1] : f[o + 1];
+  const double f2 = MIXED ? (double)f32[o + 2] : f[o + 2];
+  double v0 = v[o] + dtfm * f0, v1 = v[o + 1] + dtfm * f1, v2 = v[o + 2] + dtfm * f2;
+  v0 = v0 + dtfm * f0;
+  v1 = v1 + dtfm * f1;
+  v2 = v2 + dtfm * f2;
+  v[o] = v0; v[o + 1] = v1; v[o + 2] = v2;
+  x[o] += dtv * v0;
+  x[o + 1] += dtv * v1;
+  x[o + 2] += dtv * v2;
+}
+
+// CommSerial::TagHaloUpdateSelf (comm_serial.h:184-197) for ghost g: its
+// owner's position plus the periodic shift of each shifted dimension (image
+// code 1: -L, 2: +L), one addition per dimension
+__device__ __forceinline__ void emd_halo_update_at(int g, double *__restrict__ x,
+                                                   const int *__restrict__ root,
+                                                   const int *__restrict__ image, int n_local,
+                                                   double lx, double ly, double lz) {
+  const int r = root[g];
+  const int img = image[g];
+  const double L[3] = {lx, ly, lz};
+  const size_t dst = 3 * (size_t)(n_local + g), src = 3 * (size_t)r;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int c = (img >> (2 * d)) & 3;
+    const double xs = x[src + d];
+    x[dst + d] = (c == 0) ? xs : xs + (c == 1 ? -L[d] : L[d]);
+  }
+}
+
+// fold_phases of a halo (emd_comm.hip); returns the number of ghosts
+long emd_fold_phases_of(const emd_system *s, const emd_halo *h, fold_phases &ph);
+constexpr int FOLD_ONE_THREADS = 1024;
+constexpr int FOLD_ONE_MAX = 16384;  // ghosts; above this the per-phase launches are faster
+
 // bin grid of a neighbour build: interior bins are the non-halo ones, one
 // block per interior bin; the 27-bin stencil order is dx, dy, dz = -1..1
 // with dz fastest (stencil slot 13 is the bin itself)

@@ -901,22 +901,66 @@ __global__ void k_nve_final_initial(double *__restrict__ x, double *__restrict__
                                     const int *__restrict__ type,
                                     const double *__restrict__ mass, int n, double dtf,
                                     double dtv) {
-#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double dtfm = dtf / mass[type[i]];
-  const size_t o = 3 * (size_t)i;
-  const double f0 = MIXED ? (double)f32[o] : f[o];
-  const double f1 = MIXED ? (double)f32[o + 1] : f[o + 1];
-  const double f2 = MIXED ? (double)f32[o + 2] : f[o + 2];
-  double v0 = v[o] + dtfm * f0, v1 = v[o + 1] + dtfm * f1, v2 = v[o + 2] + dtfm * f2;
-  v0 = v0 + dtfm * f0;
-  v1 = v1 + dtfm * f1;
-  v2 = v2 + dtfm * f2;
-  v[o] = v0; v[o + 1] = v1; v[o + 2] = v2;
-  x[o] += dtv * v0;
-  x[o + 1] += dtv * v1;
-  x[o + 2] += dtv * v2;
+  emd_nve_final_initial_at<MIXED>(i, x, v, f, f32, type, mass, dtf, dtv);
+}
+
+// The step boundary of a small system in one workgroup: CommSerial's
+// update_force of step s (comm_serial.h:200-213; src/examinimd.cpp:243),
+// IntegratorNVE's final_integrate of step s and initial_integrate of step
+// s+1 (src/examinimd.cpp:249 then 196) and, when s+1 rebuilds no list,
+// CommSerial's update_halo of the new positions (comm_serial.h:184-197;
+// src/examinimd.cpp:226).  Workgroup barriers separate the three parts, so
+// every atom sees the same operations in the same order as the separate
+// launches (k_halo_fold_one, k_nve_final_initial, k_halo_update): the same
+// bits, in one launch instead of three.
+template <bool MIXED>
+__global__ void __launch_bounds__(FOLD_ONE_THREADS)
+k_step_boundary_one(double *__restrict__ x, double *__restrict__ v, double *__restrict__ f,
+                    float *__restrict__ f32, const int *__restrict__ type,
+                    const double *__restrict__ mass, int n, double dtf, double dtv,
+                    fold_phases ph, const int *__restrict__ root, const int *__restrict__ image,
+                    int n_halo, double lx, double ly, double lz) {
+  if (MIXED) emd_fold_phases_one<float, FOLD_ONE_THREADS>(f32, ph);
+  else emd_fold_phases_one<double, FOLD_ONE_THREADS>(f, ph);
+  for (int i = threadIdx.x; i < n; i += FOLD_ONE_THREADS)
+    emd_nve_final_initial_at<MIXED>(i, x, v, f, f32, type, mass, dtf, dtv);
+  if (n_halo <= 0) return;
+  __syncthreads();
+  for (int g = threadIdx.x; g < n_halo; g += FOLD_ONE_THREADS)
+    emd_halo_update_at(g, x, root, image, n, lx, ly, lz);
+}
+
+constexpr int STEP_ONE_MAX = 16384;  // local atoms of a one-workgroup step boundary
+
+extern "C" emd_status emd_nve_step_boundary(emd_ctx *ctx, emd_system *s, const emd_halo *h,
+                                            int fold, int update_halo, double dtf, double dtv,
+                                            int precision) {
+  if (!ctx || !s || !h) return EMD_ERR_ARG;
+  const int n = s->n_local;
+  fold_phases ph;
+  long ghosts = emd_fold_phases_of(s, h, ph);
+  if (!fold)
+    for (int p = 0; p < 6; p++) ph.num[p] = 0;
+  const int n_halo = update_halo ? h->n_ghost : 0;
+  if (!ctx->fold_one || n > STEP_ONE_MAX || ghosts > FOLD_ONE_MAX || n_halo > FOLD_ONE_MAX) {
+    // larger systems: the separate launches (their grids fill the GPU)
+    if (fold) EMD_TRY(emd_comm_update_force(ctx, s, h, precision));
+    EMD_TRY(emd_nve_final_initial(ctx, s, dtf, dtv, precision));
+    if (update_halo) EMD_TRY(emd_comm_update_halo(ctx, s, h));
+    return EMD_OK;
+  }
+  if (n <= 0 && n_halo <= 0 && (!fold || ghosts == 0)) return EMD_OK;
+  if (precision == EMD_MIXED)
+    EMD_LAUNCH(ctx, "step_boundary", k_step_boundary_one<true>, 1, FOLD_ONE_THREADS, 0, s->x,
+               s->v, s->f, s->f32, s->type, s->mass, n, dtf, dtv, ph, h->root, h->image, n_halo,
+               s->domain[0], s->domain[1], s->domain[2]);
+  else
+    EMD_LAUNCH(ctx, "step_boundary", k_step_boundary_one<false>, 1, FOLD_ONE_THREADS, 0, s->x,
+               s->v, s->f, s->f32, s->type, s->mass, n, dtf, dtv, ph, h->root, h->image, n_halo,
+               s->domain[0], s->domain[1], s->domain[2]);
+  return EMD_OK;
 }
 
 extern "C" emd_status emd_nve_final_initial(emd_ctx *ctx, emd_system *s, double dtf, double dtv,

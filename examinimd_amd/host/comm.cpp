@@ -8,6 +8,7 @@ void Comm::init() {}
 void Comm::exchange() {}
 void Comm::exchange_halo() {}
 void Comm::update_halo() {}
+const emd_halo* Comm::device_halo() const { return nullptr; }
 void Comm::update_force() {}
 void Comm::reduce_float(T_FLOAT*, T_INT) {}
 void Comm::reduce_int(T_INT*, T_INT) {}

@@ -22,6 +22,9 @@ public:
   virtual void update_halo();
   // Reverse communication of forces
   virtual void update_force();
+  // the device halo that update_force / update_halo use, for a fused step
+  // boundary (null: this module's exchanges cannot be fused)
+  virtual const emd_halo* device_halo() const;
   virtual void reduce_float(T_FLOAT* values, T_INT N);
   virtual void reduce_int(T_INT* values, T_INT N);
   virtual void reduce_max_float(T_FLOAT* values, T_INT N);

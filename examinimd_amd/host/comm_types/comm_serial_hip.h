@@ -27,6 +27,7 @@ public:
   void update_force() override;
   const char* name() override;
   const emd_halo* halo() const { return &h; }
+  const emd_halo* device_halo() const override { return &h; }
 };
 #endif
 #endif

@@ -154,20 +154,34 @@ void ExaMiniMD::run(int nsteps) {
   };
 
   // the final half-kick of a step whose end nothing observes is deferred and
-  // fused with the next step's initial integration (same arithmetic)
-  bool deferred_final = false;
+  // fused with the next step's initial integration (same arithmetic); with
+  // newton on, the ghost-force fold before it is deferred too, and the
+  // integrator may run fold, kicks, drift and (without a rebuild) the halo
+  // position update as one device step boundary (same operations, same order)
+  bool deferred_final = false, deferred_fold = false;
   for (int step = 1; step <= nsteps; step++) {
+    const bool rebuild = step % input->comm_exchange_rate == 0 && step > 0;
+    bool halo_updated = false;
     other_timer.reset();
     {
       TraceRange r("Integrator::initial_integrate");
-      if (deferred_final) integrator->final_initial_integrate();
-      else integrator->initial_integrate();
+      if (deferred_final) {
+        if (integrator->final_initial_integrate_boundary(comm->device_halo(), deferred_fold,
+                                                         !rebuild)) {
+          halo_updated = !rebuild;
+        } else {
+          if (deferred_fold) comm->update_force();
+          integrator->final_initial_integrate();
+        }
+      } else {
+        integrator->initial_integrate();
+      }
     }
-    deferred_final = false;
+    deferred_final = deferred_fold = false;
     fence();
     other_time += other_timer.seconds();
 
-    if (step % input->comm_exchange_rate == 0 && step > 0) {
+    if (rebuild) {
       comm_timer.reset();
       {
         TraceRange r("Comm::exchange");
@@ -200,7 +214,7 @@ void ExaMiniMD::run(int nsteps) {
       }
       fence();
       neigh_time += neigh_timer.seconds();
-    } else {
+    } else if (!halo_updated) {
       comm_timer.reset();
       {
         TraceRange r("Comm::update_halo");
@@ -220,23 +234,28 @@ void ExaMiniMD::run(int nsteps) {
     fence();
     force_time += force_timer.seconds();
 
-    if (input->comm_newton) {
-      comm_timer.reset();
-      {
-        TraceRange r("Comm::update_force");
-        comm->update_force();
-      }
-      fence();
-      comm_time += comm_timer.seconds();
-    }
-
-    other_timer.reset();
     const bool observed =
         (input->thermo_rate > 0 && step % input->thermo_rate == 0) ||
         (input->dumpbinaryflag && input->dumpbinary_rate > 0 && step % input->dumpbinary_rate == 0) ||
         (input->correctnessflag && input->correctness_rate > 0 &&
          step % input->correctness_rate == 0);
-    if (fuse_integrate && step < nsteps && !observed) {
+    const bool defer = fuse_integrate && step < nsteps && !observed;
+    if (input->comm_newton) {
+      if (defer && comm->device_halo()) {
+        deferred_fold = true;
+      } else {
+        comm_timer.reset();
+        {
+          TraceRange r("Comm::update_force");
+          comm->update_force();
+        }
+        fence();
+        comm_time += comm_timer.seconds();
+      }
+    }
+
+    other_timer.reset();
+    if (defer) {
       deferred_final = true;
     } else {
       TraceRange r("Integrator::final_integrate");

@@ -10,6 +10,7 @@ void Integrator::final_initial_integrate() {
   final_integrate();
   initial_integrate();
 }
+bool Integrator::final_initial_integrate_boundary(const emd_halo*, bool, bool) { return false; }
 
 IntegratorNVEHIP::IntegratorNVEHIP(System* s) : Integrator(s) {
   dtv = system->dt;
@@ -27,4 +28,12 @@ void IntegratorNVEHIP::final_integrate() {
 void IntegratorNVEHIP::final_initial_integrate() {
   EMD_CALL(system->ctx,
            emd_nve_final_initial(system->ctx, system->view(), dtf, dtv, system->precision));
+}
+
+bool IntegratorNVEHIP::final_initial_integrate_boundary(const emd_halo* h, bool fold,
+                                                        bool update_halo) {
+  if (!h) return false;
+  EMD_CALL(system->ctx, emd_nve_step_boundary(system->ctx, system->view(), h, fold ? 1 : 0,
+                                              update_halo ? 1 : 0, dtf, dtv, system->precision));
+  return true;
 }

@@ -15,6 +15,10 @@ public:
   // final_integrate of one step then initial_integrate of the next, for a
   // step boundary with nothing in between; a module may fuse the two
   virtual void final_initial_integrate();
+  // the whole step boundary on the device: the halo's force fold (if
+  // `fold`), final_integrate, initial_integrate and the halo's position
+  // update (if `update_halo`); false if this module cannot (nothing done)
+  virtual bool final_initial_integrate_boundary(const emd_halo* h, bool fold, bool update_halo);
 };
 
 #include "modules_integrator.h"

@@ -12,5 +12,6 @@ public:
   void initial_integrate() override;
   void final_integrate() override;
   void final_initial_integrate() override;
+  bool final_initial_integrate_boundary(const emd_halo* h, bool fold, bool update_halo) override;
 };
 #endif

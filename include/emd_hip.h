@@ -287,6 +287,17 @@ emd_status emd_nve_final(emd_ctx *ctx, emd_system *s, double dtf, int precision)
  * launch; results bitwise equal to the two calls */
 emd_status emd_nve_final_initial(emd_ctx *ctx, emd_system *s, double dtf, double dtv,
                                  int precision);
+/* The step boundary as the driver runs it when nothing observes it:
+ * comm->update_force() of step s if `fold` (src/examinimd.cpp:243,
+ * comm_serial.h:200-213), final_integrate of s and initial_integrate of s+1
+ * (src/examinimd.cpp:249 then 196), and comm->update_halo() of s+1 if
+ * `update_halo` (src/examinimd.cpp:226, comm_serial.h:184-197).  Up to 16384
+ * local atoms and 16384 ghosts the three run in one workgroup launch
+ * separated by barriers; larger systems get the three separate launches.
+ * Results are bitwise those of emd_comm_update_force, emd_nve_final_initial
+ * and emd_comm_update_halo called in turn. */
+emd_status emd_nve_step_boundary(emd_ctx *ctx, emd_system *s, const emd_halo *h, int fold,
+                                 int update_halo, double dtf, double dtv, int precision);
 
 /* Temperature / KinE reductions, src/property_temperature.cpp:43-62,
  * src/property_kine.cpp:43-61: returns sum_i m_i v_i^2 (compensated) */
