@@ -75,15 +75,25 @@ struct halo_phase_op {
 // All six exchange_halo phases in one launch of one workgroup, for small
 // systems where the per-phase scans are launch- and sync-bound.  Each phase
 // is the same stable compaction as the scan path (ascending source index,
-// halo_phase_op's predicate and emit), 1024 candidates per round; a device
-// fence and a barrier separate phases, whose candidates include the ghosts
-// the previous phases wrote.  Capacities (system arrays, pack lists, the
-// root/image tables) are checked per ghost: a phase that does not fit is
-// still counted but not written, and the kernel stops there, reporting the
-// phase; the host grows and runs it again.
+// halo_phase_op's predicate), over up to 16 rounds of 1024 candidates:
+// every round's predicate first, one ballot count per (round, wavefront),
+// one scan of those counts, then the emits.  A phase writes only the ghost
+// indices (pack list, root, image code); the predicate of a ghost candidate
+// reads its root's coordinate, which equals the ghost's own in every
+// dimension a later phase tests (a ghost is shifted only in the dimensions
+// of the phases that made it, and the -d phase does not take the +d
+// phase's ghosts).  After the six phases one pass writes every ghost's
+// position (its root's plus one shift per shifted dimension, the single
+// rounding the phase-by-phase copy performs), velocity, charge, id and type.
+// Capacities (system arrays, pack lists, the root/image tables) are checked
+// per ghost: a phase that does not fit is still counted but not written, and
+// the kernel stops there, reporting the phase; the host grows and runs it
+// again.
 constexpr int HALO_ONE_THREADS = 1024;
 constexpr int HALO_ONE_MAX = 16384;  // local + ghost atoms; above, the per-phase scans
 constexpr int HALO_ONE_ROUNDS = HALO_ONE_MAX / HALO_ONE_THREADS;
+constexpr int HALO_ONE_WAVES = HALO_ONE_THREADS / EMD_WAVE;
+static_assert(HALO_ONE_ROUNDS * HALO_ONE_WAVES <= 4 * EMD_WAVE, "one wavefront scans the counts");
 
 struct halo_one_args {
   halo_phase_op op[6];
@@ -94,52 +104,122 @@ struct halo_one_args {
 };
 
 __global__ void __launch_bounds__(HALO_ONE_THREADS) k_halo_one(halo_one_args a) {
-  __shared__ int s_w[HALO_ONE_THREADS / EMD_WAVE];
+  __shared__ int s_cnt[HALO_ONE_ROUNDS * HALO_ONE_WAVES];
+  __shared__ int s_tot;
   const int tid = threadIdx.x, wid = tid / EMD_WAVE, lane = tid % EMD_WAVE;
+  const int n_local = a.n_local;
   int n_ghost = 0, prev = 0, failed = 0;
   for (int phase = 0; phase < 6 && !failed; phase++) {
-    halo_phase_op op = a.op[phase];
-    const int ncand = a.n_local + n_ghost - ((phase & 1) ? prev : 0);
+    const halo_phase_op &op = a.op[phase];
+    const int ncand = n_local + n_ghost - ((phase & 1) ? prev : 0);
     if (ncand > HALO_ONE_MAX) {  // more candidates than one workgroup's rounds
       failed = 7;
       break;
     }
-    op.dst0 = a.n_local + n_ghost;
-    const int room = min(min(a.n_max - op.dst0, a.cap_pack[phase]), a.root_cap - n_ghost);
-    // the predicates of all rounds first (their loads in flight together)
+    const int dst0 = n_local + n_ghost;
+    const int room = min(min(a.n_max - dst0, a.cap_pack[phase]), a.root_cap - n_ghost);
+    const int nr = (ncand + HALO_ONE_THREADS - 1) / HALO_ONE_THREADS;
+    // the predicates of all rounds first (their loads in flight together);
+    // a ghost candidate is tested on its root's coordinate
     unsigned flags = 0;
 #pragma unroll
     for (int r = 0; r < HALO_ONE_ROUNDS; r++) {
       const int i = r * HALO_ONE_THREADS + tid;
-      if (i < ncand && op.value(i)) flags |= 1u << r;
-    }
-    int k = 0;
-    for (int base = 0, r = 0; base < ncand; base += HALO_ONE_THREADS, r++) {
-      const int i = base + tid;
-      const int v = (flags >> r) & 1;
-      const int inc = emd_wave_incl_scan(v);
-      if (lane == EMD_WAVE - 1) s_w[wid] = inc;
-      __syncthreads();
-      int woff = 0, tot = 0;
-#pragma unroll
-      for (int w = 0; w < HALO_ONE_THREADS / EMD_WAVE; w++) {
-        const int sw = s_w[w];
-        woff += (w < wid) ? sw : 0;
-        tot += sw;
+      if (i < ncand) {
+        const int src = i < n_local ? i : op.root[i - n_local];
+        const double xd = op.x[3 * (size_t)src + op.dim];
+        if (op.hi_side ? (xd >= op.thr) : (xd <= op.thr)) flags |= 1u << r;
       }
-      const int pos = k + woff + inc - v;
-      if (v && pos < room) op.emit(i, pos, v);
-      k += tot;
-      __syncthreads();  // s_w is rewritten by the next round
+    }
+    // kept candidates per (round, wavefront), then their exclusive scan in
+    // (round, wavefront) order, which is ascending candidate order
+#pragma unroll
+    for (int r = 0; r < HALO_ONE_ROUNDS; r++) {
+      if (r < nr) {
+        const unsigned long long m = __ballot((flags >> r) & 1);
+        if (lane == 0) s_cnt[r * HALO_ONE_WAVES + wid] = __popcll(m);
+      }
+    }
+    __syncthreads();
+    if (wid == 0) {
+      const int nc = nr * HALO_ONE_WAVES;
+      int c[4], sum = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int t = 4 * lane + q;
+        c[q] = t < nc ? s_cnt[t] : 0;
+        sum += c[q];
+      }
+      const int inc = emd_wave_incl_scan(sum);
+      int run = inc - sum;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int t = 4 * lane + q;
+        if (t < nc) s_cnt[t] = run;
+        run += c[q];
+      }
+      if (lane == EMD_WAVE - 1) s_tot = inc;
+    }
+    __syncthreads();
+    const int k = s_tot;
+    // emits: the sources' root and image first (ghost sources only), then
+    // the pack entries and the new ghosts' root and image
+    int sroot[HALO_ONE_ROUNDS], simg[HALO_ONE_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < HALO_ONE_ROUNDS; r++) {
+      const int i = r * HALO_ONE_THREADS + tid;
+      sroot[r] = i;
+      simg[r] = 0;
+      if (((flags >> r) & 1) && i >= n_local) {
+        sroot[r] = op.root[i - n_local];
+        simg[r] = op.image[i - n_local];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < HALO_ONE_ROUNDS; r++) {
+      if (r < nr) {
+        const int v = (flags >> r) & 1;
+        const unsigned long long m = __ballot(v);
+        if (v) {
+          const int pos = s_cnt[r * HALO_ONE_WAVES + wid] + __popcll(m & emd_lanemask_lt());
+          if (pos < room) {
+            const int gi = n_ghost + pos;
+            op.pack[pos] = r * HALO_ONE_THREADS + tid;
+            op.root[gi] = sroot[r];
+            op.image[gi] = simg[r] | op.code;
+          }
+        }
+      }
     }
     if (tid == 0) a.out[phase] = k;
     if (k > room) failed = phase + 1;
     prev = k;
     n_ghost += k;
-    __threadfence();   // this phase's ghosts are the next phase's candidates
+    __threadfence();   // this phase's roots are read by the next phases
     __syncthreads();
   }
   if (tid == 0) a.out[6] = failed;
+  if (failed) return;
+  // the ghosts' data from their roots (CommSerial::TagHaloSelf copies, the
+  // position as k_halo_update forms it)
+  const halo_phase_op &op = a.op[0];
+  // box lengths: the -d phase's shift is +L_d (image code 2), the +d
+  // phase's -L_d (code 1)
+  const double L[3] = {a.op[1].shift, a.op[3].shift, a.op[5].shift};
+  for (int g = tid; g < n_ghost; g += HALO_ONE_THREADS) {
+    const int r = op.root[g], img = op.image[g];
+    const size_t dst = 3 * (size_t)(n_local + g), src = 3 * (size_t)r;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const int c = (img >> (2 * d)) & 3;
+      const double xs = op.x[src + d];
+      op.x[dst + d] = (c == 0) ? xs : xs + (c == 1 ? -L[d] : L[d]);
+      op.vel[dst + d] = op.vel[src + d];
+    }
+    op.q[n_local + g] = op.q[r];
+    op.id[n_local + g] = op.id[r];
+    op.type[n_local + g] = op.type[r];
+  }
 }
 
 static emd_status exchange_halo_one(emd_ctx *ctx, emd_system *s, double comm_depth,
@@ -154,7 +234,7 @@ static emd_status exchange_halo_one(emd_ctx *ctx, emd_system *s, double comm_dep
       op.hi_side = hi_side;
       op.thr = hi_side ? s->hi[dim] - comm_depth : s->lo[dim] + comm_depth;
       op.shift = hi_side ? -s->domain[dim] : s->domain[dim];
-      op.dst0 = 0;  // set per phase by the kernel
+      op.dst0 = 0;  // unused: k_halo_one places each phase itself
       op.n_local = n_local;
       op.code = (hi_side ? 1 : 2) << (2 * dim);
       op.x = s->x; op.vel = s->v; op.q = s->q; op.type = s->type; op.id = s->id;

@@ -198,7 +198,7 @@ def test_ghost_force_fold_against_numpy(monkeypatch, nc, cut, mixed):
 def test_halo_exchange_one_launch_equals_scans(monkeypatch, nc, cut):
     """exchange_halo of a small system in one workgroup launch gives the
     ghosts of the six per-phase scans: counts, pack lists, positions,
-    velocities, ids, roots and image codes, bit for bit (including the first
+    velocities, ids, types, charges, roots and image codes, bit for bit (including the first
     call, which starts with no capacity and grows).  BCC 8^3 with a 20 A
     halo depth has ~27 000 local + ghost atoms, more than one workgroup
     handles: that call falls back to the scans mid-way."""
@@ -223,10 +223,14 @@ def test_halo_exchange_one_launch_equals_scans(monkeypatch, nc, cut):
         xs = np.zeros(3 * tot)
         vs = np.zeros(3 * tot)
         ids = np.zeros(tot, np.int32)
+        types = np.full(tot, -1, np.int32)
+        qs = np.full(tot, np.nan)
         pl.chk(L.emd_system_download(c, s, tot, xs.ctypes.data_as(C.POINTER(C.c_double)),
-                                     vs.ctypes.data_as(C.POINTER(C.c_double)), None, None,
-                                     ids.ctypes.data_as(C.POINTER(C.c_int)), None), "download")
-        rec["x"], rec["v"], rec["id"] = xs, vs, ids
+                                     vs.ctypes.data_as(C.POINTER(C.c_double)), None,
+                                     types.ctypes.data_as(C.POINTER(C.c_int)),
+                                     ids.ctypes.data_as(C.POINTER(C.c_int)),
+                                     qs.ctypes.data_as(C.POINTER(C.c_double))), "download")
+        rec["x"], rec["v"], rec["id"], rec["type"], rec["q"] = xs, vs, ids, types, qs
         for name, ptr, cnt in [("root", pl.h.root, ng), ("image", pl.h.image, ng)] + \
                 [(f"pack{p}", pl.h.pack[p], pl.h.num_ghost[p]) for p in range(6)]:
             arr = np.zeros(cnt, np.int32)
