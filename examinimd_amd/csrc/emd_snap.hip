@@ -1456,6 +1456,8 @@ struct emd_snap {
   unsigned short *yf_key = nullptr, *yf_meta = nullptr;
   unsigned short *yf_centry = nullptr;  // entry of each (chunk, thread), 0 if none
   unsigned short *yf_keytid = nullptr;  // adjoint triple id of each key
+  double *s_part = nullptr;  // the beta adjoint's per-block key sums (k_snap_s_flat)
+  size_t cap_spart = 0;
   double *yf_cgw = nullptr, *yf_w64 = nullptr, *yf_kcoef = nullptr;
   float *yf_w32 = nullptr;
   int *yf_rstart = nullptr;
@@ -1941,7 +1943,7 @@ extern "C" emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn) {
                   sn->w_i, sn->sbuf,
                   sn->u_r, sn->u_i, sn->y_r, sn->y_i, sn->eatom,
                   sn->yf_code32, sn->yf_code64, sn->yf_key, sn->yf_meta, sn->yf_cgw,
-                  sn->yf_centry, sn->yf_keytid,
+                  sn->yf_centry, sn->yf_keytid, sn->s_part,
                   sn->yf_w64, sn->yf_w32, sn->yf_rstart, sn->yf_kcoef};
   for (void *q : ptrs)
     if (q) hipFree(q);
@@ -2480,7 +2482,9 @@ k_snap_s(snap_dev_tables t, const double *__restrict__ cg, const stab_entry *__r
 // same bank-shifted U_tot copies as k_snap_yflat; AT atoms per block share
 // each term's loads, W of the chunk's entry is read once per chunk, a run of
 // one key within a chunk is summed in a register and added to the block's
-// LDS sums by one atomic, and those go to S once per key and block.
+// LDS sums by one atomic, and the block stores its key sums to its row of
+// part (k_snap_s_reduce adds the rows up: no global atomics on the few key
+// addresses, and a fixed order).
 template <class T, int J, int AT>
 __global__ void __launch_bounds__(YF_THREADS)
 k_snap_s_flat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
@@ -2488,7 +2492,7 @@ k_snap_s_flat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
               const unsigned short *__restrict__ centry, const unsigned short *__restrict__ keytid,
               int nkeys, int n, const T *__restrict__ utot_r, const T *__restrict__ utot_i,
               const double *__restrict__ w_r, const double *__restrict__ w_i,
-              double *__restrict__ S) {
+              double *__restrict__ part) {
   using L = yf_layout<T, J>;
   static_assert(L::UBYTES % sizeof(cpair<T>) == 0, "copies aligned");
   constexpr int UP = L::UBYTES / sizeof(cpair<T>);
@@ -2568,7 +2572,25 @@ k_snap_s_flat(int nu, int ny, int nch, const uint32_t *__restrict__ code,
     atomicAdd(&s_S[run_key], run);
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < nkeys; k += YF_THREADS) atomicAdd(&S[keytid[k]], s_S[k]);
+  for (int k = threadIdx.x; k < nkeys; k += YF_THREADS) part[(size_t)blockIdx.x * nkeys + k] = s_S[k];
+}
+
+// S of each key: its column of the per-block sums, added by one block in a
+// fixed order (strided partial sums, then a tree)
+__global__ void __launch_bounds__(256)
+k_snap_s_reduce(const double *__restrict__ part, int nrows, int nkeys,
+                const unsigned short *__restrict__ keytid, double *__restrict__ S) {
+  __shared__ double s_v[256];
+  const int k = blockIdx.x;
+  double v = 0.0;
+  for (int r = threadIdx.x; r < nrows; r += 256) v += part[(size_t)r * nkeys + k];
+  s_v[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s_v[threadIdx.x] += s_v[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S[keytid[k]] += s_v[0];  // one key per block: no other writer
 }
 
 // sum over atoms of every bispectrum component B_k (dE/dbeta_k), FP64
@@ -2634,10 +2656,16 @@ static emd_status snap_adjoint_impl(emd_ctx *ctx, const emd_system *s, const emd
   const int ntrip = (int)sn->tid.size();
   if (sn->t.twojmax <= SNAP_MAXJ) {
     const uint32_t *code = std::is_same<T, float>::value ? sn->yf_code32 : sn->yf_code64;
-    EMD_LAUNCH(ctx, "snap_adjoint_s", (k_snap_s_flat<T, SNAP_MAXJ, 2>), (n + 1) / 2, YF_THREADS, 0,
+    const int nb = (n + 1) / 2, nkeys = (int)sn->yf_keys.size();
+    EMD_TRY(emd_ensure(ctx, (void **)&sn->s_part, &sn->cap_spart,
+                       (size_t)nb * nkeys * sizeof(double)));
+    EMD_LAUNCH(ctx, "snap_adjoint_s", (k_snap_s_flat<T, SNAP_MAXJ, 2>), nb, YF_THREADS, 0,
                sn->t.nu, sn->t.ny, sn->yf_nch, code, sn->yf_key, sn->yf_cgw, sn->yf_centry,
-               sn->yf_keytid, (int)sn->yf_keys.size(), n, (const T *)sn->u_r, (const T *)sn->u_i,
-               sn->w_r, sn->w_i, S);
+               sn->yf_keytid, nkeys, n, (const T *)sn->u_r, (const T *)sn->u_i, sn->w_r, sn->w_i,
+               sn->s_part);
+    if (nkeys > 0)
+      EMD_LAUNCH(ctx, "snap_adjoint_s", k_snap_s_reduce, nkeys, 256, 0, sn->s_part, nb, nkeys,
+                 sn->yf_keytid, S);
   } else {
     EMD_LAUNCH(ctx, "snap_adjoint_s", k_snap_s<T>, n, 192, 0, sn->t, sn->cg, sn->stab,
                (const int *)(sn->sbuf + SNAP_MAXT + SNAP_MAXC), ntrip, n, (const T *)sn->u_r,
