@@ -135,6 +135,7 @@ HIP_PROTOS = {
     "emd_snap_create": (C.c_int, [vp, P(EmdSnapParams), P(vp)]),
     "emd_snap_destroy": (C.c_int, [vp, vp]),
     "emd_snap_set_coeff": (C.c_int, [vp, vp, dptr]),
+    "emd_snap_static_geometry": (C.c_int, [vp, vp, C.c_int]),
     "emd_snap_force": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, C.c_int]),
     "emd_snap_force_zeroed": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, C.c_int]),
     "emd_snap_energy": (C.c_int, [vp, P(EmdSystem), P(EmdNeigh), vp, P(C.c_double), dptr]),

@@ -1458,6 +1458,13 @@ struct emd_snap {
   unsigned short *yf_keytid = nullptr;  // adjoint triple id of each key
   double *s_part = nullptr;  // the beta adjoint's per-block key sums (k_snap_s_flat)
   size_t cap_spart = 0;
+  // emd_snap_static_geometry: the compaction, neighbour parameters and U_tot
+  // of the last force or adjoint call (twojmax <= 8) are reused while the
+  // caller keeps positions and list fixed: their precision (-1: none), atom
+  // count, and the position and list arrays they came from
+  int geom_static = 0, geom_prec = -1, geom_n = -1;
+  const double *geom_x = nullptr;
+  const int *geom_ent = nullptr;
   double *yf_cgw = nullptr, *yf_w64 = nullptr, *yf_kcoef = nullptr;
   float *yf_w32 = nullptr;
   int *yf_rstart = nullptr;
@@ -1951,6 +1958,13 @@ extern "C" emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn) {
   return EMD_OK;
 }
 
+extern "C" emd_status emd_snap_static_geometry(emd_ctx *ctx, emd_snap *sn, int on) {
+  if (!ctx || !sn) return EMD_ERR_ARG;
+  sn->geom_static = on != 0;
+  sn->geom_prec = -1;
+  return EMD_OK;
+}
+
 extern "C" emd_status emd_snap_set_coeff(emd_ctx *ctx, emd_snap *sn, const double *coeff) {
   if (!ctx || !sn || !coeff) return EMD_ERR_ARG;
   EMD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -2101,6 +2115,22 @@ static emd_status launch_fij(emd_ctx *ctx, emd_snap *sn, int n, const T *yr, con
   return EMD_OK;
 }
 
+// with static geometry on: are the compaction, neighbour parameters and U_tot
+// in precision prec those of this system and list?
+static bool snap_geom_cached(const emd_snap *sn, const emd_system *s, const emd_neigh *nl,
+                             int prec) {
+  return sn->geom_static && sn->geom_prec == prec && sn->par_prec == prec &&
+         sn->geom_n == s->n_local && sn->geom_x == s->x && sn->geom_ent == nl->entries;
+}
+
+static void snap_geom_record(emd_snap *sn, const emd_system *s, const emd_neigh *nl, int prec) {
+  if (!sn->geom_static) return;
+  sn->geom_prec = prec;
+  sn->geom_n = s->n_local;
+  sn->geom_x = s->x;
+  sn->geom_ent = nl->entries;
+}
+
 static emd_status snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, emd_snap *sn,
                              int precision, bool zero) {
   if (!ctx || !s || !nl || !sn) return EMD_ERR_ARG;
@@ -2109,8 +2139,14 @@ static emd_status snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, e
   if (n <= 0) return zero ? emd_force_zero(ctx, s, precision) : EMD_OK;
   const int prec = precision == EMD_MIXED ? EMD_MIXED : EMD_FP64;
   if (sn->t.twojmax <= SNAP_MAXJ) {
-    // compaction, neighbour parameters and U_tot in one launch, then Y
-    EMD_TRY(snap_neigh_ui(ctx, s, nl, sn, zero ? prec : -1, prec));
+    // compaction, neighbour parameters and U_tot in one launch (or, with
+    // static geometry, those of the previous call), then Y
+    if (snap_geom_cached(sn, s, nl, prec)) {
+      if (zero) EMD_TRY(emd_force_zero(ctx, s, prec));
+    } else {
+      EMD_TRY(snap_neigh_ui(ctx, s, nl, sn, zero ? prec : -1, prec));
+      snap_geom_record(sn, s, nl, prec);
+    }
     if (precision == EMD_MIXED) EMD_TRY(snap_y<float>(ctx, sn, n));
     else EMD_TRY(snap_y<double>(ctx, sn, n));
   } else {
@@ -2144,6 +2180,7 @@ extern "C" emd_status emd_snap_energy(emd_ctx *ctx, const emd_system *s, const e
   const int n = s->n_local;
   *energy = 0.0;
   if (n <= 0) return EMD_OK;
+  sn->geom_prec = -1;  // the compaction and U_tot are rewritten
   EMD_TRY(snap_neigh(ctx, s, nl, sn, -1, EMD_FP64));
   EMD_TRY(launch_ui<double>(ctx, sn, n, (double *)sn->u_r, (double *)sn->u_i));
   double *ea = eatom;
@@ -2684,8 +2721,14 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
   EMD_ARG(ctx, s->n_ghost == 0 || (h && h->root && h->n_ghost == s->n_ghost),
           "ghost atoms need the halo (owner of each ghost)");
   const int aprec = precision == EMD_MIXED ? EMD_MIXED : EMD_FP64;
-  if (sn->t.twojmax <= SNAP_MAXJ) EMD_TRY(snap_neigh_ui(ctx, s, nl, sn, -1, aprec));
-  else EMD_TRY(snap_neigh(ctx, s, nl, sn, -1, aprec));
+  if (sn->t.twojmax <= SNAP_MAXJ) {
+    if (!snap_geom_cached(sn, s, nl, aprec)) {
+      EMD_TRY(snap_neigh_ui(ctx, s, nl, sn, -1, aprec));
+      snap_geom_record(sn, s, nl, aprec);
+    }
+  } else {
+    EMD_TRY(snap_neigh(ctx, s, nl, sn, -1, aprec));
+  }
   EMD_TRY(emd_ensure(ctx, (void **)&sn->w_r, &sn->cap_w, (size_t)n * sn->t.ny * sizeof(double)));
   EMD_TRY(emd_ensure(ctx, (void **)&sn->w_i, &sn->cap_wi, (size_t)n * sn->t.ny * sizeof(double)));
   // sbuf: S[SNAP_MAXT] | Bsum[SNAP_MAXC] | sjstart ints (twojmax + 2 of them)
@@ -2699,6 +2742,7 @@ extern "C" emd_status emd_snap_adjoint_beta(emd_ctx *ctx, const emd_system *s, c
   else EMD_TRY(snap_adjoint_impl<double>(ctx, s, h, sn, dLdF, S));
   // dE/dbeta_k = sum_i B_k^i in FP64
   if (dLdE != 0.0) {
+    sn->geom_prec = -1;  // U_tot is rewritten in FP64
     EMD_TRY(launch_ui<double>(ctx, sn, n, (double *)sn->u_r, (double *)sn->u_i));
     EMD_LAUNCH(ctx, "snap_adjoint_bsum", k_snap_bsum, n, 256, 0, sn->t, sn->cg, sn->etab,
                sn->ncoeff, sn->bzeroflag, n, (const double *)sn->u_r, (const double *)sn->u_i, B,
@@ -2725,6 +2769,7 @@ extern "C" emd_status emd_snap_descriptors(emd_ctx *ctx, const emd_system *s, co
   if (!ctx || !s || !nl || !sn || !B) return EMD_ERR_ARG;
   const int n = s->n_local;
   if (n <= 0) return EMD_OK;
+  sn->geom_prec = -1;  // the compaction and U_tot are rewritten
   EMD_TRY(snap_neigh(ctx, s, nl, sn, -1, EMD_FP64));
   EMD_TRY(launch_ui<double>(ctx, sn, n, (double *)sn->u_r, (double *)sn->u_i));
   EMD_LAUNCH(ctx, "snap_descriptors", k_snap_bsum, n, 256, 0, sn->t, sn->cg, sn->etab,

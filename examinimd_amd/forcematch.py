@@ -267,6 +267,9 @@ class SnapForceMatchMerged:
         self.R = np.add.reduceat(m._dl(eatom, n), m.starts)
         self._g = m._put(np.zeros(3 * n))
         self.pl.snap_set_coeff(beta_true)
+        # the configurations never move: the compaction, neighbour parameters
+        # and U_tot depend on them only, so force and adjoint calls reuse them
+        m.chk(L.emd_snap_static_geometry(c, self.pl.snap, 1), "static geometry")
         self.Ft = self._forces(EMD_FP64)
         self.Et = self.energies(beta_true)
 

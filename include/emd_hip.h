@@ -319,6 +319,14 @@ typedef struct emd_snap emd_snap;
 emd_status emd_snap_create(emd_ctx *ctx, const emd_snap_params *p, emd_snap **out);
 emd_status emd_snap_destroy(emd_ctx *ctx, emd_snap *sn);
 emd_status emd_snap_set_coeff(emd_ctx *ctx, emd_snap *sn, const double *coeff);
+/* No counterpart in the reference (it serves repeated force and beta-adjoint
+ * evaluations of fixed configurations, e.g. force matching): with on != 0
+ * the caller promises that positions, atom count and neighbour list stay as
+ * they are; for twojmax <= 8 the compaction, neighbour parameters and U_tot
+ * of the last force or adjoint call of a precision are then reused by later
+ * calls of that precision (they depend on the geometry only, not on beta).
+ * Calling it again (on or off) discards what was kept. */
+emd_status emd_snap_static_geometry(emd_ctx *ctx, emd_snap *sn, int on);
 /* ForceSNAP::compute, force_snap_neigh_impl.h:159-208 (+ operator() 589-725);
  * forces land on locals and ghosts, fold with emd_comm_update_force */
 emd_status emd_snap_force(emd_ctx *ctx, emd_system *s, const emd_neigh *nl, emd_snap *sn,

@@ -178,3 +178,47 @@ def test_lj_eps_sigma_gradient(oracle):
     assert np.abs(grads[emd.EMD_FP64] - o_fd).max() <= 1e-8 * scale
     assert np.abs(grads[emd.EMD_MIXED] - o_fd).max() <= 1e-5 * scale
     pl.close()
+
+
+@pytest.mark.parametrize("precision", [emd.EMD_FP64, emd.EMD_MIXED])
+def test_snap_static_geometry_reuse_matches_recomputation(precision):
+    """emd_snap_static_geometry: forces and beta adjoints of a fixed
+    configuration that reuse the kept compaction, neighbour parameters and
+    U_tot equal those recomputed from scratch (1e-12 relative in FP64, the
+    atomic force scatter aside), for a second beta too, and an energy call
+    in between (which rewrites U_tot in FP64) does not leave stale values"""
+    from helpers import rel_err
+    print(f"case: SNAP static geometry reuse, precision {precision}", flush=True)
+    a = 3.1803
+    x = lattice("bcc", 6, a) + np.random.default_rng(12).normal(0, 0.08, (432, 3))
+    beta1 = np.array(decks.snap_beta(seed=3, sigma=0.05))
+    beta2 = np.array(decks.snap_beta(seed=4, sigma=0.05))
+    gF = np.random.default_rng(5).normal(size=(432, 3))
+    tol = 1e-12 if precision == emd.EMD_FP64 else 1e-5
+    pl = Pipeline()
+    try:
+        pl.load(x, (6 * a,) * 3, mass=183.84)
+        pl.build(4.73442 + 1.0, newton=1)
+        pl.snap_setup(list(beta1))
+        L, c = pl.L, pl.ctx.h
+        want = {}
+        for b in (1, 2):
+            pl.snap_set_coeff(beta1 if b == 1 else beta2)
+            want[b] = (pl.snap_force(precision), pl.snap_adjoint(gF, 0.0, precision))
+        pl.chk(L.emd_snap_static_geometry(c, pl.snap, 1), "static on")
+        got = {}
+        for b in (1, 2, 1):
+            pl.snap_set_coeff(beta1 if b == 1 else beta2)
+            got[b] = (pl.snap_force(precision), pl.snap_adjoint(gF, 0.0, precision))
+        pl.snap_energy()
+        pl.snap_set_coeff(beta2)
+        after_energy = pl.snap_force(precision)
+        pl.chk(L.emd_snap_static_geometry(c, pl.snap, 0), "static off")
+    finally:
+        pl.close()
+    for b in (1, 2):
+        ef = rel_err(got[b][0], want[b][0])
+        ea = np.abs(got[b][1] - want[b][1]).max() / np.abs(want[b][1]).max()
+        print("beta", b, "force rel err", ef, "adjoint rel err", ea)
+        assert ef <= tol and ea <= tol
+    assert rel_err(after_energy, want[2][0]) <= tol
